@@ -1,0 +1,102 @@
+/* psx.h -- C ABI of the MI355X pseudo-spectral solver (libpsx.so).
+ *
+ * Drop-in boundary for PySpectral's serial hot path (PySpectral/src).  All state
+ * arrays are device pointers to complex128 values stored as interleaved
+ * (re, im) doubles in the reference's per-field Fourier layout
+ * (N1, N2, N3/2+1), C order, k1/k2 in FFT order, k3 >= 0
+ * (src/Classes.py:19-25,565-567), with the three velocity components stacked:
+ * shape (3, N1, N2, N3/2+1).  The reference interleaves the components into one
+ * (3N1, 3N2, 3(N3/2+1)) array Q (src/Classes.py:30-34); the Python layer maps
+ * that view onto this compact one.
+ *
+ * Every call is asynchronous on the context's HIP stream and returns 0 on
+ * success or a negative PSX_E* code; psx_last_error() gives the message.  No
+ * call allocates device memory except psx_create.  A context is not
+ * thread-safe; separate contexts are independent.
+ */
+#ifndef PSX_H
+#define PSX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSX_OK 0
+#define PSX_E_ARG (-1)
+#define PSX_E_HIP (-2)
+#define PSX_E_OOM (-3)
+#define PSX_E_UNSUPPORTED (-4)
+
+/* turb_model values (src/Classes.py:28,77,101,129,192) */
+#define PSX_MODEL_DNS 0          /* computeRHS_NOSGS (src/RHSfunctions.py:3-50)        */
+#define PSX_MODEL_SMAGORINSKY 1  /* computeRHS_SMAG  (src/RHSfunctions.py:146-237)     */
+#define PSX_MODEL_TMODEL 2       /* computeRHS_tmodel (src/RHSfunctions.py:240-363)    */
+#define PSX_MODEL_STATIC_TAU 3   /* computeRHS_staumodel (src/RHSfunctions.py:366-497) */
+#define PSX_MODEL_DYNAMIC_TAU 4  /* computeRHS_Dtaumodel (src/RHSfunctions.py:592-878) */
+
+typedef struct psx_params {
+  double nu;              /* viscosity (main.nu)                                     */
+  double L1, L2, L3;      /* box lengths (gridclass L1..L3, src/PySpec_3d.py:61-72)   */
+  double dx, dy, dz;      /* case grid spacings (src/Classes.py:562-564): CFL and Delta */
+  double cfl;             /* > 0: CFL-limited dt; <= 0: fixed dt = -cfl (src/Classes.py:687-698) */
+  double Cs;              /* Smagorinsky constant, 0.16 in the reference (src/RHSfunctions.py:197) */
+  double Ct;              /* t-model constant (src/Classes.py:103-108)                */
+  double tau_a, tau_b;    /* static tau-model tau = tau_a + tau_b t (src/Classes.py:194-195) */
+  int kc;                 /* sharp cutoff of Gf (src/Classes.py:584-588)              */
+  int rotate;             /* Coriolis terms on/off (src/RHSfunctions.py:33-50)        */
+  double Om1, Om2, Om3;   /* rotation vector                                         */
+} psx_params;
+
+typedef struct psx_context psx_context;
+
+/* Replaces FFTclass + gridclass + the turb_model dispatch of variables
+ * (src/Classes.py:590-625, 548-588, 28-229): plans, twiddle tables and pipeline
+ * scratch for an N1 x N2 x N3 grid.  stream is a hipStream_t (NULL = default). */
+int psx_create(psx_context** ctx, int n1, int n2, int n3, int model, const psx_params* params, int device,
+               void* stream);
+int psx_destroy(psx_context* ctx);
+int psx_set_stream(psx_context* ctx, void* stream);
+int psx_set_params(psx_context* ctx, const psx_params* params);
+const char* psx_last_error(const psx_context* ctx);
+/* Bytes of device scratch the context holds. */
+long long psx_scratch_bytes(const psx_context* ctx);
+
+/* main.computeRHS(main, grid, myFFT, utilities) (src/RHSfunctions.py:3,146,240,366,592):
+ * rhs = RHS(uhat) at step-start time t.  uhat_model (nullable) receives the
+ * velocity as the model leaves it in main.uhat (Nyquist entries zeroed for the
+ * models that truncate, src/RHSfunctions.py:10-12).  w0 (nullable) receives the
+ * subgrid term main.w0_{u,v,w} (closures only). */
+int psx_rhs(psx_context* ctx, const double* uhat, double* rhs, double* uhat_model, double* w0, double t);
+
+/* One stage of advanceQ_RK4 (src/PySpec_3d.py:102-104): out = q0 + dt*c*RHS(uin),
+ * dt read from the device scalar *dt_dev.  out may alias uin or q0. */
+int psx_rk_stage(psx_context* ctx, const double* q0, const double* uin, double* out, const double* dt_dev,
+                 double c, double t, int zero_nyquist_out);
+
+/* utilitiesClass.compute_dt (src/Classes.py:687-698) on uhat, result in *dt_dev. */
+int psx_compute_dt(psx_context* ctx, const double* uhat, double* dt_dev);
+
+/* advanceQ_RK4 (src/PySpec_3d.py:96-105): dt from the stale velocity qs (the
+ * reference's main.uhat), then the four low-storage stages.  On return q holds
+ * the new state and qs the stage-4 input as the model left it (main.uhat after
+ * the step).  Before the first step qs must hold the initial velocity. */
+int psx_step(psx_context* ctx, double* q, double* qs, double* dt_dev, double t);
+
+/* utilitiesClass.computeEnergy / computeEnstrophy / computeEnergy_resolved
+ * (src/Classes.py:652-676,733-744): out_dev[0..2] = E, Omega, E_resolved. */
+int psx_stats(psx_context* ctx, const double* uhat, double* out_dev);
+
+/* computeSpectrum / computeSpectrum_resolved (src/Classes.py:762-797,825-844):
+ * spec_dev[(kmax+1) x 3], kmax from psx_spectrum_bins(). */
+int psx_spectrum_bins(const psx_context* ctx);
+int psx_spectrum(psx_context* ctx, const double* uhat, double* spec_dev, int resolved);
+
+/* Scalar written by the dynamic tau-model each RHS evaluation (main.tau,
+ * src/RHSfunctions.py:857-861), as a device pointer to 4 doubles:
+ * tau, LE, PLQLE_filt, filt_PLQLE (real parts; magnitudes for the complex sums). */
+const double* psx_tau_device(const psx_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSX_H */

@@ -1,0 +1,89 @@
+"""ctypes binding of libpsx.so (the C ABI declared in include/psx.h).
+
+The library is loaded from this package directory; there is no fallback: if it
+is missing or cannot be loaded every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsx.so")
+
+PSX_OK = 0
+MODEL_IDS = {
+    0: 0, "dns": 0,
+    1: 1, "smagorinsky": 1,
+    2: 2, "tmodel": 2,
+    "Static tau-model": 3, "static_tau": 3,
+    "Dynamic tau-model": 4, "dynamic_tau": 4,
+}
+
+# every symbol include/psx.h declares
+EXPORTS = (
+    "psx_create", "psx_destroy", "psx_set_stream", "psx_set_params", "psx_last_error",
+    "psx_scratch_bytes", "psx_rhs", "psx_rk_stage", "psx_compute_dt", "psx_step", "psx_stats",
+    "psx_spectrum_bins", "psx_spectrum", "psx_tau_device",
+)
+
+
+class psx_params(C.Structure):
+    _fields_ = [
+        ("nu", C.c_double),
+        ("L1", C.c_double), ("L2", C.c_double), ("L3", C.c_double),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("cfl", C.c_double),
+        ("Cs", C.c_double),
+        ("Ct", C.c_double),
+        ("tau_a", C.c_double), ("tau_b", C.c_double),
+        ("kc", C.c_int),
+        ("rotate", C.c_int),
+        ("Om1", C.c_double), ("Om2", C.c_double), ("Om3", C.c_double),
+    ]
+
+
+class PsxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libpsx.so and declare its signatures (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PsxError(f"{LIB_PATH} is not built; run pyspectral_amd/build_lib.py (or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    vp, dp, i, d = C.c_void_p, C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "psx_create": (i, [C.POINTER(vp), i, i, i, i, C.POINTER(psx_params), i, vp]),
+        "psx_destroy": (i, [vp]),
+        "psx_set_stream": (i, [vp, vp]),
+        "psx_set_params": (i, [vp, C.POINTER(psx_params)]),
+        "psx_last_error": (C.c_char_p, [vp]),
+        "psx_scratch_bytes": (C.c_longlong, [vp]),
+        "psx_rhs": (i, [vp, dp, dp, dp, dp, d]),
+        "psx_rk_stage": (i, [vp, dp, dp, dp, dp, d, d, i]),
+        "psx_compute_dt": (i, [vp, dp, dp]),
+        "psx_step": (i, [vp, dp, dp, dp, d]),
+        "psx_stats": (i, [vp, dp, dp]),
+        "psx_spectrum_bins": (i, [vp]),
+        "psx_spectrum": (i, [vp, dp, dp, i]),
+        "psx_tau_device": (C.c_void_p, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(ctx, rc: int, what: str) -> None:
+    if rc != PSX_OK:
+        msg = load().psx_last_error(ctx).decode() if ctx else ""
+        raise PsxError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,176 @@
+// Batched FP64 complex FFTs staged in LDS, for gfx950 (wave64).
+//
+// Algorithm: mixed-radix Stockham autosort (Govindaraju et al., "High Performance
+// Discrete Fourier Transforms on Graphics Processors", SC'08, Fig. 2): stage s
+// with radix R reads butterfly j's inputs at j + r*L/R, applies the twiddles
+// W_{Ns*R}^{(j mod Ns) r}, does an R-point DFT and writes to
+// (j / Ns)*Ns*R + (j mod Ns) + r*Ns.  Output is in natural order.  Each stage
+// reads all of a thread's butterflies into registers, barriers, and writes them
+// back in place, so one LDS buffer of B*LS complex values holds the batch.
+//
+// Twiddles come from a precomputed table tw[m] = exp(-2*pi*i*m/L), m in [0, L),
+// built on the host in double precision (no trigonometry in the kernels).
+// SIGN = -1 is the forward transform exp(-i...), SIGN = +1 the unnormalised
+// inverse exp(+i...).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace psx {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// multiply by +i (S = +1) or -i (S = -1)
+template <int S>
+__device__ __forceinline__ double2 cmul_i(double2 a) {
+  return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- radix plans
+template <int L> struct Plan;
+#define PSX_PLAN(LEN, ...)                                        \
+  template <> struct Plan<LEN> {                                  \
+    static constexpr int r[] = {__VA_ARGS__};                     \
+    static constexpr int n = sizeof(r) / sizeof(int);             \
+  };
+PSX_PLAN(8, 8)
+PSX_PLAN(12, 4, 3)
+PSX_PLAN(16, 4, 4)
+PSX_PLAN(24, 8, 3)
+PSX_PLAN(32, 8, 4)
+PSX_PLAN(36, 4, 3, 3)
+PSX_PLAN(48, 4, 4, 3)
+PSX_PLAN(64, 8, 8)
+PSX_PLAN(72, 8, 3, 3)
+PSX_PLAN(96, 8, 4, 3)
+PSX_PLAN(128, 8, 4, 4)
+PSX_PLAN(144, 4, 4, 3, 3)
+PSX_PLAN(192, 8, 8, 3)
+PSX_PLAN(256, 8, 8, 4)
+PSX_PLAN(288, 8, 4, 3, 3)
+PSX_PLAN(384, 8, 4, 4, 3)
+PSX_PLAN(512, 8, 8, 8)
+PSX_PLAN(576, 8, 8, 3, 3)
+PSX_PLAN(768, 8, 8, 4, 3)
+#undef PSX_PLAN
+
+// ---------------------------------------------------------------- small DFTs
+template <int R, int SIGN> struct Dft;
+
+template <int SIGN> struct Dft<2, SIGN> {
+  static __device__ __forceinline__ void run(double2* v) {
+    double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int SIGN> struct Dft<3, SIGN> {
+  static __device__ __forceinline__ void run(double2* v) {
+    // exp(SIGN*2*pi*i/3) = -1/2 + SIGN*i*sqrt(3)/2
+    constexpr double h = 0.86602540378443864676372317075294;  // sqrt(3)/2
+    double2 t = cadd(v[1], v[2]);
+    double2 d = csub(v[1], v[2]);
+    double2 m = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
+    double2 r = cmul_i<SIGN>(cscale(d, h));
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+
+template <int SIGN> struct Dft<4, SIGN> {
+  static __device__ __forceinline__ void run(double2* v) {
+    double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    double2 t2 = cadd(v[1], v[3]), t3 = cmul_i<SIGN>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <int SIGN> struct Dft<8, SIGN> {
+  static __device__ __forceinline__ void run(double2* v) {
+    constexpr double s = 0.70710678118654752440084436210485;  // 1/sqrt(2)
+    double2 e[4] = {v[0], v[2], v[4], v[6]};
+    double2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, SIGN>::run(e);
+    Dft<4, SIGN>::run(o);
+    // W^k = exp(SIGN*2*pi*i*k/8)
+    double2 w1 = make_double2(s * (o[1].x - SIGN * o[1].y), s * (o[1].y + SIGN * o[1].x));
+    double2 w2 = cmul_i<SIGN>(o[2]);
+    double2 w3 = make_double2(s * (-o[3].x - SIGN * o[3].y), s * (-o[3].y + SIGN * o[3].x));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
+  }
+};
+
+// ---------------------------------------------------------------- Stockham batch
+// B transforms of length L, transform b at buf[b*LS .. b*LS+L), NT threads.
+template <int L, int B, int NT, int LS, int SIGN, int S, int NS>
+__device__ __forceinline__ void fft_stages(double2* buf, const double2* __restrict__ tw, int tid) {
+  if constexpr (S < Plan<L>::n) {
+    constexpr int R = Plan<L>::r[S];
+    constexpr int NBF = L / R;
+    constexpr int TOT = B * NBF;
+    constexpr int IT = (TOT + NT - 1) / NT;
+    constexpr int TWS = L / (NS * R);
+    double2 v[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int j = tid + it * NT;
+      if ((TOT % NT) == 0 || j < TOT) {
+        const int b = j / NBF;
+        const int jj = j - b * NBF;
+        const double2* src = buf + b * LS + jj;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[it][r] = src[r * NBF];
+        if constexpr (NS > 1) {
+          const int k = jj % NS;
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            double2 w = tw[k * r * TWS];
+            if constexpr (SIGN > 0) w.y = -w.y;
+            v[it][r] = cmul(v[it][r], w);
+          }
+        }
+        Dft<R, SIGN>::run(v[it]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int j = tid + it * NT;
+      if ((TOT % NT) == 0 || j < TOT) {
+        const int b = j / NBF;
+        const int jj = j - b * NBF;
+        const int k = jj % NS;
+        double2* dst = buf + b * LS + (jj / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[r * NS] = v[it][r];
+      }
+    }
+    __syncthreads();
+    fft_stages<L, B, NT, LS, SIGN, S + 1, NS * R>(buf, tw, tid);
+  }
+}
+
+// Run the batch.  Caller must __syncthreads() before (data staged) — the routine
+// ends with a barrier, so results are visible on return.
+template <int L, int B, int NT, int LS, int SIGN>
+__device__ __forceinline__ void fft_batch(double2* buf, const double2* __restrict__ tw, int tid) {
+  fft_stages<L, B, NT, LS, SIGN, 0, 1>(buf, tw, tid);
+}
+
+}  // namespace psx

@@ -1,0 +1,70 @@
+// Fused z-pass instantiations, mode Z_SMAG.
+#include "launch.hpp"
+
+namespace psx {
+
+constexpr int zp_smag(int L) { return z_P(L, ZShape<Z_SMAG>::nin > ZShape<Z_SMAG>::nout ? ZShape<Z_SMAG>::nin : ZShape<Z_SMAG>::nout); }
+
+int zpass_positions_smag(int L) {
+  switch (L) {
+    case 24: return zp_smag(24);
+    case 48: return zp_smag(48);
+    case 72: return zp_smag(72);
+    case 96: return zp_smag(96);
+    case 144: return zp_smag(144);
+    case 192: return zp_smag(192);
+    case 288: return zp_smag(288);
+    case 384: return zp_smag(384);
+    default: return 0;
+  }
+}
+
+hipError_t launch_zpass_smag(int L, const ZArgs& a, const double2* tw, hipStream_t s) {
+  switch (L) {
+    case 24: {
+      constexpr int P = zp_smag(24);
+      hipLaunchKernelGGL((zpass<24, P, Z_SMAG>), dim3((unsigned)(a.npos / P)), dim3(NT), 0, s, a, tw);
+      break;
+    }
+    case 48: {
+      constexpr int P = zp_smag(48);
+      hipLaunchKernelGGL((zpass<48, P, Z_SMAG>), dim3((unsigned)(a.npos / P)), dim3(NT), 0, s, a, tw);
+      break;
+    }
+    case 72: {
+      constexpr int P = zp_smag(72);
+      hipLaunchKernelGGL((zpass<72, P, Z_SMAG>), dim3((unsigned)(a.npos / P)), dim3(NT), 0, s, a, tw);
+      break;
+    }
+    case 96: {
+      constexpr int P = zp_smag(96);
+      hipLaunchKernelGGL((zpass<96, P, Z_SMAG>), dim3((unsigned)(a.npos / P)), dim3(NT), 0, s, a, tw);
+      break;
+    }
+    case 144: {
+      constexpr int P = zp_smag(144);
+      hipLaunchKernelGGL((zpass<144, P, Z_SMAG>), dim3((unsigned)(a.npos / P)), dim3(NT), 0, s, a, tw);
+      break;
+    }
+    case 192: {
+      constexpr int P = zp_smag(192);
+      hipLaunchKernelGGL((zpass<192, P, Z_SMAG>), dim3((unsigned)(a.npos / P)), dim3(NT), 0, s, a, tw);
+      break;
+    }
+    case 288: {
+      constexpr int P = zp_smag(288);
+      hipLaunchKernelGGL((zpass<288, P, Z_SMAG>), dim3((unsigned)(a.npos / P)), dim3(NT), 0, s, a, tw);
+      break;
+    }
+    case 384: {
+      constexpr int P = zp_smag(384);
+      hipLaunchKernelGGL((zpass<384, P, Z_SMAG>), dim3((unsigned)(a.npos / P)), dim3(NT), 0, s, a, tw);
+      break;
+    }
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace psx

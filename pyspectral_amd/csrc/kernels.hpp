@@ -1,0 +1,556 @@
+// Pseudo-spectral pipeline kernels for gfx950.
+//
+// Layouts (complex128 = double2, kz contiguous everywhere):
+//   state  S[c][k1][k2][k3]       (3, N1, N2, N3/2+1)  -- the reference's per-field layout
+//   X      X[f][x ][k2][k3<H]     after the x-inverse pass (x padded to M1)
+//   Y      Y[f][x ][y ][k3<H]     after the y-inverse pass (y padded to M2)
+//   Z      Z[f][x ][y ][k3<Ho]    after the fused z pass (products transformed back in z)
+//   W      W[f][x ][k2][k3<Ho]    after the y-forward pass (truncated in k2)
+// The x-forward pass is fused with the spectral epilogue (projection, closures,
+// viscous term, RK stage update) and writes the state layout.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_core.hpp"
+
+namespace psx {
+
+constexpr int NT = 256;       // threads per block
+constexpr int MAXJOBS = 16;   // input fields per pencil pass
+constexpr int MAXREC = 12;    // real pencils built by the z pass
+
+// ------------------------------------------------------------------ index maps
+// Signed frequency of FFT-ordered spectral index i on an n-point axis; index n/2
+// is the -n/2 "oddball" mode, as in the reference's wavenumber arrays
+// (src/Classes.py:565-566).
+__device__ __forceinline__ int signed_freq(int i, int n) { return i < n / 2 ? i : i - n; }
+
+// Slot of spectral index i (of n) in an L-point transform, or -1 when the index
+// is dropped.  keep_nyq = 0 drops index n/2 (3/2 rule, src/padding.py:8-24);
+// keep_nyq = 1 keeps it at L - n/2 (2x rule, src/padding.py:34-48) and, for L == n,
+// gives the identity.
+__device__ __forceinline__ int pad_slot(int i, int n, int L, int keep_nyq) {
+  if (i < n / 2) return i;
+  if (i == n / 2 && !keep_nyq) return -1;
+  return L + i - n;
+}
+
+// ------------------------------------------------------------------ pencil passes
+struct PencilArgs {
+  const double2* in;
+  double2* out;
+  long long in_sf, in_sa, in_sl;     // strides (complex units): field, outer, transform axis
+  long long out_sf, out_sa, out_sl;
+  int A;         // outer extent
+  int C;         // contiguous (k3) extent
+  int Nspec;     // spectral entries along the transform axis
+  int keep_nyq;  // see pad_slot
+  double kscale; // 2*pi/L_axis, the derivative factor
+  double scale;  // applied to every output
+  int job_in[MAXJOBS];
+  int job_plain[MAXJOBS];
+  int job_deriv[MAXJOBS];
+};
+
+// Inverse along a strided axis: Nspec spectral entries -> L physical points, with
+// the zero-pad fused in; optionally a second output multiplied by i*k_axis.
+// grid = (A * ceil(C/T), njobs)
+template <int L, int T>
+__global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2* __restrict__ tw) {
+  constexpr int LS = L + 1;
+  __shared__ double2 buf[T * LS];
+  const int tid = threadIdx.x;
+  const int ct = (a.C + T - 1) / T;
+  const int aidx = blockIdx.x / ct;
+  const int c0 = (blockIdx.x - aidx * ct) * T;
+  const int job = blockIdx.y;
+  const double2* src = a.in + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
+  constexpr int IT = (L * T + NT - 1) / NT;
+  double2 r[IT];
+  const int nload = a.Nspec * T;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * NT;
+    r[it] = make_double2(0.0, 0.0);
+    if (e < nload) {
+      const int l = e / T, c = e - (e / T) * T;
+      if (c0 + c < a.C) r[it] = src[(long long)l * a.in_sl + c];
+    }
+  }
+  for (int var = 0; var < 2; ++var) {
+    const int fo = var == 0 ? a.job_plain[job] : a.job_deriv[job];
+    if (fo < 0) continue;
+    for (int e = tid; e < T * L; e += NT) buf[(e / L) * LS + (e % L)] = make_double2(0.0, 0.0);
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (e < nload) {
+        const int l = e / T, c = e - (e / T) * T;
+        const int p = pad_slot(l, a.Nspec, L, a.keep_nyq);
+        if (p >= 0) {
+          double2 v = r[it];
+          if (var == 1) v = cscale(cmul_i<1>(v), a.kscale * signed_freq(l, a.Nspec));
+          buf[c * LS + p] = v;
+        }
+      }
+    }
+    __syncthreads();
+    fft_batch<L, T, NT, LS, +1>(buf, tw, tid);
+    double2* dst = a.out + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
+    for (int e = tid; e < L * T; e += NT) {
+      const int p = e / T, c = e - (e / T) * T;
+      if (c0 + c < a.C) dst[(long long)p * a.out_sl + c] = cscale(buf[c * LS + p], a.scale);
+    }
+    __syncthreads();
+  }
+}
+
+// Forward along a strided axis: L physical points -> Nspec spectral entries, with
+// the truncation fused in.  grid = (A * ceil(C/T), njobs)
+template <int L, int T>
+__global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2* __restrict__ tw) {
+  constexpr int LS = L + 1;
+  __shared__ double2 buf[T * LS];
+  const int tid = threadIdx.x;
+  const int ct = (a.C + T - 1) / T;
+  const int aidx = blockIdx.x / ct;
+  const int c0 = (blockIdx.x - aidx * ct) * T;
+  const int job = blockIdx.y;
+  const double2* src = a.in + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
+  for (int e = tid; e < L * T; e += NT) {
+    const int l = e / T, c = e - (e / T) * T;
+    buf[c * LS + l] = (c0 + c < a.C) ? src[(long long)l * a.in_sl + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
+  const int fo = a.job_plain[job];
+  double2* dst = a.out + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
+  for (int e = tid; e < a.Nspec * T; e += NT) {
+    const int i = e / T, c = e - (e / T) * T;
+    if (c0 + c < a.C) {
+      const int p = pad_slot(i, a.Nspec, L, a.keep_nyq);
+      dst[(long long)i * a.out_sl + c] = p >= 0 ? cscale(buf[c * LS + p], a.scale) : make_double2(0.0, 0.0);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ fused z pass
+enum ZMode { Z_DNS = 0, Z_SMAG = 1, Z_TM2 = 2, Z_DT = 3 };
+
+template <int MODE> struct ZShape;
+template <> struct ZShape<Z_DNS> { static constexpr int nin = 3, nout = 6; };
+template <> struct ZShape<Z_SMAG> { static constexpr int nin = 9, nout = 12; };
+template <> struct ZShape<Z_TM2> { static constexpr int nin = 6, nout = 9; };
+template <> struct ZShape<Z_DT> { static constexpr int nin = 3, nout = 0; };
+
+// A real z-pencil's half spectrum is built as sum_t coef_t (i k3)^{dz_t} field_{f_t}.
+struct Recipe {
+  int nterm;
+  int f[2];
+  double coef[2];
+  int dz[2];
+};
+
+struct ZArgs {
+  const double2* in0;     // input fields f < nin0
+  const double2* in1;     // input fields f >= nin0
+  long long in_sf0, in_sf1;
+  int Hin0, Hin1, nin0;
+  double2* out;
+  long long out_sf;
+  int Hout;
+  int npos;               // physical (x, y) positions
+  Recipe rec[MAXREC];
+  double kzscale;         // 2*pi/L3
+  double out_scale;       // multiplies every product before the forward transform
+  double c2;              // (Cs*Delta)^2 for the Smagorinsky stress
+  double dinv[3];         // 1/dx, 1/dy, 1/dz (CFL reduction)
+  unsigned long long* red;  // CFL max accumulator (bit pattern of a non-negative double)
+};
+
+__device__ __forceinline__ double2 zin(const ZArgs& a, int f, long long pos, int k) {
+  if (f < a.nin0) {
+    return k < a.Hin0 ? a.in0[(long long)f * a.in_sf0 + pos * a.Hin0 + k] : make_double2(0.0, 0.0);
+  }
+  const int g = f - a.nin0;
+  return k < a.Hin1 ? a.in1[(long long)g * a.in_sf1 + pos * a.Hin1 + k] : make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ double2 zspec(const ZArgs& a, int r, long long pos, int k) {
+  const Recipe& R = a.rec[r];
+  double2 acc = make_double2(0.0, 0.0);
+  for (int t = 0; t < R.nterm; ++t) {
+    double2 v = zin(a, R.f[t], pos, k);
+    double s = R.coef[t];
+    if (R.dz[t]) {
+      v = cmul_i<1>(v);
+      s *= a.kzscale * k;
+    }
+    acc.x += s * v.x;
+    acc.y += s * v.y;
+  }
+  return acc;
+}
+
+template <int MODE>
+__device__ __forceinline__ void zpoint(const ZArgs& a, const double* in, double* out) {
+  if constexpr (MODE == Z_DNS) {
+    const double u = in[0], v = in[1], w = in[2], s = a.out_scale;
+    out[0] = s * (u * u);
+    out[1] = s * (v * v);
+    out[2] = s * (w * w);
+    out[3] = s * (u * v);
+    out[4] = s * (u * w);
+    out[5] = s * (v * w);
+  } else if constexpr (MODE == Z_SMAG) {
+    // src/RHSfunctions.py:195-212: |S| = sqrt(2 S_ij S_ij), nu_t = (Cs Delta)^2 |S|,
+    // tau_ij = -2 nu_t S_ij; the stress is quadratic in the velocity, so it takes
+    // the same normalisation factor as the products.
+    const double u = in[0], v = in[1], w = in[2], s = a.out_scale;
+    const double S11 = in[3], S22 = in[4], S33 = in[5], S12 = in[6], S13 = in[7], S23 = in[8];
+    const double Smag = sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * S12 * S12 +
+                                    2.0 * S13 * S13 + 2.0 * S23 * S23));
+    const double m2nut = -2.0 * a.c2 * Smag * s;
+    out[0] = s * (u * u);
+    out[1] = s * (v * v);
+    out[2] = s * (w * w);
+    out[3] = s * (u * v);
+    out[4] = s * (u * w);
+    out[5] = s * (v * w);
+    out[6] = m2nut * S11;
+    out[7] = m2nut * S22;
+    out[8] = m2nut * S33;
+    out[9] = m2nut * S12;
+    out[10] = m2nut * S13;
+    out[11] = m2nut * S23;
+  } else if constexpr (MODE == Z_TM2) {
+    // u_j * (PL_q)_i, stored at index 3 i + j (src/RHSfunctions.py:325-335)
+    const double s = a.out_scale;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) out[3 * i + j] = s * (in[j] * in[3 + i]);
+  }
+}
+
+// One block = P consecutive (x, y) positions (P even).  Real pencils of field f at
+// positions 2pp and 2pp+1 share complex pencil f*(P/2)+pp: a single complex FFT
+// transforms both (inverse: Z = A + iB with the Hermitian halves mirrored;
+// forward: A = (Z_k + conj Z_{L-k})/2, B = (Z_k - conj Z_{L-k})/(2i)).
+template <int L, int P, int MODE>
+__global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__ tw) {
+  constexpr int NIN = ZShape<MODE>::nin;
+  constexpr int NOUT = ZShape<MODE>::nout;
+  constexpr int HP = P / 2;
+  constexpr int NCI = NIN * HP;
+  constexpr int NCO = NOUT * HP;
+  constexpr int NCMAX = NCI > NCO ? NCI : NCO;
+  constexpr int LS = L + 1;
+  constexpr int L2 = L / 2;
+  __shared__ double2 buf[NCMAX * LS];
+  const int tid = threadIdx.x;
+  const long long pos0 = (long long)blockIdx.x * P;
+
+  // --- build the inverse spectra (Re-only at k = 0 and k = L/2, as c2r does)
+  for (int e = tid; e < NCI * (L2 + 1); e += NT) {
+    const int q = e / (L2 + 1), k = e - q * (L2 + 1);
+    const int f = q / HP, pp = q - f * HP;
+    const double2 A = zspec(a, f, pos0 + 2 * pp, k);
+    const double2 B = zspec(a, f, pos0 + 2 * pp + 1, k);
+    if (k == 0 || k == L2) {
+      buf[q * LS + k] = make_double2(A.x, B.x);
+    } else {
+      buf[q * LS + k] = make_double2(A.x - B.y, A.y + B.x);
+      buf[q * LS + L - k] = make_double2(A.x + B.y, B.x - A.y);
+    }
+  }
+  __syncthreads();
+  fft_batch<L, NCI, NT, LS, +1>(buf, tw, tid);
+
+  if constexpr (MODE == Z_DT) {
+    double m = 0.0;
+    for (int e = tid; e < HP * L; e += NT) {
+      const int pp = e / L, z = e - pp * L;
+      const double2 u = buf[(0 * HP + pp) * LS + z];
+      const double2 v = buf[(1 * HP + pp) * LS + z];
+      const double2 w = buf[(2 * HP + pp) * LS + z];
+      const double m0 = fabs(u.x) * a.dinv[0] + fabs(v.x) * a.dinv[1] + fabs(w.x) * a.dinv[2];
+      const double m1 = fabs(u.y) * a.dinv[0] + fabs(v.y) * a.dinv[1] + fabs(w.y) * a.dinv[2];
+      m = fmax(m, fmax(m0, m1));
+    }
+    // wave reduction then one atomic per wave (max is order independent)
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((tid & 63) == 0) atomicMax(a.red, (unsigned long long)__double_as_longlong(m));
+    return;
+  } else {
+    // --- pointwise products / closure on pairs of positions
+    constexpr int NPTS = HP * L;
+    constexpr int IT = (NPTS + NT - 1) / NT;
+    double2 res[IT][NOUT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if ((NPTS % NT) == 0 || e < NPTS) {
+        const int pp = e / L, z = e - pp * L;
+        double in0[NIN], in1[NIN], o0[NOUT], o1[NOUT];
+#pragma unroll
+        for (int f = 0; f < NIN; ++f) {
+          const double2 v = buf[(f * HP + pp) * LS + z];
+          in0[f] = v.x;
+          in1[f] = v.y;
+        }
+        zpoint<MODE>(a, in0, o0);
+        zpoint<MODE>(a, in1, o1);
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) res[it][o] = make_double2(o0[o], o1[o]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if ((NPTS % NT) == 0 || e < NPTS) {
+        const int pp = e / L, z = e - pp * L;
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) buf[(o * HP + pp) * LS + z] = res[it][o];
+      }
+    }
+    __syncthreads();
+    fft_batch<L, NCO, NT, LS, -1>(buf, tw, tid);
+    for (int e = tid; e < NCO * a.Hout; e += NT) {
+      const int q = e / a.Hout, k = e - q * a.Hout;
+      const int o = q / HP, pp = q - o * HP;
+      const double2 z1 = buf[q * LS + k];
+      const double2 z2 = buf[q * LS + (k == 0 ? 0 : L - k)];
+      const double2 A = make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
+      const double2 B = make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
+      double2* dst = a.out + (long long)o * a.out_sf + (pos0 + 2 * pp) * a.Hout + k;
+      dst[0] = A;
+      dst[a.Hout] = B;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ x-forward + epilogue
+enum EpiMode { E_DNS = 0, E_SMAG = 1, E_TAU2 = 2 };
+template <int EPI> struct EpiShape;
+template <> struct EpiShape<E_DNS> { static constexpr int nf = 6; };
+template <> struct EpiShape<E_SMAG> { static constexpr int nf = 12; };
+template <> struct EpiShape<E_TAU2> { static constexpr int nf = 9; };
+
+struct EpiArgs {
+  const double2* in;          // W layout [f][x][k2][k3<H]
+  long long in_sf;            // field stride
+  int N1, N2, N3, Nc, H;
+  int keep_nyq;               // x truncation rule (see pad_slot)
+  const double2* uprev;       // state-layout velocity the RHS is evaluated at
+  const double2* q0;          // stage base Q0 (nullptr: write the RHS itself)
+  double2* out;               // state layout
+  double2* w0;                // optional: the SGS term w0 (state layout), or nullptr
+  const double2* pl;          // E_TAU2: truncated PL (state layout)
+  const double* dt;           // device dt
+  double c;                   // stage coefficient
+  double nu;
+  double ks[3];               // 2*pi/L_i
+  int trunc_u;                // the model zeroes Nyquist entries of uhat before use
+  int zero_nyq_out;           // zero Nyquist entries of the output
+  double coef;                // E_TAU2: w0 coefficient (host value) ...
+  const double* coef_dev;     // ... or a device scalar (overrides coef when set)
+  int rotate;
+  double Om[3];
+};
+
+// Accumulate the contribution of transformed field f (value v) to the three RHS
+// components at wavenumber (k1, k2, k3), kinv = 1/(|k|^2 + 1e-50).
+template <int EPI>
+__device__ __forceinline__ void epi_accumulate(int f, const double* k, double kinv, double2 v, double2* acc,
+                                               double2* acc_sgs) {
+  if constexpr (EPI == E_DNS || EPI == E_SMAG) {
+    // fields 0..5: N_mn = u_m u_n, 6..11: tau_mn; (mn) in 11,22,33,12,13,23
+    constexpr int MI[6] = {0, 1, 2, 0, 0, 1};
+    constexpr int NI[6] = {0, 1, 2, 1, 2, 2};
+    const int g = f % 6;
+    const int m = MI[g], n = NI[g];
+    const double s = (m == n) ? 1.0 : 2.0;
+    const double kk = s * k[m] * k[n] * kinv;
+    const double2 mi = cmul_i<-1>(v);  // -i v
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 d = make_double2(0.0, 0.0);
+      if (i == m) d = cadd(d, cscale(mi, k[n]));
+      if (i == n && m != n) d = cadd(d, cscale(mi, k[m]));
+      if (f < 6) {
+        // + i k_i (k_m k_n N_mn)/k^2: the Leray projection (src/RHSfunctions.py:29-45)
+        d = cadd(d, cscale(cmul_i<1>(v), k[i] * kk));
+      } else {
+        // + 1j*k_i*(1j*ksqr_i*k_m k_n tau_mn) as written (src/RHSfunctions.py:215-222)
+        d = csub(d, cscale(v, k[i] * kk));
+      }
+      acc[i] = cadd(acc[i], d);
+      if (acc_sgs != nullptr && f >= 6) acc_sgs[i] = cadd(acc_sgs[i], d);
+    }
+  } else if constexpr (EPI == E_TAU2) {
+    // field 3 i + j holds u_j (PL_q)_i; PLQL_m = sum_j -i k_j (A_ji + A_ij) + i k_m pterm,
+    // pterm = 2/k^2 sum_mn k_m k_n A_mn (src/RHSfunctions.py:338-352)
+    const int i = f / 3, j = f - 3 * (f / 3);
+    const double2 mi = cmul_i<-1>(v);
+    const double pk = 2.0 * k[i] * k[j] * kinv;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double2 d = cscale(cmul_i<1>(v), k[m] * pk);
+      if (m == i) d = cadd(d, cscale(mi, k[j]));
+      if (m == j) d = cadd(d, cscale(mi, k[i]));
+      acc[m] = cadd(acc[m], d);
+    }
+  }
+}
+
+// One block = (k2, k3-tile of T) running the x-forward FFT of every field and
+// accumulating the spectral RHS in registers; blocks past N2*ceil(H/T) handle the
+// k3 >= H planes (no transformed data there) elementwise.
+template <int L, int NS1, int T, int EPI, bool W0>
+__global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2* __restrict__ tw) {
+  constexpr int LS = L + 1;
+  constexpr int NF = EpiShape<EPI>::nf;
+  constexpr int IT = (NS1 * T + NT - 1) / NT;
+  __shared__ double2 buf[T * LS];
+  const int tid = threadIdx.x;
+  const int ct = (a.H + T - 1) / T;
+  const int nmain = a.N2 * ct;
+  const long long plane = (long long)a.N2 * a.Nc;
+  const long long fstride = (long long)a.N1 * plane;  // component stride in the state layout
+
+  if ((int)blockIdx.x >= nmain) {
+    // k3 in [H, Nc): elementwise (no transformed fields)
+    const int nrest = a.Nc - a.H;
+    const long long total = (long long)a.N1 * a.N2 * nrest;
+    for (long long e = (long long)(blockIdx.x - nmain) * NT + tid; e < total;
+         e += (long long)(gridDim.x - nmain) * NT) {
+      const int kz = a.H + (int)(e % nrest);
+      const long long r = e / nrest;
+      const int ky = (int)(r % a.N2), kx = (int)(r / a.N2);
+      const long long idx = (long long)kx * plane + (long long)ky * a.Nc + kz;
+      double k[3] = {a.ks[0] * signed_freq(kx, a.N1), a.ks[1] * signed_freq(ky, a.N2), a.ks[2] * kz};
+      const double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+      const bool nyq = (kx == a.N1 / 2) || (ky == a.N2 / 2) || (kz == a.N3 / 2);
+      double2 u[3], rhs[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        u[c] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : a.uprev[c * fstride + idx];
+        rhs[c] = EPI == E_TAU2 ? a.pl[c * fstride + idx] : cscale(u[c], -a.nu * (ksq + 1e-50));
+        if (EPI == E_TAU2 && a.w0) a.w0[c * fstride + idx] = make_double2(0.0, 0.0);
+        if (EPI == E_SMAG && a.w0) a.w0[c * fstride + idx] = make_double2(0.0, 0.0);
+      }
+      if (a.rotate) {
+        const double kinv = 1.0 / (ksq + 1e-50);
+        double2 cr[3];
+        cr[0] = csub(cscale(u[1], a.Om[2]), cscale(u[2], a.Om[1]));
+        cr[1] = csub(cscale(u[2], a.Om[0]), cscale(u[0], a.Om[2]));
+        cr[2] = csub(cscale(u[0], a.Om[1]), cscale(u[1], a.Om[0]));
+        const double2 X = cadd(cadd(cscale(cr[0], k[0]), cscale(cr[1], k[1])), cscale(cr[2], k[2]));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rhs[c] = cadd(rhs[c], csub(cscale(cr[c], 2.0), cscale(X, 2.0 * k[c] * kinv)));
+      }
+      const double dtc = a.q0 ? (*a.dt) * a.c : 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double2 o = a.q0 ? cadd(a.q0[c * fstride + idx], cscale(rhs[c], dtc)) : rhs[c];
+        if (a.zero_nyq_out && nyq) o = make_double2(0.0, 0.0);
+        a.out[c * fstride + idx] = o;
+      }
+    }
+    return;
+  }
+
+  const int ky = blockIdx.x / ct;
+  const int c0 = (blockIdx.x - ky * ct) * T;
+  const long long in_sl = (long long)a.N2 * a.H;  // x stride in W layout
+  const double2* src0 = a.in + (long long)ky * a.H + c0;
+  constexpr bool SGS = W0 && (EPI == E_SMAG);
+  double2 acc[IT][3];
+  double2 accs[SGS ? IT : 1][3];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) acc[it][0] = acc[it][1] = acc[it][2] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int it = 0; it < (SGS ? IT : 1); ++it) accs[it][0] = accs[it][1] = accs[it][2] = make_double2(0.0, 0.0);
+
+  const double k2 = a.ks[1] * signed_freq(ky, a.N2);
+#pragma unroll 1
+  for (int f = 0; f < NF; ++f) {
+    const double2* src = src0 + (long long)f * a.in_sf;
+    for (int e = tid; e < L * T; e += NT) {
+      const int l = e / T, c = e - (e / T) * T;
+      buf[c * LS + l] = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (e < NS1 * T) {
+        const int kx = e / T, c = e - (e / T) * T;
+        const int p = pad_slot(kx, NS1, L, a.keep_nyq);
+        if (p >= 0 && c0 + c < a.H) {
+          const double k[3] = {a.ks[0] * signed_freq(kx, NS1), k2, a.ks[2] * (c0 + c)};
+          const double kinv = 1.0 / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + 1e-50);
+          epi_accumulate<EPI>(f, k, kinv, buf[c * LS + p], acc[it], SGS ? accs[SGS ? it : 0] : nullptr);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const double dtc = a.q0 ? (*a.dt) * a.c : 0.0;
+  const double coef = a.coef_dev ? *a.coef_dev : a.coef;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * NT;
+    if (e < NS1 * T) {
+      const int kx = e / T, c = e - (e / T) * T;
+      const int kz = c0 + c;
+      if (kz < a.H) {
+        const long long idx = (long long)kx * plane + (long long)ky * a.Nc + kz;
+        const double k[3] = {a.ks[0] * signed_freq(kx, NS1), k2, a.ks[2] * kz};
+        const double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const bool nyq = (kx == a.N1 / 2) || (ky == a.N2 / 2) || (kz == a.N3 / 2);
+        double2 u[3], rhs[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          u[cc] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : a.uprev[cc * fstride + idx];
+          if constexpr (EPI == E_TAU2) {
+            const double2 w = cscale(acc[it][cc], coef);
+            if (a.w0) a.w0[cc * fstride + idx] = w;
+            rhs[cc] = cadd(a.pl[cc * fstride + idx], w);
+          } else {
+            rhs[cc] = csub(acc[it][cc], cscale(u[cc], a.nu * (ksq + 1e-50)));
+          }
+        }
+        if constexpr (SGS) {
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) a.w0[cc * fstride + idx] = accs[it][cc];
+        }
+        if (a.rotate) {
+          const double kinv = 1.0 / (ksq + 1e-50);
+          double2 cr[3];
+          cr[0] = csub(cscale(u[1], a.Om[2]), cscale(u[2], a.Om[1]));
+          cr[1] = csub(cscale(u[2], a.Om[0]), cscale(u[0], a.Om[2]));
+          cr[2] = csub(cscale(u[0], a.Om[1]), cscale(u[1], a.Om[0]));
+          const double2 X = cadd(cadd(cscale(cr[0], k[0]), cscale(cr[1], k[1])), cscale(cr[2], k[2]));
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc)
+            rhs[cc] = cadd(rhs[cc], csub(cscale(cr[cc], 2.0), cscale(X, 2.0 * k[cc] * kinv)));
+        }
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          double2 o = a.q0 ? cadd(a.q0[cc * fstride + idx], cscale(rhs[cc], dtc)) : rhs[cc];
+          if (a.zero_nyq_out && nyq) o = make_double2(0.0, 0.0);
+          a.out[cc * fstride + idx] = o;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace psx

@@ -1,0 +1,27 @@
+// Host-side launchers: each kernel family is instantiated for the supported
+// transform lengths in its own translation unit; these entry points pick the
+// instantiation for a runtime length.  All return a hipError_t.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace psx {
+
+// Tile widths (contiguous k3 columns per block) and z-pass position counts,
+// sized so a block's LDS buffer stays near 48 KiB.
+constexpr int pencil_T(int L) { return L <= 192 ? 16 : (L <= 384 ? 8 : 4); }
+constexpr int epi_T(int L) { return L <= 192 ? 8 : (L <= 384 ? 4 : 2); }
+constexpr int z_P(int L, int nmax) { return (nmax * 2 * (L + 1) * 16 <= 49152) ? 4 : 2; }
+
+bool length_supported(int L);
+
+hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s);
+hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s);
+// mode: ZMode; returns hipErrorInvalidValue for an unsupported (L, mode)
+hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipStream_t s);
+int zpass_positions(int L, int mode);
+// epi: EpiMode; ns1 = spectral length along x
+hipError_t launch_epilogue(int L, int ns1, int epi, bool w0, const EpiArgs& a, const double2* tw, hipStream_t s);
+
+}  // namespace psx

@@ -1,0 +1,529 @@
+// libpsx: context, plans and the C ABI declared in include/psx.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/psx.h"
+#include "launch.hpp"
+#include "small_kernels.hpp"
+
+namespace psx {
+hipError_t launch_zpass_dns(int L, const ZArgs& a, const double2* tw, hipStream_t s);
+hipError_t launch_zpass_smag(int L, const ZArgs& a, const double2* tw, hipStream_t s);
+hipError_t launch_zpass_tm2(int L, const ZArgs& a, const double2* tw, hipStream_t s);
+hipError_t launch_zpass_dt(int L, const ZArgs& a, const double2* tw, hipStream_t s);
+int zpass_positions_dns(int L);
+int zpass_positions_smag(int L);
+int zpass_positions_tm2(int L);
+int zpass_positions_dt(int L);
+
+int zpass_positions(int L, int mode) {
+  switch (mode) {
+    case Z_DNS: return zpass_positions_dns(L);
+    case Z_SMAG: return zpass_positions_smag(L);
+    case Z_TM2: return zpass_positions_tm2(L);
+    case Z_DT: return zpass_positions_dt(L);
+  }
+  return 0;
+}
+
+hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipStream_t s) {
+  switch (mode) {
+    case Z_DNS: return launch_zpass_dns(L, a, tw, s);
+    case Z_SMAG: return launch_zpass_smag(L, a, tw, s);
+    case Z_TM2: return launch_zpass_tm2(L, a, tw, s);
+    case Z_DT: return launch_zpass_dt(L, a, tw, s);
+  }
+  return hipErrorInvalidValue;
+}
+}  // namespace psx
+
+using namespace psx;
+
+namespace {
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+struct Err {
+  int code;
+};
+}  // namespace
+
+struct psx_context {
+  int N1 = 0, N2 = 0, N3 = 0, Nc = 0, H = 0;
+  int M1 = 0, M2 = 0, M3 = 0;  // 3/2-padded lengths
+  int model = 0;
+  psx_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // pipeline scratch
+  double2* bufA = nullptr;
+  double2* bufB = nullptr;
+  size_t nA = 0, nB = 0;
+  double2* w0_scratch = nullptr;  // unused placeholder for models without w0
+  unsigned long long* red = nullptr;
+  double* scal = nullptr;  // device scalars: [0] dt (internal), [1..4] tau family
+  double* part = nullptr;
+  int npart = 0;
+  std::map<int, double2*> tw;
+  long long scratch_bytes = 0;
+  std::string err;
+};
+
+namespace {
+
+void check(psx_context* c, hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    throw Err{e == hipErrorOutOfMemory ? PSX_E_OOM : PSX_E_HIP};
+  }
+}
+
+void fail(psx_context* c, int code, const std::string& msg) {
+  c->err = msg;
+  throw Err{code};
+}
+
+template <typename F>
+int guarded(psx_context* c, F&& f) {
+  if (!c) return PSX_E_ARG;
+  try {
+    c->err.clear();
+    f();
+    return PSX_OK;
+  } catch (const Err& e) {
+    return e.code;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return PSX_E_HIP;
+  }
+}
+
+bool truncating(int model) { return model != PSX_MODEL_SMAGORINSKY; }
+
+const double2* twiddles(psx_context* c, int L) {
+  auto it = c->tw.find(L);
+  if (it != c->tw.end()) return it->second;
+  if (!length_supported(L)) fail(c, PSX_E_UNSUPPORTED, "transform length " + std::to_string(L) + " not supported");
+  std::vector<double2> h(L);
+  for (int m = 0; m < L; ++m) {
+    // exp(-2 pi i m / L), evaluated on the reduced angle for accuracy
+    const double ang = 2.0 * kPi * (double)m / (double)L;
+    h[m] = make_double2(std::cos(ang), -std::sin(ang));
+  }
+  double2* d = nullptr;
+  check(c, hipMalloc(&d, sizeof(double2) * L), "hipMalloc(twiddles)");
+  c->scratch_bytes += sizeof(double2) * L;
+  check(c, hipMemcpy(d, h.data(), sizeof(double2) * L, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+  c->tw[L] = d;
+  return d;
+}
+
+PencilArgs pencil_base() {
+  PencilArgs a;
+  std::memset(&a, 0, sizeof(a));
+  for (int j = 0; j < MAXJOBS; ++j) a.job_in[j] = a.job_plain[j] = a.job_deriv[j] = -1;
+  a.scale = 1.0;
+  return a;
+}
+
+// ---- pipeline building blocks --------------------------------------------------
+// x-inverse from the state layout: (nf fields of uin) -> X[f][L1][N2][C]
+void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, int keep_nyq, int njobs,
+               const int* jin, const int* jplain, const int* jderiv) {
+  PencilArgs a = pencil_base();
+  a.in = uin;
+  a.out = X;
+  a.in_sf = (long long)c->N1 * c->N2 * c->Nc;
+  a.in_sa = c->Nc;
+  a.in_sl = (long long)c->N2 * c->Nc;
+  a.out_sf = (long long)L1 * c->N2 * C;
+  a.out_sa = C;
+  a.out_sl = (long long)c->N2 * C;
+  a.A = c->N2;
+  a.C = C;
+  a.Nspec = c->N1;
+  a.keep_nyq = keep_nyq;
+  a.kscale = 2.0 * kPi / c->p.L1;
+  for (int j = 0; j < njobs; ++j) {
+    a.job_in[j] = jin[j];
+    a.job_plain[j] = jplain[j];
+    a.job_deriv[j] = jderiv ? jderiv[j] : -1;
+  }
+  check(c, launch_pencil_inverse(L1, a, twiddles(c, L1), njobs, c->stream), "x_inverse");
+}
+
+// y-inverse: X[f][L1][N2][C] -> Y[f][L1][L2][C]
+void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int C, int keep_nyq, int njobs,
+               const int* jin, const int* jplain, const int* jderiv) {
+  PencilArgs a = pencil_base();
+  a.in = X;
+  a.out = Y;
+  a.in_sf = (long long)L1 * c->N2 * C;
+  a.in_sa = (long long)c->N2 * C;
+  a.in_sl = C;
+  a.out_sf = (long long)L1 * L2 * C;
+  a.out_sa = (long long)L2 * C;
+  a.out_sl = C;
+  a.A = L1;
+  a.C = C;
+  a.Nspec = c->N2;
+  a.keep_nyq = keep_nyq;
+  a.kscale = 2.0 * kPi / c->p.L2;
+  for (int j = 0; j < njobs; ++j) {
+    a.job_in[j] = jin[j];
+    a.job_plain[j] = jplain[j];
+    a.job_deriv[j] = jderiv ? jderiv[j] : -1;
+  }
+  check(c, launch_pencil_inverse(L2, a, twiddles(c, L2), njobs, c->stream), "y_inverse");
+}
+
+// y-forward with truncation: Z[f][L1][L2][C] -> W[f][L1][N2][C]
+void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int C, int keep_nyq, int nf) {
+  PencilArgs a = pencil_base();
+  a.in = Z;
+  a.out = W;
+  a.in_sf = (long long)L1 * L2 * C;
+  a.in_sa = (long long)L2 * C;
+  a.in_sl = C;
+  a.out_sf = (long long)L1 * c->N2 * C;
+  a.out_sa = (long long)c->N2 * C;
+  a.out_sl = C;
+  a.A = L1;
+  a.C = C;
+  a.Nspec = c->N2;
+  a.keep_nyq = keep_nyq;
+  for (int j = 0; j < nf; ++j) {
+    a.job_in[j] = j;
+    a.job_plain[j] = j;
+  }
+  check(c, launch_pencil_forward(L2, a, twiddles(c, L2), nf, c->stream), "y_forward");
+}
+
+// RHS / stage for the 3/2-rule models (DNS, Smagorinsky)
+void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double coef,
+              double2* w0, int zero_nyq_out) {
+  const bool smag = c->model == PSX_MODEL_SMAGORINSKY;
+  const int M1 = c->M1, M2 = c->M2, M3 = c->M3, H = c->H;
+  double2* A = c->bufA;
+  double2* B = c->bufB;
+  // K1: x-inverse (+ d/dx for the closure)
+  {
+    const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2}, jd[3] = {3, 4, 5};
+    x_inverse(c, uin, A, M1, H, 0, 3, jin, jp, smag ? jd : nullptr);
+  }
+  // K2: y-inverse (+ d/dy of the velocity for the closure)
+  const int nY = smag ? 9 : 3;
+  {
+    const int jin[6] = {0, 1, 2, 3, 4, 5}, jp[6] = {0, 1, 2, 3, 4, 5}, jd[6] = {6, 7, 8, -1, -1, -1};
+    y_inverse(c, A, B, M1, M2, H, 0, smag ? 6 : 3, jin, jp, smag ? jd : nullptr);
+  }
+  // K3: z c2r -> products (+ Smagorinsky stress) -> z r2c
+  {
+    ZArgs z;
+    std::memset(&z, 0, sizeof(z));
+    z.in0 = B;
+    z.in_sf0 = (long long)M1 * M2 * H;
+    z.Hin0 = H;
+    z.nin0 = nY;
+    z.in1 = nullptr;
+    z.out = A;
+    z.out_sf = (long long)M1 * M2 * H;
+    z.Hout = H;
+    z.npos = M1 * M2;
+    z.kzscale = 2.0 * kPi / c->p.L3;
+    const double Nt = (double)c->N1 * c->N2 * c->N3;
+    z.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
+    const double delta = std::cbrt(c->p.dx * c->p.dy * c->p.dz);
+    z.c2 = (c->p.Cs * delta) * (c->p.Cs * delta);
+    auto one = [](Recipe& r, int f) { r.nterm = 1; r.f[0] = f; r.coef[0] = 1.0; r.dz[0] = 0; };
+    one(z.rec[0], 0);
+    one(z.rec[1], 1);
+    one(z.rec[2], 2);
+    if (smag) {
+      // Y fields: u v w ux vx wx uy vy wy; pencils u v w S11 S22 S33 S12 S13 S23
+      one(z.rec[3], 3);
+      one(z.rec[4], 7);
+      z.rec[5] = Recipe{1, {2, 0}, {1.0, 0.0}, {1, 0}};
+      z.rec[6] = Recipe{2, {6, 4}, {0.5, 0.5}, {0, 0}};
+      z.rec[7] = Recipe{2, {0, 5}, {0.5, 0.5}, {1, 0}};
+      z.rec[8] = Recipe{2, {1, 8}, {0.5, 0.5}, {1, 0}};
+    }
+    const int mode = smag ? Z_SMAG : Z_DNS;
+    const int P = zpass_positions(M3, mode);
+    if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
+    check(c, launch_zpass(M3, mode, z, twiddles(c, M3), c->stream), "z_pass");
+  }
+  const int nF = smag ? 12 : 6;
+  // K4: y-forward + truncation
+  y_forward(c, A, B, M1, M2, H, 0, nF);
+  // K5: x-forward + truncation + projection/closure/viscous + stage update
+  {
+    EpiArgs e;
+    std::memset(&e, 0, sizeof(e));
+    e.in = B;
+    e.in_sf = (long long)M1 * c->N2 * H;
+    e.N1 = c->N1;
+    e.N2 = c->N2;
+    e.N3 = c->N3;
+    e.Nc = c->Nc;
+    e.H = H;
+    e.keep_nyq = 0;
+    e.uprev = uin;
+    e.q0 = q0;
+    e.out = out;
+    e.w0 = w0;
+    e.dt = dt;
+    e.c = coef;
+    e.nu = c->p.nu;
+    e.ks[0] = 2.0 * kPi / c->p.L1;
+    e.ks[1] = 2.0 * kPi / c->p.L2;
+    e.ks[2] = 2.0 * kPi / c->p.L3;
+    e.trunc_u = truncating(c->model) ? 1 : 0;
+    e.zero_nyq_out = zero_nyq_out;
+    e.rotate = c->p.rotate;
+    e.Om[0] = c->p.Om1;
+    e.Om[1] = c->p.Om2;
+    e.Om[2] = c->p.Om3;
+    check(c, launch_epilogue(M1, c->N1, smag ? E_SMAG : E_DNS, smag && w0 != nullptr, e, twiddles(c, M1), c->stream),
+          "x_forward_epilogue");
+  }
+}
+
+void run_stage(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double coef,
+               double2* w0, int zero_nyq_out, double t) {
+  (void)t;
+  switch (c->model) {
+    case PSX_MODEL_DNS:
+    case PSX_MODEL_SMAGORINSKY:
+      stage_32(c, uin, q0, out, dt, coef, w0, zero_nyq_out);
+      break;
+    default:
+      fail(c, PSX_E_UNSUPPORTED, "model not available in this build");
+  }
+}
+
+void compute_dt(psx_context* c, const double2* u, double* dt) {
+  if (c->p.cfl <= 0) {
+    hipLaunchKernelGGL(set_f64, dim3(1), dim3(1), 0, c->stream, dt, -c->p.cfl);
+    check(c, hipGetLastError(), "set_f64");
+    return;
+  }
+  const int N1 = c->N1, N2 = c->N2, N3 = c->N3, Nc = c->Nc;
+  hipLaunchKernelGGL(set_u64, dim3(1), dim3(1), 0, c->stream, c->red, 0ULL);
+  check(c, hipGetLastError(), "set_u64");
+  const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2};
+  x_inverse(c, u, c->bufA, N1, Nc, 1, 3, jin, jp, nullptr);
+  y_inverse(c, c->bufA, c->bufB, N1, N2, Nc, 1, 3, jin, jp, nullptr);
+  ZArgs z;
+  std::memset(&z, 0, sizeof(z));
+  z.in0 = c->bufB;
+  z.in_sf0 = (long long)N1 * N2 * Nc;
+  z.Hin0 = Nc;
+  z.nin0 = 3;
+  z.npos = N1 * N2;
+  z.kzscale = 2.0 * kPi / c->p.L3;
+  for (int r = 0; r < 3; ++r) z.rec[r] = Recipe{1, {r, 0}, {1.0, 0.0}, {0, 0}};
+  z.dinv[0] = 1.0 / c->p.dx;
+  z.dinv[1] = 1.0 / c->p.dy;
+  z.dinv[2] = 1.0 / c->p.dz;
+  z.red = c->red;
+  const int P = zpass_positions(N3, Z_DT);
+  if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "CFL pass: unsupported length");
+  check(c, launch_zpass(N3, Z_DT, z, twiddles(c, N3), c->stream), "cfl_pass");
+  const double Nt = (double)N1 * N2 * N3;
+  const double dx = c->p.dx, dy = c->p.dy, dz = c->p.dz;
+  const double visc = c->p.nu > 0
+                          ? 0.634 / (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)) / c->p.nu * c->p.cfl / 1.35
+                          : 0.0;
+  hipLaunchKernelGGL(cfl_finalize, dim3(1), dim3(1), 0, c->stream, c->red, dt, 1.0 / std::sqrt(Nt), c->p.cfl,
+                     c->p.nu, visc);
+  check(c, hipGetLastError(), "cfl_finalize");
+}
+
+void validate_params(psx_context* c, const psx_params* p) {
+  if (!p) fail(c, PSX_E_ARG, "params is NULL");
+  if (!(p->L1 > 0 && p->L2 > 0 && p->L3 > 0)) fail(c, PSX_E_ARG, "box lengths must be positive");
+  if (!(p->dx > 0 && p->dy > 0 && p->dz > 0)) fail(c, PSX_E_ARG, "grid spacings must be positive");
+  if (p->rotate != 0 && p->rotate != 1) fail(c, PSX_E_ARG, "rotate must be 0 or 1");
+}
+
+}  // namespace
+
+extern "C" {
+
+int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_params* params, int device,
+               void* stream) {
+  if (!out) return PSX_E_ARG;
+  *out = nullptr;
+  psx_context* c = new psx_context();
+  int rc = guarded(c, [&] {
+    if (n1 < 4 || n2 < 4 || n3 < 4 || (n1 % 2) || (n2 % 2) || (n3 % 4))
+      fail(c, PSX_E_ARG, "grid sizes must be even (N3 a multiple of 4)");
+    if (model < PSX_MODEL_DNS || model > PSX_MODEL_DYNAMIC_TAU) fail(c, PSX_E_ARG, "unknown model");
+    if (model != PSX_MODEL_DNS && model != PSX_MODEL_SMAGORINSKY)
+      fail(c, PSX_E_UNSUPPORTED, "model not available in this build");
+    validate_params(c, params);
+    c->N1 = n1;
+    c->N2 = n2;
+    c->N3 = n3;
+    c->Nc = n3 / 2 + 1;
+    c->H = n3 / 2;
+    c->M1 = 3 * n1 / 2;
+    c->M2 = 3 * n2 / 2;
+    c->M3 = 3 * n3 / 2;
+    c->model = model;
+    c->p = *params;
+    c->device = device;
+    c->stream = (hipStream_t)stream;
+    check(c, hipSetDevice(device), "hipSetDevice");
+    for (int L : {c->M1, c->M2, c->M3, n1, n2, n3})
+      if (!length_supported(L)) fail(c, PSX_E_UNSUPPORTED, "grid size not supported: length " + std::to_string(L));
+    const bool smag = model == PSX_MODEL_SMAGORINSKY;
+    const size_t MMH = (size_t)c->M1 * c->M2 * c->H;
+    const size_t MNH = (size_t)c->M1 * c->N2 * c->H;
+    const size_t state = (size_t)3 * n1 * n2 * c->Nc;
+    const size_t nZ = (smag ? 12 : 6) * MMH;
+    const size_t nX = (smag ? 6 : 3) * MNH;
+    const size_t nY = (smag ? 9 : 3) * MMH;
+    const size_t nW = (smag ? 12 : 6) * MNH;
+    c->nA = std::max(std::max(nX, nZ), state);
+    c->nB = std::max(std::max(nY, nW), state);
+    check(c, hipMalloc(&c->bufA, c->nA * sizeof(double2)), "hipMalloc(bufA)");
+    check(c, hipMalloc(&c->bufB, c->nB * sizeof(double2)), "hipMalloc(bufB)");
+    check(c, hipMalloc(&c->red, 64), "hipMalloc(red)");
+    check(c, hipMalloc(&c->scal, 16 * sizeof(double)), "hipMalloc(scal)");
+    check(c, hipMemset(c->scal, 0, 16 * sizeof(double)), "hipMemset(scal)");
+    c->npart = 1024;
+    check(c, hipMalloc(&c->part, 3 * c->npart * sizeof(double)), "hipMalloc(part)");
+    c->scratch_bytes += (long long)(c->nA + c->nB) * sizeof(double2) + 64 + 16 * 8 + 3 * c->npart * 8;
+    for (int L : {c->M1, c->M2, c->M3, n1, n2, n3}) twiddles(c, L);
+  });
+  if (rc != PSX_OK) {
+    // keep the message reachable through a static buffer
+    static thread_local std::string last;
+    last = c->err;
+    psx_destroy(c);
+    std::fprintf(stderr, "psx_create: %s\n", last.c_str());
+    return rc;
+  }
+  *out = c;
+  return PSX_OK;
+}
+
+int psx_destroy(psx_context* c) {
+  if (!c) return PSX_E_ARG;
+  hipFree(c->bufA);
+  hipFree(c->bufB);
+  hipFree(c->red);
+  hipFree(c->scal);
+  hipFree(c->part);
+  for (auto& kv : c->tw) hipFree(kv.second);
+  delete c;
+  return PSX_OK;
+}
+
+int psx_set_stream(psx_context* c, void* stream) {
+  return guarded(c, [&] { c->stream = (hipStream_t)stream; });
+}
+
+int psx_set_params(psx_context* c, const psx_params* params) {
+  return guarded(c, [&] {
+    validate_params(c, params);
+    c->p = *params;
+  });
+}
+
+const char* psx_last_error(const psx_context* c) { return c ? c->err.c_str() : "null context"; }
+
+long long psx_scratch_bytes(const psx_context* c) { return c ? c->scratch_bytes : 0; }
+
+int psx_rhs(psx_context* c, const double* uhat, double* rhs, double* uhat_model, double* w0, double t) {
+  return guarded(c, [&] {
+    if (!uhat || !rhs) fail(c, PSX_E_ARG, "null pointer");
+    if ((const double*)rhs == uhat) fail(c, PSX_E_ARG, "rhs must not alias uhat");
+    run_stage(c, (const double2*)uhat, nullptr, (double2*)rhs, nullptr, 0.0, (double2*)w0, 0, t);
+    if (uhat_model) {
+      const long long total = 3LL * c->N1 * c->N2 * c->Nc;
+      const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 4096);
+      hipLaunchKernelGGL(copy_truncate, dim3(blocks), dim3(256), 0, c->stream, (const double2*)uhat,
+                         (double2*)uhat_model, c->N1, c->N2, c->Nc, c->N3, truncating(c->model) ? 1 : 0);
+      check(c, hipGetLastError(), "copy_truncate");
+    }
+  });
+}
+
+int psx_rk_stage(psx_context* c, const double* q0, const double* uin, double* out, const double* dt_dev, double coef,
+                 double t, int zero_nyquist_out) {
+  return guarded(c, [&] {
+    if (!q0 || !uin || !out || !dt_dev) fail(c, PSX_E_ARG, "null pointer");
+    run_stage(c, (const double2*)uin, (const double2*)q0, (double2*)out, dt_dev, coef, nullptr,
+              zero_nyquist_out ? 1 : 0, t);
+  });
+}
+
+int psx_compute_dt(psx_context* c, const double* uhat, double* dt_dev) {
+  return guarded(c, [&] {
+    if (!uhat || !dt_dev) fail(c, PSX_E_ARG, "null pointer");
+    compute_dt(c, (const double2*)uhat, dt_dev);
+  });
+}
+
+int psx_step(psx_context* c, double* q, double* qs, double* dt_dev, double t) {
+  return guarded(c, [&] {
+    if (!q || !qs || !dt_dev) fail(c, PSX_E_ARG, "null pointer");
+    if (q == qs) fail(c, PSX_E_ARG, "q and qs must be distinct buffers");
+    double2* Q = (double2*)q;
+    double2* S = (double2*)qs;
+    compute_dt(c, S, dt_dev);
+    const int z = truncating(c->model) ? 1 : 0;
+    // src/PySpec_3d.py:99-104: Q = Q0 + dt*c_i*RHS(Q), c = (1/4, 1/3, 1/2, 1)
+    run_stage(c, Q, Q, S, dt_dev, 0.25, nullptr, z, t);
+    run_stage(c, S, Q, S, dt_dev, 1.0 / 3.0, nullptr, z, t);
+    run_stage(c, S, Q, S, dt_dev, 0.5, nullptr, z, t);
+    run_stage(c, S, Q, Q, dt_dev, 1.0, nullptr, 0, t);
+  });
+}
+
+int psx_stats(psx_context* c, const double* uhat, double* out_dev) {
+  return guarded(c, [&] {
+    if (!uhat || !out_dev) fail(c, PSX_E_ARG, "null pointer");
+    const long long total = (long long)c->N1 * c->N2 * c->Nc;
+    int nblk = (int)std::min<long long>((total + NT - 1) / NT, c->npart);
+    hipLaunchKernelGGL(stats_partial, dim3(nblk), dim3(NT), 0, c->stream, (const double2*)uhat, c->part, c->N1, c->N2,
+                       c->N3, c->Nc, 2.0 * kPi / c->p.L1, 2.0 * kPi / c->p.L2, 2.0 * kPi / c->p.L3, c->p.kc);
+    check(c, hipGetLastError(), "stats_partial");
+    const double Nt = (double)c->N1 * c->N2 * c->N3;
+    hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, c->stream, c->part, nblk, out_dev, 1.0 / Nt);
+    check(c, hipGetLastError(), "stats_final");
+  });
+}
+
+int psx_spectrum_bins(const psx_context* c) {
+  if (!c) return PSX_E_ARG;
+  const double a = c->N1 / 2, b = c->N2 / 2, d = c->N3 / 2 - 1;
+  return (int)std::lrint(std::sqrt(a * a + b * b + d * d)) + 1;
+}
+
+int psx_spectrum(psx_context* c, const double* uhat, double* spec_dev, int resolved) {
+  return guarded(c, [&] {
+    if (!uhat || !spec_dev) fail(c, PSX_E_ARG, "null pointer");
+    const int nb = psx_spectrum_bins(c);
+    hipLaunchKernelGGL(fill_f64, dim3((3 * nb + 255) / 256), dim3(256), 0, c->stream, spec_dev, 3 * nb, 0.0);
+    check(c, hipGetLastError(), "fill_f64");
+    const long long total = (long long)c->N1 * c->N2 * c->Nc;
+    const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 4096);
+    const double Nt = (double)c->N1 * c->N2 * c->N3;
+    hipLaunchKernelGGL(spectrum_kernel, dim3(blocks), dim3(256), 0, c->stream, (const double2*)uhat, spec_dev, nb,
+                       c->N1, c->N2, c->N3, c->Nc, c->p.kc, resolved ? 1 : 0, 1.0 / Nt);
+    check(c, hipGetLastError(), "spectrum");
+  });
+}
+
+const double* psx_tau_device(const psx_context* c) { return c ? c->scal + 1 : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+"""GPU-resident pseudo-spectral solver over libpsx (one context per grid/model/GPU).
+
+State tensors are ``torch.complex128`` device tensors of shape ``(3, N1, N2, N3//2+1)``
+in the reference's per-field Fourier layout (``src/Classes.py:19-25``), with
+``uhat = rfftn(u)/sqrt(N1 N2 N3)``.  PyTorch provides device memory and the stream;
+every arithmetic operation runs in libpsx's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import threading
+
+import torch
+
+from . import _lib
+
+_REGISTRY: dict[int, "SpectralSolver"] = {}
+_REG_LOCK = threading.Lock()
+_NEXT_ID = [1]
+
+
+def _ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+class SpectralSolver:
+    """A libpsx context: plans, twiddle tables and pipeline scratch for one grid.
+
+    Parameters mirror the reference's case globals (``src/PySpec_3d.py:9-72``):
+    ``nu``, ``L1..L3``, ``dx..dz`` (case grid spacings, used for Delta and the
+    CFL limit), ``cfl`` (<= 0 means fixed dt = -cfl), ``Cs`` (0.16 in
+    ``src/RHSfunctions.py:197``), ``Ct``, ``tau_a``/``tau_b`` (``dt0``/``dt1``),
+    ``kc``, ``rotate`` and ``Om``.
+    """
+
+    def __init__(self, N1, N2, N3, model=0, *, nu, L=(2 * math.pi,) * 3, dx=None, dy=None, dz=None,
+                 cfl=-0.01, Cs=0.16, Ct=0.1, tau_a=-10.0, tau_b=-10.0, kc=None, rotate=0,
+                 Om=(0.0, 0.0, 0.0), device=None):
+        if not torch.cuda.is_available():
+            raise _lib.PsxError("SpectralSolver needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.lib = _lib.load()
+        self.N1, self.N2, self.N3 = int(N1), int(N2), int(N3)
+        self.Nc = self.N3 // 2 + 1
+        self.model = _lib.MODEL_IDS[model]
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
+        L1, L2, L3 = (float(x) for x in L)
+        self.params = _lib.psx_params(
+            nu=float(nu), L1=L1, L2=L2, L3=L3,
+            dx=float(L1 / N1 if dx is None else dx), dy=float(L2 / N2 if dy is None else dy),
+            dz=float(L3 / N3 if dz is None else dz), cfl=float(cfl), Cs=float(Cs), Ct=float(Ct),
+            tau_a=float(tau_a), tau_b=float(tau_b), kc=int(N1 // 2 if kc is None else kc),
+            rotate=int(rotate), Om1=float(Om[0]), Om2=float(Om[1]), Om3=float(Om[2]))
+        ctx = C.c_void_p()
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            rc = self.lib.psx_create(C.byref(ctx), self.N1, self.N2, self.N3, self.model,
+                                     C.byref(self.params), self.device.index, C.c_void_p(stream))
+        if rc != 0:
+            raise _lib.PsxError(f"psx_create failed ({rc}) for {N1}x{N2}x{N3} model {model}")
+        self.ctx = ctx
+        self._dt = torch.zeros(1, dtype=torch.float64, device=self.device)
+        with _REG_LOCK:
+            self.id = _NEXT_ID[0]
+            _NEXT_ID[0] += 1
+            _REGISTRY[self.id] = self
+
+    # ------------------------------------------------------------------ plumbing
+    @property
+    def shape(self):
+        return (3, self.N1, self.N2, self.Nc)
+
+    def empty_state(self) -> torch.Tensor:
+        return torch.empty(self.shape, dtype=torch.complex128, device=self.device)
+
+    def _sync_stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.ctx, self.lib.psx_set_stream(self.ctx, C.c_void_p(s)), "psx_set_stream")
+
+    def _state_arg(self, t: torch.Tensor, name: str) -> torch.Tensor:
+        if t.dtype != torch.complex128 or tuple(t.shape) != self.shape:
+            raise ValueError(f"{name} must be complex128 of shape {self.shape}, got {t.dtype} {tuple(t.shape)}")
+        if t.device != self.device:
+            raise ValueError(f"{name} must live on {self.device}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+        return t
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if k == "Om":
+                self.params.Om1, self.params.Om2, self.params.Om3 = (float(x) for x in v)
+            else:
+                setattr(self.params, k, type(getattr(self.params, k))(v))
+        _lib.check(self.ctx, self.lib.psx_set_params(self.ctx, C.byref(self.params)), "psx_set_params")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.psx_destroy(self.ctx)
+            self.ctx = None
+            with _REG_LOCK:
+                _REGISTRY.pop(self.id, None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ operations
+    def rhs(self, uhat: torch.Tensor, t: float = 0.0, *, out=None, uhat_model=None, w0=None) -> torch.Tensor:
+        """``computeRHS_<model>``: RHS(uhat) at step-start time ``t``."""
+        self._state_arg(uhat, "uhat")
+        out = self.empty_state() if out is None else self._state_arg(out, "out")
+        um = 0 if uhat_model is None else _ptr(self._state_arg(uhat_model, "uhat_model"))
+        w = 0 if w0 is None else _ptr(self._state_arg(w0, "w0"))
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_rhs(self.ctx, _ptr(uhat), _ptr(out), um or None, w or None, float(t)),
+                   "psx_rhs")
+        return out
+
+    def rk_stage(self, q0, uin, out, dt: torch.Tensor, c: float, t: float = 0.0, zero_nyquist_out=False):
+        for n, x in (("q0", q0), ("uin", uin), ("out", out)):
+            self._state_arg(x, n)
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_rk_stage(self.ctx, _ptr(q0), _ptr(uin), _ptr(out), _ptr(dt), float(c),
+                                                   float(t), int(bool(zero_nyquist_out))), "psx_rk_stage")
+        return out
+
+    def compute_dt(self, uhat: torch.Tensor, dt: torch.Tensor | None = None) -> torch.Tensor:
+        self._state_arg(uhat, "uhat")
+        dt = self._dt if dt is None else dt
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_compute_dt(self.ctx, _ptr(uhat), _ptr(dt)), "psx_compute_dt")
+        return dt
+
+    def step(self, q: torch.Tensor, qs: torch.Tensor, t: float = 0.0, dt: torch.Tensor | None = None) -> torch.Tensor:
+        """``advanceQ_RK4``: q <- new state, qs <- stage-4 input (stale main.uhat).
+        Returns the device dt tensor used for the step."""
+        self._state_arg(q, "q")
+        self._state_arg(qs, "qs")
+        dt = self._dt if dt is None else dt
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_step(self.ctx, _ptr(q), _ptr(qs), _ptr(dt), float(t)), "psx_step")
+        return dt
+
+    def stats(self, uhat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """(E, Omega, E_resolved) on the device."""
+        self._state_arg(uhat, "uhat")
+        out = torch.empty(3, dtype=torch.float64, device=self.device) if out is None else out
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_stats(self.ctx, _ptr(uhat), _ptr(out)), "psx_stats")
+        return out
+
+    def spectrum(self, uhat: torch.Tensor, resolved: bool = False) -> torch.Tensor:
+        self._state_arg(uhat, "uhat")
+        nb = self.lib.psx_spectrum_bins(self.ctx)
+        out = torch.empty((nb, 3), dtype=torch.float64, device=self.device)
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_spectrum(self.ctx, _ptr(uhat), _ptr(out), int(bool(resolved))),
+                   "psx_spectrum")
+        return out
+
+    def scratch_bytes(self) -> int:
+        return int(self.lib.psx_scratch_bytes(self.ctx))
+
+
+def get(solver_id: int) -> SpectralSolver:
+    return _REGISTRY[solver_id]

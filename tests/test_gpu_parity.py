@@ -1,0 +1,149 @@
+"""GPU parity: libpsx's HIP path against the oracle on the same seeded inputs.
+
+Tolerances: single RHS evaluations agree to 1e-12 relative L2 (FFT summation
+order is the only difference); the 100-step Taylor-Green run of the plain
+method agrees to 1e-10 relative L2 on uhat (the stated parity bar).
+"""
+import numpy as np
+import pytest
+
+from oracle import spectral_oracle as so
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FFT = so.FFTBackend()
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _solver(N, model, nu, **kw):
+    from pyspectral_amd.solver import SpectralSolver
+
+    return SpectralSolver(N, N, N, model, nu=nu, **kw)
+
+
+def _field(N, seed, nyquist_noise=False):
+    u = np.stack(so.random_isotropic(N, seed=seed, kp=max(2.0, N / 8)))
+    if nyquist_noise:
+        rng = np.random.default_rng(seed + 100)
+        for c in range(3):
+            u[c, N // 2] += 0.1 * rng.standard_normal((N, N // 2 + 1))
+            u[c, :, N // 2] += 0.1 * rng.standard_normal((N, N // 2 + 1))
+            u[c, :, :, N // 2] += 0.1 * rng.standard_normal((N, N))
+    return u
+
+
+@pytest.mark.parametrize("N", [16, 32, 48])
+@pytest.mark.parametrize("model", [0, 1])
+def test_rhs_matches_oracle(N, model):
+    print(f"\nrhs model={model} N={N}", flush=True)
+    nu = 0.013
+    u = _field(N, 3 + N, nyquist_noise=True)
+    g = so.Grid(N, N, N)
+    st = so.State(g, *u, nu=nu, turb_model=model)
+    so.compute_rhs(st, FFT)
+    ref = st.compact()
+    s = _solver(N, model, nu)
+    ud = _dev(u)
+    um = torch.empty_like(ud)
+    w0 = torch.empty_like(ud) if model == 1 else None
+    r = _host(s.rhs(ud, 0.0, uhat_model=um, w0=w0))
+    assert _rel(r, ref) < 1e-12, _rel(r, ref)
+    # main.uhat as the model leaves it
+    exp_u = np.stack([st.uhat, st.vhat, st.what])
+    assert np.array_equal(_host(um), exp_u) or _rel(_host(um), exp_u) == 0.0
+    if model == 1:
+        w_ref = np.stack(st.w0)
+        assert _rel(_host(w0), w_ref) < 1e-12, _rel(_host(w0), w_ref)
+
+
+def test_rhs_tgv_closed_form():
+    N, nu = 32, 1.0 / 400
+    u, v, w, _ = so.taylor_green(N, N, N)
+    s = _solver(N, 0, nu)
+    r = _host(s.rhs(_dev(np.stack([u, v, w])), 0.0))
+    x = np.arange(N) * 2 * np.pi / N
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    fu = -3 * nu * np.sin(X) * np.cos(Y) * np.cos(Z) - np.sin(2 * X) * np.cos(2 * Z) / 8
+    fv = 3 * nu * np.cos(X) * np.sin(Y) * np.cos(Z) - np.sin(2 * Y) * np.cos(2 * Z) / 8
+    fw = (np.cos(2 * X) + np.cos(2 * Y)) * np.sin(2 * Z) / 8
+    ref = np.stack([np.fft.rfftn(f) / np.sqrt(N ** 3) for f in (fu, fv, fw)])
+    assert _rel(r, ref) < 1e-13
+
+
+@pytest.mark.parametrize("N", [16, 32])
+def test_compute_dt_matches_oracle(N):
+    u = _field(N, 7, nyquist_noise=True)
+    g = so.Grid(N, N, N)
+    st = so.State(g, *u, nu=0.01, cfl=0.5)
+    so.compute_dt(st)
+    s = _solver(N, 1, 0.01, cfl=0.5)
+    dt = _host(s.compute_dt(_dev(u)))[0]
+    assert abs(dt - st.dt) < 1e-14 * st.dt, (dt, st.dt)
+
+
+@pytest.mark.parametrize("model,N,steps,cfl", [(0, 16, 5, 0.5), (1, 16, 5, 0.5), (0, 32, 3, -0.002),
+                                                 (1, 32, 3, 0.5)])
+def test_steps_match_oracle(model, N, steps, cfl):
+    print(f"\nstep model={model} N={N}", flush=True)
+    nu = 0.01
+    u = _field(N, 11, nyquist_noise=True)
+    g = so.Grid(N, N, N)
+    st = so.State(g, *u, nu=nu, turb_model=model, cfl=cfl, dt=0.002)
+    s = _solver(N, model, nu, cfl=cfl)
+    q = _dev(u)
+    qs = q.clone()
+    for _ in range(steps):
+        so.advance_rk4(st, FFT)
+        dt = s.step(q, qs, 0.0)
+        assert abs(_host(dt)[0] - st.dt) <= 1e-13 * st.dt
+    assert _rel(_host(q), st.compact()) < 1e-12
+    stale = np.stack([st.uhat, st.vhat, st.what])
+    assert _rel(_host(qs), stale) < 1e-12
+
+
+def test_tgv_100_steps_parity():
+    """Plain method, 32^3 TGV, nu = 1/400, dt = 0.01, 100 steps: <= 1e-10 relative L2."""
+    N, nu, dt = 32, 1.0 / 400, 0.01
+    u, v, w, _ = so.taylor_green(N, N, N)
+    g = so.Grid(N, N, N)
+    st = so.State(g, u, v, w, nu=nu, dt=dt)
+    s = _solver(N, 0, nu, cfl=-dt)
+    q = _dev(np.stack([u, v, w]))
+    qs = q.clone()
+    for _ in range(100):
+        so.advance_rk4(st, FFT)
+        s.step(q, qs, 0.0)
+    err = _rel(_host(q), st.compact())
+    print(f"\nTGV 100-step relative L2 = {err:.3e}")
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("N", [16, 32])
+def test_stats_and_spectrum_match_oracle(N):
+    u = _field(N, 5, nyquist_noise=True)
+    g = so.Grid(N, N, N, kc=N // 3)
+    st = so.State(g, *u, nu=0.01)
+    s = _solver(N, 0, 0.01, kc=N // 3)
+    ud = _dev(u)
+    E, Om, Er = _host(s.stats(ud))
+    assert abs(E - so.compute_energy(st)) < 1e-13 * abs(E)
+    assert abs(Om - so.compute_enstrophy(st)) < 1e-13 * abs(Om)
+    assert abs(Er - so.compute_energy_resolved(st)) < 1e-13 * abs(Er)
+    for res in (False, True):
+        k, spec = so.compute_spectrum(st, resolved=res)
+        got = _host(s.spectrum(ud, resolved=res))
+        assert got.shape == spec.shape
+        assert np.allclose(got, spec, rtol=1e-12, atol=1e-15 * spec.max())
