@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Throughput of the FP64 pseudo-spectral RK4 advance on one MI355X.
+
+Workload (default): decaying isotropic turbulence LES at 128^3 modes (192^3 with
+3/2 padding), FP64 throughout, Smagorinsky closure Cs = 0.16, nu = 1e-3, CFL-limited
+dt (CFL = 0.5), seeded random-phase solenoidal field with E(k) ~ k^4 exp(-2 (k/4)^2),
+u_rms = 1.  One "step" is one full advanceQ_RK4: the CFL dt on the stale velocity
+and four low-storage stages, all on the GPU.
+
+Prints one JSON line: Mpts*RK4-steps/s (N^3 * steps / wall second) with the
+dominant kernel's HBM roofline figure (algorithmic bytes per launch / mean launch
+time from HIP events) and the CPU baseline (the oracle's numpy/scipy restatement
+of the reference on this host's cores, timed on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+C16 = 16               # bytes per complex128
+
+
+def kernel_bytes(N, closure):
+    """Compulsory (algorithmic) bytes per launch of each kernel class at N^3, 3/2 rule:
+    every input field read once, every output field written once."""
+    M, H, Nc = 3 * N // 2, N // 2, N // 2 + 1
+    smag = closure == "smagorinsky"
+    nX, nY, nZ = (6, 9, 12) if smag else (3, 3, 6)
+    state = N * N * Nc * C16
+    return {
+        "x_inverse": 3 * N * N * H * C16 + nX * M * N * H * C16,
+        "y_inverse": nX * M * N * H * C16 + nY * M * M * H * C16,
+        "z_pass": nY * M * M * H * C16 + nZ * M * M * H * C16,
+        "y_forward": nZ * M * M * H * C16 + nZ * M * N * H * C16,
+        "x_forward_epilogue": nZ * M * N * H * C16 + 3 * 3 * state,
+        "cfl_x": 6 * state,
+        "cfl_y": 6 * state,
+        "cfl_z": 3 * state,
+    }
+
+
+def survey_bytes_per_step(N, closure):
+    """The 3-pass pencil model's algorithmic bytes per RK4 step (4 stages):
+    DNS 16[h(3N^2+18MN+18M^2)+9N^2Nc], Smagorinsky 16[h(3N^2+24MN+30M^2)+9N^2Nc]."""
+    M, h, Nc = 1.5 * N, N / 2, N // 2 + 1
+    if closure == "smagorinsky":
+        b = 16 * (h * (3 * N * N + 24 * M * N + 30 * M * M) + 9 * N * N * Nc)
+    else:
+        b = 16 * (h * (3 * N * N + 18 * M * N + 18 * M * M) + 9 * N * N * Nc)
+    return 4 * b
+
+
+def cpu_baseline(N, closure, nu, cfl, steps, seed):
+    """The oracle (numpy restatement of the reference's src/ path, scipy.fft with one
+    worker per available core) advancing the same field; bounded sample of `steps`."""
+    import numpy as np
+
+    from oracle import spectral_oracle as so
+    from pyspectral_amd import cases
+
+    cores = so.default_workers()
+    uh = cases.decaying_isotropic(N, seed=seed)
+    g = so.Grid(N, N, N)
+    st = so.State(g, *uh, nu=nu, turb_model=1 if closure == "smagorinsky" else 0, cfl=cfl)
+    fft = so.FFTBackend(workers=cores)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        so.advance_rk4(st, fft)
+    el = time.perf_counter() - t0
+    del np
+    return {"value": N ** 3 * steps / el / 1e6, "unit": "Mpts*RK4-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} RK4 steps of the same {N}^3 {closure} case (oracle, {fft.name}), "
+                      f"{el:.1f} s wall", "seconds": el}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--N", type=int, default=128)
+    ap.add_argument("--closure", default="smagorinsky", choices=["smagorinsky", "dns"])
+    ap.add_argument("--event-steps", type=int, default=50, help="steps timed per kernel with HIP events")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--seed", type=int, default=1234)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build advances one grid on one GPU; runs on more than one GPU are not part "
+              "of it (see DESIGN.md for how the grid would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes as C
+
+    import torch
+
+    from pyspectral_amd import _lib, cases
+    from pyspectral_amd.solver import SpectralSolver
+
+    N, nu, cfl = args.N, 1e-3, 0.5
+    model = 1 if args.closure == "smagorinsky" else 0
+    uh = cases.decaying_isotropic(N, seed=args.seed)
+    solver = SpectralSolver(N, N, N, model, nu=nu, cfl=cfl, Cs=0.16)
+    q = torch.from_numpy(uh).to("cuda")
+    qs = q.clone()
+    for _ in range(args.warmup):
+        solver.step(q, qs)
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.step(q, qs)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_per_step = el / args.steps * 1e3
+    value = N ** 3 * args.steps / el / 1e6
+
+    # per-kernel durations from HIP events on the library's stream
+    lib = solver.lib
+    _lib.check(solver.ctx, lib.psx_timing_begin(solver.ctx), "psx_timing_begin")
+    for _ in range(args.event_steps):
+        solver.step(q, qs)
+    ms = (C.c_double * 9)()
+    cnt = (C.c_int * 9)()
+    _lib.check(solver.ctx, lib.psx_timing_end(solver.ctx, ms, cnt), "psx_timing_end")
+    kb = kernel_bytes(N, args.closure)
+    kernels = {}
+    for i, name in enumerate(_lib.KERNEL_NAMES):
+        if cnt[i] == 0:
+            continue
+        avg_ms = ms[i] / cnt[i]
+        entry = {"launches": cnt[i], "avg_us": round(avg_ms * 1e3, 2),
+                 "share": round(ms[i] / max(sum(ms), 1e-30), 4)}
+        if name in kb:
+            entry["GBs"] = round(kb[name] / (avg_ms * 1e-3) / 1e9, 1)
+        kernels[name] = entry
+    dom = max(kernels, key=lambda k: kernels[k]["share"])
+    ach = kb[dom] / (kernels[dom]["avg_us"] * 1e-6) / 1e9
+    E = solver.stats(qs).cpu().tolist()
+    finite = all(map(lambda x: x == x and abs(x) < 1e300, E))
+
+    sb = survey_bytes_per_step(N, args.closure)
+    out = {
+        "metric": "Mpts*RK4-steps/s at 128^3 FP64 LES",
+        "value": round(value, 2),
+        "unit": "Mpts*RK4-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random-phase isotropic field)",
+        "config": {"workload": f"decaying isotropic turbulence LES, {args.closure}, {N}^3 modes "
+                               f"({3 * N // 2}^3 padded), nu={nu}, CFL={cfl}, RK4 (low-storage 4-stage)",
+                   "grid": N, "closure": args.closure, "cfl": cfl, "nu": nu, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": kb[dom]},
+        "step_roofline": {"model_bytes_per_step": int(sb), "achieved": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
+                          "unit": "GB/s", "frac": round(sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels": kernels,
+        "stats_after": {"E": E[0], "Omega": E[1], "finite": finite},
+    }
+    if args.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(N, args.closure, nu, cfl, args.cpu_steps, args.seed)
+        out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

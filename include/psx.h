@@ -91,10 +91,29 @@ int psx_stats(psx_context* ctx, const double* uhat, double* out_dev);
 int psx_spectrum_bins(const psx_context* ctx);
 int psx_spectrum(psx_context* ctx, const double* uhat, double* spec_dev, int resolved);
 
-/* Scalar written by the dynamic tau-model each RHS evaluation (main.tau,
- * src/RHSfunctions.py:857-861), as a device pointer to 4 doubles:
- * tau, LE, PLQLE_filt, filt_PLQLE (real parts; magnitudes for the complex sums). */
-const double* psx_tau_device(const psx_context* ctx);
+/* The dynamic tau-model's coefficient from the last RHS evaluation (main.tau,
+ * src/RHSfunctions.py:857-861), copied to host memory (synchronizes the stream):
+ * out[0] tau as computed, out[1] tau clamped at 0 (the value used), out[2..7] the
+ * reductions LE, PLQLE_filt, filt_PLQLE as (re, im) pairs. */
+int psx_tau(psx_context* ctx, double* out_host8);
+
+/* Instrumentation (no counterpart in the reference, which only prints wall time,
+ * src/PySpec_3d.py:136-145): bracket every kernel launch with HIP events on the
+ * context stream between begin and end.  psx_timing_end synchronizes the
+ * stream and returns, per kernel class, the summed milliseconds and launch
+ * counts in ms[PSX_NKERNEL], count[PSX_NKERNEL]. */
+#define PSX_KERNEL_X_INVERSE 0
+#define PSX_KERNEL_Y_INVERSE 1
+#define PSX_KERNEL_Z_PASS 2
+#define PSX_KERNEL_Y_FORWARD 3
+#define PSX_KERNEL_X_FORWARD_EPILOGUE 4
+#define PSX_KERNEL_CFL_X 5
+#define PSX_KERNEL_CFL_Y 6
+#define PSX_KERNEL_CFL_Z 7
+#define PSX_KERNEL_OTHER 8
+#define PSX_NKERNEL 9
+int psx_timing_begin(psx_context* ctx);
+int psx_timing_end(psx_context* ctx, double* ms, int* count);
 
 #ifdef __cplusplus
 }

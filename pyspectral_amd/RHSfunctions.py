@@ -1,0 +1,53 @@
+"""Drop-in ``computeRHS_*(main, grid, myFFT, utilities)`` (``src/RHSfunctions.py``).
+
+Each function has the reference's signature and side effects: ``main.Q2U()``,
+then the right-hand side of ``main.uhat`` written into ``main.Q``, with
+``main.uhat`` left as the model leaves it (Nyquist entries zeroed for the models
+that truncate, ``src/RHSfunctions.py:10-12``) and, for the closures,
+``main.w0_{u,v,w}`` set.  The work runs in libpsx's HIP kernels on ``main.Qd`` /
+``main.Ud``; which closure runs is fixed by the context ``variables`` created
+for ``turb_model``.
+"""
+from __future__ import annotations
+
+import torch
+
+
+def _rhs(main, want_w0: bool):
+    main.Q2U()
+    uin = main.Ud.clone()
+    w0 = None
+    if want_w0:
+        if main.w0d is None:
+            main.w0d = torch.empty_like(main.Ud)
+        w0 = main.w0d
+    main.solver.rhs(uin, main.t, out=main.Qd, uhat_model=main.Ud, w0=w0)
+
+
+def computeRHS_NOSGS(main, grid, myFFT, utilities):
+    """``src/RHSfunctions.py:3-50``."""
+    _rhs(main, False)
+
+
+# The scipy.weave variant computes the same arithmetic (src/RHSfunctionsWEAVE.py:5-77).
+computeRHS_NOSGS_WEAVE = computeRHS_NOSGS
+
+
+def computeRHS_SMAG(main, grid, myFFT, utilities):
+    """``src/RHSfunctions.py:146-237``."""
+    _rhs(main, True)
+
+
+def computeRHS_tmodel(main, grid, myFFT, utilities):
+    """``src/RHSfunctions.py:240-363``."""
+    _rhs(main, True)
+
+
+def computeRHS_staumodel(main, grid, myFFT, utilities):
+    """``src/RHSfunctions.py:366-497``."""
+    _rhs(main, True)
+
+
+def computeRHS_Dtaumodel(main, grid, myFFT, utilities):
+    """``src/RHSfunctions.py:592-878``."""
+    _rhs(main, True)

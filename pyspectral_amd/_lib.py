@@ -24,8 +24,11 @@ MODEL_IDS = {
 EXPORTS = (
     "psx_create", "psx_destroy", "psx_set_stream", "psx_set_params", "psx_last_error",
     "psx_scratch_bytes", "psx_rhs", "psx_rk_stage", "psx_compute_dt", "psx_step", "psx_stats",
-    "psx_spectrum_bins", "psx_spectrum", "psx_tau_device",
+    "psx_spectrum_bins", "psx_spectrum", "psx_tau", "psx_timing_begin", "psx_timing_end",
 )
+
+KERNEL_NAMES = ("x_inverse", "y_inverse", "z_pass", "y_forward", "x_forward_epilogue",
+                "cfl_x", "cfl_y", "cfl_z", "other")
 
 
 class psx_params(C.Structure):
@@ -73,7 +76,9 @@ def load() -> C.CDLL:
         "psx_stats": (i, [vp, dp, dp]),
         "psx_spectrum_bins": (i, [vp]),
         "psx_spectrum": (i, [vp, dp, dp, i]),
-        "psx_tau_device": (C.c_void_p, [vp]),
+        "psx_tau": (i, [vp, C.POINTER(C.c_double)]),
+        "psx_timing_begin": (i, [vp]),
+        "psx_timing_end": (i, [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -7,7 +7,7 @@ namespace psx {
 #define PSX_PAIRS32(X) X(24, 16) X(48, 32) X(72, 48) X(96, 64) X(144, 96) X(192, 128) X(288, 192) X(384, 256) X(576, 384) X(768, 512)
 #define PSX_PAIRS2X(X) X(32, 16) X(64, 32) X(96, 48) X(128, 64) X(192, 96) X(256, 128) X(384, 192) X(512, 256) X(768, 384)
 
-static unsigned epi_blocks(int L, const EpiArgs& a) {
+unsigned epilogue_blocks(int L, const EpiArgs& a) {
   const int T = epi_T(L);
   unsigned main_blocks = (unsigned)(a.N2 * ((a.H + T - 1) / T));
   long long rest = (long long)a.N1 * a.N2 * (a.Nc - a.H);
@@ -20,7 +20,7 @@ static unsigned epi_blocks(int L, const EpiArgs& a) {
 }
 
 hipError_t launch_epilogue(int L, int ns1, int epi, bool w0, const EpiArgs& a, const double2* tw, hipStream_t s) {
-  const dim3 grid(epi_blocks(L, a));
+  const dim3 grid(epilogue_blocks(L, a));
   const int key = L * 4096 + ns1;
   if (epi == E_DNS) {
     switch (key) {

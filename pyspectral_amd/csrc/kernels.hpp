@@ -333,6 +333,19 @@ __global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__
   }
 }
 
+// Sum a complex value over the block (deterministic tree order); uses >= NT
+// entries of the scratch LDS buffer, which must be free.  Result valid in tid 0.
+__device__ __forceinline__ double2 block_sum2(double2 v, double2* scratch, int tid) {
+  __syncthreads();
+  scratch[tid] = v;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) scratch[tid] = cadd(scratch[tid], scratch[tid + s]);
+    __syncthreads();
+  }
+  return scratch[0];
+}
+
 // ------------------------------------------------------------------ x-forward + epilogue
 enum EpiMode { E_DNS = 0, E_SMAG = 1, E_TAU2 = 2 };
 template <int EPI> struct EpiShape;
@@ -360,6 +373,12 @@ struct EpiArgs {
   const double* coef_dev;     // ... or a device scalar (overrides coef when set)
   int rotate;
   double Om[3];
+  // E_TAU2 reductions for the dynamic tau-model: per-block partial (re, im) of
+  // sum_w PLQL . conj(red_u) with the half-spectrum weights; store_only writes
+  // PLQL (coefficient 1) to w0 (if set) and skips the state update.
+  const double2* red_u;
+  double* red_part;
+  int store_only;
 };
 
 // Accumulate the contribution of transformed field f (value v) to the three RHS
@@ -439,10 +458,11 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         u[c] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : a.uprev[c * fstride + idx];
-        rhs[c] = EPI == E_TAU2 ? a.pl[c * fstride + idx] : cscale(u[c], -a.nu * (ksq + 1e-50));
+        rhs[c] = EPI == E_TAU2 ? make_double2(0.0, 0.0) : cscale(u[c], -a.nu * (ksq + 1e-50));
         if (EPI == E_TAU2 && a.w0) a.w0[c * fstride + idx] = make_double2(0.0, 0.0);
         if (EPI == E_SMAG && a.w0) a.w0[c * fstride + idx] = make_double2(0.0, 0.0);
       }
+      if (EPI == E_TAU2 && a.store_only) continue;
       if (a.rotate) {
         const double kinv = 1.0 / (ksq + 1e-50);
         double2 cr[3];
@@ -461,6 +481,10 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
         a.out[c * fstride + idx] = o;
       }
     }
+    if (EPI == E_TAU2 && a.red_part && tid == 0) {
+      a.red_part[2 * blockIdx.x] = 0.0;
+      a.red_part[2 * blockIdx.x + 1] = 0.0;
+    }
     return;
   }
 
@@ -469,6 +493,7 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
   const long long in_sl = (long long)a.N2 * a.H;  // x stride in W layout
   const double2* src0 = a.in + (long long)ky * a.H + c0;
   constexpr bool SGS = W0 && (EPI == E_SMAG);
+  static_assert(EPI != E_TAU2 || T * LS >= NT, "block_sum2 needs NT LDS entries");
   double2 acc[IT][3];
   double2 accs[SGS ? IT : 1][3];
 #pragma unroll
@@ -503,7 +528,8 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
   }
 
   const double dtc = a.q0 ? (*a.dt) * a.c : 0.0;
-  const double coef = a.coef_dev ? *a.coef_dev : a.coef;
+  const double coef = (EPI == E_TAU2 && a.store_only) ? 1.0 : (a.coef_dev ? *a.coef_dev : a.coef);
+  double2 red = make_double2(0.0, 0.0);
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int e = tid + it * NT;
@@ -522,7 +548,13 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
           if constexpr (EPI == E_TAU2) {
             const double2 w = cscale(acc[it][cc], coef);
             if (a.w0) a.w0[cc * fstride + idx] = w;
-            rhs[cc] = cadd(a.pl[cc * fstride + idx], w);
+            if (a.red_u) {
+              const double wz = kz == 0 ? 1.0 : (kz < a.N3 / 2 ? 2.0 : 0.0);
+              const double2 r = a.red_u[cc * fstride + idx];
+              red.x += wz * (w.x * r.x + w.y * r.y);
+              red.y += wz * (w.y * r.x - w.x * r.y);
+            }
+            rhs[cc] = a.store_only ? make_double2(0.0, 0.0) : cadd(a.pl[cc * fstride + idx], w);
           } else {
             rhs[cc] = csub(acc[it][cc], cscale(u[cc], a.nu * (ksq + 1e-50)));
           }
@@ -542,13 +574,188 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
           for (int cc = 0; cc < 3; ++cc)
             rhs[cc] = cadd(rhs[cc], csub(cscale(cr[cc], 2.0), cscale(X, 2.0 * k[cc] * kinv)));
         }
+        if (!(EPI == E_TAU2 && a.store_only)) {
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-          double2 o = a.q0 ? cadd(a.q0[cc * fstride + idx], cscale(rhs[cc], dtc)) : rhs[cc];
-          if (a.zero_nyq_out && nyq) o = make_double2(0.0, 0.0);
-          a.out[cc * fstride + idx] = o;
+          for (int cc = 0; cc < 3; ++cc) {
+            double2 o = a.q0 ? cadd(a.q0[cc * fstride + idx], cscale(rhs[cc], dtc)) : rhs[cc];
+            if (a.zero_nyq_out && nyq) o = make_double2(0.0, 0.0);
+            a.out[cc * fstride + idx] = o;
+          }
         }
       }
+    }
+  }
+  if constexpr (EPI == E_TAU2) {
+    if (a.red_part) {
+      const double2 tot = block_sum2(red, buf, tid);
+      if (tid == 0) {
+        a.red_part[2 * blockIdx.x] = tot.x;
+        a.red_part[2 * blockIdx.x + 1] = tot.y;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ 2x-grid PL pass
+// First round of the t-model family (src/RHSfunctions.py:269-323): the x-forward
+// of the six 2x-grid product spectra (no truncation), the projected RHS
+//   PL_i = -i k_j N_ij - i k_i p - nu k^2 pad_2x(u)_i,  p = -k_m k_n N_mn / k^2,
+// its truncation to the base grid (unpad_2x, kept for the final sum), the q
+// part PL_q = PL outside the resolved cube, and the x-inverse of PL_q for the
+// second round -- all in one kernel per (k2, k3-tile).
+struct PLArgs {
+  const double2* in;      // [6][P1][P2][C] x physical, k2 and k3 spectral
+  long long in_sf;
+  int P1, P2, C;          // 2x-grid lengths; C = k3 modes carried (N3)
+  int N1, N2, N3, Nc;     // base grid
+  const double2* u;       // base-grid velocity for the viscous term (already truncated/filtered)
+  double nu;
+  double ks[3];
+  int kpx, kpy, kpz;      // resolved cube: |s1| < kpx, |s2| < kpy, k3 < kpz (index units)
+  double2* pln;           // out: PL on the base grid (state layout, k3 < N3/2), or nullptr
+  double2* xq;            // out: [3][P1][P2][C] x-inverse of PL_q
+  // dynamic tau-model Leonard term: G_i = -i k_j N_ij + i k_i p restricted to
+  // |k_i| < kf (src/RHSfunctions.py:814-819)
+  double2* g_out;         // store G (state layout) or nullptr
+  const double2* g_ref;   // subtract this G and reduce against uf, or nullptr
+  const double2* uf;      // filtered velocity for the LE reduction
+  double kf;
+  double* part;           // per-block (re, im) partial of sum_w (G - g_ref) . conj(uf)
+};
+
+template <int L, int T, bool DYN>
+__global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restrict__ tw) {
+  constexpr int LS = L + 1;
+  constexpr int IT = (L * T + NT - 1) / NT;
+  static_assert(T * LS >= NT, "block_sum2 needs NT LDS entries");
+  __shared__ double2 buf[T * LS];
+  const int tid = threadIdx.x;
+  const int ct = (a.C + T - 1) / T;
+  const int ky = blockIdx.x / ct;
+  const int c0 = (blockIdx.x - ky * ct) * T;
+  const long long in_sl = (long long)a.P2 * a.C;
+  const double2* src0 = a.in + (long long)ky * a.C + c0;
+  const int s2 = signed_freq(ky, a.P2);
+  const double k2 = a.ks[1] * s2;
+  double2 D[IT][3], Pj[IT][3];
+#pragma unroll
+  for (int it = 0; it < IT; ++it)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) D[it][i] = Pj[it][i] = make_double2(0.0, 0.0);
+  constexpr int MI[6] = {0, 1, 2, 0, 0, 1};
+  constexpr int NI[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll 1
+  for (int f = 0; f < 6; ++f) {
+    const double2* src = src0 + (long long)f * a.in_sf;
+    for (int e = tid; e < L * T; e += NT) {
+      const int l = e / T, c = e - (e / T) * T;
+      buf[c * LS + l] = (c0 + c < a.C) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
+    const int m = MI[f], n = NI[f];
+    const double s = (m == n) ? 1.0 : 2.0;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      const int kx = e / T, c = e - (e / T) * T;
+      if (e < L * T && c0 + c < a.C) {
+        const double k[3] = {a.ks[0] * signed_freq(kx, L), k2, a.ks[2] * (c0 + c)};
+        const double kinv = 1.0 / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + 1e-50);
+        const double2 v = buf[c * LS + kx];
+        const double2 mi = cmul_i<-1>(v);
+        const double kk = s * k[m] * k[n] * kinv;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          if (i == m) D[it][i] = cadd(D[it][i], cscale(mi, k[n]));
+          if (i == n && m != n) D[it][i] = cadd(D[it][i], cscale(mi, k[m]));
+          Pj[it][i] = cadd(Pj[it][i], cscale(cmul_i<1>(v), k[i] * kk));
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // PL = D + P - nu k^2 u; base-grid copy; Leonard reduction; q part into LDS rows
+  const long long plane = (long long)a.N2 * a.Nc;
+  const long long fs = (long long)a.N1 * plane;
+  double2 red = make_double2(0.0, 0.0);
+  double2 q[IT][3];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * NT;
+    const int kx = e / T, c = e - (e / T) * T;
+    const int kz = c0 + c;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) q[it][i] = make_double2(0.0, 0.0);
+    if (e < L * T && kz < a.C) {
+      const int s1 = signed_freq(kx, L);
+      const double k[3] = {a.ks[0] * s1, k2, a.ks[2] * kz};
+      const double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+      const bool in_base = (s1 >= -a.N1 / 2 && s1 < a.N1 / 2 && s2 >= -a.N2 / 2 && s2 < a.N2 / 2 && kz < a.N3 / 2);
+      const long long bidx = in_base ? ((long long)((s1 + a.N1) % a.N1) * plane +
+                                        (long long)((s2 + a.N2) % a.N2) * a.Nc + kz)
+                                     : 0;
+      const bool has_u = in_base && s1 > -a.N1 / 2 && s2 > -a.N2 / 2;
+      double2 pl[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        pl[i] = cadd(D[it][i], Pj[it][i]);
+        if (has_u) pl[i] = csub(pl[i], cscale(a.u[i * fs + bidx], a.nu * (ksq + 1e-50)));
+        if (in_base && a.pln) a.pln[i * fs + bidx] = pl[i];
+      }
+      if constexpr (DYN) {
+        const bool in_f = in_base && fabs(k[0]) < a.kf && fabs(k[1]) < a.kf && fabs(k[2]) < a.kf;
+        if (in_f) {
+          const double wz = kz == 0 ? 1.0 : (kz < a.N3 / 2 ? 2.0 : 0.0);
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            // G = -i k_j N_ij + i k_i p = D - P
+            const double2 G = csub(D[it][i], Pj[it][i]);
+            if (a.g_out) a.g_out[i * fs + bidx] = G;
+            if (a.g_ref) {
+              const double2 d = csub(G, a.g_ref[i * fs + bidx]);
+              const double2 r = a.uf[i * fs + bidx];
+              red.x += wz * (d.x * r.x + d.y * r.y);
+              red.y += wz * (d.y * r.x - d.x * r.y);
+            }
+          }
+        }
+      }
+      const bool inside = (s1 > -a.kpx && s1 < a.kpx && s2 > -a.kpy && s2 < a.kpy && kz < a.kpz);
+      if (!inside) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) q[it][i] = pl[i];
+      }
+    }
+  }
+  if constexpr (DYN) {
+    if (a.part) {
+      const double2 tot = block_sum2(red, buf, tid);
+      if (tid == 0) {
+        a.part[2 * blockIdx.x] = tot.x;
+        a.part[2 * blockIdx.x + 1] = tot.y;
+      }
+    }
+  }
+  // x-inverse of PL_q, one component at a time
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (e < L * T) {
+        const int kx = e / T, c = e - (e / T) * T;
+        buf[c * LS + kx] = q[it][i];
+      }
+    }
+    __syncthreads();
+    fft_batch<L, T, NT, LS, +1>(buf, tw, tid);
+    double2* dst = a.xq + (long long)i * a.P1 * a.P2 * a.C + (long long)ky * a.C + c0;
+    for (int e = tid; e < L * T; e += NT) {
+      const int x = e / T, c = e - (e / T) * T;
+      if (c0 + c < a.C) dst[(long long)x * in_sl + c] = buf[c * LS + x];
     }
   }
 }

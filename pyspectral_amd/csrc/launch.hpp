@@ -23,5 +23,8 @@ hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipS
 int zpass_positions(int L, int mode);
 // epi: EpiMode; ns1 = spectral length along x
 hipError_t launch_epilogue(int L, int ns1, int epi, bool w0, const EpiArgs& a, const double2* tw, hipStream_t s);
+unsigned epilogue_blocks(int L, const EpiArgs& a);
+hipError_t launch_pl_pass(int L, bool dyn, const PLArgs& a, const double2* tw, hipStream_t s);
+unsigned pl_pass_blocks(int L, const PLArgs& a);
 
 }  // namespace psx

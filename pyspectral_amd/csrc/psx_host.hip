@@ -1,6 +1,7 @@
 // libpsx: context, plans and the C ABI declared in include/psx.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -64,7 +65,18 @@ struct psx_context {
   double2* bufA = nullptr;
   double2* bufB = nullptr;
   size_t nA = 0, nB = 0;
-  double2* w0_scratch = nullptr;  // unused placeholder for models without w0
+  // t-model family: Y1 (velocity after the 2x y-inverse), PL and PLQL on the base
+  // grid, the filtered velocity and the filtered Leonard block (dynamic tau)
+  double2* bufC = nullptr;
+  double2* pln = nullptr;
+  double2* plql = nullptr;
+  double2* uf = nullptr;
+  double2* glf = nullptr;
+  size_t nC = 0;
+  double* partLE = nullptr;
+  double* partA = nullptr;
+  double* partB = nullptr;
+  int npartT = 0;
   unsigned long long* red = nullptr;
   double* scal = nullptr;  // device scalars: [0] dt (internal), [1..4] tau family
   double* part = nullptr;
@@ -72,6 +84,11 @@ struct psx_context {
   std::map<int, double2*> tw;
   long long scratch_bytes = 0;
   std::string err;
+  // optional per-kernel timing with HIP events on the context stream
+  bool timing = false;
+  std::vector<hipEvent_t> events;
+  size_t events_used = 0;
+  std::vector<int> event_kinds;
 };
 
 namespace {
@@ -105,6 +122,32 @@ int guarded(psx_context* c, F&& f) {
 
 bool truncating(int model) { return model != PSX_MODEL_SMAGORINSKY; }
 
+// Kernel classes reported by psx_timing_end (order of PSX_KERNEL_* in psx.h).
+enum Kind { K_XINV = 0, K_YINV, K_ZPASS, K_YFWD, K_XEPI, K_CFL_X, K_CFL_Y, K_CFL_Z, K_OTHER, K_NKIND };
+
+hipEvent_t next_event(psx_context* c) {
+  if (c->events_used == c->events.size()) {
+    hipEvent_t e;
+    check(c, hipEventCreate(&e), "hipEventCreate");
+    c->events.push_back(e);
+  }
+  return c->events[c->events_used++];
+}
+
+// Brackets one launch with start/stop events when timing is on.
+struct TimeScope {
+  psx_context* c;
+  TimeScope(psx_context* ctx, int kind) : c(ctx) {
+    if (c->timing) {
+      check(c, hipEventRecord(next_event(c), c->stream), "hipEventRecord");
+      c->event_kinds.push_back(kind);
+    }
+  }
+  ~TimeScope() {
+    if (c->timing) hipEventRecord(next_event(c), c->stream);
+  }
+};
+
 const double2* twiddles(psx_context* c, int L) {
   auto it = c->tw.find(L);
   if (it != c->tw.end()) return it->second;
@@ -134,7 +177,7 @@ PencilArgs pencil_base() {
 // ---- pipeline building blocks --------------------------------------------------
 // x-inverse from the state layout: (nf fields of uin) -> X[f][L1][N2][C]
 void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, int keep_nyq, int njobs,
-               const int* jin, const int* jplain, const int* jderiv) {
+               const int* jin, const int* jplain, const int* jderiv, int kind = K_XINV) {
   PencilArgs a = pencil_base();
   a.in = uin;
   a.out = X;
@@ -154,24 +197,27 @@ void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, in
     a.job_plain[j] = jplain[j];
     a.job_deriv[j] = jderiv ? jderiv[j] : -1;
   }
-  check(c, launch_pencil_inverse(L1, a, twiddles(c, L1), njobs, c->stream), "x_inverse");
+  const double2* tw = twiddles(c, L1);
+  TimeScope ts(c, kind);
+  check(c, launch_pencil_inverse(L1, a, tw, njobs, c->stream), "x_inverse");
 }
 
 // y-inverse: X[f][L1][N2][C] -> Y[f][L1][L2][C]
 void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int C, int keep_nyq, int njobs,
-               const int* jin, const int* jplain, const int* jderiv) {
+               const int* jin, const int* jplain, const int* jderiv, int kind = K_YINV, int ns2 = 0) {
+  if (ns2 == 0) ns2 = c->N2;
   PencilArgs a = pencil_base();
   a.in = X;
   a.out = Y;
-  a.in_sf = (long long)L1 * c->N2 * C;
-  a.in_sa = (long long)c->N2 * C;
+  a.in_sf = (long long)L1 * ns2 * C;
+  a.in_sa = (long long)ns2 * C;
   a.in_sl = C;
   a.out_sf = (long long)L1 * L2 * C;
   a.out_sa = (long long)L2 * C;
   a.out_sl = C;
   a.A = L1;
   a.C = C;
-  a.Nspec = c->N2;
+  a.Nspec = ns2;
   a.keep_nyq = keep_nyq;
   a.kscale = 2.0 * kPi / c->p.L2;
   for (int j = 0; j < njobs; ++j) {
@@ -179,29 +225,35 @@ void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int
     a.job_plain[j] = jplain[j];
     a.job_deriv[j] = jderiv ? jderiv[j] : -1;
   }
-  check(c, launch_pencil_inverse(L2, a, twiddles(c, L2), njobs, c->stream), "y_inverse");
+  const double2* tw = twiddles(c, L2);
+  TimeScope ts(c, kind);
+  check(c, launch_pencil_inverse(L2, a, tw, njobs, c->stream), "y_inverse");
 }
 
 // y-forward with truncation: Z[f][L1][L2][C] -> W[f][L1][N2][C]
-void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int C, int keep_nyq, int nf) {
+void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int C, int keep_nyq, int nf,
+               int ns2 = 0) {
+  if (ns2 == 0) ns2 = c->N2;
   PencilArgs a = pencil_base();
   a.in = Z;
   a.out = W;
   a.in_sf = (long long)L1 * L2 * C;
   a.in_sa = (long long)L2 * C;
   a.in_sl = C;
-  a.out_sf = (long long)L1 * c->N2 * C;
-  a.out_sa = (long long)c->N2 * C;
+  a.out_sf = (long long)L1 * ns2 * C;
+  a.out_sa = (long long)ns2 * C;
   a.out_sl = C;
   a.A = L1;
   a.C = C;
-  a.Nspec = c->N2;
+  a.Nspec = ns2;
   a.keep_nyq = keep_nyq;
   for (int j = 0; j < nf; ++j) {
     a.job_in[j] = j;
     a.job_plain[j] = j;
   }
-  check(c, launch_pencil_forward(L2, a, twiddles(c, L2), nf, c->stream), "y_forward");
+  const double2* tw = twiddles(c, L2);
+  TimeScope ts(c, K_YFWD);
+  check(c, launch_pencil_forward(L2, a, tw, nf, c->stream), "y_forward");
 }
 
 // RHS / stage for the 3/2-rule models (DNS, Smagorinsky)
@@ -256,7 +308,9 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
     const int mode = smag ? Z_SMAG : Z_DNS;
     const int P = zpass_positions(M3, mode);
     if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
-    check(c, launch_zpass(M3, mode, z, twiddles(c, M3), c->stream), "z_pass");
+    const double2* tw = twiddles(c, M3);
+    TimeScope ts(c, K_ZPASS);
+    check(c, launch_zpass(M3, mode, z, tw, c->stream), "z_pass");
   }
   const int nF = smag ? 12 : 6;
   // K4: y-forward + truncation
@@ -289,21 +343,219 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
     e.Om[0] = c->p.Om1;
     e.Om[1] = c->p.Om2;
     e.Om[2] = c->p.Om3;
-    check(c, launch_epilogue(M1, c->N1, smag ? E_SMAG : E_DNS, smag && w0 != nullptr, e, twiddles(c, M1), c->stream),
+    const double2* tw = twiddles(c, M1);
+    TimeScope ts(c, K_XEPI);
+    check(c, launch_epilogue(M1, c->N1, smag ? E_SMAG : E_DNS, smag && w0 != nullptr, e, tw, c->stream),
           "x_forward_epilogue");
   }
 }
 
+// One pass of the 2x-grid t-model pipeline on velocity u (already filtered for the
+// test-filtered pass of the dynamic model): round 1 (products on the 2x grid, PL,
+// PL_q), round 2 (u_j (PL_q)_i on the 2x grid) and the final x-forward epilogue.
+// src/RHSfunctions.py:240-363 (t-model), 366-497 (static tau), 592-878 (dynamic).
+void tau_pass(psx_context* c, const double2* u, bool filtered, const double2* uin, const double2* q0, double2* out,
+              const double* dt, double stage_c, double2* w0, int zero_nyq_out, double tcoef) {
+  const bool dyn = c->model == PSX_MODEL_DYNAMIC_TAU;
+  const int N1 = c->N1, N2 = c->N2, N3 = c->N3, H = c->H;
+  const int P1 = 2 * N1, P2 = 2 * N2, P3 = 2 * N3;
+  double2* A = c->bufA;
+  double2* B = c->bufB;
+  double2* Y1 = c->bufC;
+  const double Nt = (double)N1 * N2 * N3;
+  const double kfd = (double)c->p.kc * 2.0 / 3.0;
+  const int kf = (int)kfd;  // int(kc*2/3), src/Classes.py:131
+  {
+    const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2};
+    x_inverse(c, u, A, P1, H, 0, 3, jin, jp, nullptr);
+    y_inverse(c, A, Y1, P1, P2, H, 0, 3, jin, jp, nullptr);
+  }
+  {
+    ZArgs z;
+    std::memset(&z, 0, sizeof(z));
+    z.in0 = Y1;
+    z.in_sf0 = (long long)P1 * P2 * H;
+    z.Hin0 = H;
+    z.nin0 = 3;
+    z.out = A;
+    z.out_sf = (long long)P1 * P2 * N3;
+    z.Hout = N3;
+    z.npos = P1 * P2;
+    z.kzscale = 2.0 * kPi / c->p.L3;
+    z.out_scale = 1.0 / (std::sqrt(Nt) * (double)P1 * P2 * P3);
+    for (int r = 0; r < 3; ++r) z.rec[r] = Recipe{1, {r, 0}, {1.0, 0.0}, {0, 0}};
+    const int P = zpass_positions(P3, Z_DNS);
+    if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported 2x length");
+    const double2* tw = twiddles(c, P3);
+    TimeScope ts(c, K_ZPASS);
+    check(c, launch_zpass(P3, Z_DNS, z, tw, c->stream), "z_pass(2x)");
+  }
+  y_forward(c, A, B, P1, P2, N3, 1, 6, P2);
+  {
+    PLArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = B;
+    a.in_sf = (long long)P1 * P2 * N3;
+    a.P1 = P1;
+    a.P2 = P2;
+    a.C = N3;
+    a.N1 = N1;
+    a.N2 = N2;
+    a.N3 = N3;
+    a.Nc = c->Nc;
+    a.u = u;
+    a.nu = c->p.nu;
+    a.ks[0] = 2.0 * kPi / c->p.L1;
+    a.ks[1] = 2.0 * kPi / c->p.L2;
+    a.ks[2] = 2.0 * kPi / c->p.L3;
+    if (filtered) {
+      a.kpx = a.kpy = a.kpz = kf;  // seperateModes_testFilter (src/padding.py:52-64)
+    } else {
+      a.kpx = N1 / 2;              // seperateModes (src/padding.py:67-79)
+      a.kpy = N2 / 2;
+      a.kpz = N3 / 2;
+    }
+    a.pln = filtered ? nullptr : c->pln;
+    a.xq = A;
+    if (dyn) {
+      a.kf = kfd;
+      a.uf = c->uf;
+      if (filtered) {
+        a.g_out = c->glf;
+      } else {
+        a.g_ref = c->glf;
+        a.part = c->partLE;
+      }
+    }
+    const double2* tw = twiddles(c, P1);
+    TimeScope ts(c, K_OTHER);
+    check(c, launch_pl_pass(P1, dyn, a, tw, c->stream), "pl_pass");
+  }
+  {
+    const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2};
+    y_inverse(c, A, B, P1, P2, N3, 1, 3, jin, jp, nullptr, K_YINV, P2);
+  }
+  {
+    ZArgs z;
+    std::memset(&z, 0, sizeof(z));
+    z.in0 = Y1;
+    z.in_sf0 = (long long)P1 * P2 * H;
+    z.Hin0 = H;
+    z.nin0 = 3;
+    z.in1 = B;
+    z.in_sf1 = (long long)P1 * P2 * N3;
+    z.Hin1 = N3;
+    z.out = A;
+    z.out_sf = (long long)P1 * P2 * H;
+    z.Hout = H;
+    z.npos = P1 * P2;
+    z.kzscale = 2.0 * kPi / c->p.L3;
+    z.out_scale = 1.0 / (std::sqrt(Nt) * (double)P1 * P2 * P3);
+    for (int r = 0; r < 6; ++r) z.rec[r] = Recipe{1, {r, 0}, {1.0, 0.0}, {0, 0}};
+    const int P = zpass_positions(P3, Z_TM2);
+    if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported 2x length");
+    const double2* tw = twiddles(c, P3);
+    TimeScope ts(c, K_ZPASS);
+    check(c, launch_zpass(P3, Z_TM2, z, tw, c->stream), "z_pass(2x, round 2)");
+  }
+  y_forward(c, A, B, P1, P2, H, 1, 9, N2);
+  {
+    EpiArgs e;
+    std::memset(&e, 0, sizeof(e));
+    e.in = B;
+    e.in_sf = (long long)P1 * N2 * H;
+    e.N1 = N1;
+    e.N2 = N2;
+    e.N3 = N3;
+    e.Nc = c->Nc;
+    e.H = H;
+    e.keep_nyq = 1;
+    e.uprev = uin;
+    e.pl = c->pln;
+    e.dt = dt;
+    e.c = stage_c;
+    e.nu = c->p.nu;
+    e.ks[0] = 2.0 * kPi / c->p.L1;
+    e.ks[1] = 2.0 * kPi / c->p.L2;
+    e.ks[2] = 2.0 * kPi / c->p.L3;
+    e.trunc_u = 1;
+    e.zero_nyq_out = zero_nyq_out;
+    e.rotate = c->p.rotate;
+    e.Om[0] = c->p.Om1;
+    e.Om[1] = c->p.Om2;
+    e.Om[2] = c->p.Om3;
+    if (dyn) {
+      e.store_only = 1;
+      e.w0 = filtered ? nullptr : c->plql;
+      e.red_u = c->uf;
+      e.red_part = filtered ? c->partA : c->partB;
+    } else {
+      e.q0 = q0;
+      e.out = out;
+      e.w0 = w0;
+      e.coef = tcoef;
+    }
+    const double2* tw = twiddles(c, P1);
+    TimeScope ts(c, K_XEPI);
+    check(c, launch_epilogue(P1, N1, E_TAU2, false, e, tw, c->stream), "x_forward_epilogue(2x)");
+  }
+}
+
+void stage_tau(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double stage_c,
+               double2* w0, int zero_nyq_out, double t) {
+  const int N1 = c->N1, N2 = c->N2, N3 = c->N3, Nc = c->Nc;
+  const long long total = 3LL * N1 * N2 * Nc;
+  const unsigned eb = (unsigned)std::min<long long>((total + 255) / 256, 4096);
+  const double ks0 = 2.0 * kPi / c->p.L1, ks1 = 2.0 * kPi / c->p.L2, ks2 = 2.0 * kPi / c->p.L3;
+  if (c->model == PSX_MODEL_TMODEL) {
+    tau_pass(c, uin, false, uin, q0, out, dt, stage_c, w0, zero_nyq_out, c->p.Ct * t);
+    return;
+  }
+  if (c->model == PSX_MODEL_STATIC_TAU) {
+    tau_pass(c, uin, false, uin, q0, out, dt, stage_c, w0, zero_nyq_out, std::max(0.0, c->p.tau_a + c->p.tau_b * t));
+    return;
+  }
+  // dynamic tau-model: filtered pass, unfiltered pass, tau, update
+  const double kf = (double)(int)((double)c->p.kc * 2.0 / 3.0);
+  hipLaunchKernelGGL(filter_copy, dim3(eb), dim3(256), 0, c->stream, uin, c->uf, N1, N2, N3, Nc, ks0, ks1, ks2, kf);
+  check(c, hipGetLastError(), "filter_copy");
+  tau_pass(c, c->uf, true, uin, nullptr, nullptr, dt, stage_c, nullptr, 0, 0.0);
+  tau_pass(c, uin, false, uin, nullptr, nullptr, dt, stage_c, nullptr, 0, 0.0);
+  const int P1 = 2 * N1, P2 = 2 * N2;
+  PLArgs pa;
+  std::memset(&pa, 0, sizeof(pa));
+  pa.P2 = P2;
+  pa.C = N3;
+  EpiArgs ea;
+  std::memset(&ea, 0, sizeof(ea));
+  ea.N1 = N1;
+  ea.N2 = N2;
+  ea.Nc = Nc;
+  ea.H = c->H;
+  const int nLE = (int)pl_pass_blocks(P1, pa);
+  const int nE = (int)epilogue_blocks(P1, ea);
+  hipLaunchKernelGGL(tau_finalize, dim3(1), dim3(1), 0, c->stream, c->partLE, nLE, c->partA, nE, c->partB, nE,
+                     1.0 / ((double)N1 * N2 * N3), c->scal + 1);
+  check(c, hipGetLastError(), "tau_finalize");
+  hipLaunchKernelGGL(tau_update, dim3(eb), dim3(256), 0, c->stream, c->pln, c->plql, c->scal + 2, uin, q0, out, w0, dt,
+                     stage_c, N1, N2, N3, Nc, ks0, ks1, ks2, zero_nyq_out, c->p.rotate, c->p.Om1, c->p.Om2, c->p.Om3);
+  check(c, hipGetLastError(), "tau_update");
+}
+
 void run_stage(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double coef,
                double2* w0, int zero_nyq_out, double t) {
-  (void)t;
   switch (c->model) {
     case PSX_MODEL_DNS:
     case PSX_MODEL_SMAGORINSKY:
       stage_32(c, uin, q0, out, dt, coef, w0, zero_nyq_out);
       break;
+    case PSX_MODEL_TMODEL:
+    case PSX_MODEL_STATIC_TAU:
+    case PSX_MODEL_DYNAMIC_TAU:
+      stage_tau(c, uin, q0, out, dt, coef, w0, zero_nyq_out, t);
+      break;
     default:
-      fail(c, PSX_E_UNSUPPORTED, "model not available in this build");
+      fail(c, PSX_E_UNSUPPORTED, "unknown model");
   }
 }
 
@@ -317,8 +569,8 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   hipLaunchKernelGGL(set_u64, dim3(1), dim3(1), 0, c->stream, c->red, 0ULL);
   check(c, hipGetLastError(), "set_u64");
   const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2};
-  x_inverse(c, u, c->bufA, N1, Nc, 1, 3, jin, jp, nullptr);
-  y_inverse(c, c->bufA, c->bufB, N1, N2, Nc, 1, 3, jin, jp, nullptr);
+  x_inverse(c, u, c->bufA, N1, Nc, 1, 3, jin, jp, nullptr, K_CFL_X);
+  y_inverse(c, c->bufA, c->bufB, N1, N2, Nc, 1, 3, jin, jp, nullptr, K_CFL_Y);
   ZArgs z;
   std::memset(&z, 0, sizeof(z));
   z.in0 = c->bufB;
@@ -334,7 +586,11 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   z.red = c->red;
   const int P = zpass_positions(N3, Z_DT);
   if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "CFL pass: unsupported length");
-  check(c, launch_zpass(N3, Z_DT, z, twiddles(c, N3), c->stream), "cfl_pass");
+  {
+    const double2* tw = twiddles(c, N3);
+    TimeScope ts(c, K_CFL_Z);
+    check(c, launch_zpass(N3, Z_DT, z, tw, c->stream), "cfl_pass");
+  }
   const double Nt = (double)N1 * N2 * N3;
   const double dx = c->p.dx, dy = c->p.dy, dz = c->p.dz;
   const double visc = c->p.nu > 0
@@ -365,9 +621,9 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
     if (n1 < 4 || n2 < 4 || n3 < 4 || (n1 % 2) || (n2 % 2) || (n3 % 4))
       fail(c, PSX_E_ARG, "grid sizes must be even (N3 a multiple of 4)");
     if (model < PSX_MODEL_DNS || model > PSX_MODEL_DYNAMIC_TAU) fail(c, PSX_E_ARG, "unknown model");
-    if (model != PSX_MODEL_DNS && model != PSX_MODEL_SMAGORINSKY)
-      fail(c, PSX_E_UNSUPPORTED, "model not available in this build");
     validate_params(c, params);
+    if (model == PSX_MODEL_DYNAMIC_TAU && params->rotate)
+      fail(c, PSX_E_UNSUPPORTED, "the dynamic tau-model is not available with rotation");
     c->N1 = n1;
     c->N2 = n2;
     c->N3 = n3;
@@ -381,18 +637,48 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
     c->device = device;
     c->stream = (hipStream_t)stream;
     check(c, hipSetDevice(device), "hipSetDevice");
-    for (int L : {c->M1, c->M2, c->M3, n1, n2, n3})
+    const bool tau = model >= PSX_MODEL_TMODEL;
+    std::vector<int> lens = {n1, n2, n3};
+    if (tau) {
+      for (int L : {2 * n1, 2 * n2, 2 * n3}) lens.push_back(L);
+    } else {
+      for (int L : {c->M1, c->M2, c->M3}) lens.push_back(L);
+    }
+    for (int L : lens)
       if (!length_supported(L)) fail(c, PSX_E_UNSUPPORTED, "grid size not supported: length " + std::to_string(L));
     const bool smag = model == PSX_MODEL_SMAGORINSKY;
-    const size_t MMH = (size_t)c->M1 * c->M2 * c->H;
-    const size_t MNH = (size_t)c->M1 * c->N2 * c->H;
     const size_t state = (size_t)3 * n1 * n2 * c->Nc;
-    const size_t nZ = (smag ? 12 : 6) * MMH;
-    const size_t nX = (smag ? 6 : 3) * MNH;
-    const size_t nY = (smag ? 9 : 3) * MMH;
-    const size_t nW = (smag ? 12 : 6) * MNH;
-    c->nA = std::max(std::max(nX, nZ), state);
-    c->nB = std::max(std::max(nY, nW), state);
+    if (!tau) {
+      const size_t MMH = (size_t)c->M1 * c->M2 * c->H;
+      const size_t MNH = (size_t)c->M1 * c->N2 * c->H;
+      const size_t nZ = (smag ? 12 : 6) * MMH;
+      const size_t nX = (smag ? 6 : 3) * MNH;
+      const size_t nY = (smag ? 9 : 3) * MMH;
+      const size_t nW = (smag ? 12 : 6) * MNH;
+      c->nA = std::max(std::max(nX, nZ), state);
+      c->nB = std::max(std::max(nY, nW), state);
+    } else {
+      const size_t P1 = 2 * (size_t)n1, P2 = 2 * (size_t)n2;
+      const size_t full = P1 * P2 * (size_t)n3;       // one 2x-grid field, k3 < N3
+      const size_t half = P1 * P2 * (size_t)c->H;     // k3 < N3/2
+      const size_t X1 = 3 * P1 * (size_t)n2 * c->H, W2 = 9 * P1 * (size_t)n2 * c->H;
+      c->nA = std::max({X1, 6 * full, 3 * full, 9 * half, state});
+      c->nB = std::max({6 * full, 3 * full, W2, state});
+      c->nC = 3 * half;
+      check(c, hipMalloc(&c->bufC, c->nC * sizeof(double2)), "hipMalloc(bufC)");
+      check(c, hipMalloc(&c->pln, state * sizeof(double2)), "hipMalloc(pl)");
+      c->scratch_bytes += (long long)(c->nC + state) * sizeof(double2);
+      if (model == PSX_MODEL_DYNAMIC_TAU) {
+        check(c, hipMalloc(&c->plql, state * sizeof(double2)), "hipMalloc(plql)");
+        check(c, hipMalloc(&c->uf, state * sizeof(double2)), "hipMalloc(uf)");
+        check(c, hipMalloc(&c->glf, state * sizeof(double2)), "hipMalloc(glf)");
+        c->npartT = 1 << 20;
+        check(c, hipMalloc(&c->partLE, 2 * sizeof(double) * c->npartT), "hipMalloc(partLE)");
+        check(c, hipMalloc(&c->partA, 2 * sizeof(double) * c->npartT), "hipMalloc(partA)");
+        check(c, hipMalloc(&c->partB, 2 * sizeof(double) * c->npartT), "hipMalloc(partB)");
+        c->scratch_bytes += (long long)3 * state * sizeof(double2) + 6LL * sizeof(double) * c->npartT;
+      }
+    }
     check(c, hipMalloc(&c->bufA, c->nA * sizeof(double2)), "hipMalloc(bufA)");
     check(c, hipMalloc(&c->bufB, c->nB * sizeof(double2)), "hipMalloc(bufB)");
     check(c, hipMalloc(&c->red, 64), "hipMalloc(red)");
@@ -401,7 +687,7 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
     c->npart = 1024;
     check(c, hipMalloc(&c->part, 3 * c->npart * sizeof(double)), "hipMalloc(part)");
     c->scratch_bytes += (long long)(c->nA + c->nB) * sizeof(double2) + 64 + 16 * 8 + 3 * c->npart * 8;
-    for (int L : {c->M1, c->M2, c->M3, n1, n2, n3}) twiddles(c, L);
+    for (int L : lens) twiddles(c, L);
   });
   if (rc != PSX_OK) {
     // keep the message reachable through a static buffer
@@ -422,7 +708,16 @@ int psx_destroy(psx_context* c) {
   hipFree(c->red);
   hipFree(c->scal);
   hipFree(c->part);
+  hipFree(c->bufC);
+  hipFree(c->pln);
+  hipFree(c->plql);
+  hipFree(c->uf);
+  hipFree(c->glf);
+  hipFree(c->partLE);
+  hipFree(c->partA);
+  hipFree(c->partB);
   for (auto& kv : c->tw) hipFree(kv.second);
+  for (auto e : c->events) hipEventDestroy(e);
   delete c;
   return PSX_OK;
 }
@@ -524,6 +819,40 @@ int psx_spectrum(psx_context* c, const double* uhat, double* spec_dev, int resol
   });
 }
 
-const double* psx_tau_device(const psx_context* c) { return c ? c->scal + 1 : nullptr; }
+int psx_tau(psx_context* c, double* out) {
+  return guarded(c, [&] {
+    if (!out) fail(c, PSX_E_ARG, "null pointer");
+    check(c, hipMemcpyAsync(out, c->scal + 1, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+int psx_timing_begin(psx_context* c) {
+  return guarded(c, [&] {
+    c->timing = true;
+    c->events_used = 0;
+    c->event_kinds.clear();
+  });
+}
+
+int psx_timing_end(psx_context* c, double* ms, int* count) {
+  return guarded(c, [&] {
+    if (!ms || !count) fail(c, PSX_E_ARG, "null pointer");
+    c->timing = false;
+    check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (int k = 0; k < K_NKIND; ++k) {
+      ms[k] = 0.0;
+      count[k] = 0;
+    }
+    for (size_t i = 0; i < c->event_kinds.size(); ++i) {
+      float t = 0.f;
+      check(c, hipEventElapsedTime(&t, c->events[2 * i], c->events[2 * i + 1]), "hipEventElapsedTime");
+      ms[c->event_kinds[i]] += t;
+      count[c->event_kinds[i]] += 1;
+    }
+    c->events_used = 0;
+    c->event_kinds.clear();
+  });
+}
 
 }  // extern "C"

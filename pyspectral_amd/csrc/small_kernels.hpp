@@ -138,4 +138,93 @@ __global__ void spectrum_kernel(const double2* __restrict__ u, double* __restric
   }
 }
 
+// Velocity as the t-model family sees it: Nyquist entries zeroed and, with
+// kf > 0, the sharp test filter |k_i| < kf (src/Classes.py:132-150).
+__global__ void filter_copy(const double2* __restrict__ in, double2* __restrict__ out, int N1, int N2, int N3,
+                            int Nc, double ks0, double ks1, double ks2, double kf) {
+  const long long total = 3LL * N1 * N2 * Nc;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(e % Nc);
+    const long long r = e / Nc;
+    const int ky = (int)(r % N2);
+    const int kx = (int)((r / N2) % N1);
+    bool keep = kx != N1 / 2 && ky != N2 / 2 && kz != N3 / 2;
+    if (kf > 0) {
+      keep = keep && fabs(ks0 * signed_freq(kx, N1)) < kf && fabs(ks1 * signed_freq(ky, N2)) < kf &&
+             fabs(ks2 * kz) < kf;
+    }
+    out[e] = keep ? in[e] : make_double2(0.0, 0.0);
+  }
+}
+
+// Dynamic tau from the three reductions (src/RHSfunctions.py:822-861):
+// tau = |LE| / (|PLQLE_filt| - |filt_PLQLE| + 1e-5), used clamped at 0.
+// scal[0] raw tau, scal[1] clamped tau, scal[2..7] LE, PLQLE_filt, filt_PLQLE (re, im).
+__global__ void tau_finalize(const double* __restrict__ pLE, int nLE, const double* __restrict__ pA, int nA,
+                             const double* __restrict__ pB, int nB, double inv_nt, double* __restrict__ scal) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double le[2] = {0.0, 0.0}, a[2] = {0.0, 0.0}, b[2] = {0.0, 0.0};
+    for (int i = 0; i < nLE; ++i) { le[0] += pLE[2 * i]; le[1] += pLE[2 * i + 1]; }
+    for (int i = 0; i < nA; ++i) { a[0] += pA[2 * i]; a[1] += pA[2 * i + 1]; }
+    for (int i = 0; i < nB; ++i) { b[0] += pB[2 * i]; b[1] += pB[2 * i + 1]; }
+    for (int j = 0; j < 2; ++j) { le[j] *= inv_nt; a[j] *= inv_nt; b[j] *= inv_nt; }
+    const double tau = hypot(le[0], le[1]) / (hypot(a[0], a[1]) - hypot(b[0], b[1]) + 1.0e-5);
+    scal[0] = tau;
+    scal[1] = fmax(0.0, tau);
+    scal[2] = le[0]; scal[3] = le[1];
+    scal[4] = a[0]; scal[5] = a[1];
+    scal[6] = b[0]; scal[7] = b[1];
+  }
+}
+
+// out = q0 + dt*c*(PL + tau*PLQL + Coriolis) (or the RHS itself when q0 is null);
+// w0 = tau*PLQL when requested.  pl/plql hold k3 < N3/2; the k3 = N3/2 plane is zero
+// (unpad_2x drops it).  src/RHSfunctions.py:480-491.
+__global__ void tau_update(const double2* __restrict__ pl, const double2* __restrict__ plql,
+                           const double* __restrict__ tau, const double2* __restrict__ uin,
+                           const double2* __restrict__ q0, double2* __restrict__ out, double2* __restrict__ w0,
+                           const double* __restrict__ dt, double c, int N1, int N2, int N3, int Nc, double ks0,
+                           double ks1, double ks2, int zero_nyq_out, int rotate, double om0, double om1, double om2) {
+  const long long fs = (long long)N1 * N2 * Nc;
+  const double t = *tau;
+  const double dtc = q0 ? (*dt) * c : 0.0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < fs; e += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(e % Nc);
+    const long long r = e / Nc;
+    const int ky = (int)(r % N2);
+    const int kx = (int)(r / N2);
+    const bool nyq = kx == N1 / 2 || ky == N2 / 2 || kz == N3 / 2;
+    double2 rhs[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 w = make_double2(0.0, 0.0), p = make_double2(0.0, 0.0);
+      if (kz < N3 / 2) {
+        w = cscale(plql[i * fs + e], t);
+        p = pl[i * fs + e];
+      }
+      if (w0) w0[i * fs + e] = w;
+      rhs[i] = cadd(p, w);
+    }
+    if (rotate && !nyq) {
+      const double k[3] = {ks0 * signed_freq(kx, N1), ks1 * signed_freq(ky, N2), ks2 * kz};
+      const double kinv = 1.0 / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + 1e-50);
+      const double2 u0 = uin[e], u1 = uin[fs + e], u2 = uin[2 * fs + e];
+      double2 cr[3];
+      cr[0] = csub(cscale(u1, om2), cscale(u2, om1));
+      cr[1] = csub(cscale(u2, om0), cscale(u0, om2));
+      cr[2] = csub(cscale(u0, om1), cscale(u1, om0));
+      const double2 X = cadd(cadd(cscale(cr[0], k[0]), cscale(cr[1], k[1])), cscale(cr[2], k[2]));
+#pragma unroll
+      for (int i = 0; i < 3; ++i) rhs[i] = cadd(rhs[i], csub(cscale(cr[i], 2.0), cscale(X, 2.0 * k[i] * kinv)));
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 o = q0 ? cadd(q0[i * fs + e], cscale(rhs[i], dtc)) : rhs[i];
+      if (zero_nyq_out && nyq) o = make_double2(0.0, 0.0);
+      out[i * fs + e] = o;
+    }
+  }
+}
+
 }  // namespace psx

@@ -161,6 +161,12 @@ class SpectralSolver:
                    "psx_spectrum")
         return out
 
+    def tau_values(self):
+        """Dynamic tau-model: (tau, tau used, LE re, im, PLQLE_filt re, im, filt_PLQLE re, im)."""
+        out = (C.c_double * 8)()
+        _lib.check(self.ctx, self.lib.psx_tau(self.ctx, out), "psx_tau")
+        return list(out)
+
     def scratch_bytes(self) -> int:
         return int(self.lib.psx_scratch_bytes(self.ctx))
 
