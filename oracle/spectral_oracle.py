@@ -723,11 +723,13 @@ def taylor_green(N1, N2, N3, grid: str = "exact"):
     by pi/2 in x and y, with identical energy and dissipation histories on an
     exact grid.  Returns ``(uhat, vhat, what, (dx, dy, dz))``."""
     def axis(n):
+        # the spacing gridclass reads off the case arrays, x[1] - x[0] (src/Classes.py:562-564)
         if grid == "exact":
             d = 2 * np.pi / n
             return np.arange(n) * d, d
         d = 2 * np.pi / (n - 1)
-        return np.linspace(0, 2 * np.pi - d, n), d
+        x = np.linspace(0, 2 * np.pi - d, n)
+        return x, x[1] - x[0]
 
     x, dx = axis(N1)
     y, dy = axis(N2)

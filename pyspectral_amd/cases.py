@@ -14,14 +14,15 @@ def taylor_green(N: int, grid: str = "exact"):
     ``grid='exact'``: u = (sin x cos y cos z, -cos x sin y cos z, 0) on x_j = 2 pi j/N.
     ``grid='reference'``: the case grid of ``run/TGV_Validate100/inputfile.py:31-44``,
     ``linspace(0, 2pi - dx, N)`` with dx = 2pi/(N-1) and u = (cos x sin y cos z,
-    -sin x cos y cos z, 0).  Returns ``(uhat, (dx, dy, dz))``.
+    -sin x cos y cos z, 0).  Returns ``(uhat, (dx, dy, dz))`` with the spacing of the
+    sample points.
     """
     if grid == "exact":
         d = 2 * np.pi / N
         x = np.arange(N) * d
     elif grid == "reference":
-        d = 2 * np.pi / (N - 1)
-        x = np.linspace(0, 2 * np.pi - d, N)
+        x = np.linspace(0, 2 * np.pi - 2 * np.pi / (N - 1), N)
+        d = x[1] - x[0]  # the spacing gridclass reads off the case arrays (src/Classes.py:562-564)
     else:
         raise ValueError(grid)
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")

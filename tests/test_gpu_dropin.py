@@ -16,7 +16,7 @@ def _case(N, nu, dt, grid="reference", turb_model=0, et=1.0, **extra):
 
     uh, (dx, dy, dz) = cases.taylor_green(N, grid=grid)
     if grid == "reference":
-        x = np.linspace(0, 2 * np.pi - dx, N)
+        x = np.linspace(0, 2 * np.pi - 2 * np.pi / (N - 1), N)
     else:
         x = np.arange(N) * dx
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
