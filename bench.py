@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--event-steps", type=int, default=50, help="steps timed per kernel with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size dispatch "
+                         "that calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -100,6 +103,14 @@ def main():
 
     from pyspectral_amd import _lib, cases
     from pyspectral_amd.solver import SpectralSolver
+
+    if args.copy_mib > 0:
+        n = args.copy_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     N, nu, cfl = args.N, 1e-3, 0.5
     model = 1 if args.closure == "smagorinsky" else 0

@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpsx.so")
+LIB_PATH = os.environ.get("PSX_LIBRARY") or os.path.join(HERE, "libpsx.so")
 
 PSX_OK = 0
 MODEL_IDS = {

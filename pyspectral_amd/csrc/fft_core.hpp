@@ -116,6 +116,18 @@ template <int SIGN> struct Dft<8, SIGN> {
   }
 };
 
+// ---------------------------------------------------------------- LDS row layout
+// Rows of L complex values are stored with one pad slot after every 8 values (so
+// the stride-R writes of a Stockham stage with Ns = 1 fall in distinct banks) and
+// one extra slot per row (so column-wise accesses across rows do too).
+#ifndef PSX_LDS_PAD
+#define PSX_LDS_PAD 1
+#endif
+template <int L> struct Row {
+  static constexpr int LS = PSX_LDS_PAD ? L + L / 8 + 1 : L + 1;
+  static __device__ __forceinline__ int at(int i) { return PSX_LDS_PAD ? i + (i >> 3) : i; }
+};
+
 // ---------------------------------------------------------------- Stockham batch
 // B transforms of length L, transform b at buf[b*LS .. b*LS+L), NT threads.
 template <int L, int B, int NT, int LS, int SIGN, int S, int NS>
@@ -133,9 +145,9 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
       if ((TOT % NT) == 0 || j < TOT) {
         const int b = j / NBF;
         const int jj = j - b * NBF;
-        const double2* src = buf + b * LS + jj;
+        const double2* src = buf + b * LS;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[it][r] = src[r * NBF];
+        for (int r = 0; r < R; ++r) v[it][r] = src[Row<L>::at(jj + r * NBF)];
         if constexpr (NS > 1) {
           const int k = jj % NS;
 #pragma unroll
@@ -156,9 +168,10 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
         const int b = j / NBF;
         const int jj = j - b * NBF;
         const int k = jj % NS;
-        double2* dst = buf + b * LS + (jj / NS) * NS * R + k;
+        double2* dst = buf + b * LS;
+        const int base = (jj / NS) * NS * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[r * NS] = v[it][r];
+        for (int r = 0; r < R; ++r) dst[Row<L>::at(base + r * NS)] = v[it][r];
       }
     }
     __syncthreads();

@@ -16,6 +16,9 @@
 
 namespace psx {
 
+// in-row LDS index of a kernel templated on transform length L
+#define RW(i) (::psx::Row<L>::at(i))
+
 constexpr int NT = 256;       // threads per block
 constexpr int MAXJOBS = 16;   // input fields per pencil pass
 constexpr int MAXREC = 12;    // real pencils built by the z pass
@@ -58,7 +61,7 @@ struct PencilArgs {
 // grid = (A * ceil(C/T), njobs)
 template <int L, int T>
 __global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2* __restrict__ tw) {
-  constexpr int LS = L + 1;
+  constexpr int LS = Row<L>::LS;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
   const int ct = (a.C + T - 1) / T;
@@ -81,7 +84,7 @@ __global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2
   for (int var = 0; var < 2; ++var) {
     const int fo = var == 0 ? a.job_plain[job] : a.job_deriv[job];
     if (fo < 0) continue;
-    for (int e = tid; e < T * L; e += NT) buf[(e / L) * LS + (e % L)] = make_double2(0.0, 0.0);
+    for (int e = tid; e < T * L; e += NT) buf[(e / L) * LS + RW((e % L))] = make_double2(0.0, 0.0);
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2
         if (p >= 0) {
           double2 v = r[it];
           if (var == 1) v = cscale(cmul_i<1>(v), a.kscale * signed_freq(l, a.Nspec));
-          buf[c * LS + p] = v;
+          buf[c * LS + RW(p)] = v;
         }
       }
     }
@@ -101,7 +104,7 @@ __global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2
     double2* dst = a.out + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
     for (int e = tid; e < L * T; e += NT) {
       const int p = e / T, c = e - (e / T) * T;
-      if (c0 + c < a.C) dst[(long long)p * a.out_sl + c] = cscale(buf[c * LS + p], a.scale);
+      if (c0 + c < a.C) dst[(long long)p * a.out_sl + c] = cscale(buf[c * LS + RW(p)], a.scale);
     }
     __syncthreads();
   }
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2
 // the truncation fused in.  grid = (A * ceil(C/T), njobs)
 template <int L, int T>
 __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2* __restrict__ tw) {
-  constexpr int LS = L + 1;
+  constexpr int LS = Row<L>::LS;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
   const int ct = (a.C + T - 1) / T;
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   const double2* src = a.in + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
   for (int e = tid; e < L * T; e += NT) {
     const int l = e / T, c = e - (e / T) * T;
-    buf[c * LS + l] = (c0 + c < a.C) ? src[(long long)l * a.in_sl + c] : make_double2(0.0, 0.0);
+    buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * a.in_sl + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
     const int i = e / T, c = e - (e / T) * T;
     if (c0 + c < a.C) {
       const int p = pad_slot(i, a.Nspec, L, a.keep_nyq);
-      dst[(long long)i * a.out_sl + c] = p >= 0 ? cscale(buf[c * LS + p], a.scale) : make_double2(0.0, 0.0);
+      dst[(long long)i * a.out_sl + c] = p >= 0 ? cscale(buf[c * LS + RW(p)], a.scale) : make_double2(0.0, 0.0);
     }
   }
 }
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__
   constexpr int NCI = NIN * HP;
   constexpr int NCO = NOUT * HP;
   constexpr int NCMAX = NCI > NCO ? NCI : NCO;
-  constexpr int LS = L + 1;
+  constexpr int LS = Row<L>::LS;
   constexpr int L2 = L / 2;
   __shared__ double2 buf[NCMAX * LS];
   const int tid = threadIdx.x;
@@ -260,10 +263,10 @@ __global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__
     const double2 A = zspec(a, f, pos0 + 2 * pp, k);
     const double2 B = zspec(a, f, pos0 + 2 * pp + 1, k);
     if (k == 0 || k == L2) {
-      buf[q * LS + k] = make_double2(A.x, B.x);
+      buf[q * LS + RW(k)] = make_double2(A.x, B.x);
     } else {
-      buf[q * LS + k] = make_double2(A.x - B.y, A.y + B.x);
-      buf[q * LS + L - k] = make_double2(A.x + B.y, B.x - A.y);
+      buf[q * LS + RW(k)] = make_double2(A.x - B.y, A.y + B.x);
+      buf[q * LS + RW(L - k)] = make_double2(A.x + B.y, B.x - A.y);
     }
   }
   __syncthreads();
@@ -273,9 +276,9 @@ __global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__
     double m = 0.0;
     for (int e = tid; e < HP * L; e += NT) {
       const int pp = e / L, z = e - pp * L;
-      const double2 u = buf[(0 * HP + pp) * LS + z];
-      const double2 v = buf[(1 * HP + pp) * LS + z];
-      const double2 w = buf[(2 * HP + pp) * LS + z];
+      const double2 u = buf[(0 * HP + pp) * LS + RW(z)];
+      const double2 v = buf[(1 * HP + pp) * LS + RW(z)];
+      const double2 w = buf[(2 * HP + pp) * LS + RW(z)];
       const double m0 = fabs(u.x) * a.dinv[0] + fabs(v.x) * a.dinv[1] + fabs(w.x) * a.dinv[2];
       const double m1 = fabs(u.y) * a.dinv[0] + fabs(v.y) * a.dinv[1] + fabs(w.y) * a.dinv[2];
       m = fmax(m, fmax(m0, m1));
@@ -297,7 +300,7 @@ __global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__
         double in0[NIN], in1[NIN], o0[NOUT], o1[NOUT];
 #pragma unroll
         for (int f = 0; f < NIN; ++f) {
-          const double2 v = buf[(f * HP + pp) * LS + z];
+          const double2 v = buf[(f * HP + pp) * LS + RW(z)];
           in0[f] = v.x;
           in1[f] = v.y;
         }
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__
       if ((NPTS % NT) == 0 || e < NPTS) {
         const int pp = e / L, z = e - pp * L;
 #pragma unroll
-        for (int o = 0; o < NOUT; ++o) buf[(o * HP + pp) * LS + z] = res[it][o];
+        for (int o = 0; o < NOUT; ++o) buf[(o * HP + pp) * LS + RW(z)] = res[it][o];
       }
     }
     __syncthreads();
@@ -322,8 +325,8 @@ __global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__
     for (int e = tid; e < NCO * a.Hout; e += NT) {
       const int q = e / a.Hout, k = e - q * a.Hout;
       const int o = q / HP, pp = q - o * HP;
-      const double2 z1 = buf[q * LS + k];
-      const double2 z2 = buf[q * LS + (k == 0 ? 0 : L - k)];
+      const double2 z1 = buf[q * LS + RW(k)];
+      const double2 z2 = buf[q * LS + RW((k == 0 ? 0 : L - k))];
       const double2 A = make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
       const double2 B = make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
       double2* dst = a.out + (long long)o * a.out_sf + (pos0 + 2 * pp) * a.Hout + k;
@@ -431,7 +434,7 @@ __device__ __forceinline__ void epi_accumulate(int f, const double* k, double ki
 // k3 >= H planes (no transformed data there) elementwise.
 template <int L, int NS1, int T, int EPI, bool W0>
 __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2* __restrict__ tw) {
-  constexpr int LS = L + 1;
+  constexpr int LS = Row<L>::LS;
   constexpr int NF = EpiShape<EPI>::nf;
   constexpr int IT = (NS1 * T + NT - 1) / NT;
   __shared__ double2 buf[T * LS];
@@ -507,7 +510,7 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
     const double2* src = src0 + (long long)f * a.in_sf;
     for (int e = tid; e < L * T; e += NT) {
       const int l = e / T, c = e - (e / T) * T;
-      buf[c * LS + l] = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
+      buf[c * LS + RW(l)] = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
@@ -520,7 +523,7 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
         if (p >= 0 && c0 + c < a.H) {
           const double k[3] = {a.ks[0] * signed_freq(kx, NS1), k2, a.ks[2] * (c0 + c)};
           const double kinv = 1.0 / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + 1e-50);
-          epi_accumulate<EPI>(f, k, kinv, buf[c * LS + p], acc[it], SGS ? accs[SGS ? it : 0] : nullptr);
+          epi_accumulate<EPI>(f, k, kinv, buf[c * LS + RW(p)], acc[it], SGS ? accs[SGS ? it : 0] : nullptr);
         }
       }
     }
@@ -625,7 +628,7 @@ struct PLArgs {
 
 template <int L, int T, bool DYN>
 __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restrict__ tw) {
-  constexpr int LS = L + 1;
+  constexpr int LS = Row<L>::LS;
   constexpr int IT = (L * T + NT - 1) / NT;
   static_assert(T * LS >= NT, "block_sum2 needs NT LDS entries");
   __shared__ double2 buf[T * LS];
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
     const double2* src = src0 + (long long)f * a.in_sf;
     for (int e = tid; e < L * T; e += NT) {
       const int l = e / T, c = e - (e / T) * T;
-      buf[c * LS + l] = (c0 + c < a.C) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
+      buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
@@ -662,7 +665,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       if (e < L * T && c0 + c < a.C) {
         const double k[3] = {a.ks[0] * signed_freq(kx, L), k2, a.ks[2] * (c0 + c)};
         const double kinv = 1.0 / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + 1e-50);
-        const double2 v = buf[c * LS + kx];
+        const double2 v = buf[c * LS + RW(kx)];
         const double2 mi = cmul_i<-1>(v);
         const double kk = s * k[m] * k[n] * kinv;
 #pragma unroll
@@ -747,7 +750,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       const int e = tid + it * NT;
       if (e < L * T) {
         const int kx = e / T, c = e - (e / T) * T;
-        buf[c * LS + kx] = q[it][i];
+        buf[c * LS + RW(kx)] = q[it][i];
       }
     }
     __syncthreads();
@@ -755,7 +758,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
     double2* dst = a.xq + (long long)i * a.P1 * a.P2 * a.C + (long long)ky * a.C + c0;
     for (int e = tid; e < L * T; e += NT) {
       const int x = e / T, c = e - (e / T) * T;
-      if (c0 + c < a.C) dst[(long long)x * in_sl + c] = buf[c * LS + x];
+      if (c0 + c < a.C) dst[(long long)x * in_sl + c] = buf[c * LS + RW(x)];
     }
   }
 }

@@ -12,7 +12,9 @@ namespace psx {
 // sized so a block's LDS buffer stays near 48 KiB.
 constexpr int pencil_T(int L) { return L <= 192 ? 16 : (L <= 384 ? 8 : 4); }
 constexpr int epi_T(int L) { return L <= 192 ? 8 : (L <= 384 ? 4 : 2); }
-constexpr int z_P(int L, int nmax) { return (nmax * 2 * (L + 1) * 16 <= 49152) ? 4 : 2; }
+constexpr int z_P(int L, int nmax) {
+  return (nmax * 2 * (PSX_LDS_PAD ? L + L / 8 + 1 : L + 1) * 16 <= 49152) ? 4 : 2;
+}
 
 bool length_supported(int L);
 
