@@ -82,6 +82,14 @@ int psx_compute_dt(psx_context* ctx, const double* uhat, double* dt_dev);
  * the step).  Before the first step qs must hold the initial velocity. */
 int psx_step(psx_context* ctx, double* q, double* qs, double* dt_dev, double t);
 
+/* advanceQ_RK4 followed by the driver's t += dt (src/PySpec_3d.py:124-125) with the
+ * step-start time held on the device (*t_dev, advanced by dt), as one hipGraph.
+ * The first call for a given (q, qs, dt_dev, t_dev, stream) captures the step on
+ * the context's stream, which must not be the default stream; later calls replay
+ * it.  No counterpart in the reference (whose loop is host Python); for callers
+ * that run many steps without host work in between. */
+int psx_step_graph(psx_context* ctx, double* q, double* qs, double* dt_dev, double* t_dev);
+
 /* utilitiesClass.computeEnergy / computeEnstrophy / computeEnergy_resolved
  * (src/Classes.py:652-676,733-744): out_dev[0..2] = E, Omega, E_resolved. */
 int psx_stats(psx_context* ctx, const double* uhat, double* out_dev);

@@ -23,7 +23,7 @@ MODEL_IDS = {
 # every symbol include/psx.h declares
 EXPORTS = (
     "psx_create", "psx_destroy", "psx_set_stream", "psx_set_params", "psx_last_error",
-    "psx_scratch_bytes", "psx_rhs", "psx_rk_stage", "psx_compute_dt", "psx_step", "psx_stats",
+    "psx_scratch_bytes", "psx_rhs", "psx_rk_stage", "psx_compute_dt", "psx_step", "psx_step_graph", "psx_stats",
     "psx_spectrum_bins", "psx_spectrum", "psx_tau", "psx_timing_begin", "psx_timing_end",
 )
 
@@ -73,6 +73,7 @@ def load() -> C.CDLL:
         "psx_rk_stage": (i, [vp, dp, dp, dp, dp, d, d, i]),
         "psx_compute_dt": (i, [vp, dp, dp]),
         "psx_step": (i, [vp, dp, dp, dp, d]),
+        "psx_step_graph": (i, [vp, dp, dp, dp, dp]),
         "psx_stats": (i, [vp, dp, dp]),
         "psx_spectrum_bins": (i, [vp]),
         "psx_spectrum": (i, [vp, dp, dp, i]),

@@ -123,6 +123,10 @@ template <int SIGN> struct Dft<8, SIGN> {
 #ifndef PSX_LDS_PAD
 #define PSX_LDS_PAD 1
 #endif
+// Twiddles as powers of one table entry per butterfly (1) or one load each (0).
+#ifndef PSX_TW_POW
+#define PSX_TW_POW 1
+#endif
 template <int L> struct Row {
   static constexpr int LS = PSX_LDS_PAD ? L + L / 8 + 1 : L + 1;
   static __device__ __forceinline__ int at(int i) { return PSX_LDS_PAD ? i + (i >> 3) : i; }
@@ -150,12 +154,24 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
         for (int r = 0; r < R; ++r) v[it][r] = src[Row<L>::at(jj + r * NBF)];
         if constexpr (NS > 1) {
           const int k = jj % NS;
+#if PSX_TW_POW
+          // one table load per butterfly; the other twiddles are its powers
+          // (products of depth <= 3, a few ulp)
+          double2 w[R];
+          w[1] = tw[k * TWS];
+          if constexpr (SIGN > 0) w[1].y = -w[1].y;
+#pragma unroll
+          for (int r = 2; r < R; ++r) w[r] = cmul(w[r / 2], w[r - r / 2]);
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[it][r] = cmul(v[it][r], w[r]);
+#else
 #pragma unroll
           for (int r = 1; r < R; ++r) {
             double2 w = tw[k * r * TWS];
             if constexpr (SIGN > 0) w.y = -w.y;
             v[it][r] = cmul(v[it][r], w);
           }
+#endif
         }
         Dft<R, SIGN>::run(v[it]);
       }

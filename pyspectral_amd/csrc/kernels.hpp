@@ -170,7 +170,7 @@ struct ZArgs {
   double out_scale;       // multiplies every product before the forward transform
   double c2;              // (Cs*Delta)^2 for the Smagorinsky stress
   double dinv[3];         // 1/dx, 1/dy, 1/dz (CFL reduction)
-  unsigned long long* red;  // CFL max accumulator (bit pattern of a non-negative double)
+  double* red;            // CFL: per-block maxima, reduced by cfl_finalize
 };
 
 __device__ __forceinline__ double2 zin(const ZArgs& a, int f, long long pos, int k) {
@@ -283,9 +283,16 @@ __global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__
       const double m1 = fabs(u.y) * a.dinv[0] + fabs(v.y) * a.dinv[1] + fabs(w.y) * a.dinv[2];
       m = fmax(m, fmax(m0, m1));
     }
-    // wave reduction then one atomic per wave (max is order independent)
+    // wave reduction, then the block's maximum into its own slot (no atomics)
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((tid & 63) == 0) atomicMax(a.red, (unsigned long long)__double_as_longlong(m));
+    __shared__ double wmax[NT / 64];
+    if ((tid & 63) == 0) wmax[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      double b = wmax[0];
+      for (int w = 1; w < NT / 64; ++w) b = fmax(b, wmax[w]);
+      a.red[blockIdx.x] = b;
+    }
     return;
   } else {
     // --- pointwise products / closure on pairs of positions

@@ -10,7 +10,10 @@ namespace psx {
 
 // Tile widths (contiguous k3 columns per block) and z-pass position counts,
 // sized so a block's LDS buffer stays near 48 KiB.
-constexpr int pencil_T(int L) { return L <= 192 ? 16 : (L <= 384 ? 8 : 4); }
+#ifndef PSX_PT_SMALL
+#define PSX_PT_SMALL 16
+#endif
+constexpr int pencil_T(int L) { return L <= 96 ? 16 : (L <= 192 ? PSX_PT_SMALL : (L <= 384 ? 8 : 4)); }
 constexpr int epi_T(int L) { return L <= 192 ? 8 : (L <= 384 ? 4 : 2); }
 constexpr int z_P(int L, int nmax) {
   return (nmax * 2 * (PSX_LDS_PAD ? L + L / 8 + 1 : L + 1) * 16 <= 49152) ? 4 : 2;

@@ -77,13 +77,17 @@ struct psx_context {
   double* partA = nullptr;
   double* partB = nullptr;
   int npartT = 0;
-  unsigned long long* red = nullptr;
+  double* cfl_part = nullptr;  // per-block CFL maxima (N1*N2/P entries)
   double* scal = nullptr;  // device scalars: [0] dt (internal), [1..4] tau family
   double* part = nullptr;
   int npart = 0;
   std::map<int, double2*> tw;
   long long scratch_bytes = 0;
   std::string err;
+  // captured step (psx_step_graph) and the pointers it was captured for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const void* graph_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   // optional per-kernel timing with HIP events on the context stream
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -355,7 +359,8 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
 // PL_q), round 2 (u_j (PL_q)_i on the 2x grid) and the final x-forward epilogue.
 // src/RHSfunctions.py:240-363 (t-model), 366-497 (static tau), 592-878 (dynamic).
 void tau_pass(psx_context* c, const double2* u, bool filtered, const double2* uin, const double2* q0, double2* out,
-              const double* dt, double stage_c, double2* w0, int zero_nyq_out, double tcoef) {
+              const double* dt, double stage_c, double2* w0, int zero_nyq_out, double tcoef,
+              const double* tcoef_dev = nullptr) {
   const bool dyn = c->model == PSX_MODEL_DYNAMIC_TAU;
   const int N1 = c->N1, N2 = c->N2, N3 = c->N3, H = c->H;
   const int P1 = 2 * N1, P2 = 2 * N2, P3 = 2 * N3;
@@ -494,6 +499,7 @@ void tau_pass(psx_context* c, const double2* u, bool filtered, const double2* ui
       e.out = out;
       e.w0 = w0;
       e.coef = tcoef;
+      e.coef_dev = tcoef_dev;
     }
     const double2* tw = twiddles(c, P1);
     TimeScope ts(c, K_XEPI);
@@ -502,17 +508,23 @@ void tau_pass(psx_context* c, const double2* u, bool filtered, const double2* ui
 }
 
 void stage_tau(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double stage_c,
-               double2* w0, int zero_nyq_out, double t) {
+               double2* w0, int zero_nyq_out, double t, const double* t_dev) {
   const int N1 = c->N1, N2 = c->N2, N3 = c->N3, Nc = c->Nc;
   const long long total = 3LL * N1 * N2 * Nc;
   const unsigned eb = (unsigned)std::min<long long>((total + 255) / 256, 4096);
   const double ks0 = 2.0 * kPi / c->p.L1, ks1 = 2.0 * kPi / c->p.L2, ks2 = 2.0 * kPi / c->p.L3;
-  if (c->model == PSX_MODEL_TMODEL) {
-    tau_pass(c, uin, false, uin, q0, out, dt, stage_c, w0, zero_nyq_out, c->p.Ct * t);
-    return;
-  }
-  if (c->model == PSX_MODEL_STATIC_TAU) {
-    tau_pass(c, uin, false, uin, q0, out, dt, stage_c, w0, zero_nyq_out, std::max(0.0, c->p.tau_a + c->p.tau_b * t));
+  if (c->model == PSX_MODEL_TMODEL || c->model == PSX_MODEL_STATIC_TAU) {
+    const bool tm = c->model == PSX_MODEL_TMODEL;
+    const double* coef_dev = nullptr;
+    if (t_dev) {
+      // the step-start time lives on the device (graph replay): coefficient from it
+      hipLaunchKernelGGL(time_coef, dim3(1), dim3(1), 0, c->stream, t_dev, tm ? 1 : 0, c->p.Ct, c->p.tau_a,
+                         c->p.tau_b, c->scal + 9);
+      check(c, hipGetLastError(), "time_coef");
+      coef_dev = c->scal + 9;
+    }
+    const double coef = tm ? c->p.Ct * t : std::max(0.0, c->p.tau_a + c->p.tau_b * t);
+    tau_pass(c, uin, false, uin, q0, out, dt, stage_c, w0, zero_nyq_out, coef, coef_dev);
     return;
   }
   // dynamic tau-model: filtered pass, unfiltered pass, tau, update
@@ -543,7 +555,7 @@ void stage_tau(psx_context* c, const double2* uin, const double2* q0, double2* o
 }
 
 void run_stage(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double coef,
-               double2* w0, int zero_nyq_out, double t) {
+               double2* w0, int zero_nyq_out, double t, const double* t_dev = nullptr) {
   switch (c->model) {
     case PSX_MODEL_DNS:
     case PSX_MODEL_SMAGORINSKY:
@@ -552,7 +564,7 @@ void run_stage(psx_context* c, const double2* uin, const double2* q0, double2* o
     case PSX_MODEL_TMODEL:
     case PSX_MODEL_STATIC_TAU:
     case PSX_MODEL_DYNAMIC_TAU:
-      stage_tau(c, uin, q0, out, dt, coef, w0, zero_nyq_out, t);
+      stage_tau(c, uin, q0, out, dt, coef, w0, zero_nyq_out, t, t_dev);
       break;
     default:
       fail(c, PSX_E_UNSUPPORTED, "unknown model");
@@ -566,8 +578,6 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
     return;
   }
   const int N1 = c->N1, N2 = c->N2, N3 = c->N3, Nc = c->Nc;
-  hipLaunchKernelGGL(set_u64, dim3(1), dim3(1), 0, c->stream, c->red, 0ULL);
-  check(c, hipGetLastError(), "set_u64");
   const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2};
   x_inverse(c, u, c->bufA, N1, Nc, 1, 3, jin, jp, nullptr, K_CFL_X);
   y_inverse(c, c->bufA, c->bufB, N1, N2, Nc, 1, 3, jin, jp, nullptr, K_CFL_Y);
@@ -583,9 +593,10 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   z.dinv[0] = 1.0 / c->p.dx;
   z.dinv[1] = 1.0 / c->p.dy;
   z.dinv[2] = 1.0 / c->p.dz;
-  z.red = c->red;
+  z.red = c->cfl_part;
   const int P = zpass_positions(N3, Z_DT);
   if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "CFL pass: unsupported length");
+  const int nparts = z.npos / P;
   {
     const double2* tw = twiddles(c, N3);
     TimeScope ts(c, K_CFL_Z);
@@ -596,8 +607,8 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   const double visc = c->p.nu > 0
                           ? 0.634 / (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)) / c->p.nu * c->p.cfl / 1.35
                           : 0.0;
-  hipLaunchKernelGGL(cfl_finalize, dim3(1), dim3(1), 0, c->stream, c->red, dt, 1.0 / std::sqrt(Nt), c->p.cfl,
-                     c->p.nu, visc);
+  hipLaunchKernelGGL(cfl_finalize, dim3(1), dim3(NT), 0, c->stream, c->cfl_part, nparts, dt, 1.0 / std::sqrt(Nt),
+                     c->p.cfl, c->p.nu, visc);
   check(c, hipGetLastError(), "cfl_finalize");
 }
 
@@ -681,12 +692,13 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
     }
     check(c, hipMalloc(&c->bufA, c->nA * sizeof(double2)), "hipMalloc(bufA)");
     check(c, hipMalloc(&c->bufB, c->nB * sizeof(double2)), "hipMalloc(bufB)");
-    check(c, hipMalloc(&c->red, 64), "hipMalloc(red)");
+    check(c, hipMalloc(&c->cfl_part, sizeof(double) * ((size_t)n1 * n2 / 2 + 1)), "hipMalloc(cfl_part)");
     check(c, hipMalloc(&c->scal, 16 * sizeof(double)), "hipMalloc(scal)");
     check(c, hipMemset(c->scal, 0, 16 * sizeof(double)), "hipMemset(scal)");
     c->npart = 1024;
     check(c, hipMalloc(&c->part, 3 * c->npart * sizeof(double)), "hipMalloc(part)");
-    c->scratch_bytes += (long long)(c->nA + c->nB) * sizeof(double2) + 64 + 16 * 8 + 3 * c->npart * 8;
+    c->scratch_bytes += (long long)(c->nA + c->nB) * sizeof(double2) + 8LL * (n1 * n2 / 2 + 1) + 16 * 8 +
+                        3 * c->npart * 8;
     for (int L : lens) twiddles(c, L);
   });
   if (rc != PSX_OK) {
@@ -705,7 +717,7 @@ int psx_destroy(psx_context* c) {
   if (!c) return PSX_E_ARG;
   hipFree(c->bufA);
   hipFree(c->bufB);
-  hipFree(c->red);
+  hipFree(c->cfl_part);
   hipFree(c->scal);
   hipFree(c->part);
   hipFree(c->bufC);
@@ -718,6 +730,8 @@ int psx_destroy(psx_context* c) {
   hipFree(c->partB);
   for (auto& kv : c->tw) hipFree(kv.second);
   for (auto e : c->events) hipEventDestroy(e);
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
   delete c;
   return PSX_OK;
 }
@@ -768,19 +782,63 @@ int psx_compute_dt(psx_context* c, const double* uhat, double* dt_dev) {
   });
 }
 
+namespace {
+// The body of advanceQ_RK4 (src/PySpec_3d.py:96-105).  With t_dev the step-start
+// time is read from, and advanced by dt on, the device.
+void step_body(psx_context* c, double2* Q, double2* S, double* dt_dev, double t, double* t_dev) {
+  compute_dt(c, S, dt_dev);
+  const int z = truncating(c->model) ? 1 : 0;
+  // src/PySpec_3d.py:99-104: Q = Q0 + dt*c_i*RHS(Q), c = (1/4, 1/3, 1/2, 1)
+  run_stage(c, Q, Q, S, dt_dev, 0.25, nullptr, z, t, t_dev);
+  run_stage(c, S, Q, S, dt_dev, 1.0 / 3.0, nullptr, z, t, t_dev);
+  run_stage(c, S, Q, S, dt_dev, 0.5, nullptr, z, t, t_dev);
+  run_stage(c, S, Q, Q, dt_dev, 1.0, nullptr, 0, t, t_dev);
+  if (t_dev) {
+    hipLaunchKernelGGL(advance_time, dim3(1), dim3(1), 0, c->stream, t_dev, dt_dev);
+    check(c, hipGetLastError(), "advance_time");
+  }
+}
+}  // namespace
+
 int psx_step(psx_context* c, double* q, double* qs, double* dt_dev, double t) {
   return guarded(c, [&] {
     if (!q || !qs || !dt_dev) fail(c, PSX_E_ARG, "null pointer");
     if (q == qs) fail(c, PSX_E_ARG, "q and qs must be distinct buffers");
-    double2* Q = (double2*)q;
-    double2* S = (double2*)qs;
-    compute_dt(c, S, dt_dev);
-    const int z = truncating(c->model) ? 1 : 0;
-    // src/PySpec_3d.py:99-104: Q = Q0 + dt*c_i*RHS(Q), c = (1/4, 1/3, 1/2, 1)
-    run_stage(c, Q, Q, S, dt_dev, 0.25, nullptr, z, t);
-    run_stage(c, S, Q, S, dt_dev, 1.0 / 3.0, nullptr, z, t);
-    run_stage(c, S, Q, S, dt_dev, 0.5, nullptr, z, t);
-    run_stage(c, S, Q, Q, dt_dev, 1.0, nullptr, 0, t);
+    step_body(c, (double2*)q, (double2*)qs, dt_dev, t, nullptr);
+  });
+}
+
+int psx_step_graph(psx_context* c, double* q, double* qs, double* dt_dev, double* t_dev) {
+  return guarded(c, [&] {
+    if (!q || !qs || !dt_dev || !t_dev) fail(c, PSX_E_ARG, "null pointer");
+    if (q == qs) fail(c, PSX_E_ARG, "q and qs must be distinct buffers");
+    if (c->stream == nullptr) fail(c, PSX_E_ARG, "graph capture needs a non-default stream (psx_set_stream)");
+    const void* key[5] = {q, qs, dt_dev, t_dev, c->stream};
+    bool same = c->graph_exec != nullptr;
+    for (int i = 0; i < 5 && same; ++i) same = c->graph_key[i] == key[i];
+    if (!same) {
+      if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+      if (c->graph) hipGraphDestroy(c->graph);
+      c->graph_exec = nullptr;
+      c->graph = nullptr;
+      const bool timing = c->timing;
+      c->timing = false;
+      check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        step_body(c, (double2*)q, (double2*)qs, dt_dev, 0.0, t_dev);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(c->stream, &g);
+        if (g) hipGraphDestroy(g);
+        c->timing = timing;
+        throw;
+      }
+      c->timing = timing;
+      check(c, hipStreamEndCapture(c->stream, &c->graph), "hipStreamEndCapture");
+      check(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      for (int i = 0; i < 5; ++i) c->graph_key[i] = key[i];
+    }
+    check(c, hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
   });
 }
 

@@ -5,12 +5,21 @@
 namespace psx {
 
 // ------------------------------------------------------------------ small kernels
-// dt from the CFL maximum (src/Classes.py:687-698).  red holds max over the grid
-// of |u|/dx+|v|/dy+|w|/dz in raw (unnormalised inverse) units.
-__global__ void cfl_finalize(const unsigned long long* red, double* dt, double inv_sqrt_nt, double cfl,
-                             double nu, double visc_lim) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const double mv = __longlong_as_double((long long)*red) * inv_sqrt_nt;
+// dt from the CFL maximum (src/Classes.py:687-698).  part holds per-block maxima of
+// |u|/dx+|v|/dy+|w|/dz in raw (unnormalised inverse) units; one block of NT threads.
+__global__ __launch_bounds__(NT) void cfl_finalize(const double* __restrict__ part, int nparts, double* dt,
+                                                   double inv_sqrt_nt, double cfl, double nu, double visc_lim) {
+  __shared__ double sh[NT];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += NT) m = fmax(m, part[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double mv = sh[0] * inv_sqrt_nt;
     double d = cfl / (mv + 1e-10);
     if (nu > 0) d = fmin(d, visc_lim);
     *dt = d;
@@ -225,6 +234,17 @@ __global__ void tau_update(const double2* __restrict__ pl, const double2* __rest
       out[i * fs + e] = o;
     }
   }
+}
+
+// Closure coefficient from the device step-start time: Ct*t (t-model,
+// src/RHSfunctions.py:342) or max(0, tau_a + tau_b t) (static tau, :480-485).
+__global__ void time_coef(const double* t, int tmodel, double Ct, double ta, double tb, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = tmodel ? Ct * (*t) : fmax(0.0, ta + tb * (*t));
+}
+
+// t += dt on the device (src/PySpec_3d.py:125).
+__global__ void advance_time(double* t, const double* dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *t += *dt;
 }
 
 }  // namespace psx

@@ -144,6 +144,15 @@ class SpectralSolver:
         _lib.check(self.ctx, self.lib.psx_step(self.ctx, _ptr(q), _ptr(qs), _ptr(dt), float(t)), "psx_step")
         return dt
 
+    def step_graph(self, q: torch.Tensor, qs: torch.Tensor, dt: torch.Tensor, t: torch.Tensor) -> None:
+        """One step replayed from a captured hipGraph; ``t`` (device float64, the step-start
+        time) is advanced by dt on the device.  Needs a non-default current stream."""
+        self._state_arg(q, "q")
+        self._state_arg(qs, "qs")
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_step_graph(self.ctx, _ptr(q), _ptr(qs), _ptr(dt), _ptr(t)),
+                   "psx_step_graph")
+
     def stats(self, uhat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """(E, Omega, E_resolved) on the device."""
         self._state_arg(uhat, "uhat")
