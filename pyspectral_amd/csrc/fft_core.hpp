@@ -127,14 +127,14 @@ template <int SIGN> struct Dft<8, SIGN> {
 #ifndef PSX_TW_POW
 #define PSX_TW_POW 1
 #endif
-template <int L> struct Row {
-  static constexpr int LS = PSX_LDS_PAD ? L + L / 8 + 1 : L + 1;
-  static __device__ __forceinline__ int at(int i) { return PSX_LDS_PAD ? i + (i >> 3) : i; }
+template <int L, bool PAD = (PSX_LDS_PAD != 0)> struct Row {
+  static constexpr int LS = PAD ? L + L / 8 + 1 : L + 1;
+  static __device__ __forceinline__ int at(int i) { return PAD ? i + (i >> 3) : i; }
 };
 
 // ---------------------------------------------------------------- Stockham batch
 // B transforms of length L, transform b at buf[b*LS .. b*LS+L), NT threads.
-template <int L, int B, int NT, int LS, int SIGN, int S, int NS>
+template <int L, int B, int NT, int LS, int SIGN, int S, int NS, bool PAD>
 __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restrict__ tw, int tid) {
   if constexpr (S < Plan<L>::n) {
     constexpr int R = Plan<L>::r[S];
@@ -151,7 +151,7 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
         const int jj = j - b * NBF;
         const double2* src = buf + b * LS;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[it][r] = src[Row<L>::at(jj + r * NBF)];
+        for (int r = 0; r < R; ++r) v[it][r] = src[Row<L, PAD>::at(jj + r * NBF)];
         if constexpr (NS > 1) {
           const int k = jj % NS;
 #if PSX_TW_POW
@@ -187,19 +187,19 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
         double2* dst = buf + b * LS;
         const int base = (jj / NS) * NS * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[Row<L>::at(base + r * NS)] = v[it][r];
+        for (int r = 0; r < R; ++r) dst[Row<L, PAD>::at(base + r * NS)] = v[it][r];
       }
     }
     __syncthreads();
-    fft_stages<L, B, NT, LS, SIGN, S + 1, NS * R>(buf, tw, tid);
+    fft_stages<L, B, NT, LS, SIGN, S + 1, NS * R, PAD>(buf, tw, tid);
   }
 }
 
 // Run the batch.  Caller must __syncthreads() before (data staged) — the routine
 // ends with a barrier, so results are visible on return.
-template <int L, int B, int NT, int LS, int SIGN>
+template <int L, int B, int NT, int LS, int SIGN, bool PAD = (PSX_LDS_PAD != 0)>
 __device__ __forceinline__ void fft_batch(double2* buf, const double2* __restrict__ tw, int tid) {
-  fft_stages<L, B, NT, LS, SIGN, 0, 1>(buf, tw, tid);
+  fft_stages<L, B, NT, LS, SIGN, 0, 1, PAD>(buf, tw, tid);
 }
 
 }  // namespace psx

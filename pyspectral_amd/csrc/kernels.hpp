@@ -20,8 +20,32 @@ namespace psx {
 #define RW(i) (::psx::Row<L>::at(i))
 
 constexpr int NT = 256;       // threads per block
+
+// Twiddle tables read from global memory (0) or staged in LDS per block (1).
+#ifndef PSX_TW_LDS
+#define PSX_TW_LDS 0
+#endif
+#if PSX_TW_LDS
+#define PSX_STAGE_TW(NTH)                                              \
+  __shared__ double2 tw_lds[L];                                        \
+  for (int i_ = threadIdx.x; i_ < L; i_ += (NTH)) tw_lds[i_] = tw[i_]; \
+  const double2* twp = tw_lds;
+#else
+#define PSX_STAGE_TW(NTH) const double2* twp = tw;
+#endif
+#ifndef PSX_ZPAD
+#define PSX_ZPAD 0
+#endif
 constexpr int MAXJOBS = 16;   // input fields per pencil pass
 constexpr int MAXREC = 12;    // real pencils built by the z pass
+
+// ------------------------------------------------------------------ small selects
+// Component index pairs (11, 22, 33, 12, 13, 23) of the symmetric tensors, and a
+// select of one of three values, written as selects so no per-thread array is
+// indexed at run time (which would place it in scratch memory).
+__device__ __forceinline__ int sym_m(int g) { return g == 1 ? 1 : (g == 2 ? 2 : (g == 5 ? 1 : 0)); }
+__device__ __forceinline__ int sym_n(int g) { return g == 0 ? 0 : (g == 1 ? 1 : (g == 3 ? 1 : 2)); }
+__device__ __forceinline__ double sel3(int i, double a, double b, double c) { return i == 0 ? a : (i == 1 ? b : c); }
 
 // ------------------------------------------------------------------ index maps
 // Signed frequency of FFT-ordered spectral index i on an n-point axis; index n/2
@@ -59,11 +83,18 @@ struct PencilArgs {
 // Inverse along a strided axis: Nspec spectral entries -> L physical points, with
 // the zero-pad fused in; optionally a second output multiplied by i*k_axis.
 // grid = (A * ceil(C/T), njobs)
+#ifndef PSX_PINV_MINW
+#define PSX_PINV_MINW 1
+#endif
+#ifndef PSX_EPI_MINW
+#define PSX_EPI_MINW 1
+#endif
 template <int L, int T>
-__global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a, const double2* __restrict__ tw) {
   constexpr int LS = Row<L>::LS;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
+  PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
   const int aidx = blockIdx.x / ct;
   const int c0 = (blockIdx.x - aidx * ct) * T;
@@ -100,7 +131,7 @@ __global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2
       }
     }
     __syncthreads();
-    fft_batch<L, T, NT, LS, +1>(buf, tw, tid);
+    fft_batch<L, T, NT, LS, +1>(buf, twp, tid);
     double2* dst = a.out + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
     for (int e = tid; e < L * T; e += NT) {
       const int p = e / T, c = e - (e / T) * T;
@@ -117,6 +148,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   constexpr int LS = Row<L>::LS;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
+  PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
   const int aidx = blockIdx.x / ct;
   const int c0 = (blockIdx.x - aidx * ct) * T;
@@ -127,7 +159,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
     buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * a.in_sl + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
+  fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
   const int fo = a.job_plain[job];
   double2* dst = a.out + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
   for (int e = tid; e < a.Nspec * T; e += NT) {
@@ -238,109 +270,121 @@ __device__ __forceinline__ void zpoint(const ZArgs& a, const double* in, double*
   }
 }
 
-// One block = P consecutive (x, y) positions (P even).  Real pencils of field f at
-// positions 2pp and 2pp+1 share complex pencil f*(P/2)+pp: a single complex FFT
-// transforms both (inverse: Z = A + iB with the Hermitian halves mirrored;
-// forward: A = (Z_k + conj Z_{L-k})/2, B = (Z_k - conj Z_{L-k})/(2i)).
-template <int L, int P, int MODE>
-__global__ __launch_bounds__(NT) void zpass(ZArgs a, const double2* __restrict__ tw) {
+// One block = P consecutive (x, y) positions.  Real pencil r = f*P + p (field f,
+// position p) lives in half (r & 1) of complex pencil r >> 1, so one complex FFT
+// transforms two real pencils -- two positions of a field when P is even, two
+// fields of the one position when P = 1.  Inverse: Z = A + iB with the Hermitian
+// halves mirrored (Re only at k = 0 and L/2, as c2r does); forward:
+// A = (Z_k + conj Z_{L-k})/2, B = (Z_k - conj Z_{L-k})/(2i).  NTZ threads; LDS rows
+// unpadded (PAD = false measured faster for this kernel).
+template <int L, int P, int MODE, int NTZ>
+__global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict__ tw) {
+  constexpr bool PADZ = PSX_ZPAD != 0;
   constexpr int NIN = ZShape<MODE>::nin;
   constexpr int NOUT = ZShape<MODE>::nout;
-  constexpr int HP = P / 2;
-  constexpr int NCI = NIN * HP;
-  constexpr int NCO = NOUT * HP;
+  constexpr int NRI = NIN * P, NRO = NOUT * P;  // real pencils in / out
+  constexpr int NCI = (NRI + 1) / 2;
+  constexpr int NCO = (NRO + 1) / 2;
   constexpr int NCMAX = NCI > NCO ? NCI : NCO;
-  constexpr int LS = Row<L>::LS;
+  constexpr int LS = Row<L, PADZ>::LS;
   constexpr int L2 = L / 2;
   __shared__ double2 buf[NCMAX * LS];
+  double* bufd = reinterpret_cast<double*>(buf);
   const int tid = threadIdx.x;
+  PSX_STAGE_TW(NTZ)
   const long long pos0 = (long long)blockIdx.x * P;
+#define ZRW(i) (::psx::Row<L, PADZ>::at(i))
 
-  // --- build the inverse spectra (Re-only at k = 0 and k = L/2, as c2r does)
-  for (int e = tid; e < NCI * (L2 + 1); e += NT) {
+  // --- build the inverse spectra
+  for (int e = tid; e < NCI * (L2 + 1); e += NTZ) {
     const int q = e / (L2 + 1), k = e - q * (L2 + 1);
-    const int f = q / HP, pp = q - f * HP;
-    const double2 A = zspec(a, f, pos0 + 2 * pp, k);
-    const double2 B = zspec(a, f, pos0 + 2 * pp + 1, k);
+    const int r0 = 2 * q, r1 = 2 * q + 1;
+    const double2 A = zspec(a, r0 / P, pos0 + r0 % P, k);
+    const double2 B = r1 < NRI ? zspec(a, r1 / P, pos0 + r1 % P, k) : make_double2(0.0, 0.0);
     if (k == 0 || k == L2) {
-      buf[q * LS + RW(k)] = make_double2(A.x, B.x);
+      buf[q * LS + ZRW(k)] = make_double2(A.x, B.x);
     } else {
-      buf[q * LS + RW(k)] = make_double2(A.x - B.y, A.y + B.x);
-      buf[q * LS + RW(L - k)] = make_double2(A.x + B.y, B.x - A.y);
+      buf[q * LS + ZRW(k)] = make_double2(A.x - B.y, A.y + B.x);
+      buf[q * LS + ZRW(L - k)] = make_double2(A.x + B.y, B.x - A.y);
     }
   }
   __syncthreads();
-  fft_batch<L, NCI, NT, LS, +1>(buf, tw, tid);
+  fft_batch<L, NCI, NTZ, LS, +1, PADZ>(buf, twp, tid);
 
   if constexpr (MODE == Z_DT) {
     double m = 0.0;
-    for (int e = tid; e < HP * L; e += NT) {
-      const int pp = e / L, z = e - pp * L;
-      const double2 u = buf[(0 * HP + pp) * LS + RW(z)];
-      const double2 v = buf[(1 * HP + pp) * LS + RW(z)];
-      const double2 w = buf[(2 * HP + pp) * LS + RW(z)];
-      const double m0 = fabs(u.x) * a.dinv[0] + fabs(v.x) * a.dinv[1] + fabs(w.x) * a.dinv[2];
-      const double m1 = fabs(u.y) * a.dinv[0] + fabs(v.y) * a.dinv[1] + fabs(w.y) * a.dinv[2];
-      m = fmax(m, fmax(m0, m1));
+    for (int e = tid; e < P * L; e += NTZ) {
+      const int p = e / L, z = e - p * L;
+      double v[3];
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        const int r = f * P + p;
+        v[f] = bufd[2 * ((r >> 1) * LS + ZRW(z)) + (r & 1)];
+      }
+      m = fmax(m, fabs(v[0]) * a.dinv[0] + fabs(v[1]) * a.dinv[1] + fabs(v[2]) * a.dinv[2]);
     }
     // wave reduction, then the block's maximum into its own slot (no atomics)
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    __shared__ double wmax[NT / 64];
+    __shared__ double wmax[NTZ / 64];
     if ((tid & 63) == 0) wmax[tid >> 6] = m;
     __syncthreads();
     if (tid == 0) {
       double b = wmax[0];
-      for (int w = 1; w < NT / 64; ++w) b = fmax(b, wmax[w]);
+      for (int w = 1; w < NTZ / 64; ++w) b = fmax(b, wmax[w]);
       a.red[blockIdx.x] = b;
     }
     return;
   } else {
-    // --- pointwise products / closure on pairs of positions
-    constexpr int NPTS = HP * L;
-    constexpr int IT = (NPTS + NT - 1) / NT;
-    double2 res[IT][NOUT];
+    // --- pointwise products / closure
+    constexpr int NPTS = P * L;
+    constexpr int IT = (NPTS + NTZ - 1) / NTZ;
+    double res[IT][NOUT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * NT;
-      if ((NPTS % NT) == 0 || e < NPTS) {
-        const int pp = e / L, z = e - pp * L;
-        double in0[NIN], in1[NIN], o0[NOUT], o1[NOUT];
+      const int e = tid + it * NTZ;
+      if ((NPTS % NTZ) == 0 || e < NPTS) {
+        const int p = e / L, z = e - p * L;
+        double in[NIN];
 #pragma unroll
         for (int f = 0; f < NIN; ++f) {
-          const double2 v = buf[(f * HP + pp) * LS + RW(z)];
-          in0[f] = v.x;
-          in1[f] = v.y;
+          const int r = f * P + p;
+          in[f] = bufd[2 * ((r >> 1) * LS + ZRW(z)) + (r & 1)];
         }
-        zpoint<MODE>(a, in0, o0);
-        zpoint<MODE>(a, in1, o1);
-#pragma unroll
-        for (int o = 0; o < NOUT; ++o) res[it][o] = make_double2(o0[o], o1[o]);
+        zpoint<MODE>(a, in, res[it]);
       }
     }
     __syncthreads();
+    if constexpr ((NRO & 1) != 0) {
+      // odd number of real output pencils: the unused half of the last one is zero
+      for (int z = tid; z < L; z += NTZ) bufd[2 * ((NCO - 1) * LS + ZRW(z)) + 1] = 0.0;
+    }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * NT;
-      if ((NPTS % NT) == 0 || e < NPTS) {
-        const int pp = e / L, z = e - pp * L;
+      const int e = tid + it * NTZ;
+      if ((NPTS % NTZ) == 0 || e < NPTS) {
+        const int p = e / L, z = e - p * L;
 #pragma unroll
-        for (int o = 0; o < NOUT; ++o) buf[(o * HP + pp) * LS + RW(z)] = res[it][o];
+        for (int o = 0; o < NOUT; ++o) {
+          const int r = o * P + p;
+          bufd[2 * ((r >> 1) * LS + ZRW(z)) + (r & 1)] = res[it][o];
+        }
       }
     }
     __syncthreads();
-    fft_batch<L, NCO, NT, LS, -1>(buf, tw, tid);
-    for (int e = tid; e < NCO * a.Hout; e += NT) {
+    fft_batch<L, NCO, NTZ, LS, -1, PADZ>(buf, twp, tid);
+    for (int e = tid; e < NCO * a.Hout; e += NTZ) {
       const int q = e / a.Hout, k = e - q * a.Hout;
-      const int o = q / HP, pp = q - o * HP;
-      const double2 z1 = buf[q * LS + RW(k)];
-      const double2 z2 = buf[q * LS + RW((k == 0 ? 0 : L - k))];
-      const double2 A = make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
-      const double2 B = make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
-      double2* dst = a.out + (long long)o * a.out_sf + (pos0 + 2 * pp) * a.Hout + k;
-      dst[0] = A;
-      dst[a.Hout] = B;
+      const double2 z1 = buf[q * LS + ZRW(k)];
+      const double2 z2 = buf[q * LS + ZRW((k == 0 ? 0 : L - k))];
+      const int r0 = 2 * q, r1 = 2 * q + 1;
+      a.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k] =
+          make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
+      if (r1 < NRO)
+        a.out[(long long)(r1 / P) * a.out_sf + (pos0 + r1 % P) * a.Hout + k] =
+            make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
     }
   }
+#undef ZRW
 }
 
 // Sum a complex value over the block (deterministic tree order); uses >= NT
@@ -398,18 +442,17 @@ __device__ __forceinline__ void epi_accumulate(int f, const double* k, double ki
                                                double2* acc_sgs) {
   if constexpr (EPI == E_DNS || EPI == E_SMAG) {
     // fields 0..5: N_mn = u_m u_n, 6..11: tau_mn; (mn) in 11,22,33,12,13,23
-    constexpr int MI[6] = {0, 1, 2, 0, 0, 1};
-    constexpr int NI[6] = {0, 1, 2, 1, 2, 2};
     const int g = f % 6;
-    const int m = MI[g], n = NI[g];
+    const int m = sym_m(g), n = sym_n(g);
+    const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
     const double s = (m == n) ? 1.0 : 2.0;
-    const double kk = s * k[m] * k[n] * kinv;
+    const double kk = s * km * kn * kinv;
     const double2 mi = cmul_i<-1>(v);  // -i v
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       double2 d = make_double2(0.0, 0.0);
-      if (i == m) d = cadd(d, cscale(mi, k[n]));
-      if (i == n && m != n) d = cadd(d, cscale(mi, k[m]));
+      if (i == m) d = cadd(d, cscale(mi, kn));
+      if (i == n && m != n) d = cadd(d, cscale(mi, km));
       if (f < 6) {
         // + i k_i (k_m k_n N_mn)/k^2: the Leray projection (src/RHSfunctions.py:29-45)
         d = cadd(d, cscale(cmul_i<1>(v), k[i] * kk));
@@ -424,13 +467,14 @@ __device__ __forceinline__ void epi_accumulate(int f, const double* k, double ki
     // field 3 i + j holds u_j (PL_q)_i; PLQL_m = sum_j -i k_j (A_ji + A_ij) + i k_m pterm,
     // pterm = 2/k^2 sum_mn k_m k_n A_mn (src/RHSfunctions.py:338-352)
     const int i = f / 3, j = f - 3 * (f / 3);
+    const double ki = sel3(i, k[0], k[1], k[2]), kj = sel3(j, k[0], k[1], k[2]);
     const double2 mi = cmul_i<-1>(v);
-    const double pk = 2.0 * k[i] * k[j] * kinv;
+    const double pk = 2.0 * ki * kj * kinv;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       double2 d = cscale(cmul_i<1>(v), k[m] * pk);
-      if (m == i) d = cadd(d, cscale(mi, k[j]));
-      if (m == j) d = cadd(d, cscale(mi, k[i]));
+      if (m == i) d = cadd(d, cscale(mi, kj));
+      if (m == j) d = cadd(d, cscale(mi, ki));
       acc[m] = cadd(acc[m], d);
     }
   }
@@ -440,12 +484,13 @@ __device__ __forceinline__ void epi_accumulate(int f, const double* k, double ki
 // accumulating the spectral RHS in registers; blocks past N2*ceil(H/T) handle the
 // k3 >= H planes (no transformed data there) elementwise.
 template <int L, int NS1, int T, int EPI, bool W0>
-__global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a, const double2* __restrict__ tw) {
   constexpr int LS = Row<L>::LS;
   constexpr int NF = EpiShape<EPI>::nf;
   constexpr int IT = (NS1 * T + NT - 1) / NT;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
+  PSX_STAGE_TW(NT)
   const int ct = (a.H + T - 1) / T;
   const int nmain = a.N2 * ct;
   const long long plane = (long long)a.N2 * a.Nc;
@@ -520,7 +565,7 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
       buf[c * LS + RW(l)] = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
+    fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = tid + it * NT;
@@ -640,6 +685,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   static_assert(T * LS >= NT, "block_sum2 needs NT LDS entries");
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
+  PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
   const int ky = blockIdx.x / ct;
   const int c0 = (blockIdx.x - ky * ct) * T;
@@ -647,13 +693,17 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   const double2* src0 = a.in + (long long)ky * a.C + c0;
   const int s2 = signed_freq(ky, a.P2);
   const double k2 = a.ks[1] * s2;
-  double2 D[IT][3], Pj[IT][3];
+  // acc: divergence part D (+ projection part when !DYN); Pj: projection part (DYN)
+  double2 acc[IT][3];
+  double2 Pj[DYN ? IT : 1][3];
 #pragma unroll
   for (int it = 0; it < IT; ++it)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) D[it][i] = Pj[it][i] = make_double2(0.0, 0.0);
-  constexpr int MI[6] = {0, 1, 2, 0, 0, 1};
-  constexpr int NI[6] = {0, 1, 2, 1, 2, 2};
+    for (int i = 0; i < 3; ++i) acc[it][i] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int it = 0; it < (DYN ? IT : 1); ++it)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Pj[it][i] = make_double2(0.0, 0.0);
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
     const double2* src = src0 + (long long)f * a.in_sf;
@@ -662,8 +712,8 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
-    const int m = MI[f], n = NI[f];
+    fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
+    const int m = sym_m(f), n = sym_n(f);
     const double s = (m == n) ? 1.0 : 2.0;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -672,32 +722,35 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       if (e < L * T && c0 + c < a.C) {
         const double k[3] = {a.ks[0] * signed_freq(kx, L), k2, a.ks[2] * (c0 + c)};
         const double kinv = 1.0 / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + 1e-50);
+        const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
         const double2 v = buf[c * LS + RW(kx)];
         const double2 mi = cmul_i<-1>(v);
-        const double kk = s * k[m] * k[n] * kinv;
+        const double kk = s * km * kn * kinv;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          if (i == m) D[it][i] = cadd(D[it][i], cscale(mi, k[n]));
-          if (i == n && m != n) D[it][i] = cadd(D[it][i], cscale(mi, k[m]));
-          Pj[it][i] = cadd(Pj[it][i], cscale(cmul_i<1>(v), k[i] * kk));
+          if (i == m) acc[it][i] = cadd(acc[it][i], cscale(mi, kn));
+          if (i == n && m != n) acc[it][i] = cadd(acc[it][i], cscale(mi, km));
+          const double2 pv = cscale(cmul_i<1>(v), k[i] * kk);
+          if constexpr (DYN) {
+            Pj[it][i] = cadd(Pj[it][i], pv);
+          } else {
+            acc[it][i] = cadd(acc[it][i], pv);
+          }
         }
       }
     }
     __syncthreads();
   }
 
-  // PL = D + P - nu k^2 u; base-grid copy; Leonard reduction; q part into LDS rows
+  // PL = D + P - nu k^2 u (left in acc); base-grid copy; Leonard reduction
   const long long plane = (long long)a.N2 * a.Nc;
   const long long fs = (long long)a.N1 * plane;
   double2 red = make_double2(0.0, 0.0);
-  double2 q[IT][3];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int e = tid + it * NT;
     const int kx = e / T, c = e - (e / T) * T;
     const int kz = c0 + c;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) q[it][i] = make_double2(0.0, 0.0);
     if (e < L * T && kz < a.C) {
       const int s1 = signed_freq(kx, L);
       const double k[3] = {a.ks[0] * s1, k2, a.ks[2] * kz};
@@ -707,13 +760,6 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
                                         (long long)((s2 + a.N2) % a.N2) * a.Nc + kz)
                                      : 0;
       const bool has_u = in_base && s1 > -a.N1 / 2 && s2 > -a.N2 / 2;
-      double2 pl[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        pl[i] = cadd(D[it][i], Pj[it][i]);
-        if (has_u) pl[i] = csub(pl[i], cscale(a.u[i * fs + bidx], a.nu * (ksq + 1e-50)));
-        if (in_base && a.pln) a.pln[i * fs + bidx] = pl[i];
-      }
       if constexpr (DYN) {
         const bool in_f = in_base && fabs(k[0]) < a.kf && fabs(k[1]) < a.kf && fabs(k[2]) < a.kf;
         if (in_f) {
@@ -721,7 +767,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
 #pragma unroll
           for (int i = 0; i < 3; ++i) {
             // G = -i k_j N_ij + i k_i p = D - P
-            const double2 G = csub(D[it][i], Pj[it][i]);
+            const double2 G = csub(acc[it][i], Pj[it][i]);
             if (a.g_out) a.g_out[i * fs + bidx] = G;
             if (a.g_ref) {
               const double2 d = csub(G, a.g_ref[i * fs + bidx]);
@@ -732,10 +778,13 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
           }
         }
       }
-      const bool inside = (s1 > -a.kpx && s1 < a.kpx && s2 > -a.kpy && s2 < a.kpy && kz < a.kpz);
-      if (!inside) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) q[it][i] = pl[i];
+      for (int i = 0; i < 3; ++i) {
+        double2 pl = acc[it][i];
+        if constexpr (DYN) pl = cadd(pl, Pj[it][i]);
+        if (has_u) pl = csub(pl, cscale(a.u[i * fs + bidx], a.nu * (ksq + 1e-50)));
+        if (in_base && a.pln) a.pln[i * fs + bidx] = pl;
+        acc[it][i] = pl;
       }
     }
   }
@@ -748,8 +797,9 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       }
     }
   }
-  // x-inverse of PL_q, one component at a time
-#pragma unroll 1
+  // x-inverse of PL_q (PL outside the resolved cube), one component at a time
+  // (unrolled: acc is indexed by the component)
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     __syncthreads();
 #pragma unroll
@@ -757,11 +807,14 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       const int e = tid + it * NT;
       if (e < L * T) {
         const int kx = e / T, c = e - (e / T) * T;
-        buf[c * LS + RW(kx)] = q[it][i];
+        const int s1 = signed_freq(kx, L);
+        const bool inside = (s1 > -a.kpx && s1 < a.kpx && s2 > -a.kpy && s2 < a.kpy && c0 + c < a.kpz);
+        const bool valid = c0 + c < a.C;
+        buf[c * LS + RW(kx)] = (valid && !inside) ? acc[it][i] : make_double2(0.0, 0.0);
       }
     }
     __syncthreads();
-    fft_batch<L, T, NT, LS, +1>(buf, tw, tid);
+    fft_batch<L, T, NT, LS, +1>(buf, twp, tid);
     double2* dst = a.xq + (long long)i * a.P1 * a.P2 * a.C + (long long)ky * a.C + c0;
     for (int e = tid; e < L * T; e += NT) {
       const int x = e / T, c = e - (e / T) * T;

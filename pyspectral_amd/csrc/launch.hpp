@@ -11,12 +11,25 @@ namespace psx {
 // Tile widths (contiguous k3 columns per block) and z-pass position counts,
 // sized so a block's LDS buffer stays near 48 KiB.
 #ifndef PSX_PT_SMALL
-#define PSX_PT_SMALL 16
+#define PSX_PT_SMALL 8
 #endif
 constexpr int pencil_T(int L) { return L <= 96 ? 16 : (L <= 192 ? PSX_PT_SMALL : (L <= 384 ? 8 : 4)); }
-constexpr int epi_T(int L) { return L <= 192 ? 8 : (L <= 384 ? 4 : 2); }
-constexpr int z_P(int L, int nmax) {
-  return (nmax * 2 * (PSX_LDS_PAD ? L + L / 8 + 1 : L + 1) * 16 <= 49152) ? 4 : 2;
+#ifndef PSX_ET_SMALL
+#define PSX_ET_SMALL 8
+#endif
+constexpr int epi_T(int L) { return L <= 64 ? 8 : (L <= 192 ? PSX_ET_SMALL : (L <= 384 ? 4 : 2)); }
+// z pass: positions per block (pairing two positions, or two fields when P = 1)
+// and threads per block, sized to the first (radix-8 or -4) stage's butterflies.
+template <int MODE> constexpr int z_P(int L) {
+  return MODE == Z_SMAG ? 1 : (MODE == Z_TM2 ? 1 : 2);
+}
+template <int MODE> constexpr int z_NT(int L) {
+  constexpr int nin = ZShape<MODE>::nin, nout = ZShape<MODE>::nout;
+  const int P = z_P<MODE>(L);
+  const int nc = ((nin > nout ? nin : nout) * P + 1) / 2;
+  const int bf = nc * (L / 8);
+  const int nt = ((bf + 63) / 64) * 64;
+  return nt < 64 ? 64 : (nt > 256 ? 256 : nt);
 }
 
 bool length_supported(int L);
