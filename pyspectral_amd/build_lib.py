@@ -66,5 +66,25 @@ def build(jobs: int | None = None, verbose: bool = True, out: str = OUT, defines
     return out
 
 
+HOSTTEST_OUT = os.path.join(HERE, "libpsx_hosttest.so")
+
+
+def build_hosttest(verbose: bool = True) -> str:
+    """Host-only library running the FFT butterflies and plans on the CPU
+    (``csrc/host_fft_test.cpp``), used by the CPU test suite."""
+    hipcc = _hipcc()
+    src = os.path.join(CSRC, "host_fft_test.cpp")
+    tmp = HOSTTEST_OUT + ".tmp"
+    r = subprocess.run([hipcc, "-x", "hip", f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-shared",
+                        src, "-o", tmp], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host test library failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, HOSTTEST_OUT)
+    if verbose:
+        print(f"built {HOSTTEST_OUT}")
+    return HOSTTEST_OUT
+
+
 if __name__ == "__main__":
     build(jobs=int(sys.argv[1]) if len(sys.argv) > 1 else None)
+    build_hosttest()

@@ -17,52 +17,78 @@
 
 namespace psx {
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+__host__ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // multiply by +i (S = +1) or -i (S = -1)
 template <int S>
-__device__ __forceinline__ double2 cmul_i(double2 a) {
+__host__ __device__ __forceinline__ double2 cmul_i(double2 a) {
   return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
 
 // ---------------------------------------------------------------- radix plans
-template <int L> struct Plan;
-#define PSX_PLAN(LEN, ...)                                        \
-  template <> struct Plan<LEN> {                                  \
+// Two plan families per length: V = 0 uses radices <= 8, V = 1 prefers radix 16
+// and 12 (fewer LDS passes, more work per butterfly).  Plan<L> is the family
+// selected at build time by PSX_FFT_PLAN.
+template <int L, int V> struct PlanV;
+#define PSX_PLAN(V, LEN, ...)                                     \
+  template <> struct PlanV<LEN, V> {                              \
     static constexpr int r[] = {__VA_ARGS__};                     \
     static constexpr int n = sizeof(r) / sizeof(int);             \
   };
-PSX_PLAN(8, 8)
-PSX_PLAN(12, 4, 3)
-PSX_PLAN(16, 4, 4)
-PSX_PLAN(24, 8, 3)
-PSX_PLAN(32, 8, 4)
-PSX_PLAN(36, 4, 3, 3)
-PSX_PLAN(48, 4, 4, 3)
-PSX_PLAN(64, 8, 8)
-PSX_PLAN(72, 8, 3, 3)
-PSX_PLAN(96, 8, 4, 3)
-PSX_PLAN(128, 8, 4, 4)
-PSX_PLAN(144, 4, 4, 3, 3)
-PSX_PLAN(192, 8, 8, 3)
-PSX_PLAN(256, 8, 8, 4)
-PSX_PLAN(288, 8, 4, 3, 3)
-PSX_PLAN(384, 8, 4, 4, 3)
-PSX_PLAN(512, 8, 8, 8)
-PSX_PLAN(576, 8, 8, 3, 3)
-PSX_PLAN(768, 8, 8, 4, 3)
+PSX_PLAN(0, 8, 8)
+PSX_PLAN(0, 12, 4, 3)
+PSX_PLAN(0, 16, 4, 4)
+PSX_PLAN(0, 24, 8, 3)
+PSX_PLAN(0, 32, 8, 4)
+PSX_PLAN(0, 36, 4, 3, 3)
+PSX_PLAN(0, 48, 4, 4, 3)
+PSX_PLAN(0, 64, 8, 8)
+PSX_PLAN(0, 72, 8, 3, 3)
+PSX_PLAN(0, 96, 8, 4, 3)
+PSX_PLAN(0, 128, 8, 4, 4)
+PSX_PLAN(0, 144, 4, 4, 3, 3)
+PSX_PLAN(0, 192, 8, 8, 3)
+PSX_PLAN(0, 256, 8, 8, 4)
+PSX_PLAN(0, 288, 8, 4, 3, 3)
+PSX_PLAN(0, 384, 8, 4, 4, 3)
+PSX_PLAN(0, 512, 8, 8, 8)
+PSX_PLAN(0, 576, 8, 8, 3, 3)
+PSX_PLAN(0, 768, 8, 8, 4, 3)
+PSX_PLAN(1, 8, 8)
+PSX_PLAN(1, 12, 12)
+PSX_PLAN(1, 16, 16)
+PSX_PLAN(1, 24, 6, 4)
+PSX_PLAN(1, 32, 8, 4)
+PSX_PLAN(1, 36, 12, 3)
+PSX_PLAN(1, 48, 16, 3)
+PSX_PLAN(1, 64, 8, 8)
+PSX_PLAN(1, 72, 12, 6)
+PSX_PLAN(1, 96, 16, 6)
+PSX_PLAN(1, 128, 16, 8)
+PSX_PLAN(1, 144, 12, 12)
+PSX_PLAN(1, 192, 16, 12)
+PSX_PLAN(1, 256, 16, 16)
+PSX_PLAN(1, 288, 16, 6, 3)
+PSX_PLAN(1, 384, 16, 8, 3)
+PSX_PLAN(1, 512, 8, 8, 8)
+PSX_PLAN(1, 576, 16, 12, 3)
+PSX_PLAN(1, 768, 16, 16, 3)
 #undef PSX_PLAN
+#ifndef PSX_FFT_PLAN
+#define PSX_FFT_PLAN 0
+#endif
+template <int L> struct Plan : PlanV<L, PSX_FFT_PLAN> {};
 
 // ---------------------------------------------------------------- small DFTs
 template <int R, int SIGN> struct Dft;
 
 template <int SIGN> struct Dft<2, SIGN> {
-  static __device__ __forceinline__ void run(double2* v) {
+  static __host__ __device__ __forceinline__ void run(double2* v) {
     double2 a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -70,7 +96,7 @@ template <int SIGN> struct Dft<2, SIGN> {
 };
 
 template <int SIGN> struct Dft<3, SIGN> {
-  static __device__ __forceinline__ void run(double2* v) {
+  static __host__ __device__ __forceinline__ void run(double2* v) {
     // exp(SIGN*2*pi*i/3) = -1/2 + SIGN*i*sqrt(3)/2
     constexpr double h = 0.86602540378443864676372317075294;  // sqrt(3)/2
     double2 t = cadd(v[1], v[2]);
@@ -84,7 +110,7 @@ template <int SIGN> struct Dft<3, SIGN> {
 };
 
 template <int SIGN> struct Dft<4, SIGN> {
-  static __device__ __forceinline__ void run(double2* v) {
+  static __host__ __device__ __forceinline__ void run(double2* v) {
     double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
     double2 t2 = cadd(v[1], v[3]), t3 = cmul_i<SIGN>(csub(v[1], v[3]));
     v[0] = cadd(t0, t2);
@@ -95,7 +121,7 @@ template <int SIGN> struct Dft<4, SIGN> {
 };
 
 template <int SIGN> struct Dft<8, SIGN> {
-  static __device__ __forceinline__ void run(double2* v) {
+  static __host__ __device__ __forceinline__ void run(double2* v) {
     constexpr double s = 0.70710678118654752440084436210485;  // 1/sqrt(2)
     double2 e[4] = {v[0], v[2], v[4], v[6]};
     double2 o[4] = {v[1], v[3], v[5], v[7]};
@@ -113,6 +139,96 @@ template <int SIGN> struct Dft<8, SIGN> {
     v[6] = csub(e[2], w2);
     v[3] = cadd(e[3], w3);
     v[7] = csub(e[3], w3);
+  }
+};
+
+// 6 = 2 x 3: n = 3 n1 + n2, k = k1 + 2 k2; DFT2 over n1, twiddle W6^(n2 k1), DFT3 over n2.
+template <int SIGN> struct Dft<6, SIGN> {
+  static __host__ __device__ __forceinline__ void run(double2* v) {
+    constexpr double h = 0.86602540378443864676372317075294;  // sqrt(3)/2
+    double2 b0[3], b1[3];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+      b0[n2] = cadd(v[n2], v[n2 + 3]);
+      b1[n2] = csub(v[n2], v[n2 + 3]);
+    }
+    // W6^1 = 1/2 + SIGN i sqrt(3)/2, W6^2 = -1/2 + SIGN i sqrt(3)/2
+    b1[1] = cmul(b1[1], make_double2(0.5, SIGN * h));
+    b1[2] = cmul(b1[2], make_double2(-0.5, SIGN * h));
+    Dft<3, SIGN>::run(b0);
+    Dft<3, SIGN>::run(b1);
+    v[0] = b0[0];
+    v[2] = b0[1];
+    v[4] = b0[2];
+    v[1] = b1[0];
+    v[3] = b1[1];
+    v[5] = b1[2];
+  }
+};
+
+// 12 = 4 x 3: n = 3 n1 + n2, k = k1 + 4 k2; DFT4 over n1, twiddle W12^(n2 k1), DFT3 over n2.
+template <int SIGN> struct Dft<12, SIGN> {
+  static __host__ __device__ __forceinline__ void run(double2* v) {
+    constexpr double h = 0.86602540378443864676372317075294;  // sqrt(3)/2
+    double2 b[3][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+      double2 t[4] = {v[n2], v[n2 + 3], v[n2 + 6], v[n2 + 9]};
+      Dft<4, SIGN>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) b[n2][k1] = t[k1];
+    }
+    // W12^m, m = n2 k1 in {1, 2, 3} (n2 = 1) and {2, 4, 6} (n2 = 2)
+    b[1][1] = cmul(b[1][1], make_double2(h, SIGN * 0.5));
+    b[1][2] = cmul(b[1][2], make_double2(0.5, SIGN * h));
+    b[1][3] = cmul_i<SIGN>(b[1][3]);
+    b[2][1] = cmul(b[2][1], make_double2(0.5, SIGN * h));
+    b[2][2] = cmul(b[2][2], make_double2(-0.5, SIGN * h));
+    b[2][3] = make_double2(-b[2][3].x, -b[2][3].y);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      double2 t[3] = {b[0][k1], b[1][k1], b[2][k1]};
+      Dft<3, SIGN>::run(t);
+      v[k1] = t[0];
+      v[k1 + 4] = t[1];
+      v[k1 + 8] = t[2];
+    }
+  }
+};
+
+// 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2; DFT4 over n1, twiddle W16^(n2 k1), DFT4 over n2.
+template <int SIGN> struct Dft<16, SIGN> {
+  static __host__ __device__ __forceinline__ void run(double2* v) {
+    constexpr double c1 = 0.92387953251128675612818318939679;  // cos(pi/8)
+    constexpr double s1 = 0.38268343236508977172845998403040;  // sin(pi/8)
+    constexpr double r = 0.70710678118654752440084436210485;   // 1/sqrt(2)
+    double2 b[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      double2 t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      Dft<4, SIGN>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) b[n2][k1] = t[k1];
+    }
+    // W16^m, m = n2 k1: (1,2,3), (2,4,6), (3,6,9)
+    b[1][1] = cmul(b[1][1], make_double2(c1, SIGN * s1));
+    b[1][2] = cmul(b[1][2], make_double2(r, SIGN * r));
+    b[1][3] = cmul(b[1][3], make_double2(s1, SIGN * c1));
+    b[2][1] = cmul(b[2][1], make_double2(r, SIGN * r));
+    b[2][2] = cmul_i<SIGN>(b[2][2]);
+    b[2][3] = cmul(b[2][3], make_double2(-r, SIGN * r));
+    b[3][1] = cmul(b[3][1], make_double2(s1, SIGN * c1));
+    b[3][2] = cmul(b[3][2], make_double2(-r, SIGN * r));
+    b[3][3] = cmul(b[3][3], make_double2(-c1, -SIGN * s1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      double2 t[4] = {b[0][k1], b[1][k1], b[2][k1], b[3][k1]};
+      Dft<4, SIGN>::run(t);
+      v[k1] = t[0];
+      v[k1 + 4] = t[1];
+      v[k1 + 8] = t[2];
+      v[k1 + 12] = t[3];
+    }
   }
 };
 
