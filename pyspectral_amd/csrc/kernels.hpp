@@ -36,6 +36,11 @@ constexpr int NT = 256;       // threads per block
 #ifndef PSX_ZPAD
 #define PSX_ZPAD 0
 #endif
+// z pass: stage the block's input spectra in LDS with one coalesced load pass (1)
+// or load them inside the recipe loop (0).
+#ifndef PSX_ZSTAGE
+#define PSX_ZSTAGE 1
+#endif
 constexpr int MAXJOBS = 16;   // input fields per pencil pass
 constexpr int MAXREC = 12;    // real pencils built by the z pass
 
@@ -289,23 +294,79 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
   constexpr int LS = Row<L, PADZ>::LS;
   constexpr int L2 = L / 2;
   __shared__ double2 buf[NCMAX * LS];
-  double* bufd = reinterpret_cast<double*>(buf);
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NTZ)
   const long long pos0 = (long long)blockIdx.x * P;
 #define ZRW(i) (::psx::Row<L, PADZ>::at(i))
 
   // --- build the inverse spectra
-  for (int e = tid; e < NCI * (L2 + 1); e += NTZ) {
-    const int q = e / (L2 + 1), k = e - q * (L2 + 1);
-    const int r0 = 2 * q, r1 = 2 * q + 1;
-    const double2 A = zspec(a, r0 / P, pos0 + r0 % P, k);
-    const double2 B = r1 < NRI ? zspec(a, r1 / P, pos0 + r1 % P, k) : make_double2(0.0, 0.0);
-    if (k == 0 || k == L2) {
-      buf[q * LS + ZRW(k)] = make_double2(A.x, B.x);
-    } else {
-      buf[q * LS + ZRW(k)] = make_double2(A.x - B.y, A.y + B.x);
-      buf[q * LS + ZRW(L - k)] = make_double2(A.x + B.y, B.x - A.y);
+  constexpr int HS = L2 + 1;  // staged k3 modes per real pencil
+  constexpr bool STAGE = (PSX_ZSTAGE != 0) && (NRI * HS <= NCMAX * LS);
+  if constexpr (STAGE) {
+    // one coalesced pass of independent global loads into LDS, then the recipe
+    // combinations from LDS into registers, then the Hermitian pencils
+    for (int e = tid; e < NRI * HS; e += NTZ) {
+      const int r = e / HS, k = e - r * HS;
+      buf[e] = zin(a, r / P, pos0 + r % P, k);
+    }
+    __syncthreads();
+    constexpr int IB = (NCI * (L2 + 1) + NTZ - 1) / NTZ;
+    double2 zA[IB], zB[IB];
+#pragma unroll
+    for (int ib = 0; ib < IB; ++ib) {
+      const int e = tid + ib * NTZ;
+      zA[ib] = zB[ib] = make_double2(0.0, 0.0);
+      if (e < NCI * (L2 + 1)) {
+        const int q = e / (L2 + 1), k = e - q * (L2 + 1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = 2 * q + h;
+          if (r < NRI) {
+            const int p = r % P;
+            const Recipe& R = a.rec[r / P];
+            double2 acc = make_double2(0.0, 0.0);
+            for (int t = 0; t < R.nterm; ++t) {
+              double2 v = buf[(R.f[t] * P + p) * HS + k];
+              double sc = R.coef[t];
+              if (R.dz[t]) {
+                v = cmul_i<1>(v);
+                sc *= a.kzscale * k;
+              }
+              acc.x += sc * v.x;
+              acc.y += sc * v.y;
+            }
+            if (h == 0) zA[ib] = acc; else zB[ib] = acc;
+          }
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ib = 0; ib < IB; ++ib) {
+      const int e = tid + ib * NTZ;
+      if (e < NCI * (L2 + 1)) {
+        const int q = e / (L2 + 1), k = e - q * (L2 + 1);
+        const double2 A = zA[ib], B = zB[ib];
+        if (k == 0 || k == L2) {
+          buf[q * LS + ZRW(k)] = make_double2(A.x, B.x);
+        } else {
+          buf[q * LS + ZRW(k)] = make_double2(A.x - B.y, A.y + B.x);
+          buf[q * LS + ZRW(L - k)] = make_double2(A.x + B.y, B.x - A.y);
+        }
+      }
+    }
+  } else {
+    for (int e = tid; e < NCI * (L2 + 1); e += NTZ) {
+      const int q = e / (L2 + 1), k = e - q * (L2 + 1);
+      const int r0 = 2 * q, r1 = 2 * q + 1;
+      const double2 A = zspec(a, r0 / P, pos0 + r0 % P, k);
+      const double2 B = r1 < NRI ? zspec(a, r1 / P, pos0 + r1 % P, k) : make_double2(0.0, 0.0);
+      if (k == 0 || k == L2) {
+        buf[q * LS + ZRW(k)] = make_double2(A.x, B.x);
+      } else {
+        buf[q * LS + ZRW(k)] = make_double2(A.x - B.y, A.y + B.x);
+        buf[q * LS + ZRW(L - k)] = make_double2(A.x + B.y, B.x - A.y);
+      }
     }
   }
   __syncthreads();
@@ -313,15 +374,20 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
 
   if constexpr (MODE == Z_DT) {
     double m = 0.0;
-    for (int e = tid; e < P * L; e += NTZ) {
-      const int p = e / L, z = e - p * L;
-      double v[3];
+    for (int z = tid; z < L; z += NTZ) {
+      double2 w[NCI];
 #pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        const int r = f * P + p;
-        v[f] = bufd[2 * ((r >> 1) * LS + ZRW(z)) + (r & 1)];
+      for (int q = 0; q < NCI; ++q) w[q] = buf[q * LS + ZRW(z)];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        double v[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          const int r = f * P + p;
+          v[f] = (r & 1) ? w[r >> 1].y : w[r >> 1].x;
+        }
+        m = fmax(m, fabs(v[0]) * a.dinv[0] + fabs(v[1]) * a.dinv[1] + fabs(v[2]) * a.dinv[2]);
       }
-      m = fmax(m, fabs(v[0]) * a.dinv[0] + fabs(v[1]) * a.dinv[1] + fabs(v[2]) * a.dinv[2]);
     }
     // wave reduction, then the block's maximum into its own slot (no atomics)
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
@@ -335,39 +401,44 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
     }
     return;
   } else {
-    // --- pointwise products / closure
-    constexpr int NPTS = P * L;
-    constexpr int IT = (NPTS + NTZ - 1) / NTZ;
-    double res[IT][NOUT];
+    // --- pointwise products / closure: a thread owns one z for all P positions and
+    // moves whole complex pencil entries (16 B per lane, contiguous across lanes)
+    static_assert(P <= 2, "pointwise phase handles P = 1 or 2");
+    constexpr int IT = (L + NTZ - 1) / NTZ;
+    double2 res[IT][NCO];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * NTZ;
-      if ((NPTS % NTZ) == 0 || e < NPTS) {
-        const int p = e / L, z = e - p * L;
-        double in[NIN];
+      const int z = tid + it * NTZ;
+      if ((L % NTZ) == 0 || z < L) {
+        double2 v[NCI];
 #pragma unroll
-        for (int f = 0; f < NIN; ++f) {
-          const int r = f * P + p;
-          in[f] = bufd[2 * ((r >> 1) * LS + ZRW(z)) + (r & 1)];
+        for (int q = 0; q < NCI; ++q) v[q] = buf[q * LS + ZRW(z)];
+        double o[P][NOUT];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          double in[NIN];
+#pragma unroll
+          for (int f = 0; f < NIN; ++f) {
+            const int r = f * P + p;
+            in[f] = (r & 1) ? v[r >> 1].y : v[r >> 1].x;
+          }
+          zpoint<MODE>(a, in, o[p]);
         }
-        zpoint<MODE>(a, in, res[it]);
+#pragma unroll
+        for (int q = 0; q < NCO; ++q) {
+          const int r0 = 2 * q, r1 = 2 * q + 1;
+          res[it][q].x = o[r0 % P][r0 / P];
+          res[it][q].y = r1 < NRO ? o[r1 % P][r1 / P] : 0.0;
+        }
       }
     }
     __syncthreads();
-    if constexpr ((NRO & 1) != 0) {
-      // odd number of real output pencils: the unused half of the last one is zero
-      for (int z = tid; z < L; z += NTZ) bufd[2 * ((NCO - 1) * LS + ZRW(z)) + 1] = 0.0;
-    }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * NTZ;
-      if ((NPTS % NTZ) == 0 || e < NPTS) {
-        const int p = e / L, z = e - p * L;
+      const int z = tid + it * NTZ;
+      if ((L % NTZ) == 0 || z < L) {
 #pragma unroll
-        for (int o = 0; o < NOUT; ++o) {
-          const int r = o * P + p;
-          bufd[2 * ((r >> 1) * LS + ZRW(z)) + (r & 1)] = res[it][o];
-        }
+        for (int q = 0; q < NCO; ++q) buf[q * LS + ZRW(z)] = res[it][q];
       }
     }
     __syncthreads();
