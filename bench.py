@@ -54,6 +54,23 @@ def survey_bytes_per_step(N, closure):
     return 4 * b
 
 
+def measured_traffic(kernel, N, closure):
+    """HBM bytes per launch of `kernel` as rocprofv3 FETCH_SIZE/WRITE_SIZE measured them (copy-calibrated),
+    read from the committed counter summary for this workload; None when there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{'smag' if closure == 'smagorinsky' else 'dns'}{N}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0] == kernel:
+            try:
+                return int(float(cells[5]))
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(N, closure, nu, cfl, steps, seed):
     """The oracle (numpy restatement of the reference's src/ path, scipy.fft with one
     worker per available core) advancing the same field; bounded sample of `steps`."""
@@ -172,7 +189,8 @@ def main():
                                f"({3 * N // 2}^3 padded), nu={nu}, CFL={cfl}, RK4 (low-storage 4-stage)",
                    "grid": N, "closure": args.closure, "cfl": cfl, "nu": nu, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom, N, args.closure),
                      "bytes_per_launch": kb[dom]},
         "step_roofline": {"model_bytes_per_step": int(sb), "achieved": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
                           "unit": "GB/s", "frac": round(sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
