@@ -139,38 +139,37 @@ def unpad_2x(uhat_pad: np.ndarray) -> np.ndarray:
     return out
 
 
-def _p_cube_mask(shape, kx_lo: int, kz_hi: int) -> np.ndarray:
-    """Boolean mask of the resolved cube ``0..kx_lo-1`` and ``-(kx_lo-1)..-1`` on the
-    first two axes, ``0..kz_hi-1`` on the third."""
+def _p_cube_mask(shape, k1_lo: int, k2_lo: int, kz_hi: int) -> np.ndarray:
+    """Boolean mask of the resolved box ``0..k1_lo-1`` and ``-(k1_lo-1)..-1`` on the
+    first axis (likewise with ``k2_lo`` on the second), ``0..kz_hi-1`` on the third."""
     P1, P2, P3 = shape
-    if kx_lo < 2:
+    if min(k1_lo, k2_lo) < 2:
         raise ValueError("mode split needs a cutoff of at least 2")
 
-    def ax(n):
+    def ax(n, lo):
         m = np.zeros(n, dtype=bool)
-        m[:kx_lo] = True
-        m[n - kx_lo + 1:] = True
+        m[:lo] = True
+        m[n - lo + 1:] = True
         return m
 
     mz = np.zeros(P3, dtype=bool)
     mz[:kz_hi] = True
-    return ax(P1)[:, None, None] & ax(P2)[None, :, None] & mz[None, None, :]
+    return ax(P1, k1_lo)[:, None, None] & ax(P2, k2_lo)[None, :, None] & mz[None, None, :]
 
 
 def separate_modes(uhat_pad: np.ndarray):
-    """p/q split of a 2x-grid field at N/2: p keeps |k1|,|k2| < N/2 (the -N/2 mode
-    goes to q) and k3 < N/2 (``src/padding.py:67-79``)."""
+    """p/q split of a 2x-grid field: p keeps |k1| < N1/2, |k2| < N2/2 (the -N/2 modes
+    go to q) and k3 < N3/2, each axis at its own N (``src/padding.py:65-79``)."""
     P1, P2, P3 = uhat_pad.shape
-    N1 = P1 // 2
     N3c = (P3 + 1) // 2
-    m = _p_cube_mask(uhat_pad.shape, N1 // 2, P3 - N3c)
+    m = _p_cube_mask(uhat_pad.shape, P1 // 4, P2 // 4, P3 - N3c)
     up = np.where(m, uhat_pad, 0)
     return up, uhat_pad - up
 
 
 def separate_modes_testfilter(uhat_pad: np.ndarray, kf: int):
     """p/q split at the test-filter index ``kf`` (``src/padding.py:52-64``)."""
-    m = _p_cube_mask(uhat_pad.shape, kf, kf)
+    m = _p_cube_mask(uhat_pad.shape, kf, kf, kf)
     up = np.where(m, uhat_pad, 0)
     return up, uhat_pad - up
 

@@ -87,6 +87,10 @@ template <int L> struct Plan : PlanV<L, PSX_FFT_PLAN> {};
 // ---------------------------------------------------------------- small DFTs
 template <int R, int SIGN> struct Dft;
 
+template <int SIGN> struct Dft<1, SIGN> {
+  static __host__ __device__ __forceinline__ void run(double2*) {}
+};
+
 template <int SIGN> struct Dft<2, SIGN> {
   static __host__ __device__ __forceinline__ void run(double2* v) {
     double2 a = v[0], b = v[1];

@@ -1,0 +1,264 @@
+// Fused z pass with one wavefront per position group, for z lengths L = 64 R0.
+//
+// Same contract as zpass (kernels.hpp): c2r of the real z-pencils of P positions,
+// pointwise products / closure on the physical z line, r2c of the results with the
+// truncation to Hout modes.  The difference is the data path.  A lane l owns the
+// entries k = l + 64 m (m < R0) of every complex pencil, and a length-L transform
+// x_n = sum_k Z_k w_L^{S n k} runs as
+//   Y_a(l)        = w_L^{S a l} sum_m Z_{l+64m} w_R0^{S a m}     (registers)
+//   x_{a + R0 b}  = sum_l Y_a(l) w_64^{S b l}                     (LDS, radix 8 x 8)
+// with b = b1 + 8 b2 and l = j + 8 r.  The pencils' half spectra are read from
+// global memory straight into the first (register) step, the pointwise phase
+// feeds the forward transform's register step directly, and a workgroup is one
+// wavefront, so there is no workgroup barrier and no index division anywhere.
+#pragma once
+#include "kernels.hpp"
+
+namespace psx {
+
+template <int L> struct ZWave {
+  static constexpr int R0 = L / 64;
+  static constexpr bool ok = (L % 64 == 0) && (R0 == 1 || R0 == 2 || R0 == 3 || R0 == 4 || R0 == 6 ||
+                                               R0 == 8 || R0 == 12);
+  // complex pencils transformed together through the radix-8 passes
+  static constexpr int G = R0 >= 8 ? 1 : 8 / R0;
+  // LDS slot of one 64-point sub-transform: 64 entries, pass-1 output at b1*9 + j
+  static constexpr int SUB = 72;
+  // natural-order row: one pad entry per 16
+  static constexpr int NATS = L + L / 16;
+  static __device__ __forceinline__ int nat(int n) { return n + (n >> 4); }
+};
+
+// One workgroup = one wavefront.  The compiler drops the s_barrier of a
+// single-wavefront workgroup; the fence keeps LDS accesses in program order.
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// Half-spectrum entry kk of real z-pencil r = f*P + p (field f, position p).  The
+// recipes are the z-pass's: plain fields, except for the Smagorinsky strain rates.
+template <int MODE, int P>
+__device__ __forceinline__ double2 zw_real(const ZArgs& a, int r, long long pos0, int kk) {
+  const int f = r / P;
+  const long long pos = pos0 + r % P;
+  if constexpr (MODE == Z_SMAG) {
+    // y-pass fields: 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y, 7 v_y, 8 w_y.
+    // pencils: u v w S11 S22 S33 S12 S13 S23 (src/RHSfunctions.py:174-179)
+    const double kz = a.kzscale * kk;
+    switch (f) {
+      case 0: return zin(a, 0, pos, kk);
+      case 1: return zin(a, 1, pos, kk);
+      case 2: return zin(a, 2, pos, kk);
+      case 3: return zin(a, 3, pos, kk);
+      case 4: return zin(a, 7, pos, kk);
+      case 5: return cscale(cmul_i<1>(zin(a, 2, pos, kk)), kz);
+      case 6: return cscale(cadd(zin(a, 6, pos, kk), zin(a, 4, pos, kk)), 0.5);
+      case 7: return cscale(cadd(cscale(cmul_i<1>(zin(a, 0, pos, kk)), kz), zin(a, 5, pos, kk)), 0.5);
+      default: return cscale(cadd(cscale(cmul_i<1>(zin(a, 1, pos, kk)), kz), zin(a, 8, pos, kk)), 0.5);
+    }
+  } else {
+    return zin(a, f, pos, kk);
+  }
+}
+
+// Entry k of complex pencil q = (real pencils 2q, 2q+1) of the Hermitian pair
+// Z = A + iB, A and B the half spectra (c2r semantics: real part only at k = 0 and L/2).
+template <int L, int MODE, int P, int NRI>
+__device__ __forceinline__ double2 zw_hermitian(const ZArgs& a, int q, long long pos0, int k) {
+  const bool mirror = k > L / 2;
+  const int kk = mirror ? L - k : k;
+  const double2 A = zw_real<MODE, P>(a, 2 * q, pos0, kk);
+  const double2 B = (2 * q + 1 < NRI) ? zw_real<MODE, P>(a, 2 * q + 1, pos0, kk) : make_double2(0.0, 0.0);
+  if (k == 0 || k == L / 2) return make_double2(A.x, B.x);
+  return mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
+}
+
+// The 64-point transforms of `nrows` pencils (R0 sub-transforms each) held in LDS
+// slots f*SUB + l; the result x_{a + R0 b} of pencil g lands in natural order at
+// g*NATS + nat(n).
+template <int L, int SIGN>
+__device__ __forceinline__ void zw_sub64(double2* buf, const double2* __restrict__ tw, int l, int nrows) {
+  using W = ZWave<L>;
+  constexpr int R0 = W::R0, SUB = W::SUB, NB = W::G * R0 * 8, IT = (NB + 63) / 64;
+  double2 y[IT][8];
+  // pass 1: 8-point transforms over r (l = j + 8 r), twiddle w_64^{S j b1}
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int u = l + 64 * it, f = u >> 3, j = u & 7;
+    if ((NB % 64 == 0 || u < NB) && f < nrows * R0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) y[it][r] = buf[f * SUB + j + 8 * r];
+      Dft<8, SIGN>::run(y[it]);
+      double2 w[8];
+      w[1] = tw[R0 * j];
+      if constexpr (SIGN > 0) w[1].y = -w[1].y;
+      w[2] = cmul(w[1], w[1]);
+      w[3] = cmul(w[1], w[2]);
+      w[4] = cmul(w[2], w[2]);
+      w[5] = cmul(w[2], w[3]);
+      w[6] = cmul(w[3], w[3]);
+      w[7] = cmul(w[3], w[4]);
+#pragma unroll
+      for (int b1 = 1; b1 < 8; ++b1) y[it][b1] = cmul(y[it][b1], w[b1]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int u = l + 64 * it, f = u >> 3, j = u & 7;
+    if ((NB % 64 == 0 || u < NB) && f < nrows * R0) {
+#pragma unroll
+      for (int b1 = 0; b1 < 8; ++b1) buf[f * SUB + b1 * 9 + j] = y[it][b1];
+    }
+  }
+  wave_sync();
+  // pass 2: 8-point transforms over j for each b1
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int u = l + 64 * it, f = u >> 3, b1 = u & 7;
+    if ((NB % 64 == 0 || u < NB) && f < nrows * R0) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) y[it][j] = buf[f * SUB + b1 * 9 + j];
+      Dft<8, SIGN>::run(y[it]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int u = l + 64 * it, f = u >> 3, b1 = u & 7;
+    if ((NB % 64 == 0 || u < NB) && f < nrows * R0) {
+      const int g = f / R0, av = f - g * R0;
+#pragma unroll
+      for (int b2 = 0; b2 < 8; ++b2) buf[g * W::NATS + W::nat(av + R0 * (b1 + 8 * b2))] = y[it][b2];
+    }
+  }
+  wave_sync();
+}
+
+template <int L, int P, int MODE>
+__global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__ tw) {
+  using W = ZWave<L>;
+  static_assert(W::ok, "zwave: L must be 64 * {1,2,3,4,6,8,12}");
+  constexpr int R0 = W::R0, G = W::G, SUB = W::SUB;
+  constexpr int NIN = ZShape<MODE>::nin, NOUT = ZShape<MODE>::nout;
+  constexpr int NRI = NIN * P, NRO = NOUT * P;
+  constexpr int NCI = (NRI + 1) / 2, NCO = (NRO + 1) / 2;
+  __shared__ double2 buf[G * R0 * SUB];
+  const int l = threadIdx.x;
+  const long long pos0 = (long long)blockIdx.x * P;
+
+  // register-step twiddles w_L^{-a l} (the inverse uses the conjugates)
+  double2 t1[R0];
+#pragma unroll
+  for (int av = 0; av < R0; ++av) t1[av] = tw[av * l];
+
+  // ---- inverse transforms; lane l keeps x_n, n = l + 64 m, of every pencil
+  double2 x[NCI][R0];
+#pragma unroll
+  for (int g0 = 0; g0 < NCI; g0 += G) {
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      const int q = g0 + gi;
+      if (q < NCI) {
+        double2 v[R0];
+#pragma unroll
+        for (int m = 0; m < R0; ++m) v[m] = zw_hermitian<L, MODE, P, NRI>(a, q, pos0, l + 64 * m);
+        Dft<R0, +1>::run(v);
+        buf[(gi * R0) * SUB + l] = v[0];
+#pragma unroll
+        for (int av = 1; av < R0; ++av) buf[(gi * R0 + av) * SUB + l] = cmul(v[av], cconj(t1[av]));
+      }
+    }
+    wave_sync();
+    zw_sub64<L, +1>(buf, tw, l, NCI - g0 < G ? NCI - g0 : G);
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      const int q = g0 + gi;
+      if (q < NCI) {
+#pragma unroll
+        for (int m = 0; m < R0; ++m) x[q][m] = buf[gi * W::NATS + W::nat(l + 64 * m)];
+      }
+    }
+    wave_sync();
+  }
+
+  if constexpr (MODE == Z_DT) {
+    double mx = 0.0;
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        double v[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          const int r = f * P + p;
+          v[f] = (r & 1) ? x[r >> 1][m].y : x[r >> 1][m].x;
+        }
+        mx = fmax(mx, fabs(v[0]) * a.dinv[0] + fabs(v[1]) * a.dinv[1] + fabs(v[2]) * a.dinv[2]);
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    if (l == 0) a.red[blockIdx.x] = mx;
+    return;
+  } else {
+    // ---- pointwise products / closure on the lane's R0 z points
+    double2 xo[NCO][R0];
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      double o[P][NOUT];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        double in[NIN];
+#pragma unroll
+        for (int f = 0; f < NIN; ++f) {
+          const int r = f * P + p;
+          in[f] = (r & 1) ? x[r >> 1][m].y : x[r >> 1][m].x;
+        }
+        zpoint<MODE>(a, in, o[p]);
+      }
+#pragma unroll
+      for (int q = 0; q < NCO; ++q) {
+        const int r0 = 2 * q, r1 = 2 * q + 1;
+        xo[q][m].x = o[r0 % P][r0 / P];
+        xo[q][m].y = r1 < NRO ? o[r1 % P][r1 / P] : 0.0;
+      }
+    }
+
+    // ---- forward transforms, truncation to Hout modes, split of the pencil pairs
+#pragma unroll
+    for (int g0 = 0; g0 < NCO; g0 += G) {
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi) {
+        const int q = g0 + gi;
+        if (q < NCO) {
+          double2 v[R0];
+#pragma unroll
+          for (int m = 0; m < R0; ++m) v[m] = xo[q][m];
+          Dft<R0, -1>::run(v);
+          buf[(gi * R0) * SUB + l] = v[0];
+#pragma unroll
+          for (int av = 1; av < R0; ++av) buf[(gi * R0 + av) * SUB + l] = cmul(v[av], t1[av]);
+        }
+      }
+      wave_sync();
+      zw_sub64<L, -1>(buf, tw, l, NCO - g0 < G ? NCO - g0 : G);
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi) {
+        const int q = g0 + gi;
+        if (q < NCO) {
+          const int r0 = 2 * q, r1 = 2 * q + 1;
+          for (int k = l; k < a.Hout; k += 64) {
+            const double2 z1 = buf[gi * W::NATS + W::nat(k)];
+            const double2 z2 = buf[gi * W::NATS + W::nat(k == 0 ? 0 : L - k)];
+            a.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k] =
+                make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
+            if (r1 < NRO)
+              a.out[(long long)(r1 / P) * a.out_sf + (pos0 + r1 % P) * a.Hout + k] =
+                  make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+}  // namespace psx

@@ -82,6 +82,50 @@ def test_rhs_matches_oracle(N, model):
         assert abs(complex(tau[6], tau[7]) - fp) <= 1e-10 * abs(fp)
 
 
+def _field_shape(N1, N2, N3, seed):
+    """Seeded smooth solenoidal Hermitian field on an N1 x N2 x N3 grid (any shape)."""
+    rng = np.random.default_rng(seed)
+    g = so.Grid(N1, N2, N3)
+    shp = (3, N1, N2, N3 // 2 + 1)
+    u = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) * np.exp(-0.02 * g.ksqr)
+    u = np.stack([np.fft.rfftn(np.fft.irfftn(c, s=(N1, N2, N3), axes=(0, 1, 2)), axes=(0, 1, 2)) for c in u])
+    div = g.k1 * u[0] + g.k2 * u[1] + g.k3 * u[2]
+    u = np.stack([u[0] - g.k1 * div / g.ksqr, u[1] - g.k2 * div / g.ksqr, u[2] - g.k3 * div / g.ksqr])
+    return u
+
+
+# z lengths 64 R0 (3/2 rule: 3 N3 / 2, t-model: 2 N3, CFL: N3) take the one-wavefront z pass
+@pytest.mark.parametrize("shape,models", [((16, 16, 128), MODELS), ((16, 32, 64), MODELS),
+                                          ((16, 16, 256), [0, 1, 2, "Static tau-model"]), ((16, 16, 512), [0])])
+def test_rhs_long_z_matches_oracle(shape, models):
+    N1, N2, N3 = shape
+    nu, t = 0.011, 0.29
+    u = _field_shape(N1, N2, N3, 41 + N3)
+    for model in models:
+        print(f"\nrhs shape={shape} model={model}", flush=True)
+        g = so.Grid(N1, N2, N3)
+        st = so.State(g, *u, nu=nu, turb_model=model, t=t, **TAU_KW)
+        so.compute_rhs(st, FFT)
+        ref = st.compact()
+        from pyspectral_amd.solver import SpectralSolver
+
+        s = SpectralSolver(N1, N2, N3, model, nu=nu, Ct=0.2, tau_a=0.05, tau_b=0.5)
+        w0 = torch.empty_like(_dev(u)) if model != 0 else None
+        r = _host(s.rhs(_dev(u), t, w0=w0))
+        err = _rel(r, ref)
+        print(f"rhs relative L2 = {err:.3e}", flush=True)
+        assert err < 1e-12, (model, err)
+        if model != 0:
+            assert _rel(_host(w0), np.stack(st.w0)) < 1e-11
+        st2 = so.State(g, *u, nu=nu, cfl=0.5)
+        so.compute_dt(st2)
+        s2 = SpectralSolver(N1, N2, N3, 1, nu=nu, cfl=0.5)
+        dt = _host(s2.compute_dt(_dev(u)))[0]
+        assert abs(dt - st2.dt) < 1e-14 * st2.dt, (dt, st2.dt)
+        s.close()
+        s2.close()
+
+
 def test_rhs_tgv_closed_form():
     N, nu = 32, 1.0 / 400
     u, v, w, _ = so.taylor_green(N, N, N)

@@ -130,3 +130,23 @@ def test_dynamic_tau_reductions_bruteforce():
     for c in range(3):
         expect = PLh[c] + tau * Q[c]
         assert np.abs(R[c] - expect).max() < 1e-11 * np.abs(expect).max()
+
+
+def test_separate_modes_per_axis_cutoffs():
+    """p holds exactly the modes with |k1| < N1/2, |k2| < N2/2 and 0 <= k3 < N3/2 of the
+    2x grid, each axis at its own N (src/padding.py:65-79), stated on integer wavenumbers."""
+    N1, N2, N3 = 8, 12, 16
+    rng = np.random.default_rng(2)
+    shp = (2 * N1, 2 * N2, N3 + 1)
+    u = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    up, uq = so.separate_modes(u)
+    k1 = np.fft.fftfreq(2 * N1, 1.0 / (2 * N1))[:, None, None]
+    k2 = np.fft.fftfreq(2 * N2, 1.0 / (2 * N2))[None, :, None]
+    k3 = np.arange(N3 + 1)[None, None, :]
+    inside = (np.abs(k1) < N1 / 2) & (np.abs(k2) < N2 / 2) & (k3 < N3 / 2)
+    assert np.array_equal(up != 0, inside)
+    assert np.array_equal(up + uq, u)
+    kf = 3
+    fp, fq = so.separate_modes_testfilter(u, kf)
+    inside_f = (np.abs(k1) < kf) & (np.abs(k2) < kf) & (k3 < kf)
+    assert np.array_equal(fp != 0, inside_f)
