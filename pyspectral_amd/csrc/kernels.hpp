@@ -68,6 +68,30 @@ __device__ __forceinline__ int pad_slot(int i, int n, int L, int keep_nyq) {
   return L + i - n;
 }
 
+// ------------------------------------------------------------------ block order
+// Workgroups are dispatched to the 8 XCDs round-robin (id mod 8), each XCD with its
+// own L2.  xcd_block gives every XCD a contiguous range of logical block ids, so
+// neighbouring column tiles -- which share the cache lines of rows that are not
+// 128-byte aligned, like the state's (N3/2+1)-entry rows -- run on one L2.
+// Measured: it helps the pencil passes that read the state (the CFL passes most)
+// and slows the x-forward epilogue, whose W-layout rows are aligned.
+#ifndef PSX_XCD
+#define PSX_XCD 1
+#endif
+#ifndef PSX_XCD_EPI
+#define PSX_XCD_EPI 0
+#endif
+template <bool ON>
+__device__ __forceinline__ int xcd_block() {
+  if constexpr (ON) {
+    const int w = blockIdx.x, G = gridDim.x;
+    const int q = G >> 3, r = G & 7, x = w & 7;
+    return x * q + (x < r ? x : r) + (w >> 3);
+  } else {
+    return blockIdx.x;
+  }
+}
+
 // ------------------------------------------------------------------ pencil passes
 struct PencilArgs {
   const double2* in;
@@ -91,6 +115,9 @@ struct PencilArgs {
 #ifndef PSX_PINV_MINW
 #define PSX_PINV_MINW 1
 #endif
+#ifndef PSX_EPI_PREFETCH
+#define PSX_EPI_PREFETCH 1
+#endif
 #ifndef PSX_EPI_MINW
 #define PSX_EPI_MINW 1
 #endif
@@ -101,8 +128,9 @@ __global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int aidx = blockIdx.x / ct;
-  const int c0 = (blockIdx.x - aidx * ct) * T;
+  const int bid = xcd_block<PSX_XCD != 0>();
+  const int aidx = bid / ct;
+  const int c0 = (bid - aidx * ct) * T;
   const int job = blockIdx.y;
   const double2* src = a.in + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
   constexpr int IT = (L * T + NT - 1) / NT;
@@ -155,8 +183,9 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int aidx = blockIdx.x / ct;
-  const int c0 = (blockIdx.x - aidx * ct) * T;
+  const int bid = xcd_block<PSX_XCD != 0>();
+  const int aidx = bid / ct;
+  const int c0 = (bid - aidx * ct) * T;
   const int job = blockIdx.y;
   const double2* src = a.in + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
   for (int e = tid; e < L * T; e += NT) {
@@ -563,15 +592,16 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.H + T - 1) / T;
+  const int bid = xcd_block<PSX_XCD_EPI != 0>();
   const int nmain = a.N2 * ct;
   const long long plane = (long long)a.N2 * a.Nc;
   const long long fstride = (long long)a.N1 * plane;  // component stride in the state layout
 
-  if ((int)blockIdx.x >= nmain) {
+  if ((int)bid >= nmain) {
     // k3 in [H, Nc): elementwise (no transformed fields)
     const int nrest = a.Nc - a.H;
     const long long total = (long long)a.N1 * a.N2 * nrest;
-    for (long long e = (long long)(blockIdx.x - nmain) * NT + tid; e < total;
+    for (long long e = (long long)(bid - nmain) * NT + tid; e < total;
          e += (long long)(gridDim.x - nmain) * NT) {
       const int kz = a.H + (int)(e % nrest);
       const long long r = e / nrest;
@@ -608,14 +638,14 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
       }
     }
     if (EPI == E_TAU2 && a.red_part && tid == 0) {
-      a.red_part[2 * blockIdx.x] = 0.0;
-      a.red_part[2 * blockIdx.x + 1] = 0.0;
+      a.red_part[2 * bid] = 0.0;
+      a.red_part[2 * bid + 1] = 0.0;
     }
     return;
   }
 
-  const int ky = blockIdx.x / ct;
-  const int c0 = (blockIdx.x - ky * ct) * T;
+  const int ky = bid / ct;
+  const int c0 = (bid - ky * ct) * T;
   const long long in_sl = (long long)a.N2 * a.H;  // x stride in W layout
   const double2* src0 = a.in + (long long)ky * a.H + c0;
   constexpr bool SGS = W0 && (EPI == E_SMAG);
@@ -628,13 +658,39 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   for (int it = 0; it < (SGS ? IT : 1); ++it) accs[it][0] = accs[it][1] = accs[it][2] = make_double2(0.0, 0.0);
 
   const double k2 = a.ks[1] * signed_freq(ky, a.N2);
+#if PSX_EPI_PREFETCH
+  // field f+1 is loaded into registers while field f is transformed
+  constexpr int LT = (L * T + NT - 1) / NT;
+  double2 pre[LT];
+  auto load_field = [&](int f) {
+    const double2* src = src0 + (long long)f * a.in_sf;
+#pragma unroll
+    for (int i = 0; i < LT; ++i) {
+      const int e = tid + i * NT;
+      const int l = e / T, c = e - (e / T) * T;
+      pre[i] = ((L * T) % NT == 0 || e < L * T) && (c0 + c < a.H) ? src[(long long)l * in_sl + c]
+                                                                    : make_double2(0.0, 0.0);
+    }
+  };
+  load_field(0);
+#endif
 #pragma unroll 1
   for (int f = 0; f < NF; ++f) {
+#if PSX_EPI_PREFETCH
+#pragma unroll
+    for (int i = 0; i < LT; ++i) {
+      const int e = tid + i * NT;
+      const int l = e / T, c = e - (e / T) * T;
+      if ((L * T) % NT == 0 || e < L * T) buf[c * LS + RW(l)] = pre[i];
+    }
+    if (f + 1 < NF) load_field(f + 1);
+#else
     const double2* src = src0 + (long long)f * a.in_sf;
     for (int e = tid; e < L * T; e += NT) {
       const int l = e / T, c = e - (e / T) * T;
       buf[c * LS + RW(l)] = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
     }
+#endif
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
 #pragma unroll
@@ -715,8 +771,8 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
     if (a.red_part) {
       const double2 tot = block_sum2(red, buf, tid);
       if (tid == 0) {
-        a.red_part[2 * blockIdx.x] = tot.x;
-        a.red_part[2 * blockIdx.x + 1] = tot.y;
+        a.red_part[2 * bid] = tot.x;
+        a.red_part[2 * bid + 1] = tot.y;
       }
     }
   }

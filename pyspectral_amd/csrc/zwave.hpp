@@ -33,40 +33,82 @@ template <int L> struct ZWave {
 // single-wavefront workgroup; the fence keeps LDS accesses in program order.
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
-// Half-spectrum entry kk of real z-pencil r = f*P + p (field f, position p).  The
-// recipes are the z-pass's: plain fields, except for the Smagorinsky strain rates.
-template <int MODE, int P>
-__device__ __forceinline__ double2 zw_real(const ZArgs& a, int r, long long pos0, int kk) {
-  const int f = r / P;
-  const long long pos = pos0 + r % P;
+// zin without a divergent branch: the index is clamped into the row and the loaded
+// bits are masked (an exact +0 outside the row), so the value is always used and
+// the loads of a pencil group are issued together instead of inside per-lane ifs.
+__device__ __forceinline__ double2 zw_mask(double2 v, bool keep) {
+  const unsigned long long m = keep ? ~0ull : 0ull;
+  return make_double2(__longlong_as_double(__double_as_longlong(v.x) & m),
+                      __longlong_as_double(__double_as_longlong(v.y) & m));
+}
+__device__ __forceinline__ double2 zw_in(const ZArgs& a, int f, long long pos, int k) {
+  // the field's buffer is chosen by (wave-uniform) selects, not a branch
+  const bool first = f < a.nin0;
+  const double2* base = first ? a.in0 : a.in1;
+  const long long sf = first ? a.in_sf0 : a.in_sf1;
+  const int H = first ? a.Hin0 : a.Hin1;
+  const int g = first ? f : f - a.nin0;
+  const double2 v = base[(long long)g * sf + pos * H + (k < H ? k : H - 1)];
+  return zw_mask(v, k < H);
+}
+
+// Input fields of each mode, and the recipes that build the real z-pencils from
+// them.  Smagorinsky: y-pass fields 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y,
+// 7 v_y, 8 w_y; pencils u v w S11 S22 S33 S12 S13 S23 (src/RHSfunctions.py:174-179).
+// Every other mode's pencil f is input field f.
+template <int MODE> struct ZFields {
+  static constexpr int n = MODE == Z_SMAG ? 9 : (MODE == Z_TM2 ? 6 : 3);
+};
+template <int MODE> __host__ __device__ constexpr bool zw_uses(int fr, int fi) {
+  if (MODE != Z_SMAG) return fr == fi;
+  switch (fr) {
+    case 0: return fi == 0;
+    case 1: return fi == 1;
+    case 2: return fi == 2;
+    case 3: return fi == 3;
+    case 4: return fi == 7;
+    case 5: return fi == 2;
+    case 6: return fi == 6 || fi == 4;
+    case 7: return fi == 0 || fi == 5;
+    default: return fi == 1 || fi == 8;
+  }
+}
+// is input field fi at position p read by the pencils of rows [q0, q0 + nq)?
+template <int MODE, int P, int NRI>
+__host__ __device__ constexpr bool zw_group_uses(int q0, int nq, int fi, int p) {
+  for (int r = 2 * q0; r < 2 * (q0 + nq) && r < NRI; ++r)
+    if (r % P == p && zw_uses<MODE>(r / P, fi)) return true;
+  return false;
+}
+// pencil r's half-spectrum entry from the raw field entries of its position
+template <int MODE>
+__device__ __forceinline__ double2 zw_recipe(int fr, const double2* raw, double kz) {
   if constexpr (MODE == Z_SMAG) {
-    // y-pass fields: 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y, 7 v_y, 8 w_y.
-    // pencils: u v w S11 S22 S33 S12 S13 S23 (src/RHSfunctions.py:174-179)
-    const double kz = a.kzscale * kk;
-    switch (f) {
-      case 0: return zin(a, 0, pos, kk);
-      case 1: return zin(a, 1, pos, kk);
-      case 2: return zin(a, 2, pos, kk);
-      case 3: return zin(a, 3, pos, kk);
-      case 4: return zin(a, 7, pos, kk);
-      case 5: return cscale(cmul_i<1>(zin(a, 2, pos, kk)), kz);
-      case 6: return cscale(cadd(zin(a, 6, pos, kk), zin(a, 4, pos, kk)), 0.5);
-      case 7: return cscale(cadd(cscale(cmul_i<1>(zin(a, 0, pos, kk)), kz), zin(a, 5, pos, kk)), 0.5);
-      default: return cscale(cadd(cscale(cmul_i<1>(zin(a, 1, pos, kk)), kz), zin(a, 8, pos, kk)), 0.5);
+    switch (fr) {
+      case 0: return raw[0];
+      case 1: return raw[1];
+      case 2: return raw[2];
+      case 3: return raw[3];
+      case 4: return raw[7];
+      case 5: return cscale(cmul_i<1>(raw[2]), kz);
+      case 6: return cscale(cadd(raw[6], raw[4]), 0.5);
+      case 7: return cscale(cadd(cscale(cmul_i<1>(raw[0]), kz), raw[5]), 0.5);
+      default: return cscale(cadd(cscale(cmul_i<1>(raw[1]), kz), raw[8]), 0.5);
     }
   } else {
-    return zin(a, f, pos, kk);
+    return raw[fr];
   }
 }
 
 // Entry k of complex pencil q = (real pencils 2q, 2q+1) of the Hermitian pair
-// Z = A + iB, A and B the half spectra (c2r semantics: real part only at k = 0 and L/2).
-template <int L, int MODE, int P, int NRI>
-__device__ __forceinline__ double2 zw_hermitian(const ZArgs& a, int q, long long pos0, int k) {
+// Z = A + iB, A and B the half spectra at kk = min(k, L - k) (c2r semantics: real
+// part only at k = 0 and L/2).  raw[p][fi]: input field fi of position p at kk.
+template <int L, int MODE, int P, int NRI, int NF>
+__device__ __forceinline__ double2 zw_hermitian(int q, int k, double kz, const double2 (&raw)[P][NF]) {
   const bool mirror = k > L / 2;
-  const int kk = mirror ? L - k : k;
-  const double2 A = zw_real<MODE, P>(a, 2 * q, pos0, kk);
-  const double2 B = (2 * q + 1 < NRI) ? zw_real<MODE, P>(a, 2 * q + 1, pos0, kk) : make_double2(0.0, 0.0);
+  const double2 A = zw_recipe<MODE>((2 * q) / P, raw[(2 * q) % P], kz);
+  const double2 B = (2 * q + 1 < NRI) ? zw_recipe<MODE>((2 * q + 1) / P, raw[(2 * q + 1) % P], kz)
+                                      : make_double2(0.0, 0.0);
   if (k == 0 || k == L / 2) return make_double2(A.x, B.x);
   return mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
 }
@@ -154,13 +196,28 @@ __global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__
   double2 x[NCI][R0];
 #pragma unroll
   for (int g0 = 0; g0 < NCI; g0 += G) {
+    // every raw entry the group's pencils read, loaded before any is used
+    constexpr int NF = ZFields<MODE>::n;
+    double2 raw[R0][P][NF];
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      const int k = l + 64 * m, kk = k > L / 2 ? L - k : k;
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int fi = 0; fi < NF; ++fi)
+          if (zw_group_uses<MODE, P, NRI>(g0, G, fi, p)) raw[m][p][fi] = zw_in(a, fi, pos0 + p, kk);
+    }
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
       const int q = g0 + gi;
       if (q < NCI) {
         double2 v[R0];
 #pragma unroll
-        for (int m = 0; m < R0; ++m) v[m] = zw_hermitian<L, MODE, P, NRI>(a, q, pos0, l + 64 * m);
+        for (int m = 0; m < R0; ++m) {
+          const int k = l + 64 * m, kk = k > L / 2 ? L - k : k;
+          v[m] = zw_hermitian<L, MODE, P, NRI, NF>(q, k, a.kzscale * kk, raw[m]);
+        }
         Dft<R0, +1>::run(v);
         buf[(gi * R0) * SUB + l] = v[0];
 #pragma unroll
