@@ -16,6 +16,10 @@
 
 namespace psx {
 
+#ifndef PSX_ZW_PREFETCH
+#define PSX_ZW_PREFETCH 1
+#endif
+
 template <int L> struct ZWave {
   static constexpr int R0 = L / 64;
   static constexpr bool ok = (L % 64 == 0) && (R0 == 1 || R0 == 2 || R0 == 3 || R0 == 4 || R0 == 6 ||
@@ -73,13 +77,18 @@ template <int MODE> __host__ __device__ constexpr bool zw_uses(int fr, int fi) {
     default: return fi == 1 || fi == 8;
   }
 }
-// is input field fi at position p read by the pencils of rows [q0, q0 + nq)?
-template <int MODE, int P, int NRI>
-__host__ __device__ constexpr bool zw_group_uses(int q0, int nq, int fi, int p) {
-  for (int r = 2 * q0; r < 2 * (q0 + nq) && r < NRI; ++r)
-    if (r % P == p && zw_uses<MODE>(r / P, fi)) return true;
-  return false;
-}
+// bit fi of bits[g][p]: input field fi of position p is read by the pencils of
+// complex rows [g*G, g*G + G) -- a table built at compile time
+template <int MODE, int P, int NRI, int G> struct ZGroupUse {
+  static constexpr int NG = ((NRI + 1) / 2 + G - 1) / G;
+  unsigned bits[NG][P];
+  constexpr ZGroupUse() : bits{} {
+    for (int g = 0; g < NG; ++g)
+      for (int r = 2 * g * G; r < 2 * (g + 1) * G && r < NRI; ++r)
+        for (int fi = 0; fi < ZFields<MODE>::n; ++fi)
+          if (zw_uses<MODE>(r / P, fi)) bits[g][r % P] |= 1u << fi;
+  }
+};
 // pencil r's half-spectrum entry from the raw field entries of its position
 template <int MODE>
 __device__ __forceinline__ double2 zw_recipe(int fr, const double2* raw, double kz) {
@@ -193,12 +202,11 @@ __global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__
   for (int av = 0; av < R0; ++av) t1[av] = tw[av * l];
 
   // ---- inverse transforms; lane l keeps x_n, n = l + 64 m, of every pencil
-  double2 x[NCI][R0];
-#pragma unroll
-  for (int g0 = 0; g0 < NCI; g0 += G) {
-    // every raw entry the group's pencils read, loaded before any is used
-    constexpr int NF = ZFields<MODE>::n;
-    double2 raw[R0][P][NF];
+  constexpr int NF = ZFields<MODE>::n;
+  constexpr ZGroupUse<MODE, P, NRI, G> use{};
+  // every raw entry a pencil group reads, loaded before any is used; with
+  // PSX_ZW_PREFETCH the next group's loads are issued before this group's LDS passes
+  auto load_group = [&](int g0, double2 (&raw)[R0][P][NF]) {
 #pragma unroll
     for (int m = 0; m < R0; ++m) {
       const int k = l + 64 * m, kk = k > L / 2 ? L - k : k;
@@ -206,8 +214,18 @@ __global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__
       for (int p = 0; p < P; ++p)
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi)
-          if (zw_group_uses<MODE, P, NRI>(g0, G, fi, p)) raw[m][p][fi] = zw_in(a, fi, pos0 + p, kk);
+          if ((use.bits[g0 / G][p] >> fi) & 1u) raw[m][p][fi] = zw_in(a, fi, pos0 + p, kk);
     }
+    // keep the group's loads together: nothing is scheduled across this point
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  double2 x[NCI][R0];
+  double2 raw[2][R0][P][NF];
+  load_group(0, raw[0]);
+#pragma unroll
+  for (int g0 = 0; g0 < NCI; g0 += G) {
+    const int cur = (g0 / G) & 1;
+    if (!PSX_ZW_PREFETCH && g0 > 0) load_group(g0, raw[cur]);
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
       const int q = g0 + gi;
@@ -216,7 +234,7 @@ __global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__
 #pragma unroll
         for (int m = 0; m < R0; ++m) {
           const int k = l + 64 * m, kk = k > L / 2 ? L - k : k;
-          v[m] = zw_hermitian<L, MODE, P, NRI, NF>(q, k, a.kzscale * kk, raw[m]);
+          v[m] = zw_hermitian<L, MODE, P, NRI, NF>(q, k, a.kzscale * kk, raw[cur][m]);
         }
         Dft<R0, +1>::run(v);
         buf[(gi * R0) * SUB + l] = v[0];
@@ -224,6 +242,7 @@ __global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__
         for (int av = 1; av < R0; ++av) buf[(gi * R0 + av) * SUB + l] = cmul(v[av], cconj(t1[av]));
       }
     }
+    if (PSX_ZW_PREFETCH && g0 + G < NCI) load_group(g0 + G, raw[cur ^ 1]);
     wave_sync();
     zw_sub64<L, +1>(buf, tw, l, NCI - g0 < G ? NCI - g0 : G);
 #pragma unroll
