@@ -99,6 +99,22 @@ int psx_stats(psx_context* ctx, const double* uhat, double* out_dev);
 int psx_spectrum_bins(const psx_context* ctx);
 int psx_spectrum(psx_context* ctx, const double* uhat, double* spec_dev, int resolved);
 
+/* Filters of a state-layout field (3, N1, N2, N3/2+1).  PSX_FILTER_SHARP is
+ * gridclass.Gf / utilitiesClass.myFilter (src/Classes.py:584-588, 1271-1279): keep
+ * |i1|, |i2| < kc and i3 < kc (signed indices, kc from psx_params).  PSX_FILTER_BOX
+ * (no counterpart in the reference) is the top-hat of width delta on every axis,
+ * transfer function prod_i sin(k_i delta/2) / (k_i delta/2). */
+#define PSX_FILTER_SHARP 0
+#define PSX_FILTER_BOX 1
+int psx_filter(psx_context* ctx, const double* uhat, double* out, int filter, double delta);
+
+/* utilitiesClass.computeSGS_DNS (src/Classes.py:861-929): with ubar = G uhat,
+ * tau_hat_ij = G FFT(u_i u_j) - FFT(ubar_i ubar_j) (3/2-rule products) and
+ * w0_i = -i k_j tau_ij - k_i (k_m k_j tau_mj)/k^2.  w0_out: (3, N1, N2, N3/2+1) or
+ * NULL; tau_out: (6, N1, N2, N3/2+1), order 11 22 33 12 13 23, or NULL.  Works on
+ * a context of any model; its work buffers are allocated on the first call. */
+int psx_sgs_dns(psx_context* ctx, const double* uhat, double* w0_out, double* tau_out, int filter, double delta);
+
 /* The dynamic tau-model's coefficient from the last RHS evaluation (main.tau,
  * src/RHSfunctions.py:857-861), copied to host memory (synchronizes the stream):
  * out[0] tau as computed, out[1] tau clamped at 0 (the value used), out[2..7] the

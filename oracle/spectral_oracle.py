@@ -530,6 +530,39 @@ def test_filter_2x(g: Grid, kf: int, uhat_pad):
     return f1[:, None, None] * f2[None, :, None] * f3[None, None, :] * uhat_pad
 
 
+def box_transfer(g: Grid, delta: float) -> np.ndarray:
+    """Transfer function of the top-hat (box) filter of width ``delta`` on every axis,
+    G(k) = prod_i sin(k_i delta/2) / (k_i delta/2).  Not in the reference (its filters
+    are sharp cutoffs); the filtered-DNS training data use it."""
+    s = lambda k: np.sinc(k * delta / (2.0 * np.pi))  # np.sinc(x) = sin(pi x)/(pi x)
+    return s(g.k1) * s(g.k2) * s(g.k3)
+
+
+def filter_transfer(g: Grid, kind: str = "sharp", delta: float | None = None) -> np.ndarray:
+    """``grid.Gf`` (sharp cutoff at kc, ``src/Classes.py:584-588``; the same mask as
+    ``myFilter``, ``src/Classes.py:1271-1279``) or the box filter."""
+    if kind == "sharp":
+        return g.Gf
+    if kind == "box":
+        return box_transfer(g, float(delta))
+    raise ValueError(kind)
+
+
+def sgs_dns(st: State, fft: FFTBackend, kind: str = "sharp", delta: float | None = None):
+    """``utilitiesClass.computeSGS_DNS`` (``src/Classes.py:861-929``) on the state's
+    ``uhat, vhat, what``: with ubar = G uhat, tau_hat_ij = G FFT(u_i u_j) - FFT(ubar_i ubar_j)
+    (3/2-rule products, ``:886-909``), and w0 by the same divergence/projection form as
+    the Smagorinsky closure (``:911-926``).  Returns (w0 (3 arrays), tau_hat (6 arrays,
+    order 11 22 33 12 13 23))."""
+    g = st.grid
+    G = filter_transfer(g, kind, delta)
+    u = (st.uhat, st.vhat, st.what)
+    N = _products_32(st, fft, *u)
+    Nf = _products_32(st, fft, *(G * a for a in u))
+    tau = [G * a - b for a, b in zip(N, Nf)]
+    return list(sgs_divergence(g, tau)), tau
+
+
 def _hermitian_inner(a, b, n3h, Nt):
     """sum over the half spectrum with weight 2 for 0 < k3 < N3/2 and 1 for k3 = 0
     (the weighting of ``src/RHSfunctions.py:822-852``)."""

@@ -215,6 +215,44 @@ class utilitiesClass:
         s = main.solver.spectrum(main.Ud, resolved=True).cpu().numpy()
         return np.arange(s.shape[0], dtype=float), s
 
+    def computeSGS_DNS(self, main, grid, myFFT):
+        """``computeSGS_DNS`` (``src/Classes.py:861-929``): the subgrid term of the sharp
+        filter grid.Gf (cutoff kc) from the resolved field; returns w0_u, w0_v, w0_w of
+        shape (N1, N2, N3/2+1, 1) as the reference does."""
+        w0, _ = main.solver.sgs_dns(main.Ud, "sharp")
+        w = w0.cpu().numpy()
+        return w[0][..., None], w[1][..., None], w[2][..., None]
+
+    def myFilter(self, uhat, kc):
+        """``myFilter`` (``src/Classes.py:1271-1279``): the sharp cutoff of one field at kc."""
+        solver = _filter_solver(uhat.shape)
+        import torch
+
+        x = torch.zeros(solver.shape, dtype=torch.complex128, device=solver.device)
+        x[0] = torch.from_numpy(np.ascontiguousarray(uhat)).to(solver.device)
+        old = solver.params.kc
+        solver.set_params(kc=int(kc))
+        try:
+            out = solver.filter(x, "sharp")
+        finally:
+            solver.set_params(kc=old)
+        return out[0].cpu().numpy()
+
+
+_FILTER_SOLVERS = {}
+
+
+def _filter_solver(shape):
+    """A DNS context of the field's grid, used for the filters of utilitiesClass."""
+    from .solver import SpectralSolver
+
+    N1, N2, Nc = (int(n) for n in shape[:3])
+    N3 = 2 * (Nc - 1)
+    key = (N1, N2, N3)
+    if key not in _FILTER_SOLVERS:
+        _FILTER_SOLVERS[key] = SpectralSolver(N1, N2, N3, 0, nu=0.0)
+    return _FILTER_SOLVERS[key]
+
 
 def _warn(msg):
     sys.stdout.write(msg + "\n")

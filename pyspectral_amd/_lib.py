@@ -25,7 +25,10 @@ EXPORTS = (
     "psx_create", "psx_destroy", "psx_set_stream", "psx_set_params", "psx_last_error",
     "psx_scratch_bytes", "psx_rhs", "psx_rk_stage", "psx_compute_dt", "psx_step", "psx_step_graph", "psx_stats",
     "psx_spectrum_bins", "psx_spectrum", "psx_tau", "psx_timing_begin", "psx_timing_end",
+    "psx_filter", "psx_sgs_dns",
 )
+
+PSX_FILTER_SHARP, PSX_FILTER_BOX = 0, 1
 
 KERNEL_NAMES = ("x_inverse", "y_inverse", "z_pass", "y_forward", "x_forward_epilogue",
                 "cfl_x", "cfl_y", "cfl_z", "other")
@@ -80,6 +83,8 @@ def load() -> C.CDLL:
         "psx_tau": (i, [vp, C.POINTER(C.c_double)]),
         "psx_timing_begin": (i, [vp]),
         "psx_timing_end": (i, [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "psx_filter": (i, [vp, dp, dp, i, d]),
+        "psx_sgs_dns": (i, [vp, dp, dp, dp, i, d]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

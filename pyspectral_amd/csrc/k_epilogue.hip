@@ -73,4 +73,26 @@ hipError_t launch_epilogue(int L, int ns1, int epi, bool w0, const EpiArgs& a, c
   return hipErrorInvalidValue;
 }
 
+unsigned sgs_blocks(int L, const SgsArgs& a) {
+  const int T = epi_T(L);
+  unsigned main_blocks = (unsigned)(a.N2 * ((a.H + T - 1) / T));
+  long long rest = (long long)a.N1 * a.N2 * (a.Nc - a.H);
+  long long b = (rest + NT - 1) / NT;
+  return main_blocks + (unsigned)(b < 1 ? 0 : (b < 1024 ? b : 1024));
+}
+
+hipError_t launch_xforward_sgs(int L, int ns1, const SgsArgs& a, const double2* tw, hipStream_t s) {
+  const dim3 grid(sgs_blocks(L, a));
+  switch (L * 4096 + ns1) {
+#define PSX_CASE(LL, NN)                                                                 \
+  case LL * 4096 + NN:                                                                   \
+    hipLaunchKernelGGL((xforward_sgs<LL, NN, epi_T(LL)>), grid, dim3(NT), 0, s, a, tw); \
+    return hipGetLastError();
+    PSX_PAIRS32(PSX_CASE)
+#undef PSX_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace psx

@@ -206,13 +206,14 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
 }
 
 // ------------------------------------------------------------------ fused z pass
-enum ZMode { Z_DNS = 0, Z_SMAG = 1, Z_TM2 = 2, Z_DT = 3 };
+enum ZMode { Z_DNS = 0, Z_SMAG = 1, Z_TM2 = 2, Z_DT = 3, Z_SGS = 4 };
 
 template <int MODE> struct ZShape;
 template <> struct ZShape<Z_DNS> { static constexpr int nin = 3, nout = 6; };
 template <> struct ZShape<Z_SMAG> { static constexpr int nin = 9, nout = 12; };
 template <> struct ZShape<Z_TM2> { static constexpr int nin = 6, nout = 9; };
 template <> struct ZShape<Z_DT> { static constexpr int nin = 3, nout = 0; };
+template <> struct ZShape<Z_SGS> { static constexpr int nin = 6, nout = 12; };
 
 // A real z-pencil's half spectrum is built as sum_t coef_t (i k3)^{dz_t} field_{f_t}.
 struct Recipe {
@@ -294,6 +295,20 @@ __device__ __forceinline__ void zpoint(const ZArgs& a, const double* in, double*
     out[9] = m2nut * S12;
     out[10] = m2nut * S13;
     out[11] = m2nut * S23;
+  } else if constexpr (MODE == Z_SGS) {
+    // u_i u_j of the velocity (0..5) and of the filtered velocity (6..11), order
+    // 11 22 33 12 13 23 (src/Classes.py:886-909)
+    const double s = a.out_scale;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const double u = in[3 * h], v = in[3 * h + 1], w = in[3 * h + 2];
+      out[6 * h + 0] = s * (u * u);
+      out[6 * h + 1] = s * (v * v);
+      out[6 * h + 2] = s * (w * w);
+      out[6 * h + 3] = s * (u * v);
+      out[6 * h + 4] = s * (u * w);
+      out[6 * h + 5] = s * (v * w);
+    }
   } else if constexpr (MODE == Z_TM2) {
     // u_j * (PL_q)_i, stored at index 3 i + j (src/RHSfunctions.py:325-335)
     const double s = a.out_scale;
@@ -773,6 +788,106 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
       if (tid == 0) {
         a.red_part[2 * bid] = tot.x;
         a.red_part[2 * bid + 1] = tot.y;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ filtered-DNS stress
+// Filters on the N grid.  PSX_FILTER_SHARP is grid.Gf (src/Classes.py:584-588):
+// 1 where the signed indices satisfy |i1|, |i2| < kc and i3 < kc.  PSX_FILTER_BOX is
+// the top-hat of width delta: prod_i sin(k_i delta/2)/(k_i delta/2).
+enum FilterKind { F_SHARP = 0, F_BOX = 1 };
+__device__ __forceinline__ double sinc_half(double k, double delta) {
+  const double x = 0.5 * k * delta;
+  return x == 0.0 ? 1.0 : sin(x) / x;
+}
+__device__ __forceinline__ double filter_gain(int kind, int s1, int s2, int kz, double kc, double delta,
+                                              const double* ks) {
+  if (kind == F_SHARP) return (abs(s1) < kc && abs(s2) < kc && kz < kc) ? 1.0 : 0.0;
+  return sinc_half(ks[0] * s1, delta) * sinc_half(ks[1] * s2, delta) * sinc_half(ks[2] * kz, delta);
+}
+
+struct SgsArgs {
+  const double2* in;   // W layout: 12 fields, products of u (0..5) and of G u (6..11)
+  long long in_sf;
+  int N1, N2, N3, Nc, H;
+  double2* tau;        // state layout, 6 fields (11 22 33 12 13 23)
+  int filter;
+  double kc, delta;
+  double ks[3];
+};
+
+// x-forward of the twelve product fields and tau_hat_mn = G N_mn - N~_mn into the state
+// layout (src/Classes.py:904-909); the 3/2 truncation zeroes index N1/2 and k3 >= H.
+template <int L, int NS1, int T>
+__global__ __launch_bounds__(NT) void xforward_sgs(SgsArgs a, const double2* __restrict__ tw) {
+  constexpr int LS = Row<L>::LS;
+  constexpr int IT = (NS1 * T + NT - 1) / NT;
+  __shared__ double2 buf[T * LS];
+  const int tid = threadIdx.x;
+  const int ct = (a.H + T - 1) / T;
+  const int nmain = a.N2 * ct;
+  const long long plane = (long long)a.N2 * a.Nc;
+  const long long fstride = (long long)a.N1 * plane;
+  if ((int)blockIdx.x >= nmain) {
+    const int nrest = a.Nc - a.H;
+    const long long total = (long long)a.N1 * a.N2 * nrest;
+    for (long long e = (long long)(blockIdx.x - nmain) * NT + tid; e < total;
+         e += (long long)(gridDim.x - nmain) * NT) {
+      const int kz = a.H + (int)(e % nrest);
+      const long long r = e / nrest;
+      const long long idx = (r / a.N2) * plane + (r % a.N2) * a.Nc + kz;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) a.tau[m * fstride + idx] = make_double2(0.0, 0.0);
+    }
+    return;
+  }
+  const int ky = blockIdx.x / ct;
+  const int c0 = (blockIdx.x - ky * ct) * T;
+  const long long in_sl = (long long)a.N2 * a.H;
+  const double2* src0 = a.in + (long long)ky * a.H + c0;
+  const int s2 = signed_freq(ky, a.N2);
+  double2 acc[IT][6];
+#pragma unroll
+  for (int it = 0; it < IT; ++it)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) acc[it][m] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int f = 0; f < 12; ++f) {
+    const double2* src = src0 + (long long)f * a.in_sf;
+    for (int e = tid; e < L * T; e += NT) {
+      const int l = e / T, c = e - (e / T) * T;
+      buf[c * LS + RW(l)] = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_batch<L, T, NT, LS, -1>(buf, tw, tid);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (e < NS1 * T) {
+        const int kx = e / T, c = e - (e / T) * T;
+        const int p = pad_slot(kx, NS1, L, 0);
+        if (p >= 0 && c0 + c < a.H) {
+          const double2 v = buf[c * LS + RW(p)];
+          const double g = f < 6 ? filter_gain(a.filter, signed_freq(kx, NS1), s2, c0 + c, a.kc, a.delta, a.ks)
+                                 : -1.0;
+          acc[it][f % 6] = cadd(acc[it][f % 6], cscale(v, g));
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * NT;
+    if (e < NS1 * T) {
+      const int kx = e / T, c = e - (e / T) * T;
+      const int kz = c0 + c;
+      if (kz < a.H) {
+        const long long idx = (long long)kx * plane + (long long)ky * a.Nc + kz;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) a.tau[m * fstride + idx] = acc[it][m];
       }
     }
   }

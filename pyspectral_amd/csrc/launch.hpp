@@ -21,7 +21,7 @@ constexpr int epi_T(int L) { return L <= 64 ? 8 : (L <= 192 ? PSX_ET_SMALL : (L 
 // z pass: positions per block (pairing two positions, or two fields when P = 1)
 // and threads per block, sized to the first (radix-8 or -4) stage's butterflies.
 template <int MODE> constexpr int z_P(int L) {
-  return MODE == Z_SMAG ? 1 : (MODE == Z_TM2 ? 1 : 2);
+  return (MODE == Z_SMAG || MODE == Z_TM2 || MODE == Z_SGS) ? 1 : 2;
 }
 template <int MODE> constexpr int z_NT(int L) {
   constexpr int nin = ZShape<MODE>::nin, nout = ZShape<MODE>::nout;
@@ -43,6 +43,9 @@ int zpass_positions(int L, int mode);
 hipError_t launch_epilogue(int L, int ns1, int epi, bool w0, const EpiArgs& a, const double2* tw, hipStream_t s);
 unsigned epilogue_blocks(int L, const EpiArgs& a);
 hipError_t launch_pl_pass(int L, bool dyn, const PLArgs& a, const double2* tw, hipStream_t s);
+// filtered-DNS stress x-forward (3/2 pairs only)
+hipError_t launch_xforward_sgs(int L, int ns1, const SgsArgs& a, const double2* tw, hipStream_t s);
+unsigned sgs_blocks(int L, const SgsArgs& a);
 unsigned pl_pass_blocks(int L, const PLArgs& a);
 
 }  // namespace psx

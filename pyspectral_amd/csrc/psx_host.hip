@@ -18,10 +18,12 @@ hipError_t launch_zpass_dns(int L, const ZArgs& a, const double2* tw, hipStream_
 hipError_t launch_zpass_smag(int L, const ZArgs& a, const double2* tw, hipStream_t s);
 hipError_t launch_zpass_tm2(int L, const ZArgs& a, const double2* tw, hipStream_t s);
 hipError_t launch_zpass_dt(int L, const ZArgs& a, const double2* tw, hipStream_t s);
+hipError_t launch_zpass_sgs(int L, const ZArgs& a, const double2* tw, hipStream_t s);
 int zpass_positions_dns(int L);
 int zpass_positions_smag(int L);
 int zpass_positions_tm2(int L);
 int zpass_positions_dt(int L);
+int zpass_positions_sgs(int L);
 
 int zpass_positions(int L, int mode) {
   switch (mode) {
@@ -29,6 +31,7 @@ int zpass_positions(int L, int mode) {
     case Z_SMAG: return zpass_positions_smag(L);
     case Z_TM2: return zpass_positions_tm2(L);
     case Z_DT: return zpass_positions_dt(L);
+    case Z_SGS: return zpass_positions_sgs(L);
   }
   return 0;
 }
@@ -39,6 +42,7 @@ hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipS
     case Z_SMAG: return launch_zpass_smag(L, a, tw, s);
     case Z_TM2: return launch_zpass_tm2(L, a, tw, s);
     case Z_DT: return launch_zpass_dt(L, a, tw, s);
+    case Z_SGS: return launch_zpass_sgs(L, a, tw, s);
   }
   return hipErrorInvalidValue;
 }
@@ -72,6 +76,11 @@ struct psx_context {
   double2* plql = nullptr;
   double2* uf = nullptr;
   double2* glf = nullptr;
+  // filtered-DNS stress work buffers, allocated on first use
+  double2* sgsA = nullptr;
+  double2* sgsB = nullptr;
+  double2* sgsU = nullptr;
+  double2* sgsT = nullptr;
   size_t nC = 0;
   double* partLE = nullptr;
   double* partA = nullptr;
@@ -725,6 +734,10 @@ int psx_destroy(psx_context* c) {
   hipFree(c->plql);
   hipFree(c->uf);
   hipFree(c->glf);
+  hipFree(c->sgsA);
+  hipFree(c->sgsB);
+  hipFree(c->sgsU);
+  hipFree(c->sgsT);
   hipFree(c->partLE);
   hipFree(c->partA);
   hipFree(c->partB);
@@ -853,6 +866,111 @@ int psx_stats(psx_context* c, const double* uhat, double* out_dev) {
     const double Nt = (double)c->N1 * c->N2 * c->N3;
     hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, c->stream, c->part, nblk, out_dev, 1.0 / Nt);
     check(c, hipGetLastError(), "stats_final");
+  });
+}
+
+namespace {
+void check_filter(psx_context* c, int filter, double delta) {
+  if (filter != PSX_FILTER_SHARP && filter != PSX_FILTER_BOX) fail(c, PSX_E_ARG, "unknown filter");
+  if (filter == PSX_FILTER_BOX && !(delta > 0.0)) fail(c, PSX_E_ARG, "box filter width must be positive");
+}
+
+void launch_filter(psx_context* c, const double2* in, double2* out, int filter, double delta) {
+  const long long total = 3LL * c->N1 * c->N2 * c->Nc;
+  const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL(apply_filter, dim3(blocks), dim3(256), 0, c->stream, in, out, 3, c->N1, c->N2, c->Nc, filter,
+                     (double)c->p.kc, delta, 2.0 * kPi / c->p.L1, 2.0 * kPi / c->p.L2, 2.0 * kPi / c->p.L3);
+  check(c, hipGetLastError(), "apply_filter");
+}
+}  // namespace
+
+int psx_filter(psx_context* c, const double* uhat, double* out, int filter, double delta) {
+  return guarded(c, [&] {
+    if (!uhat || !out) fail(c, PSX_E_ARG, "null pointer");
+    check_filter(c, filter, delta);
+    launch_filter(c, (const double2*)uhat, (double2*)out, filter, delta);
+  });
+}
+
+int psx_sgs_dns(psx_context* c, const double* uhat, double* w0_out, double* tau_out, int filter, double delta) {
+  return guarded(c, [&] {
+    if (!uhat) fail(c, PSX_E_ARG, "null pointer");
+    check_filter(c, filter, delta);
+    const int M1 = c->M1, M2 = c->M2, M3 = c->M3, H = c->H;
+    for (int L : {M1, M2, M3})
+      if (!length_supported(L)) fail(c, PSX_E_UNSUPPORTED, "3/2-rule length " + std::to_string(L) + " not supported");
+    const size_t state = (size_t)3 * c->N1 * c->N2 * c->Nc;
+    if (!c->sgsA) {
+      const size_t MMH = (size_t)M1 * M2 * H, MNH = (size_t)M1 * c->N2 * H;
+      const size_t nA = std::max(6 * MNH, 12 * MMH), nB = std::max(6 * MMH, 12 * MNH);
+      check(c, hipMalloc(&c->sgsA, nA * sizeof(double2)), "hipMalloc(sgsA)");
+      check(c, hipMalloc(&c->sgsB, nB * sizeof(double2)), "hipMalloc(sgsB)");
+      check(c, hipMalloc(&c->sgsU, state * sizeof(double2)), "hipMalloc(sgsU)");
+      check(c, hipMalloc(&c->sgsT, 2 * state * sizeof(double2)), "hipMalloc(sgsT)");
+      c->scratch_bytes += (long long)(nA + nB + 3 * state) * sizeof(double2);
+    }
+    double2* A = c->sgsA;
+    double2* B = c->sgsB;
+    const double2* u = (const double2*)uhat;
+    launch_filter(c, u, c->sgsU, filter, delta);
+    {
+      const int jin[3] = {0, 1, 2}, jp0[3] = {0, 1, 2}, jp1[3] = {3, 4, 5};
+      x_inverse(c, u, A, M1, H, 0, 3, jin, jp0, nullptr, K_OTHER);
+      x_inverse(c, c->sgsU, A, M1, H, 0, 3, jin, jp1, nullptr, K_OTHER);
+      const int jin6[6] = {0, 1, 2, 3, 4, 5};
+      y_inverse(c, A, B, M1, M2, H, 0, 6, jin6, jin6, nullptr, K_OTHER);
+    }
+    {
+      ZArgs z;
+      std::memset(&z, 0, sizeof(z));
+      z.in0 = B;
+      z.in_sf0 = (long long)M1 * M2 * H;
+      z.Hin0 = H;
+      z.nin0 = 6;
+      z.out = A;
+      z.out_sf = (long long)M1 * M2 * H;
+      z.Hout = H;
+      z.npos = M1 * M2;
+      z.kzscale = 2.0 * kPi / c->p.L3;
+      const double Nt = (double)c->N1 * c->N2 * c->N3;
+      z.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
+      for (int r = 0; r < 6; ++r) z.rec[r] = Recipe{1, {r, 0}, {1.0, 0.0}, {0, 0}};
+      const int P = zpass_positions(M3, Z_SGS);
+      if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
+      const double2* tw = twiddles(c, M3);
+      TimeScope ts(c, K_OTHER);
+      check(c, launch_zpass(M3, Z_SGS, z, tw, c->stream), "z_pass(sgs)");
+    }
+    y_forward(c, A, B, M1, M2, H, 0, 12);
+    double2* tau = tau_out ? (double2*)tau_out : c->sgsT;
+    {
+      SgsArgs s;
+      std::memset(&s, 0, sizeof(s));
+      s.in = B;
+      s.in_sf = (long long)M1 * c->N2 * H;
+      s.N1 = c->N1;
+      s.N2 = c->N2;
+      s.N3 = c->N3;
+      s.Nc = c->Nc;
+      s.H = H;
+      s.tau = tau;
+      s.filter = filter;
+      s.kc = (double)c->p.kc;
+      s.delta = delta;
+      s.ks[0] = 2.0 * kPi / c->p.L1;
+      s.ks[1] = 2.0 * kPi / c->p.L2;
+      s.ks[2] = 2.0 * kPi / c->p.L3;
+      const double2* tw = twiddles(c, M1);
+      TimeScope ts(c, K_OTHER);
+      check(c, launch_xforward_sgs(M1, c->N1, s, tw, c->stream), "x_forward(sgs)");
+    }
+    if (w0_out) {
+      const long long total = (long long)c->N1 * c->N2 * c->Nc;
+      const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 8192);
+      hipLaunchKernelGGL(sgs_w0, dim3(blocks), dim3(256), 0, c->stream, (const double2*)tau, (double2*)w0_out,
+                         c->N1, c->N2, c->Nc, 2.0 * kPi / c->p.L1, 2.0 * kPi / c->p.L2, 2.0 * kPi / c->p.L3);
+      check(c, hipGetLastError(), "sgs_w0");
+    }
   });
 }
 

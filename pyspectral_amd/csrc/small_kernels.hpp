@@ -248,3 +248,54 @@ __global__ void advance_time(double* t, const double* dt) {
 }
 
 }  // namespace psx
+
+namespace psx {
+
+// out = G(k) in over nf fields of the state layout (filters: filter_gain)
+__global__ void apply_filter(const double2* __restrict__ in, double2* __restrict__ out, int nf, int N1, int N2,
+                             int Nc, int kind, double kc, double delta, double ks0, double ks1, double ks2) {
+  const double ks[3] = {ks0, ks1, ks2};
+  const long long total = (long long)nf * N1 * N2 * Nc;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(e % Nc);
+    const long long r = e / Nc;
+    const int ky = (int)(r % N2);
+    const int kx = (int)((r / N2) % N1);
+    out[e] = cscale(in[e], filter_gain(kind, signed_freq(kx, N1), signed_freq(ky, N2), kz, kc, delta, ks));
+  }
+}
+
+// w0_i = -i k_j tau_ij + i k_i (i/k^2 k_m k_j tau_mj), the divergence/projection form
+// of src/Classes.py:911-918 (tau order 11 22 33 12 13 23, state layout)
+__global__ void sgs_w0(const double2* __restrict__ tau, double2* __restrict__ w0, int N1, int N2, int Nc,
+                       double ks0, double ks1, double ks2) {
+  const long long fs = (long long)N1 * N2 * Nc;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < fs;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(e % Nc);
+    const long long r = e / Nc;
+    const int ky = (int)(r % N2);
+    const int kx = (int)(r / N2);
+    const double k1 = ks0 * signed_freq(kx, N1), k2 = ks1 * signed_freq(ky, N2), k3 = ks2 * kz;
+    const double2 t11 = tau[e], t22 = tau[fs + e], t33 = tau[2 * fs + e];
+    const double2 t12 = tau[3 * fs + e], t13 = tau[4 * fs + e], t23 = tau[5 * fs + e];
+    const double kinv = 1.0 / (k1 * k1 + k2 * k2 + k3 * k3 + 1e-50);
+    double2 kk = cscale(t11, k1 * k1);
+    kk = cadd(kk, cscale(t22, k2 * k2));
+    kk = cadd(kk, cscale(t33, k3 * k3));
+    kk = cadd(kk, cscale(t12, 2.0 * k1 * k2));
+    kk = cadd(kk, cscale(t13, 2.0 * k1 * k3));
+    kk = cadd(kk, cscale(t23, 2.0 * k2 * k3));
+    // i k_i * (i kinv kk) = -k_i kinv kk
+    const double2 p = cscale(kk, -kinv);
+    const double2 d1 = cadd(cadd(cscale(t11, k1), cscale(t12, k2)), cscale(t13, k3));
+    const double2 d2 = cadd(cadd(cscale(t12, k1), cscale(t22, k2)), cscale(t23, k3));
+    const double2 d3 = cadd(cadd(cscale(t13, k1), cscale(t23, k2)), cscale(t33, k3));
+    w0[e] = cadd(cmul_i<-1>(d1), cscale(p, k1));
+    w0[fs + e] = cadd(cmul_i<-1>(d2), cscale(p, k2));
+    w0[2 * fs + e] = cadd(cmul_i<-1>(d3), cscale(p, k3));
+  }
+}
+
+}  // namespace psx

@@ -61,7 +61,7 @@ __device__ __forceinline__ double2 zw_in(const ZArgs& a, int f, long long pos, i
 // 7 v_y, 8 w_y; pencils u v w S11 S22 S33 S12 S13 S23 (src/RHSfunctions.py:174-179).
 // Every other mode's pencil f is input field f.
 template <int MODE> struct ZFields {
-  static constexpr int n = MODE == Z_SMAG ? 9 : (MODE == Z_TM2 ? 6 : 3);
+  static constexpr int n = MODE == Z_SMAG ? 9 : ((MODE == Z_TM2 || MODE == Z_SGS) ? 6 : 3);
 };
 template <int MODE> __host__ __device__ constexpr bool zw_uses(int fr, int fi) {
   if (MODE != Z_SMAG) return fr == fi;

@@ -153,6 +153,33 @@ class SpectralSolver:
         _lib.check(self.ctx, self.lib.psx_step_graph(self.ctx, _ptr(q), _ptr(qs), _ptr(dt), _ptr(t)),
                    "psx_step_graph")
 
+    _FILTERS = {"sharp": _lib.PSX_FILTER_SHARP, "box": _lib.PSX_FILTER_BOX}
+
+    def filter(self, uhat: torch.Tensor, kind: str = "sharp", delta: float = 0.0, out=None) -> torch.Tensor:
+        """G * uhat: ``kind="sharp"`` is grid.Gf / myFilter at the context's kc; ``"box"``
+        the top-hat of width ``delta``."""
+        self._state_arg(uhat, "uhat")
+        out = self.empty_state() if out is None else self._state_arg(out, "out")
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_filter(self.ctx, _ptr(uhat), _ptr(out), self._FILTERS[kind], float(delta)),
+                   "psx_filter")
+        return out
+
+    def sgs_dns(self, uhat: torch.Tensor, kind: str = "sharp", delta: float = 0.0, *, w0=True, tau=False):
+        """``computeSGS_DNS``: (w0 (3, N1, N2, Nc) or None, tau_hat (6, N1, N2, Nc) or None)."""
+        self._state_arg(uhat, "uhat")
+        w = self.empty_state() if w0 is True else (None if w0 is None or w0 is False else self._state_arg(w0, "w0"))
+        t = None
+        if tau is True:
+            t = torch.empty((6,) + self.shape[1:], dtype=torch.complex128, device=self.device)
+        elif tau is not None and tau is not False:
+            t = tau
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_sgs_dns(self.ctx, _ptr(uhat), _ptr(w) if w is not None else None,
+                                                  _ptr(t) if t is not None else None, self._FILTERS[kind],
+                                                  float(delta)), "psx_sgs_dns")
+        return w, t
+
     def stats(self, uhat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """(E, Omega, E_resolved) on the device."""
         self._state_arg(uhat, "uhat")
