@@ -33,6 +33,7 @@ extern "C" {
 #define PSX_MODEL_TMODEL 2       /* computeRHS_tmodel (src/RHSfunctions.py:240-363)    */
 #define PSX_MODEL_STATIC_TAU 3   /* computeRHS_staumodel (src/RHSfunctions.py:366-497) */
 #define PSX_MODEL_DYNAMIC_TAU 4  /* computeRHS_Dtaumodel (src/RHSfunctions.py:592-878) */
+#define PSX_MODEL_DSMAG 5        /* computeRHS_DSMAG, turb_model 7 (src/RHSfunctions.py:1663-1862) */
 
 typedef struct psx_params {
   double nu;              /* viscosity (main.nu)                                     */
@@ -118,7 +119,8 @@ int psx_sgs_dns(psx_context* ctx, const double* uhat, double* w0_out, double* ta
 /* The dynamic tau-model's coefficient from the last RHS evaluation (main.tau,
  * src/RHSfunctions.py:857-861), copied to host memory (synchronizes the stream):
  * out[0] tau as computed, out[1] tau clamped at 0 (the value used), out[2..7] the
- * reductions LE, PLQLE_filt, filt_PLQLE as (re, im) pairs. */
+ * reductions LE, PLQLE_filt, filt_PLQLE as (re, im) pairs.  For the dynamic
+ * Smagorinsky model out[0] = out[1] = Cs^2 (src/RHSfunctions.py:1818). */
 int psx_tau(psx_context* ctx, double* out_host8);
 
 /* Instrumentation (no counterpart in the reference, which only prints wall time,

@@ -232,6 +232,7 @@ MODEL_NAMES = {
     2: "tmodel",
     "Static tau-model": "static_tau",
     "Dynamic tau-model": "dynamic_tau",
+    7: "dsmag",
 }
 
 
@@ -354,11 +355,12 @@ def rhs_nosgs(st: State, fft: FFTBackend) -> None:
 SMAG_CS = 0.16  # hard-coded in src/RHSfunctions.py:197
 
 
-def smag_stress_hat(st: State, fft: FFTBackend, uhat, vhat, what, return_real=False):
+def smag_stress_hat(st: State, fft: FFTBackend, uhat, vhat, what, return_real=False, cs2=None):
     """Smagorinsky stress spectra tau_ij = unpad(FFT(-2 nu_t S_ij)), order
     (11, 22, 33, 12, 13, 23) (``src/RHSfunctions.py:174-212``).  With
     ``return_real`` also the padded-grid strain fields and nu_t (in the reference's
-    scaled units ``(3/2)^(-3/2) Nt^(-1/4)`` times physical)."""
+    scaled units ``(3/2)^(-3/2) Nt^(-1/4)`` times physical).  ``cs2`` replaces the
+    hard-coded Cs^2 = 0.16^2 (the dynamic model's coefficient)."""
     g = st.grid
     k1, k2, k3 = g.k1, g.k2, g.k3
     scale = np.sqrt((3.0 / 2.0) ** 3 * np.sqrt(g.Nt))
@@ -373,7 +375,8 @@ def smag_stress_hat(st: State, fft: FFTBackend, uhat, vhat, what, return_real=Fa
     s11, s22, s33, s12, s13, s23 = Sr
     Smag = np.sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * s12 * s12
                           + 2.0 * s13 * s13 + 2.0 * s23 * s23))
-    nut = SMAG_CS * g.Delta * SMAG_CS * g.Delta * np.abs(Smag)
+    cs2 = SMAG_CS * SMAG_CS if cs2 is None else cs2
+    nut = cs2 * g.Delta * g.Delta * np.abs(Smag)
     tau = [unpad(fft.r2c(-2.0 * nut * s)) for s in Sr]
     if return_real:
         return tau, Sr, nut
@@ -407,6 +410,72 @@ def rhs_smag(st: State, fft: FFTBackend) -> None:
     if st.rotate == 1:
         phat = phat - _coriolis_p(st, g, st.uhat, st.vhat, st.what, k1, k2, k3, ki)
     tau = smag_stress_hat(st, fft, st.uhat, st.vhat, st.what)
+    wu, wv, ww_ = sgs_divergence(g, tau)
+    st.w0 = (wu, wv, ww_)
+    ru = -1j * k1 * uu - 1j * k2 * uv - 1j * k3 * uw - 1j * k1 * phat - st.nu * g.ksqr * st.uhat + wu
+    rv = -1j * k1 * uv - 1j * k2 * vv - 1j * k3 * vw - 1j * k2 * phat - st.nu * g.ksqr * st.vhat + wv
+    rw = -1j * k1 * uw - 1j * k2 * vw - 1j * k3 * ww - 1j * k3 * phat - st.nu * g.ksqr * st.what + ww_
+    if st.rotate == 1:
+        ru, rv, rw = _coriolis_add(st, ru, rv, rw)
+    st.set_rhs(ru, rv, rw)
+
+
+DSMAG_ALPHA = 2.0  # src/RHSfunctions.py:1665
+
+
+def dsmag_coefficient(st: State, fft: FFTBackend, uhat, vhat, what):
+    """The dynamic Smagorinsky coefficient of ``computeRHS_DSMAG``
+    (``src/RHSfunctions.py:1663-1820``): test filter myFilter at kL = kc/alpha (the
+    reference slices with the float kc/2, which truncates to int(kc/2)); Leonard
+    stress L_ij = myFilter(u_i u_j) - uF_i uF_j; M_ij = -2 Delta^2 (alpha^2 FFT(|S^F| S^F_ij)
+    - myFilter(FFT(|S| S_ij))); Cs^2 = mean(M_ij L_ij) / mean(M_ij M_ij) over the six
+    stored components on the padded grid.  Returns (Cs^2, L, M, SS) with SS the
+    spectra of |S| S_ij (unfiltered)."""
+    g = st.grid
+    kL = int(g.kc / DSMAG_ALPHA)
+    flt = Grid(g.N1, g.N2, g.N3, kc=kL).Gf       # myFilter(., kL): the same index mask
+    u = (uhat, vhat, what)
+    uF = tuple(flt * a for a in u)
+    Nu = _products_32(st, fft, *u)
+    NF = _products_32(st, fft, *uF)
+    L = [flt * a - b for a, b in zip(Nu, NF)]
+    _, Sr, _ = smag_stress_hat(st, fft, *u, return_real=True, cs2=1.0)
+    _, SrF, _ = smag_stress_hat(st, fft, *uF, return_real=True, cs2=1.0)
+
+    def mag(S):
+        s11, s22, s33, s12, s13, s23 = S
+        return np.sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * s12 * s12 + 2.0 * s13 * s13
+                              + 2.0 * s23 * s23))
+
+    Sm, SmF = mag(Sr), mag(SrF)
+    SS = [unpad(fft.r2c(Sm * s)) for s in Sr]
+    SSF = [unpad(fft.r2c(SmF * s)) for s in SrF]
+    d2 = g.Delta ** 2
+    M = [-2.0 * d2 * (DSMAG_ALPHA ** 2 * a - flt * b) for a, b in zip(SSF, SS)]
+    scale = np.sqrt((3.0 / 2.0) ** 3 * np.sqrt(g.Nt))
+    Mp = ((3 * g.N1) // 2, (3 * g.N2) // 2, (3 * g.N3) // 2)
+    Mr = [fft.c2r(pad(a), Mp) * scale for a in M]
+    Lr = [fft.c2r(pad(a), Mp) * scale for a in L]
+    num = sum(m * l for m, l in zip(Mr, Lr))
+    den = sum(m * m for m in Mr)
+    return float(np.mean(num) / np.mean(den)), L, M, SS
+
+
+def rhs_dsmag(st: State, fft: FFTBackend) -> None:
+    """``computeRHS_DSMAG`` (``src/RHSfunctions.py:1663-1862``): the Smagorinsky RHS
+    (no Nyquist truncation) with nu_t = Cs^2 Delta^2 |S| and the dynamic Cs^2."""
+    g = st.grid
+    st.Q2U()
+    cs2, _, _, _ = dsmag_coefficient(st, fft, st.uhat, st.vhat, st.what)
+    st.cs2 = cs2
+    uu, vv, ww, uv, uw, vw = _products_32(st, fft, st.uhat, st.vhat, st.what)
+    k1, k2, k3 = g.k1, g.k2, g.k3
+    ki = g.ksqr_i
+    phat = (-k1 * k1 * ki * uu - k2 * k2 * ki * vv - k3 * k3 * ki * ww - 2.0 * k1 * k2 * ki * uv
+            - 2.0 * k1 * k3 * ki * uw - 2.0 * k2 * k3 * ki * vw)
+    if st.rotate == 1:
+        phat = phat - _coriolis_p(st, g, st.uhat, st.vhat, st.what, k1, k2, k3, ki)
+    tau = smag_stress_hat(st, fft, st.uhat, st.vhat, st.what, cs2=cs2)
     wu, wv, ww_ = sgs_divergence(g, tau)
     st.w0 = (wu, wv, ww_)
     ru = -1j * k1 * uu - 1j * k2 * uv - 1j * k3 * uw - 1j * k1 * phat - st.nu * g.ksqr * st.uhat + wu
@@ -620,6 +689,7 @@ RHS = {
     "tmodel": rhs_tmodel,
     "static_tau": rhs_staumodel,
     "dynamic_tau": rhs_dtaumodel,
+    "dsmag": rhs_dsmag,
 }
 
 

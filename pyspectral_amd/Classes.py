@@ -30,6 +30,7 @@ _MODEL_RHS = {
     2: "computeRHS_tmodel",
     "Static tau-model": "computeRHS_staumodel",
     "Dynamic tau-model": "computeRHS_Dtaumodel",
+    7: "computeRHS_DSMAG",
 }
 
 
@@ -96,7 +97,7 @@ class variables:
                  dt0_subintegrations, dt1, dt1_subintegrations, cfl, dim, myFFT=None):
         if turb_model not in _MODEL_RHS:
             raise NotImplementedError(f"turb_model {turb_model!r} has no GPU implementation "
-                                      "(GPU models: 0, 1, 2, 'Static tau-model', 'Dynamic tau-model')")
+                                      "(GPU models: 0, 1, 2, 'Static tau-model', 'Dynamic tau-model', 7)")
         if dim != 3:
             raise NotImplementedError("only dim = 3 is supported")
         self.turb_model = turb_model

@@ -51,3 +51,8 @@ def computeRHS_staumodel(main, grid, myFFT, utilities):
 def computeRHS_Dtaumodel(main, grid, myFFT, utilities):
     """``src/RHSfunctions.py:592-878``."""
     _rhs(main, True)
+
+
+def computeRHS_DSMAG(main, grid, myFFT, utilities):
+    """``src/RHSfunctions.py:1663-1862`` (dynamic Smagorinsky, turb_model 7)."""
+    _rhs(main, True)

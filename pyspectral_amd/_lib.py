@@ -18,6 +18,7 @@ MODEL_IDS = {
     2: 2, "tmodel": 2,
     "Static tau-model": 3, "static_tau": 3,
     "Dynamic tau-model": 4, "dynamic_tau": 4,
+    7: 5, "dsmag": 5,
 }
 
 # every symbol include/psx.h declares

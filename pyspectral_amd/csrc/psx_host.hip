@@ -76,6 +76,10 @@ struct psx_context {
   double2* plql = nullptr;
   double2* uf = nullptr;
   double2* glf = nullptr;
+  // dynamic Smagorinsky: spectral store of 24 product fields and its reduction partials
+  double2* dsD = nullptr;
+  double* dsPart = nullptr;
+  int ndsPart = 0;
   // filtered-DNS stress work buffers, allocated on first use
   double2* sgsA = nullptr;
   double2* sgsB = nullptr;
@@ -133,7 +137,7 @@ int guarded(psx_context* c, F&& f) {
   }
 }
 
-bool truncating(int model) { return model != PSX_MODEL_SMAGORINSKY; }
+bool truncating(int model) { return model != PSX_MODEL_SMAGORINSKY && model != PSX_MODEL_DSMAG; }
 
 // Kernel classes reported by psx_timing_end (order of PSX_KERNEL_* in psx.h).
 enum Kind { K_XINV = 0, K_YINV, K_ZPASS, K_YFWD, K_XEPI, K_CFL_X, K_CFL_Y, K_CFL_Z, K_OTHER, K_NKIND };
@@ -269,10 +273,10 @@ void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int
   check(c, launch_pencil_forward(L2, a, tw, nf, c->stream), "y_forward");
 }
 
-// RHS / stage for the 3/2-rule models (DNS, Smagorinsky)
-void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double coef,
-              double2* w0, int zero_nyq_out) {
-  const bool smag = c->model == PSX_MODEL_SMAGORINSKY;
+// K1-K4 of the 3/2-rule pipeline: the y-forward of the z-pass products of uin lands
+// in bufB (W layout).  smag adds the strain-rate pencils and the stress
+// -2 c2 |S| S_ij (c2 = (Cs Delta)^2 for the static closure).
+void front_32(psx_context* c, const double2* uin, bool smag, double c2) {
   const int M1 = c->M1, M2 = c->M2, M3 = c->M3, H = c->H;
   double2* A = c->bufA;
   double2* B = c->bufB;
@@ -303,8 +307,7 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
     z.kzscale = 2.0 * kPi / c->p.L3;
     const double Nt = (double)c->N1 * c->N2 * c->N3;
     z.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
-    const double delta = std::cbrt(c->p.dx * c->p.dy * c->p.dz);
-    z.c2 = (c->p.Cs * delta) * (c->p.Cs * delta);
+    z.c2 = c2;
     auto one = [](Recipe& r, int f) { r.nterm = 1; r.f[0] = f; r.coef[0] = 1.0; r.dz[0] = 0; };
     one(z.rec[0], 0);
     one(z.rec[1], 1);
@@ -328,6 +331,18 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
   const int nF = smag ? 12 : 6;
   // K4: y-forward + truncation
   y_forward(c, A, B, M1, M2, H, 0, nF);
+}
+
+double delta_grid(const psx_context* c) { return std::cbrt(c->p.dx * c->p.dy * c->p.dz); }
+
+// RHS / stage for the 3/2-rule models (DNS, Smagorinsky)
+void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double coef,
+              double2* w0, int zero_nyq_out) {
+  const bool smag = c->model == PSX_MODEL_SMAGORINSKY;
+  const int M1 = c->M1, H = c->H;
+  double2* B = c->bufB;
+  const double cd = c->p.Cs * delta_grid(c);
+  front_32(c, uin, smag, cd * cd);
   // K5: x-forward + truncation + projection/closure/viscous + stage update
   {
     EpiArgs e;
@@ -361,6 +376,88 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
     check(c, launch_epilogue(M1, c->N1, smag ? E_SMAG : E_DNS, smag && w0 != nullptr, e, tw, c->stream),
           "x_forward_epilogue");
   }
+}
+
+// computeRHS_DSMAG (src/RHSfunctions.py:1663-1862): the Smagorinsky pipeline on the
+// velocity (c2 = Delta^2) and on its test-filtered copy (c2 = alpha^2 Delta^2), each
+// x-forwarded into the spectral store; Cs^2 from the store by a deterministic
+// reduction; then the elementwise RHS with tau = Cs^2 T and the stage update.  The
+// whole stage stays on the device (graph-capturable).
+constexpr double kDsmagAlpha = 2.0;
+
+void store_x_forward(psx_context* c, int first) {
+  const int M1 = c->M1, H = c->H;
+  PencilArgs a = pencil_base();
+  a.in = c->bufB;
+  a.out = c->dsD;
+  a.in_sf = (long long)M1 * c->N2 * H;
+  a.in_sa = H;
+  a.in_sl = (long long)c->N2 * H;
+  a.out_sf = (long long)c->N1 * c->N2 * H;
+  a.out_sa = H;
+  a.out_sl = (long long)c->N2 * H;
+  a.A = c->N2;
+  a.C = H;
+  a.Nspec = c->N1;
+  a.keep_nyq = 0;
+  for (int j = 0; j < 12; ++j) {
+    a.job_in[j] = j;
+    a.job_plain[j] = first + j;
+  }
+  const double2* tw = twiddles(c, M1);
+  TimeScope ts(c, K_XEPI);
+  check(c, launch_pencil_forward(M1, a, tw, 12, c->stream), "x_forward(store)");
+}
+
+void stage_dsmag(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double coef,
+                 double2* w0) {
+  const double d = delta_grid(c);
+  const double kL = (double)(int)((double)c->p.kc / kDsmagAlpha);  // myFilter(., kc/alpha): int slice bound
+  const long long total = 3LL * c->N1 * c->N2 * c->Nc;
+  const unsigned eb = (unsigned)std::min<long long>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL(apply_filter, dim3(eb), dim3(256), 0, c->stream, uin, c->uf, 3, c->N1, c->N2, c->Nc, F_SHARP,
+                     kL, 0.0, 0.0, 0.0, 0.0);
+  check(c, hipGetLastError(), "apply_filter");
+  front_32(c, uin, true, d * d);
+  store_x_forward(c, 0);
+  front_32(c, c->uf, true, kDsmagAlpha * kDsmagAlpha * d * d);
+  store_x_forward(c, 12);
+  {
+    TimeScope ts(c, K_OTHER);
+    hipLaunchKernelGGL(dsmag_partials, dim3(c->ndsPart), dim3(NT), 0, c->stream, (const double2*)c->dsD, c->dsPart,
+                       c->N1, c->N2, c->H, kL);
+    check(c, hipGetLastError(), "dsmag_partials");
+    hipLaunchKernelGGL(dsmag_finalize, dim3(1), dim3(1), 0, c->stream, (const double*)c->dsPart, c->ndsPart,
+                       c->scal + 1);
+    check(c, hipGetLastError(), "dsmag_finalize");
+  }
+  DsmagArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.D = c->dsD;
+  a.uprev = uin;
+  a.q0 = q0;
+  a.out = out;
+  a.w0 = w0;
+  a.cs2 = c->scal + 1;
+  a.dt = dt;
+  a.c = coef;
+  a.nu = c->p.nu;
+  a.ks[0] = 2.0 * kPi / c->p.L1;
+  a.ks[1] = 2.0 * kPi / c->p.L2;
+  a.ks[2] = 2.0 * kPi / c->p.L3;
+  a.N1 = c->N1;
+  a.N2 = c->N2;
+  a.N3 = c->N3;
+  a.Nc = c->Nc;
+  a.H = c->H;
+  a.rotate = c->p.rotate;
+  a.Om[0] = c->p.Om1;
+  a.Om[1] = c->p.Om2;
+  a.Om[2] = c->p.Om3;
+  TimeScope ts(c, K_XEPI);
+  hipLaunchKernelGGL(dsmag_final, dim3((unsigned)std::min<long long>((total / 3 + 255) / 256, 8192)), dim3(256), 0,
+                     c->stream, a);
+  check(c, hipGetLastError(), "dsmag_final");
 }
 
 // One pass of the 2x-grid t-model pipeline on velocity u (already filtered for the
@@ -575,6 +672,9 @@ void run_stage(psx_context* c, const double2* uin, const double2* q0, double2* o
     case PSX_MODEL_DYNAMIC_TAU:
       stage_tau(c, uin, q0, out, dt, coef, w0, zero_nyq_out, t, t_dev);
       break;
+    case PSX_MODEL_DSMAG:
+      stage_dsmag(c, uin, q0, out, dt, coef, w0);
+      break;
     default:
       fail(c, PSX_E_UNSUPPORTED, "unknown model");
   }
@@ -640,7 +740,7 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
   int rc = guarded(c, [&] {
     if (n1 < 4 || n2 < 4 || n3 < 4 || (n1 % 2) || (n2 % 2) || (n3 % 4))
       fail(c, PSX_E_ARG, "grid sizes must be even (N3 a multiple of 4)");
-    if (model < PSX_MODEL_DNS || model > PSX_MODEL_DYNAMIC_TAU) fail(c, PSX_E_ARG, "unknown model");
+    if (model < PSX_MODEL_DNS || model > PSX_MODEL_DSMAG) fail(c, PSX_E_ARG, "unknown model");
     validate_params(c, params);
     if (model == PSX_MODEL_DYNAMIC_TAU && params->rotate)
       fail(c, PSX_E_UNSUPPORTED, "the dynamic tau-model is not available with rotation");
@@ -657,7 +757,7 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
     c->device = device;
     c->stream = (hipStream_t)stream;
     check(c, hipSetDevice(device), "hipSetDevice");
-    const bool tau = model >= PSX_MODEL_TMODEL;
+    const bool tau = model >= PSX_MODEL_TMODEL && model <= PSX_MODEL_DYNAMIC_TAU;
     std::vector<int> lens = {n1, n2, n3};
     if (tau) {
       for (int L : {2 * n1, 2 * n2, 2 * n3}) lens.push_back(L);
@@ -666,7 +766,7 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
     }
     for (int L : lens)
       if (!length_supported(L)) fail(c, PSX_E_UNSUPPORTED, "grid size not supported: length " + std::to_string(L));
-    const bool smag = model == PSX_MODEL_SMAGORINSKY;
+    const bool smag = model == PSX_MODEL_SMAGORINSKY || model == PSX_MODEL_DSMAG;
     const size_t state = (size_t)3 * n1 * n2 * c->Nc;
     if (!tau) {
       const size_t MMH = (size_t)c->M1 * c->M2 * c->H;
@@ -677,6 +777,14 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
       const size_t nW = (smag ? 12 : 6) * MNH;
       c->nA = std::max(std::max(nX, nZ), state);
       c->nB = std::max(std::max(nY, nW), state);
+      if (model == PSX_MODEL_DSMAG) {
+        const size_t nD = (size_t)24 * n1 * n2 * c->H;
+        c->ndsPart = 1024;
+        check(c, hipMalloc(&c->dsD, nD * sizeof(double2)), "hipMalloc(dsD)");
+        check(c, hipMalloc(&c->uf, state * sizeof(double2)), "hipMalloc(uf)");
+        check(c, hipMalloc(&c->dsPart, 2 * sizeof(double) * c->ndsPart), "hipMalloc(dsPart)");
+        c->scratch_bytes += (long long)(nD + state) * sizeof(double2) + 16LL * c->ndsPart;
+      }
     } else {
       const size_t P1 = 2 * (size_t)n1, P2 = 2 * (size_t)n2;
       const size_t full = P1 * P2 * (size_t)n3;       // one 2x-grid field, k3 < N3
@@ -734,6 +842,8 @@ int psx_destroy(psx_context* c) {
   hipFree(c->plql);
   hipFree(c->uf);
   hipFree(c->glf);
+  hipFree(c->dsD);
+  hipFree(c->dsPart);
   hipFree(c->sgsA);
   hipFree(c->sgsB);
   hipFree(c->sgsU);

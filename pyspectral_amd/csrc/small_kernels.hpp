@@ -299,3 +299,148 @@ __global__ void sgs_w0(const double2* __restrict__ tau, double2* __restrict__ w0
 }
 
 }  // namespace psx
+
+namespace psx {
+
+// ---- dynamic Smagorinsky (src/RHSfunctions.py:1663-1862)
+// The spectral store D[f][k1][k2][k3 < H] holds, for the velocity and its test-filtered
+// copy, the products N_ij (0..5 / 12..17) and T_ij = -2 c2 FFT(|S| S_ij) (6..11 / 18..23),
+// c2 = Delta^2 and alpha^2 Delta^2 respectively (index order 11 22 33 12 13 23).
+// Cs^2 = mean(M_ij L_ij)/mean(M_ij M_ij) over the padded grid with
+// M_ij = TF_ij - G T_ij and L_ij = G N_ij - NF_ij (G: the sharp test filter at kL); the
+// fields are band-limited, so the means are the Hermitian inner products of the
+// spectra (weight 2 for 0 < k3), computed here as per-block partial sums.
+__global__ __launch_bounds__(NT) void dsmag_partials(const double2* __restrict__ D, double* __restrict__ part,
+                                                     int N1, int N2, int H, double kL) {
+  const long long fs = (long long)N1 * N2 * H;
+  double num = 0.0, den = 0.0;
+  for (long long e = (long long)blockIdx.x * NT + threadIdx.x; e < fs; e += (long long)gridDim.x * NT) {
+    const int kz = (int)(e % H);
+    const long long r = e / H;
+    const int ky = (int)(r % N2), kx = (int)(r / N2);
+    const double g = (abs(signed_freq(kx, N1)) < kL && abs(signed_freq(ky, N2)) < kL && kz < kL) ? 1.0 : 0.0;
+    const double w = kz == 0 ? 1.0 : 2.0;
+    double sn = 0.0, sd = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const double2 Nv = D[m * fs + e], T = D[(6 + m) * fs + e];
+      const double2 NF = D[(12 + m) * fs + e], TF = D[(18 + m) * fs + e];
+      const double2 M = csub(TF, cscale(T, g));
+      const double2 L = csub(cscale(Nv, g), NF);
+      sn += M.x * L.x + M.y * L.y;
+      sd += M.x * M.x + M.y * M.y;
+    }
+    num += w * sn;
+    den += w * sd;
+  }
+  __shared__ double sh[2][NT];
+  sh[0][threadIdx.x] = num;
+  sh[1][threadIdx.x] = den;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0][0];
+    part[2 * blockIdx.x + 1] = sh[1][0];
+  }
+}
+
+// cs2[0] = cs2[1] = sum num / sum den (partials summed in a fixed order)
+__global__ void dsmag_finalize(const double* __restrict__ part, int nparts, double* __restrict__ cs2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double n = 0.0, d = 0.0;
+    for (int i = 0; i < nparts; ++i) {
+      n += part[2 * i];
+      d += part[2 * i + 1];
+    }
+    cs2[0] = n / d;
+    cs2[1] = n / d;
+  }
+}
+
+struct DsmagArgs {
+  const double2* D;      // spectral store (see dsmag_partials)
+  const double2* uprev;  // velocity the RHS is evaluated at (state layout)
+  const double2* q0;     // stage base (nullptr: write the RHS)
+  double2* out;
+  double2* w0;           // optional SGS term
+  const double* cs2;     // device Cs^2
+  const double* dt;
+  double c, nu;
+  double ks[3];
+  int N1, N2, N3, Nc, H;
+  int rotate;
+  double Om[3];
+};
+
+// RHS_i = -i k_j N_ij - i k_i p - nu k^2 u_i + Cs^2 w0_i(T) (+ Coriolis), with the
+// Smagorinsky projection form of w0 (src/RHSfunctions.py:1836-1862), then the
+// low-storage stage update q0 + dt c RHS.
+__global__ void dsmag_final(DsmagArgs a) {
+  const long long fs = (long long)a.N1 * a.N2 * a.Nc, ds = (long long)a.N1 * a.N2 * a.H;
+  const double cs2 = *a.cs2;
+  const double dtc = a.q0 ? (*a.dt) * a.c : 0.0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < fs;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(e % a.Nc);
+    const long long r = e / a.Nc;
+    const int ky = (int)(r % a.N2), kx = (int)(r / a.N2);
+    const double k[3] = {a.ks[0] * signed_freq(kx, a.N1), a.ks[1] * signed_freq(ky, a.N2), a.ks[2] * kz};
+    const double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    const double kinv = 1.0 / (ksq + 1e-50);
+    double2 Nm[6], T[6];
+    const long long di = r * a.H + kz;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      Nm[m] = kz < a.H ? a.D[m * ds + di] : make_double2(0.0, 0.0);
+      T[m] = kz < a.H ? a.D[(6 + m) * ds + di] : make_double2(0.0, 0.0);
+    }
+    double2 u[3];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) u[cc] = a.uprev[cc * fs + e];
+    // symmetric components: t(i, j) with 11 22 33 12 13 23 at 0..5
+    auto idx = [](int i, int j) { return i == j ? i : (i + j == 1 ? 3 : (i + j == 2 ? 4 : 5)); };
+    double2 kkN = make_double2(0.0, 0.0), kkT = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        kkN = cadd(kkN, cscale(Nm[idx(i, j)], k[i] * k[j]));
+        kkT = cadd(kkT, cscale(T[idx(i, j)], k[i] * k[j]));
+      }
+    double2 p = cscale(kkN, -kinv);
+    double2 cr[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+    if (a.rotate) {
+      cr[0] = csub(cscale(u[1], a.Om[2]), cscale(u[2], a.Om[1]));
+      cr[1] = csub(cscale(u[2], a.Om[0]), cscale(u[0], a.Om[2]));
+      cr[2] = csub(cscale(u[0], a.Om[1]), cscale(u[1], a.Om[0]));
+      // phat -= 2 ksqr_i 1j (k . (u x Om)) (src/RHSfunctions.py:1689-1692)
+      const double2 X = cadd(cadd(cscale(cr[0], k[0]), cscale(cr[1], k[1])), cscale(cr[2], k[2]));
+      p = csub(p, cmul_i<1>(cscale(X, 2.0 * kinv)));
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 divN = make_double2(0.0, 0.0), divT = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        divN = cadd(divN, cscale(Nm[idx(i, j)], k[j]));
+        divT = cadd(divT, cscale(T[idx(i, j)], k[j]));
+      }
+      // w0_i = -i k_j tau_ij - k_i (k k tau)/k^2 with tau = Cs^2 T
+      const double2 w = cscale(csub(cmul_i<-1>(divT), cscale(kkT, k[i] * kinv)), cs2);
+      double2 rhs = cadd(cmul_i<-1>(divN), cmul_i<-1>(cscale(p, k[i])));
+      rhs = csub(rhs, cscale(u[i], a.nu * (ksq + 1e-50)));
+      rhs = cadd(rhs, w);
+      if (a.rotate) rhs = cadd(rhs, cscale(cr[i], 2.0));
+      if (a.w0) a.w0[i * fs + e] = w;
+      a.out[i * fs + e] = a.q0 ? cadd(a.q0[i * fs + e], cscale(rhs, dtc)) : rhs;
+    }
+  }
+}
+
+}  // namespace psx

@@ -1,0 +1,66 @@
+"""Solution-file formats and loaders (src/postProcessingFunctions.py, the driver's
+np.savez calls): round trips through the files, with loads that execute nothing
+(allow_pickle=False)."""
+import types
+
+import numpy as np
+import pytest
+
+from oracle import spectral_oracle as so
+from pyspectral_amd import postProcessingFunctions as pp
+
+
+def _main(N, turb_model, seed=0):
+    rng = np.random.default_rng(seed)
+    shp = (N, N, N // 2 + 1)
+    f = lambda: rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    m = types.SimpleNamespace(uhat=f(), vhat=f(), what=f(), t=0.125, turb_model=turb_model)
+    m.w0_u, m.w0_v, m.w0_w = (f()[..., None] for _ in range(3))
+    return m
+
+
+@pytest.mark.parametrize("turb_model", [0, 1])
+def test_npsol_round_trip(tmp_path, turb_model):
+    N = 8
+    m = _main(N, turb_model)
+    path = pp.save_npsol(m, 30, str(tmp_path))
+    assert path.endswith("npsol30.npz")
+    info = {"turb_model": np.array(turb_model)}
+    with np.load(path, allow_pickle=False) as sol:
+        assert float(sol["t"]) == 0.125
+        v = pp.variablesFromFile(None, sol, info, kc=3)
+        for name in ("uhat", "vhat", "what"):
+            assert np.array_equal(getattr(v, name), getattr(m, name))
+        if turb_model:
+            assert np.array_equal(v.w0_u, m.w0_u[..., 0]) and np.array_equal(v.w0_w, m.w0_w[..., 0])
+        else:
+            assert v.w0_u.shape == (N, N, N // 2 + 1, 1) and not v.w0_u.any()
+        m2 = _main(N, turb_model, seed=1)
+        pp.save_npsol(m2, 31, str(tmp_path), compressed=False)
+    with np.load(str(tmp_path / "npsol31.npz"), allow_pickle=False) as sol2:
+        v.setFieldsFromFile(sol2)
+        assert np.array_equal(v.vhat, m2.vhat)
+
+
+def test_grid_from_file(tmp_path):
+    from pyspectral_amd.Classes import gridclass
+
+    N, kc = 12, 4
+    x = np.arange(N) * 2 * np.pi / N
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    g = gridclass(N, N, N, X, Y, Z, kc)
+    np.savez(tmp_path / "grid", k1=g.k1, k2=g.k2, k3=g.k3, x=g.x, y=g.y, z=g.z)
+    sol = {"uhat": np.zeros((N, N, N // 2 + 1), complex)}
+    with np.load(tmp_path / "grid.npz", allow_pickle=False) as gf:
+        gg = pp.gridFromFile(gf, sol, None, kc)
+    o = so.Grid(N, N, N, kc=kc)
+    for name in ("k1", "k2", "k3", "ksqr"):
+        assert np.array_equal(getattr(gg, name), getattr(o, name))
+    # integer 2x-grid wavenumbers (L = 2 pi): the oracle's k1f..k3f
+    for name in ("k1f", "k2f", "k3f"):
+        assert np.allclose(getattr(gg, name), getattr(o, name), rtol=0, atol=1e-12)
+        assert np.array_equal(getattr(gg, name), np.round(getattr(gg, name)))
+    assert np.array_equal(gg.Gf, o.Gf)
+    assert gg.Delta == np.pi / kc and gg.N3 == N
+    with pytest.raises(NotImplementedError):
+        pp.FFTFromFile(None, 1)

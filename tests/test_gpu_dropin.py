@@ -89,3 +89,33 @@ def test_tgv_re100_fixture_history(golden_dir):
     assert n >= 1990
     assert rel[0] < 5e-7
     assert rel.max() < 5e-3
+
+
+def test_solution_files_round_trip(tmp_path):
+    """A Smagorinsky run writing npsol<it>.npz through the savehook; every file the run
+    leaves loads without pickles and the loaders give back the saved state."""
+    from pyspectral_amd import postProcessingFunctions as pp
+    from pyspectral_amd.PySpec_3d import run_case
+
+    N, nu, dt = 16, 0.01, 0.01
+    g, _ = _case(N, nu, dt, turb_model=1, et=0.045)
+    out = str(tmp_path / "3DSolution")
+    seen = {}
+
+    def hook(main, grid, utilities, iteration, myFFT):
+        seen[iteration] = main.uhat.copy()
+        pp.save_npsol(main, iteration, out)
+
+    g["savehook"], g["save_freq"] = hook, 2
+    run_case(g, outdir=out, verbose=False)
+    with np.load(os.path.join(out, "runinfo.npz"), allow_pickle=False) as info, \
+            np.load(os.path.join(out, "grid.npz"), allow_pickle=False) as gf:
+        for it, u in seen.items():
+            with np.load(os.path.join(out, f"npsol{it}.npz"), allow_pickle=False) as sol:
+                v = pp.variablesFromFile(gf, sol, info, kc=N // 2)
+                assert np.array_equal(v.uhat, u)
+                assert v.w0_u.shape == u.shape
+                gg = pp.gridFromFile(gf, sol, info, N // 2)
+                assert gg.N1 == N
+    stats = pp.load_stats(os.path.join(out, "stats.npz"))
+    assert set(stats) == {"Energy", "Dissipation", "Energy_resolved", "Dissipation_resolved", "t", "Enstrophy"}

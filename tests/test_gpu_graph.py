@@ -10,7 +10,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("model", [0, 1, 2, "Dynamic tau-model"])
+@pytest.mark.parametrize("model", [0, 1, 2, "Dynamic tau-model", 7])
 def test_graph_replays_match_eager(model):
     from pyspectral_amd import cases
     from pyspectral_amd.solver import SpectralSolver

@@ -45,7 +45,7 @@ def _field(N, seed, nyquist_noise=False):
     return u
 
 
-MODELS = [0, 1, 2, "Static tau-model", "Dynamic tau-model"]
+MODELS = [0, 1, 2, "Static tau-model", "Dynamic tau-model", 7]
 TAU_KW = dict(Ct=0.2, dt0=0.05, dt1=0.5)
 
 
@@ -73,6 +73,9 @@ def test_rhs_matches_oracle(N, model):
     if model != 0:
         w_ref = np.stack(st.w0)
         assert _rel(_host(w0), w_ref) < 1e-11, _rel(_host(w0), w_ref)
+    if model == 7:
+        cs2 = s.tau_values()[0]
+        assert abs(cs2 - st.cs2) <= 1e-11 * abs(st.cs2), (cs2, st.cs2)
     if model == "Dynamic tau-model":
         tau = s.tau_values()
         assert abs(tau[0] - st.tau) <= 1e-10 * abs(st.tau), (tau, st.tau)
@@ -153,7 +156,7 @@ def test_compute_dt_matches_oracle(N):
 
 @pytest.mark.parametrize("model,N,steps,cfl", [(0, 16, 5, 0.5), (1, 16, 5, 0.5), (0, 32, 3, -0.002),
                                                  (1, 32, 3, 0.5), (2, 16, 4, 0.5), ("Static tau-model", 16, 3, 0.5),
-                                                 ("Dynamic tau-model", 16, 3, 0.5)])
+                                                 ("Dynamic tau-model", 16, 3, 0.5), (7, 16, 3, 0.5)])
 def test_steps_match_oracle(model, N, steps, cfl):
     print(f"\nstep model={model} N={N}", flush=True)
     nu = 0.01
