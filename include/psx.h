@@ -55,6 +55,37 @@ typedef struct psx_context psx_context;
  * scratch for an N1 x N2 x N3 grid.  stream is a hipStream_t (NULL = default). */
 int psx_create(psx_context** ctx, int n1, int n2, int n3, int model, const psx_params* params, int device,
                void* stream);
+
+/* No counterpart in the reference (which runs one grid per process): `members`
+ * independent instances of the same grid and model in one context, for ensembles.
+ * Every state argument then holds the members one after another,
+ * (members, 3, N1, N2, N3/2+1); dt_dev arguments hold one dt per member; nu gives
+ * each member's viscosity (NULL: params->nu for all).  Each kernel launch covers all
+ * members.  DNS and Smagorinsky only; psx_spectrum, psx_filter and psx_sgs_dns take
+ * single-member contexts; psx_stats writes 3 values per member. */
+int psx_create_batch(psx_context** ctx, int n1, int n2, int n3, int model, const psx_params* params, int members,
+                     const double* nu, int device, void* stream);
+
+/* Ensembles of uneven grid sizes (no counterpart in the reference): count members with
+ * grid dims[3*i..3*i+2] and viscosity nu[i]; the grid spacings are L/N and kc = N1/2
+ * for each member, the other parameters come from common.  Members of one size share
+ * a batch context.  The ensemble state is one array of psx_ensemble_size complex
+ * values; member i starts at psx_ensemble_offset(i) and its dt / stats entries are at
+ * slot psx_ensemble_slot(i).  psx_ensemble_step advances every member by one
+ * advanceQ_RK4 with its own CFL dt; psx_ensemble_step_graph captures the whole
+ * ensemble step as one hipGraph on the ensemble's (non-default) stream and replays it. */
+typedef struct psx_ensemble psx_ensemble;
+int psx_ensemble_create(psx_ensemble** ens, int count, const int* dims, const double* nu, int model,
+                        const psx_params* common, int device, void* stream);
+int psx_ensemble_destroy(psx_ensemble* ens);
+const char* psx_ensemble_last_error(const psx_ensemble* ens);
+long long psx_ensemble_size(const psx_ensemble* ens);
+long long psx_ensemble_offset(const psx_ensemble* ens, int member);
+int psx_ensemble_slot(const psx_ensemble* ens, int member);
+int psx_ensemble_set_stream(psx_ensemble* ens, void* stream);
+int psx_ensemble_step(psx_ensemble* ens, double* q, double* qs, double* dt_dev, double t);
+int psx_ensemble_step_graph(psx_ensemble* ens, double* q, double* qs, double* dt_dev);
+int psx_ensemble_stats(psx_ensemble* ens, const double* q, double* out_dev);
 int psx_destroy(psx_context* ctx);
 int psx_set_stream(psx_context* ctx, void* stream);
 int psx_set_params(psx_context* ctx, const psx_params* params);

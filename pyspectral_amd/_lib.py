@@ -26,7 +26,10 @@ EXPORTS = (
     "psx_create", "psx_destroy", "psx_set_stream", "psx_set_params", "psx_last_error",
     "psx_scratch_bytes", "psx_rhs", "psx_rk_stage", "psx_compute_dt", "psx_step", "psx_step_graph", "psx_stats",
     "psx_spectrum_bins", "psx_spectrum", "psx_tau", "psx_timing_begin", "psx_timing_end",
-    "psx_filter", "psx_sgs_dns",
+    "psx_filter", "psx_sgs_dns", "psx_create_batch",
+    "psx_ensemble_create", "psx_ensemble_destroy", "psx_ensemble_last_error", "psx_ensemble_size",
+    "psx_ensemble_offset", "psx_ensemble_slot", "psx_ensemble_set_stream", "psx_ensemble_step",
+    "psx_ensemble_step_graph", "psx_ensemble_stats",
 )
 
 PSX_FILTER_SHARP, PSX_FILTER_BOX = 0, 1
@@ -86,6 +89,19 @@ def load() -> C.CDLL:
         "psx_timing_end": (i, [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "psx_filter": (i, [vp, dp, dp, i, d]),
         "psx_sgs_dns": (i, [vp, dp, dp, dp, i, d]),
+        "psx_create_batch": (i, [C.POINTER(vp), i, i, i, i, C.POINTER(psx_params), i, C.POINTER(C.c_double), i,
+                                 vp]),
+        "psx_ensemble_create": (i, [C.POINTER(vp), i, C.POINTER(C.c_int), C.POINTER(C.c_double), i,
+                                    C.POINTER(psx_params), i, vp]),
+        "psx_ensemble_destroy": (i, [vp]),
+        "psx_ensemble_last_error": (C.c_char_p, [vp]),
+        "psx_ensemble_size": (C.c_longlong, [vp]),
+        "psx_ensemble_offset": (C.c_longlong, [vp, i]),
+        "psx_ensemble_slot": (i, [vp, i]),
+        "psx_ensemble_set_stream": (i, [vp, vp]),
+        "psx_ensemble_step": (i, [vp, dp, dp, dp, d]),
+        "psx_ensemble_step_graph": (i, [vp, dp, dp, dp]),
+        "psx_ensemble_stats": (i, [vp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -20,7 +20,7 @@ unsigned epilogue_blocks(int L, const EpiArgs& a) {
 }
 
 hipError_t launch_epilogue(int L, int ns1, int epi, bool w0, const EpiArgs& a, const double2* tw, hipStream_t s) {
-  const dim3 grid(epilogue_blocks(L, a));
+  const dim3 grid(epilogue_blocks(L, a), (unsigned)(a.members > 1 ? a.members : 1));
   const int key = L * 4096 + ns1;
   if (epi == E_DNS) {
     switch (key) {

@@ -107,7 +107,14 @@ struct PencilArgs {
   int job_in[MAXJOBS];
   int job_plain[MAXJOBS];
   int job_deriv[MAXJOBS];
+  // ensembles: grid.y = jpm * members; member m = blockIdx.y / jpm reads at
+  // in + m*in_ms and writes at out + m*out_ms (jpm = 0: one member)
+  int jpm;
+  long long in_ms, out_ms;
 };
+
+// (member, job) of a pencil block
+__device__ __forceinline__ int pencil_member(const PencilArgs& a) { return a.jpm ? (int)blockIdx.y / a.jpm : 0; }
 
 // Inverse along a strided axis: Nspec spectral entries -> L physical points, with
 // the zero-pad fused in; optionally a second output multiplied by i*k_axis.
@@ -131,8 +138,9 @@ __global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a
   const int bid = xcd_block<PSX_XCD != 0>();
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
-  const int job = blockIdx.y;
-  const double2* src = a.in + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
+  const int mem = pencil_member(a);
+  const int job = blockIdx.y - mem * a.jpm;
+  const double2* src = a.in + mem * a.in_ms + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
   constexpr int IT = (L * T + NT - 1) / NT;
   double2 r[IT];
   const int nload = a.Nspec * T;
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a
     }
     __syncthreads();
     fft_batch<L, T, NT, LS, +1>(buf, twp, tid);
-    double2* dst = a.out + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
+    double2* dst = a.out + mem * a.out_ms + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
     for (int e = tid; e < L * T; e += NT) {
       const int p = e / T, c = e - (e / T) * T;
       if (c0 + c < a.C) dst[(long long)p * a.out_sl + c] = cscale(buf[c * LS + RW(p)], a.scale);
@@ -186,8 +194,9 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   const int bid = xcd_block<PSX_XCD != 0>();
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
-  const int job = blockIdx.y;
-  const double2* src = a.in + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
+  const int mem = pencil_member(a);
+  const int job = blockIdx.y - mem * a.jpm;
+  const double2* src = a.in + mem * a.in_ms + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
   for (int e = tid; e < L * T; e += NT) {
     const int l = e / T, c = e - (e / T) * T;
     buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * a.in_sl + c] : make_double2(0.0, 0.0);
@@ -195,7 +204,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   __syncthreads();
   fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
   const int fo = a.job_plain[job];
-  double2* dst = a.out + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
+  double2* dst = a.out + mem * a.out_ms + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
   for (int e = tid; e < a.Nspec * T; e += NT) {
     const int i = e / T, c = e - (e / T) * T;
     if (c0 + c < a.C) {
@@ -238,21 +247,38 @@ struct ZArgs {
   double c2;              // (Cs*Delta)^2 for the Smagorinsky stress
   double dinv[3];         // 1/dx, 1/dy, 1/dz (CFL reduction)
   double* red;            // CFL: per-block maxima, reduced by cfl_finalize
+  // ensembles: npos_m positions per member; member m's fields start at in0 + m*ms_in0,
+  // in1 + m*ms_in1, out + m*ms_out (npos_m = 0: one member)
+  int npos_m;
+  long long ms_in0, ms_in1, ms_out;
 };
 
-__device__ __forceinline__ double2 zin(const ZArgs& a, int f, long long pos, int k) {
-  if (f < a.nin0) {
-    return k < a.Hin0 ? a.in0[(long long)f * a.in_sf0 + pos * a.Hin0 + k] : make_double2(0.0, 0.0);
-  }
-  const int g = f - a.nin0;
-  return k < a.Hin1 ? a.in1[(long long)g * a.in_sf1 + pos * a.Hin1 + k] : make_double2(0.0, 0.0);
+// A z-pass block's member and its data pointers
+struct ZMember {
+  const double2* in0;
+  const double2* in1;
+  double2* out;
+  long long pos0;  // first position of the block within its member
+};
+__device__ __forceinline__ ZMember zmember(const ZArgs& a, long long gpos) {
+  const long long m = a.npos_m ? gpos / a.npos_m : 0;
+  return ZMember{a.in0 + m * a.ms_in0, a.in1 ? a.in1 + m * a.ms_in1 : nullptr, a.out ? a.out + m * a.ms_out : nullptr,
+                 gpos - m * a.npos_m};
 }
 
-__device__ __forceinline__ double2 zspec(const ZArgs& a, int r, long long pos, int k) {
+__device__ __forceinline__ double2 zin(const ZArgs& a, const ZMember& zm, int f, long long pos, int k) {
+  if (f < a.nin0) {
+    return k < a.Hin0 ? zm.in0[(long long)f * a.in_sf0 + pos * a.Hin0 + k] : make_double2(0.0, 0.0);
+  }
+  const int g = f - a.nin0;
+  return k < a.Hin1 ? zm.in1[(long long)g * a.in_sf1 + pos * a.Hin1 + k] : make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ double2 zspec(const ZArgs& a, const ZMember& zm, int r, long long pos, int k) {
   const Recipe& R = a.rec[r];
   double2 acc = make_double2(0.0, 0.0);
   for (int t = 0; t < R.nterm; ++t) {
-    double2 v = zin(a, R.f[t], pos, k);
+    double2 v = zin(a, zm, R.f[t], pos, k);
     double s = R.coef[t];
     if (R.dz[t]) {
       v = cmul_i<1>(v);
@@ -340,7 +366,8 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
   __shared__ double2 buf[NCMAX * LS];
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NTZ)
-  const long long pos0 = (long long)blockIdx.x * P;
+  const ZMember zm = zmember(a, (long long)blockIdx.x * P);
+  const long long pos0 = zm.pos0;
 #define ZRW(i) (::psx::Row<L, PADZ>::at(i))
 
   // --- build the inverse spectra
@@ -351,7 +378,7 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
     // combinations from LDS into registers, then the Hermitian pencils
     for (int e = tid; e < NRI * HS; e += NTZ) {
       const int r = e / HS, k = e - r * HS;
-      buf[e] = zin(a, r / P, pos0 + r % P, k);
+      buf[e] = zin(a, zm, r / P, pos0 + r % P, k);
     }
     __syncthreads();
     constexpr int IB = (NCI * (L2 + 1) + NTZ - 1) / NTZ;
@@ -403,8 +430,8 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
     for (int e = tid; e < NCI * (L2 + 1); e += NTZ) {
       const int q = e / (L2 + 1), k = e - q * (L2 + 1);
       const int r0 = 2 * q, r1 = 2 * q + 1;
-      const double2 A = zspec(a, r0 / P, pos0 + r0 % P, k);
-      const double2 B = r1 < NRI ? zspec(a, r1 / P, pos0 + r1 % P, k) : make_double2(0.0, 0.0);
+      const double2 A = zspec(a, zm, r0 / P, pos0 + r0 % P, k);
+      const double2 B = r1 < NRI ? zspec(a, zm, r1 / P, pos0 + r1 % P, k) : make_double2(0.0, 0.0);
       if (k == 0 || k == L2) {
         buf[q * LS + ZRW(k)] = make_double2(A.x, B.x);
       } else {
@@ -492,10 +519,10 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
       const double2 z1 = buf[q * LS + ZRW(k)];
       const double2 z2 = buf[q * LS + ZRW((k == 0 ? 0 : L - k))];
       const int r0 = 2 * q, r1 = 2 * q + 1;
-      a.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k] =
+      zm.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k] =
           make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
       if (r1 < NRO)
-        a.out[(long long)(r1 / P) * a.out_sf + (pos0 + r1 % P) * a.Hout + k] =
+        zm.out[(long long)(r1 / P) * a.out_sf + (pos0 + r1 % P) * a.Hout + k] =
             make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
     }
   }
@@ -548,6 +575,11 @@ struct EpiArgs {
   const double2* red_u;
   double* red_part;
   int store_only;
+  // ensembles: grid.y = members; member m reads in + m*ms_in, the state-layout arrays
+  // at + m*ms_state, dt + m, and its viscosity nu_m[m] (nu_m = nullptr: nu)
+  long long ms_in, ms_state;
+  const double* nu_m;
+  int members;  // grid.y (0 or 1: one member)
 };
 
 // Accumulate the contribution of transformed field f (value v) to the three RHS
@@ -608,6 +640,15 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   PSX_STAGE_TW(NT)
   const int ct = (a.H + T - 1) / T;
   const int bid = xcd_block<PSX_XCD_EPI != 0>();
+  const int mem = blockIdx.y;
+  const double2* const in_ = a.in + mem * a.ms_in;
+  const double2* const uprev_ = a.uprev + mem * a.ms_state;
+  const double2* const q0_ = a.q0 ? a.q0 + mem * a.ms_state : nullptr;
+  double2* const out_ = a.out ? a.out + mem * a.ms_state : nullptr;
+  double2* const w0_ = a.w0 ? a.w0 + mem * a.ms_state : nullptr;
+  const double2* const pl_ = a.pl ? a.pl + mem * a.ms_state : nullptr;
+  const double* const dt_ = a.dt ? a.dt + mem : nullptr;
+  const double nu_ = a.nu_m ? a.nu_m[mem] : a.nu;
   const int nmain = a.N2 * ct;
   const long long plane = (long long)a.N2 * a.Nc;
   const long long fstride = (long long)a.N1 * plane;  // component stride in the state layout
@@ -628,10 +669,10 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
       double2 u[3], rhs[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        u[c] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : a.uprev[c * fstride + idx];
-        rhs[c] = EPI == E_TAU2 ? make_double2(0.0, 0.0) : cscale(u[c], -a.nu * (ksq + 1e-50));
-        if (EPI == E_TAU2 && a.w0) a.w0[c * fstride + idx] = make_double2(0.0, 0.0);
-        if (EPI == E_SMAG && a.w0) a.w0[c * fstride + idx] = make_double2(0.0, 0.0);
+        u[c] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : uprev_[c * fstride + idx];
+        rhs[c] = EPI == E_TAU2 ? make_double2(0.0, 0.0) : cscale(u[c], -nu_ * (ksq + 1e-50));
+        if (EPI == E_TAU2 && w0_) w0_[c * fstride + idx] = make_double2(0.0, 0.0);
+        if (EPI == E_SMAG && w0_) w0_[c * fstride + idx] = make_double2(0.0, 0.0);
       }
       if (EPI == E_TAU2 && a.store_only) continue;
       if (a.rotate) {
@@ -644,12 +685,12 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
 #pragma unroll
         for (int c = 0; c < 3; ++c) rhs[c] = cadd(rhs[c], csub(cscale(cr[c], 2.0), cscale(X, 2.0 * k[c] * kinv)));
       }
-      const double dtc = a.q0 ? (*a.dt) * a.c : 0.0;
+      const double dtc = q0_ ? (*dt_) * a.c : 0.0;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        double2 o = a.q0 ? cadd(a.q0[c * fstride + idx], cscale(rhs[c], dtc)) : rhs[c];
+        double2 o = q0_ ? cadd(q0_[c * fstride + idx], cscale(rhs[c], dtc)) : rhs[c];
         if (a.zero_nyq_out && nyq) o = make_double2(0.0, 0.0);
-        a.out[c * fstride + idx] = o;
+        out_[c * fstride + idx] = o;
       }
     }
     if (EPI == E_TAU2 && a.red_part && tid == 0) {
@@ -662,7 +703,7 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   const int ky = bid / ct;
   const int c0 = (bid - ky * ct) * T;
   const long long in_sl = (long long)a.N2 * a.H;  // x stride in W layout
-  const double2* src0 = a.in + (long long)ky * a.H + c0;
+  const double2* src0 = in_ + (long long)ky * a.H + c0;
   constexpr bool SGS = W0 && (EPI == E_SMAG);
   static_assert(EPI != E_TAU2 || T * LS >= NT, "block_sum2 needs NT LDS entries");
   double2 acc[IT][3];
@@ -724,7 +765,7 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
     __syncthreads();
   }
 
-  const double dtc = a.q0 ? (*a.dt) * a.c : 0.0;
+  const double dtc = q0_ ? (*dt_) * a.c : 0.0;
   const double coef = (EPI == E_TAU2 && a.store_only) ? 1.0 : (a.coef_dev ? *a.coef_dev : a.coef);
   double2 red = make_double2(0.0, 0.0);
 #pragma unroll
@@ -741,24 +782,24 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
         double2 u[3], rhs[3];
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
-          u[cc] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : a.uprev[cc * fstride + idx];
+          u[cc] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : uprev_[cc * fstride + idx];
           if constexpr (EPI == E_TAU2) {
             const double2 w = cscale(acc[it][cc], coef);
-            if (a.w0) a.w0[cc * fstride + idx] = w;
+            if (w0_) w0_[cc * fstride + idx] = w;
             if (a.red_u) {
               const double wz = kz == 0 ? 1.0 : (kz < a.N3 / 2 ? 2.0 : 0.0);
               const double2 r = a.red_u[cc * fstride + idx];
               red.x += wz * (w.x * r.x + w.y * r.y);
               red.y += wz * (w.y * r.x - w.x * r.y);
             }
-            rhs[cc] = a.store_only ? make_double2(0.0, 0.0) : cadd(a.pl[cc * fstride + idx], w);
+            rhs[cc] = a.store_only ? make_double2(0.0, 0.0) : cadd(pl_[cc * fstride + idx], w);
           } else {
-            rhs[cc] = csub(acc[it][cc], cscale(u[cc], a.nu * (ksq + 1e-50)));
+            rhs[cc] = csub(acc[it][cc], cscale(u[cc], nu_ * (ksq + 1e-50)));
           }
         }
         if constexpr (SGS) {
 #pragma unroll
-          for (int cc = 0; cc < 3; ++cc) a.w0[cc * fstride + idx] = accs[it][cc];
+          for (int cc = 0; cc < 3; ++cc) w0_[cc * fstride + idx] = accs[it][cc];
         }
         if (a.rotate) {
           const double kinv = 1.0 / (ksq + 1e-50);
@@ -774,9 +815,9 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
         if (!(EPI == E_TAU2 && a.store_only)) {
 #pragma unroll
           for (int cc = 0; cc < 3; ++cc) {
-            double2 o = a.q0 ? cadd(a.q0[cc * fstride + idx], cscale(rhs[cc], dtc)) : rhs[cc];
+            double2 o = q0_ ? cadd(q0_[cc * fstride + idx], cscale(rhs[cc], dtc)) : rhs[cc];
             if (a.zero_nyq_out && nyq) o = make_double2(0.0, 0.0);
-            a.out[cc * fstride + idx] = o;
+            out_[cc * fstride + idx] = o;
           }
         }
       }

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -68,7 +69,11 @@ struct psx_context {
   // pipeline scratch
   double2* bufA = nullptr;
   double2* bufB = nullptr;
-  size_t nA = 0, nB = 0;
+  size_t nA = 0, nB = 0;  // per member
+  // batch contexts: `members` instances of the same grid, one after another in every
+  // state and scratch array, with per-member viscosity nu_m (device) and dt
+  int members = 1;
+  double* nu_m = nullptr;
   // t-model family: Y1 (velocity after the 2x y-inverse), PL and PLQL on the base
   // grid, the filtered velocity and the filtered Leonard block (dynamic tau)
   double2* bufC = nullptr;
@@ -192,6 +197,19 @@ PencilArgs pencil_base() {
 }
 
 // ---- pipeline building blocks --------------------------------------------------
+// Per-member stride (complex elements) of an array the pipeline reads or writes:
+// the scratch buffers, or the state layout for everything else.
+long long member_stride(const psx_context* c, const void* p) {
+  if (p == c->bufA) return (long long)c->nA;
+  if (p == c->bufB) return (long long)c->nB;
+  return 3LL * c->N1 * c->N2 * c->Nc;
+}
+void set_members(const psx_context* c, PencilArgs& a, int njobs) {
+  a.jpm = njobs;
+  a.in_ms = member_stride(c, a.in);
+  a.out_ms = member_stride(c, a.out);
+}
+
 // x-inverse from the state layout: (nf fields of uin) -> X[f][L1][N2][C]
 void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, int keep_nyq, int njobs,
                const int* jin, const int* jplain, const int* jderiv, int kind = K_XINV) {
@@ -214,9 +232,10 @@ void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, in
     a.job_plain[j] = jplain[j];
     a.job_deriv[j] = jderiv ? jderiv[j] : -1;
   }
+  set_members(c, a, njobs);
   const double2* tw = twiddles(c, L1);
   TimeScope ts(c, kind);
-  check(c, launch_pencil_inverse(L1, a, tw, njobs, c->stream), "x_inverse");
+  check(c, launch_pencil_inverse(L1, a, tw, njobs * c->members, c->stream), "x_inverse");
 }
 
 // y-inverse: X[f][L1][N2][C] -> Y[f][L1][L2][C]
@@ -242,9 +261,10 @@ void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int
     a.job_plain[j] = jplain[j];
     a.job_deriv[j] = jderiv ? jderiv[j] : -1;
   }
+  set_members(c, a, njobs);
   const double2* tw = twiddles(c, L2);
   TimeScope ts(c, kind);
-  check(c, launch_pencil_inverse(L2, a, tw, njobs, c->stream), "y_inverse");
+  check(c, launch_pencil_inverse(L2, a, tw, njobs * c->members, c->stream), "y_inverse");
 }
 
 // y-forward with truncation: Z[f][L1][L2][C] -> W[f][L1][N2][C]
@@ -268,9 +288,10 @@ void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int
     a.job_in[j] = j;
     a.job_plain[j] = j;
   }
+  set_members(c, a, nf);
   const double2* tw = twiddles(c, L2);
   TimeScope ts(c, K_YFWD);
-  check(c, launch_pencil_forward(L2, a, tw, nf, c->stream), "y_forward");
+  check(c, launch_pencil_forward(L2, a, tw, nf * c->members, c->stream), "y_forward");
 }
 
 // K1-K4 of the 3/2-rule pipeline: the y-forward of the z-pass products of uin lands
@@ -303,7 +324,10 @@ void front_32(psx_context* c, const double2* uin, bool smag, double c2) {
     z.out = A;
     z.out_sf = (long long)M1 * M2 * H;
     z.Hout = H;
-    z.npos = M1 * M2;
+    z.npos = M1 * M2 * c->members;
+    z.npos_m = c->members > 1 ? M1 * M2 : 0;
+    z.ms_in0 = member_stride(c, z.in0);
+    z.ms_out = member_stride(c, z.out);
     z.kzscale = 2.0 * kPi / c->p.L3;
     const double Nt = (double)c->N1 * c->N2 * c->N3;
     z.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
@@ -323,7 +347,7 @@ void front_32(psx_context* c, const double2* uin, bool smag, double c2) {
     }
     const int mode = smag ? Z_SMAG : Z_DNS;
     const int P = zpass_positions(M3, mode);
-    if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
+    if (P == 0 || ((M1 * M2) % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
     const double2* tw = twiddles(c, M3);
     TimeScope ts(c, K_ZPASS);
     check(c, launch_zpass(M3, mode, z, tw, c->stream), "z_pass");
@@ -371,6 +395,10 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
     e.Om[0] = c->p.Om1;
     e.Om[1] = c->p.Om2;
     e.Om[2] = c->p.Om3;
+    e.members = c->members;
+    e.ms_in = member_stride(c, e.in);
+    e.ms_state = 3LL * c->N1 * c->N2 * c->Nc;
+    e.nu_m = c->nu_m;
     const double2* tw = twiddles(c, M1);
     TimeScope ts(c, K_XEPI);
     check(c, launch_epilogue(M1, c->N1, smag ? E_SMAG : E_DNS, smag && w0 != nullptr, e, tw, c->stream),
@@ -696,7 +724,9 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   z.in_sf0 = (long long)N1 * N2 * Nc;
   z.Hin0 = Nc;
   z.nin0 = 3;
-  z.npos = N1 * N2;
+  z.npos = N1 * N2 * c->members;
+  z.npos_m = c->members > 1 ? N1 * N2 : 0;
+  z.ms_in0 = member_stride(c, z.in0);
   z.kzscale = 2.0 * kPi / c->p.L3;
   for (int r = 0; r < 3; ++r) z.rec[r] = Recipe{1, {r, 0}, {1.0, 0.0}, {0, 0}};
   z.dinv[0] = 1.0 / c->p.dx;
@@ -704,8 +734,8 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   z.dinv[2] = 1.0 / c->p.dz;
   z.red = c->cfl_part;
   const int P = zpass_positions(N3, Z_DT);
-  if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "CFL pass: unsupported length");
-  const int nparts = z.npos / P;
+  if (P == 0 || ((N1 * N2) % P) != 0) fail(c, PSX_E_UNSUPPORTED, "CFL pass: unsupported length");
+  const int nparts = N1 * N2 / P;  // per member
   {
     const double2* tw = twiddles(c, N3);
     TimeScope ts(c, K_CFL_Z);
@@ -713,11 +743,9 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   }
   const double Nt = (double)N1 * N2 * N3;
   const double dx = c->p.dx, dy = c->p.dy, dz = c->p.dz;
-  const double visc = c->p.nu > 0
-                          ? 0.634 / (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)) / c->p.nu * c->p.cfl / 1.35
-                          : 0.0;
-  hipLaunchKernelGGL(cfl_finalize, dim3(1), dim3(NT), 0, c->stream, c->cfl_part, nparts, dt, 1.0 / std::sqrt(Nt),
-                     c->p.cfl, c->p.nu, visc);
+  const double ksum = 1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz);
+  hipLaunchKernelGGL(cfl_finalize, dim3(c->members), dim3(NT), 0, c->stream, c->cfl_part, nparts, dt,
+                     1.0 / std::sqrt(Nt), c->p.cfl, c->p.nu, (const double*)c->nu_m, ksum);
   check(c, hipGetLastError(), "cfl_finalize");
 }
 
@@ -734,6 +762,11 @@ extern "C" {
 
 int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_params* params, int device,
                void* stream) {
+  return psx_create_batch(out, n1, n2, n3, model, params, 1, nullptr, device, stream);
+}
+
+int psx_create_batch(psx_context** out, int n1, int n2, int n3, int model, const psx_params* params, int members,
+                     const double* nu_host, int device, void* stream) {
   if (!out) return PSX_E_ARG;
   *out = nullptr;
   psx_context* c = new psx_context();
@@ -741,7 +774,11 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
     if (n1 < 4 || n2 < 4 || n3 < 4 || (n1 % 2) || (n2 % 2) || (n3 % 4))
       fail(c, PSX_E_ARG, "grid sizes must be even (N3 a multiple of 4)");
     if (model < PSX_MODEL_DNS || model > PSX_MODEL_DSMAG) fail(c, PSX_E_ARG, "unknown model");
+    if (members < 1) fail(c, PSX_E_ARG, "members must be >= 1");
+    if (members > 1 && model != PSX_MODEL_DNS && model != PSX_MODEL_SMAGORINSKY)
+      fail(c, PSX_E_UNSUPPORTED, "batched instances are available for the DNS and Smagorinsky models");
     validate_params(c, params);
+    c->members = members;
     if (model == PSX_MODEL_DYNAMIC_TAU && params->rotate)
       fail(c, PSX_E_UNSUPPORTED, "the dynamic tau-model is not available with rotation");
     c->N1 = n1;
@@ -807,15 +844,24 @@ int psx_create(psx_context** out, int n1, int n2, int n3, int model, const psx_p
         c->scratch_bytes += (long long)3 * state * sizeof(double2) + 6LL * sizeof(double) * c->npartT;
       }
     }
-    check(c, hipMalloc(&c->bufA, c->nA * sizeof(double2)), "hipMalloc(bufA)");
-    check(c, hipMalloc(&c->bufB, c->nB * sizeof(double2)), "hipMalloc(bufB)");
-    check(c, hipMalloc(&c->cfl_part, sizeof(double) * ((size_t)n1 * n2 / 2 + 1)), "hipMalloc(cfl_part)");
+    const size_t mb = (size_t)members;
+    check(c, hipMalloc(&c->bufA, mb * c->nA * sizeof(double2)), "hipMalloc(bufA)");
+    check(c, hipMalloc(&c->bufB, mb * c->nB * sizeof(double2)), "hipMalloc(bufB)");
+    check(c, hipMalloc(&c->cfl_part, mb * sizeof(double) * ((size_t)n1 * n2 / 2 + 1)), "hipMalloc(cfl_part)");
+    if (members > 1) {
+      check(c, hipMalloc(&c->nu_m, mb * sizeof(double)), "hipMalloc(nu_m)");
+      std::vector<double> nu(mb, params->nu);
+      if (nu_host) nu.assign(nu_host, nu_host + mb);
+      for (double v : nu)
+        if (!(v >= 0.0)) fail(c, PSX_E_ARG, "member viscosities must be >= 0");
+      check(c, hipMemcpy(c->nu_m, nu.data(), mb * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(nu_m)");
+    }
     check(c, hipMalloc(&c->scal, 16 * sizeof(double)), "hipMalloc(scal)");
     check(c, hipMemset(c->scal, 0, 16 * sizeof(double)), "hipMemset(scal)");
     c->npart = 1024;
     check(c, hipMalloc(&c->part, 3 * c->npart * sizeof(double)), "hipMalloc(part)");
-    c->scratch_bytes += (long long)(c->nA + c->nB) * sizeof(double2) + 8LL * (n1 * n2 / 2 + 1) + 16 * 8 +
-                        3 * c->npart * 8;
+    c->scratch_bytes += (long long)members * ((c->nA + c->nB) * sizeof(double2) + 8LL * (n1 * n2 / 2 + 1)) +
+                        16 * 8 + 3 * c->npart * 8 + (members > 1 ? 8LL * members : 0);
     for (int L : lens) twiddles(c, L);
   });
   if (rc != PSX_OK) {
@@ -834,6 +880,7 @@ int psx_destroy(psx_context* c) {
   if (!c) return PSX_E_ARG;
   hipFree(c->bufA);
   hipFree(c->bufB);
+  hipFree(c->nu_m);
   hipFree(c->cfl_part);
   hipFree(c->scal);
   hipFree(c->part);
@@ -882,9 +929,12 @@ int psx_rhs(psx_context* c, const double* uhat, double* rhs, double* uhat_model,
     if (uhat_model) {
       const long long total = 3LL * c->N1 * c->N2 * c->Nc;
       const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 4096);
-      hipLaunchKernelGGL(copy_truncate, dim3(blocks), dim3(256), 0, c->stream, (const double2*)uhat,
-                         (double2*)uhat_model, c->N1, c->N2, c->Nc, c->N3, truncating(c->model) ? 1 : 0);
-      check(c, hipGetLastError(), "copy_truncate");
+      for (int m = 0; m < c->members; ++m) {
+        hipLaunchKernelGGL(copy_truncate, dim3(blocks), dim3(256), 0, c->stream, (const double2*)uhat + m * total,
+                           (double2*)uhat_model + m * total, c->N1, c->N2, c->Nc, c->N3,
+                           truncating(c->model) ? 1 : 0);
+        check(c, hipGetLastError(), "copy_truncate");
+      }
     }
   });
 }
@@ -970,12 +1020,15 @@ int psx_stats(psx_context* c, const double* uhat, double* out_dev) {
     if (!uhat || !out_dev) fail(c, PSX_E_ARG, "null pointer");
     const long long total = (long long)c->N1 * c->N2 * c->Nc;
     int nblk = (int)std::min<long long>((total + NT - 1) / NT, c->npart);
-    hipLaunchKernelGGL(stats_partial, dim3(nblk), dim3(NT), 0, c->stream, (const double2*)uhat, c->part, c->N1, c->N2,
-                       c->N3, c->Nc, 2.0 * kPi / c->p.L1, 2.0 * kPi / c->p.L2, 2.0 * kPi / c->p.L3, c->p.kc);
-    check(c, hipGetLastError(), "stats_partial");
     const double Nt = (double)c->N1 * c->N2 * c->N3;
-    hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, c->stream, c->part, nblk, out_dev, 1.0 / Nt);
-    check(c, hipGetLastError(), "stats_final");
+    for (int m = 0; m < c->members; ++m) {
+      const double2* u = (const double2*)uhat + (long long)m * 3 * total;
+      hipLaunchKernelGGL(stats_partial, dim3(nblk), dim3(NT), 0, c->stream, u, c->part, c->N1, c->N2, c->N3, c->Nc,
+                         2.0 * kPi / c->p.L1, 2.0 * kPi / c->p.L2, 2.0 * kPi / c->p.L3, c->p.kc);
+      check(c, hipGetLastError(), "stats_partial");
+      hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, c->stream, c->part, nblk, out_dev + 3 * m, 1.0 / Nt);
+      check(c, hipGetLastError(), "stats_final");
+    }
   });
 }
 
@@ -996,6 +1049,7 @@ void launch_filter(psx_context* c, const double2* in, double2* out, int filter, 
 
 int psx_filter(psx_context* c, const double* uhat, double* out, int filter, double delta) {
   return guarded(c, [&] {
+    if (c->members > 1) fail(c, PSX_E_UNSUPPORTED, "one instance per call: use a single-member context");
     if (!uhat || !out) fail(c, PSX_E_ARG, "null pointer");
     check_filter(c, filter, delta);
     launch_filter(c, (const double2*)uhat, (double2*)out, filter, delta);
@@ -1004,6 +1058,7 @@ int psx_filter(psx_context* c, const double* uhat, double* out, int filter, doub
 
 int psx_sgs_dns(psx_context* c, const double* uhat, double* w0_out, double* tau_out, int filter, double delta) {
   return guarded(c, [&] {
+    if (c->members > 1) fail(c, PSX_E_UNSUPPORTED, "one instance per call: use a single-member context");
     if (!uhat) fail(c, PSX_E_ARG, "null pointer");
     check_filter(c, filter, delta);
     const int M1 = c->M1, M2 = c->M2, M3 = c->M3, H = c->H;
@@ -1092,6 +1147,7 @@ int psx_spectrum_bins(const psx_context* c) {
 
 int psx_spectrum(psx_context* c, const double* uhat, double* spec_dev, int resolved) {
   return guarded(c, [&] {
+    if (c->members > 1) fail(c, PSX_E_UNSUPPORTED, "one instance per call: use a single-member context");
     if (!uhat || !spec_dev) fail(c, PSX_E_ARG, "null pointer");
     const int nb = psx_spectrum_bins(c);
     hipLaunchKernelGGL(fill_f64, dim3((3 * nb + 255) / 256), dim3(256), 0, c->stream, spec_dev, 3 * nb, 0.0);
@@ -1139,6 +1195,192 @@ int psx_timing_end(psx_context* c, double* ms, int* count) {
     c->events_used = 0;
     c->event_kinds.clear();
   });
+}
+
+}  // extern "C"
+
+// ---- ensembles of uneven sizes ------------------------------------------------------
+// Members are grouped by grid size into batch contexts (one per size class, in order of
+// first appearance); the ensemble state is class-major, each class its members in
+// order.  A step runs every class's batched step; the graph variant captures all of
+// them as one hipGraph on the ensemble's stream.
+struct psx_ensemble {
+  std::vector<psx_context*> cls;
+  std::vector<long long> cls_offset;  // complex offset of a class's first member
+  std::vector<int> cls_first_slot;
+  std::vector<int> slot;              // member -> slot (class-major order)
+  std::vector<long long> offset;      // member -> complex offset
+  long long total = 0;
+  int count = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::string err;
+};
+
+
+extern "C" {
+
+int psx_ensemble_create(psx_ensemble** out, int count, const int* dims, const double* nu, int model,
+                        const psx_params* common, int device, void* stream) {
+  if (!out || !dims || count < 1 || !common) return PSX_E_ARG;
+  *out = nullptr;
+  psx_ensemble* e = new psx_ensemble();
+  e->count = count;
+  e->stream = (hipStream_t)stream;
+  std::vector<std::array<int, 3>> sizes;
+  std::vector<std::vector<int>> members;
+  for (int i = 0; i < count; ++i) {
+    const std::array<int, 3> d = {dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]};
+    size_t k = 0;
+    while (k < sizes.size() && sizes[k] != d) ++k;
+    if (k == sizes.size()) {
+      sizes.push_back(d);
+      members.emplace_back();
+    }
+    members[k].push_back(i);
+  }
+  e->slot.assign(count, 0);
+  e->offset.assign(count, 0);
+  int next_slot = 0;
+  for (size_t k = 0; k < sizes.size(); ++k) {
+    const int n1 = sizes[k][0], n2 = sizes[k][1], n3 = sizes[k][2];
+    psx_params p = *common;
+    p.dx = p.L1 / n1;
+    p.dy = p.L2 / n2;
+    p.dz = p.L3 / n3;
+    p.kc = n1 / 2;
+    std::vector<double> nuk;
+    for (int i : members[k]) nuk.push_back(nu ? nu[i] : common->nu);
+    p.nu = nuk[0];
+    psx_context* c = nullptr;
+    const int rc = psx_create_batch(&c, n1, n2, n3, model, &p, (int)nuk.size(), nuk.data(), device, stream);
+    if (rc != PSX_OK) {
+      for (psx_context* x : e->cls) psx_destroy(x);
+      delete e;
+      return rc;
+    }
+    e->cls.push_back(c);
+    e->cls_offset.push_back(e->total);
+    e->cls_first_slot.push_back(next_slot);
+    const long long per = 3LL * n1 * n2 * (n3 / 2 + 1);
+    for (size_t j = 0; j < members[k].size(); ++j) {
+      e->slot[members[k][j]] = next_slot + (int)j;
+      e->offset[members[k][j]] = e->total + (long long)j * per;
+    }
+    next_slot += (int)members[k].size();
+    e->total += per * (long long)members[k].size();
+  }
+  *out = e;
+  return PSX_OK;
+}
+
+int psx_ensemble_destroy(psx_ensemble* e) {
+  if (!e) return PSX_E_ARG;
+  if (e->exec) hipGraphExecDestroy(e->exec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  for (psx_context* c : e->cls) psx_destroy(c);
+  delete e;
+  return PSX_OK;
+}
+
+const char* psx_ensemble_last_error(const psx_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+long long psx_ensemble_size(const psx_ensemble* e) { return e ? e->total : PSX_E_ARG; }
+long long psx_ensemble_offset(const psx_ensemble* e, int member) {
+  return (e && member >= 0 && member < e->count) ? e->offset[member] : PSX_E_ARG;
+}
+int psx_ensemble_slot(const psx_ensemble* e, int member) {
+  return (e && member >= 0 && member < e->count) ? e->slot[member] : PSX_E_ARG;
+}
+
+int psx_ensemble_set_stream(psx_ensemble* e, void* stream) {
+  if (!e) return PSX_E_ARG;
+  e->stream = (hipStream_t)stream;
+  for (psx_context* c : e->cls) c->stream = (hipStream_t)stream;
+  return PSX_OK;
+}
+
+int psx_ensemble_step(psx_ensemble* e, double* q, double* qs, double* dt_dev, double t) {
+  if (!e) return PSX_E_ARG;
+  if (!q || !qs || !dt_dev || q == qs) return PSX_E_ARG;
+  for (size_t k = 0; k < e->cls.size(); ++k) {
+    psx_context* c = e->cls[k];
+    const int rc = guarded(c, [&] {
+      step_body(c, (double2*)q + e->cls_offset[k], (double2*)qs + e->cls_offset[k], dt_dev + e->cls_first_slot[k], t,
+                nullptr);
+    });
+    if (rc != PSX_OK) {
+      e->err = c->err;
+      return rc;
+    }
+  }
+  return PSX_OK;
+}
+
+int psx_ensemble_step_graph(psx_ensemble* e, double* q, double* qs, double* dt_dev) {
+  if (!e) return PSX_E_ARG;
+  if (!q || !qs || !dt_dev || q == qs) return PSX_E_ARG;
+  if (e->stream == nullptr) {
+    e->err = "graph capture needs a non-default stream (psx_ensemble_set_stream)";
+    return PSX_E_ARG;
+  }
+  const void* key[4] = {q, qs, dt_dev, e->stream};
+  bool same = e->exec != nullptr;
+  for (int i = 0; i < 4 && same; ++i) same = e->key[i] == key[i];
+  if (!same) {
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      e->err = "hipStreamBeginCapture failed";
+      return PSX_E_HIP;
+    }
+    int rc = PSX_OK;
+    for (size_t k = 0; k < e->cls.size() && rc == PSX_OK; ++k) {
+      psx_context* c = e->cls[k];
+      const bool timing = c->timing;
+      c->timing = false;
+      rc = guarded(c, [&] {
+        step_body(c, (double2*)q + e->cls_offset[k], (double2*)qs + e->cls_offset[k], dt_dev + e->cls_first_slot[k],
+                  0.0, nullptr);
+      });
+      c->timing = timing;
+      if (rc != PSX_OK) e->err = c->err;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+    if (rc != PSX_OK || ec != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      if (rc == PSX_OK) e->err = "hipStreamEndCapture failed";
+      return rc != PSX_OK ? rc : PSX_E_HIP;
+    }
+    e->graph = g;
+    if (hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0) != hipSuccess) {
+      e->err = "hipGraphInstantiate failed";
+      return PSX_E_HIP;
+    }
+    for (int i = 0; i < 4; ++i) e->key[i] = key[i];
+  }
+  if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) {
+    e->err = "hipGraphLaunch failed";
+    return PSX_E_HIP;
+  }
+  return PSX_OK;
+}
+
+int psx_ensemble_stats(psx_ensemble* e, const double* q, double* out_dev) {
+  if (!e || !q || !out_dev) return PSX_E_ARG;
+  for (size_t k = 0; k < e->cls.size(); ++k) {
+    const int rc = psx_stats(e->cls[k], (const double*)((const double2*)q + e->cls_offset[k]),
+                             out_dev + 3 * e->cls_first_slot[k]);
+    if (rc != PSX_OK) {
+      e->err = e->cls[k]->err;
+      return rc;
+    }
+  }
+  return PSX_OK;
 }
 
 }  // extern "C"

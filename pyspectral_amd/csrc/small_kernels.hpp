@@ -6,10 +6,15 @@ namespace psx {
 
 // ------------------------------------------------------------------ small kernels
 // dt from the CFL maximum (src/Classes.py:687-698).  part holds per-block maxima of
-// |u|/dx+|v|/dy+|w|/dz in raw (unnormalised inverse) units; one block of NT threads.
+// |u|/dx+|v|/dy+|w|/dz in raw (unnormalised inverse) units; one block of NT threads
+// per ensemble member (block m reads part + m*nparts, writes dt[m], viscosity
+// nu_m[m] when nu_m is set).  ksum = 1/dx^2 + 1/dy^2 + 1/dz^2.
 __global__ __launch_bounds__(NT) void cfl_finalize(const double* __restrict__ part, int nparts, double* dt,
-                                                   double inv_sqrt_nt, double cfl, double nu, double visc_lim) {
+                                                   double inv_sqrt_nt, double cfl, double nu_all,
+                                                   const double* __restrict__ nu_m, double ksum) {
   __shared__ double sh[NT];
+  part += (long long)blockIdx.x * nparts;
+  const double nu = nu_m ? nu_m[blockIdx.x] : nu_all;
   double m = 0.0;
   for (int i = threadIdx.x; i < nparts; i += NT) m = fmax(m, part[i]);
   sh[threadIdx.x] = m;
@@ -21,8 +26,8 @@ __global__ __launch_bounds__(NT) void cfl_finalize(const double* __restrict__ pa
   if (threadIdx.x == 0) {
     const double mv = sh[0] * inv_sqrt_nt;
     double d = cfl / (mv + 1e-10);
-    if (nu > 0) d = fmin(d, visc_lim);
-    *dt = d;
+    if (nu > 0) d = fmin(d, 0.634 / ksum / nu * cfl / 1.35);
+    dt[blockIdx.x] = d;
   }
 }
 

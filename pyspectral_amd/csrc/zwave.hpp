@@ -45,10 +45,10 @@ __device__ __forceinline__ double2 zw_mask(double2 v, bool keep) {
   return make_double2(__longlong_as_double(__double_as_longlong(v.x) & m),
                       __longlong_as_double(__double_as_longlong(v.y) & m));
 }
-__device__ __forceinline__ double2 zw_in(const ZArgs& a, int f, long long pos, int k) {
+__device__ __forceinline__ double2 zw_in(const ZArgs& a, const ZMember& zm, int f, long long pos, int k) {
   // the field's buffer is chosen by (wave-uniform) selects, not a branch
   const bool first = f < a.nin0;
-  const double2* base = first ? a.in0 : a.in1;
+  const double2* base = first ? zm.in0 : zm.in1;
   const long long sf = first ? a.in_sf0 : a.in_sf1;
   const int H = first ? a.Hin0 : a.Hin1;
   const int g = first ? f : f - a.nin0;
@@ -194,7 +194,8 @@ __global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__
   constexpr int NCI = (NRI + 1) / 2, NCO = (NRO + 1) / 2;
   __shared__ double2 buf[G * R0 * SUB];
   const int l = threadIdx.x;
-  const long long pos0 = (long long)blockIdx.x * P;
+  const ZMember zm = zmember(a, (long long)blockIdx.x * P);
+  const long long pos0 = zm.pos0;
 
   // register-step twiddles w_L^{-a l} (the inverse uses the conjugates)
   double2 t1[R0];
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__
       for (int p = 0; p < P; ++p)
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi)
-          if ((use.bits[g0 / G][p] >> fi) & 1u) raw[m][p][fi] = zw_in(a, fi, pos0 + p, kk);
+          if ((use.bits[g0 / G][p] >> fi) & 1u) raw[m][p][fi] = zw_in(a, zm, fi, pos0 + p, kk);
     }
     // keep the group's loads together: nothing is scheduled across this point
     __builtin_amdgcn_sched_barrier(0);
@@ -324,10 +325,10 @@ __global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__
           for (int k = l; k < a.Hout; k += 64) {
             const double2 z1 = buf[gi * W::NATS + W::nat(k)];
             const double2 z2 = buf[gi * W::NATS + W::nat(k == 0 ? 0 : L - k)];
-            a.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k] =
+            zm.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k] =
                 make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
             if (r1 < NRO)
-              a.out[(long long)(r1 / P) * a.out_sf + (pos0 + r1 % P) * a.Hout + k] =
+              zm.out[(long long)(r1 / P) * a.out_sf + (pos0 + r1 % P) * a.Hout + k] =
                   make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
           }
         }
