@@ -94,6 +94,76 @@ def cpu_baseline(N, closure, nu, cfl, steps, seed):
                       f"{el:.1f} s wall", "seconds": el}
 
 
+def bench_ensemble(args):
+    """96 seeded decaying-turbulence members, 32 each at 32^3, 48^3 and 64^3 modes,
+    nu ~ U[5e-4, 5e-3], k0 in {3, 4, 5}, Smagorinsky, CFL 0.5; every step is one
+    replay of the captured ensemble graph.  Also times the same members stepped one
+    SpectralSolver at a time (unbatched) over a few steps."""
+    import numpy as np
+    import torch
+
+    from pyspectral_amd import cases
+    from pyspectral_amd.ensemble import Ensemble
+    from pyspectral_amd.solver import SpectralSolver
+
+    rng = np.random.default_rng(args.seed)
+    sizes = [32] * 32 + [48] * 32 + [64] * 32
+    nus = list(rng.uniform(5e-4, 5e-3, len(sizes)))
+    k0s = list(rng.choice([3.0, 4.0, 5.0], len(sizes)))
+    ens = Ensemble([(n, n, n) for n in sizes], nus, model=1, cfl=0.5)
+    q = ens.new_state()
+    for i, n in enumerate(sizes):
+        ens.member(q, i).copy_(torch.from_numpy(cases.decaying_isotropic(n, seed=args.seed + i, k0=k0s[i])))
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        qs, dt = q.clone(), ens.new_dt()
+        for _ in range(args.warmup):
+            ens.step_graph(q, qs, dt)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        for _ in range(args.steps):
+            ens.step_graph(q, qs, dt)
+    stream.synchronize()
+    el = time.perf_counter() - t0
+    pts = sum(n ** 3 for n in sizes)
+    value = pts * args.steps / el / 1e6
+    E = ens.stats(q).cpu().numpy()
+    # the same members, one solver each, eager launches
+    singles = [SpectralSolver(n, n, n, 1, nu=nu, cfl=0.5) for n, nu in zip(sizes, nus)]
+    uq = [ens.member(q, i).clone() for i in range(len(sizes))]
+    us = [x.clone() for x in uq]
+    ns = 10
+    for i, s in enumerate(singles):
+        s.step(uq[i], us[i])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(ns):
+        for i, s in enumerate(singles):
+            s.step(uq[i], us[i])
+    torch.cuda.synchronize()
+    el1 = time.perf_counter() - t1
+    for s in singles:
+        s.close()
+    bytes_step = sum(survey_bytes_per_step(n, "smagorinsky") for n in sizes)
+    return {
+        "metric": "Mpts*RK4-steps/s, ensemble of 96 LES members (32x 32^3, 48^3, 64^3)",
+        "value": round(value, 2), "unit": "Mpts*RK4-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded random-phase isotropic fields)",
+        "config": {"workload": "batched ensemble, Smagorinsky, 96 members 32x(32^3, 48^3, 64^3), "
+                               "nu ~ U[5e-4, 5e-3], k0 in {3,4,5}, CFL 0.5, one hipGraph replay per step",
+                   "members": len(sizes), "points_per_step": pts, "parallelism": "single GPU"},
+        "step_roofline": {"model_bytes_per_step": int(bytes_step),
+                          "achieved": round(bytes_step / (el / args.steps) / 1e9, 1), "unit": "GB/s",
+                          "frac": round(bytes_step / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)},
+        "unbatched": {"value": round(pts * ns / el1 / 1e6, 2), "ms_per_step": round(el1 / ns * 1e3, 3),
+                      "how": "the same 96 members, one SpectralSolver.step each, eager launches"},
+        "stats_after": {"E_mean": float(E[:, 0].mean()), "finite": bool(np.isfinite(E).all())},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +174,9 @@ def main():
     ap.add_argument("--event-steps", type=int, default=50, help="steps timed per kernel with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel shapes")
+    ap.add_argument("--workload", default="les128", choices=["les128", "ensemble"],
+                    help="les128: the 128^3 LES metric (default); ensemble: 96 members of 32^3/48^3/64^3")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size dispatch "
                          "that calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -121,6 +194,10 @@ def main():
     from pyspectral_amd import _lib, cases
     from pyspectral_amd.solver import SpectralSolver
 
+    if args.workload == "ensemble":
+        print(json.dumps(bench_ensemble(args)))
+        return
+
     if args.copy_mib > 0:
         n = args.copy_mib * (1 << 20) // 8
         a = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -135,6 +212,7 @@ def main():
     solver = SpectralSolver(N, N, N, model, nu=nu, cfl=cfl, Cs=0.16)
     q = torch.from_numpy(uh).to("cuda")
     qs = q.clone()
+    tuning = solver.tuning() if args.no_autotune else solver.autotune(q, reps=5)
     for _ in range(args.warmup):
         solver.step(q, qs)
     torch.cuda.synchronize()
@@ -195,6 +273,7 @@ def main():
         "step_roofline": {"model_bytes_per_step": int(sb), "achieved": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
                           "unit": "GB/s", "frac": round(sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
+        "tuning": tuning,
         "stats_after": {"E": E[0], "Omega": E[1], "finite": finite},
     }
     if args.cpu_steps > 0:

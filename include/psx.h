@@ -46,6 +46,8 @@ typedef struct psx_params {
   int kc;                 /* sharp cutoff of Gf (src/Classes.py:584-588)              */
   int rotate;             /* Coriolis terms on/off (src/RHSfunctions.py:33-50)        */
   double Om1, Om2, Om3;   /* rotation vector                                         */
+  double force;           /* linear forcing f = force * u (no counterpart in the reference;
+                             DNS, Smagorinsky and dynamic Smagorinsky)                */
 } psx_params;
 
 typedef struct psx_context psx_context;
@@ -153,6 +155,18 @@ int psx_sgs_dns(psx_context* ctx, const double* uhat, double* w0_out, double* ta
  * reductions LE, PLQLE_filt, filt_PLQLE as (re, im) pairs.  For the dynamic
  * Smagorinsky model out[0] = out[1] = Cs^2 (src/RHSfunctions.py:1818). */
 int psx_tau(psx_context* ctx, double* out_host8);
+
+/* Kernel shapes (no counterpart in the reference): tune[0..2] = tile widths (k3 columns
+ * per block: 4, 8 or 16; 0 = the built-in default) of the x-inverse, y-inverse and
+ * y-forward passes, tune[3] = z-pass variant (0: one wavefront per position where the
+ * z length is a multiple of 64, 1: the block-wide Stockham kernel).  psx_autotune times
+ * every candidate of each knob on reps RHS evaluations of uhat with HIP events, keeps
+ * the fastest (knob by knob), and returns the choice in tune_out (may be NULL); it
+ * allocates one temporary state.  DNS, Smagorinsky and DSMAG contexts. */
+#define PSX_TUNE_N 4
+int psx_set_tuning(psx_context* ctx, const int* tune);
+int psx_get_tuning(const psx_context* ctx, int* tune);
+int psx_autotune(psx_context* ctx, const double* uhat, int reps, int* tune_out);
 
 /* Instrumentation (no counterpart in the reference, which only prints wall time,
  * src/PySpec_3d.py:136-145): bracket every kernel launch with HIP events on the

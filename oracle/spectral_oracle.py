@@ -240,8 +240,10 @@ class State:
     """Solution state with the reference's interleaved ``Q`` and its ``Q2U/U2Q``."""
 
     def __init__(self, grid: Grid, uhat, vhat, what, *, nu, turb_model=0, t=0.0, dt=0.01,
-                 cfl=None, Ct=-10, dt0=-10, dt1=-10, rotate=0, Om=(0.0, 0.0, 0.0)):
+                 cfl=None, Ct=-10, dt0=-10, dt1=-10, rotate=0, Om=(0.0, 0.0, 0.0), force=0.0):
         self.grid = grid
+        # linear forcing f = force * u (not in the reference; the forced-turbulence cases)
+        self.force = float(force)
         self.turb_model = turb_model
         self.model = MODEL_NAMES[turb_model]
         self.nu = float(nu)
@@ -349,6 +351,8 @@ def rhs_nosgs(st: State, fft: FFTBackend) -> None:
     rw = -1j * k1 * uw - 1j * k2 * vw - 1j * k3 * ww - 1j * k3 * phat - st.nu * g.ksqr * st.what
     if st.rotate == 1:
         ru, rv, rw = _coriolis_add(st, ru, rv, rw)
+    if st.force != 0.0:
+        ru, rv, rw = ru + st.force * st.uhat, rv + st.force * st.vhat, rw + st.force * st.what
     st.set_rhs(ru, rv, rw)
 
 
@@ -417,6 +421,8 @@ def rhs_smag(st: State, fft: FFTBackend) -> None:
     rw = -1j * k1 * uw - 1j * k2 * vw - 1j * k3 * ww - 1j * k3 * phat - st.nu * g.ksqr * st.what + ww_
     if st.rotate == 1:
         ru, rv, rw = _coriolis_add(st, ru, rv, rw)
+    if st.force != 0.0:
+        ru, rv, rw = ru + st.force * st.uhat, rv + st.force * st.vhat, rw + st.force * st.what
     st.set_rhs(ru, rv, rw)
 
 
@@ -483,6 +489,8 @@ def rhs_dsmag(st: State, fft: FFTBackend) -> None:
     rw = -1j * k1 * uw - 1j * k2 * vw - 1j * k3 * ww - 1j * k3 * phat - st.nu * g.ksqr * st.what + ww_
     if st.rotate == 1:
         ru, rv, rw = _coriolis_add(st, ru, rv, rw)
+    if st.force != 0.0:
+        ru, rv, rw = ru + st.force * st.uhat, rv + st.force * st.vhat, rw + st.force * st.what
     st.set_rhs(ru, rv, rw)
 
 

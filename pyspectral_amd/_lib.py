@@ -29,7 +29,7 @@ EXPORTS = (
     "psx_filter", "psx_sgs_dns", "psx_create_batch",
     "psx_ensemble_create", "psx_ensemble_destroy", "psx_ensemble_last_error", "psx_ensemble_size",
     "psx_ensemble_offset", "psx_ensemble_slot", "psx_ensemble_set_stream", "psx_ensemble_step",
-    "psx_ensemble_step_graph", "psx_ensemble_stats",
+    "psx_ensemble_step_graph", "psx_ensemble_stats", "psx_set_tuning", "psx_get_tuning", "psx_autotune",
 )
 
 PSX_FILTER_SHARP, PSX_FILTER_BOX = 0, 1
@@ -50,6 +50,7 @@ class psx_params(C.Structure):
         ("kc", C.c_int),
         ("rotate", C.c_int),
         ("Om1", C.c_double), ("Om2", C.c_double), ("Om3", C.c_double),
+        ("force", C.c_double),
     ]
 
 
@@ -102,6 +103,9 @@ def load() -> C.CDLL:
         "psx_ensemble_step": (i, [vp, dp, dp, dp, d]),
         "psx_ensemble_step_graph": (i, [vp, dp, dp, dp]),
         "psx_ensemble_stats": (i, [vp, dp, dp]),
+        "psx_set_tuning": (i, [vp, C.POINTER(C.c_int)]),
+        "psx_get_tuning": (i, [vp, C.POINTER(C.c_int)]),
+        "psx_autotune": (i, [vp, dp, i, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -17,39 +17,72 @@ bool length_supported(int L) {
   }
 }
 
-static inline unsigned pencil_blocks(int L, const PencilArgs& a) {
-  const int T = pencil_T(L);
-  return (unsigned)(a.A * ((a.C + T - 1) / T));
+// Tile widths (k3 columns per block) a pencil pass may use: the length's default and
+// its neighbours, as long as the block's LDS rows fit in 64 KiB.
+template <int L, int T> constexpr bool tile_ok() { return T * Row<L>::LS * 16 <= 65536; }
+
+int pencil_tile(int L, int T) {
+  if (T <= 0) return pencil_T(L);
+  if (T != 4 && T != 8 && T != 16) return -1;
+  switch (L) {
+#define PSX_CASE(LL)                                                                    \
+  case LL:                                                                              \
+    return (T == 4 ? tile_ok<LL, 4>() : (T == 8 ? tile_ok<LL, 8>() : tile_ok<LL, 16>())) ? T : -1;
+    PSX_LENGTHS(PSX_CASE)
+#undef PSX_CASE
+    default:
+      return -1;
+  }
 }
 
-hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s) {
-  const dim3 grid(pencil_blocks(L, a), (unsigned)njobs);
+template <int L, int T, bool INV>
+static hipError_t launch_tile(const PencilArgs& a, const double2* tw, int njobs, hipStream_t s) {
+  if constexpr (tile_ok<L, T>()) {
+    const dim3 grid((unsigned)(a.A * ((a.C + T - 1) / T)), (unsigned)njobs);
+    if constexpr (INV)
+      hipLaunchKernelGGL((pencil_inverse<L, T>), grid, dim3(NT), 0, s, a, tw);
+    else
+      hipLaunchKernelGGL((pencil_forward<L, T>), grid, dim3(NT), 0, s, a, tw);
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
+template <int L, bool INV>
+static hipError_t launch_len(int T, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s) {
+  switch (T) {
+    case 4: return launch_tile<L, 4, INV>(a, tw, njobs, s);
+    case 8: return launch_tile<L, 8, INV>(a, tw, njobs, s);
+    case 16: return launch_tile<L, 16, INV>(a, tw, njobs, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T) {
+  const int t = pencil_tile(L, T);
   switch (L) {
-#define PSX_CASE(LL)                                                                  \
-  case LL:                                                                            \
-    hipLaunchKernelGGL((pencil_inverse<LL, pencil_T(LL)>), grid, dim3(NT), 0, s, a, tw); \
-    break;
+#define PSX_CASE(LL) \
+  case LL:           \
+    return launch_len<LL, true>(t, a, tw, njobs, s);
     PSX_LENGTHS(PSX_CASE)
 #undef PSX_CASE
     default:
       return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
-hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s) {
-  const dim3 grid(pencil_blocks(L, a), (unsigned)njobs);
+hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T) {
+  const int t = pencil_tile(L, T);
   switch (L) {
-#define PSX_CASE(LL)                                                                  \
-  case LL:                                                                            \
-    hipLaunchKernelGGL((pencil_forward<LL, pencil_T(LL)>), grid, dim3(NT), 0, s, a, tw); \
-    break;
+#define PSX_CASE(LL) \
+  case LL:           \
+    return launch_len<LL, false>(t, a, tw, njobs, s);
     PSX_LENGTHS(PSX_CASE)
 #undef PSX_CASE
     default:
       return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 }  // namespace psx

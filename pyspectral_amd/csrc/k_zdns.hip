@@ -24,23 +24,23 @@ int zpass_positions_dns(int L) {
   }
 }
 
-hipError_t launch_zpass_dns(int L, const ZArgs& a, const double2* tw, hipStream_t s) {
+hipError_t launch_zpass_dns(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
   switch (L) {
-    case 24: return zpass_launch<24, Z_DNS>(a, tw, s);
-    case 32: return zpass_launch<32, Z_DNS>(a, tw, s);
-    case 48: return zpass_launch<48, Z_DNS>(a, tw, s);
-    case 64: return zpass_launch<64, Z_DNS>(a, tw, s);
-    case 72: return zpass_launch<72, Z_DNS>(a, tw, s);
-    case 96: return zpass_launch<96, Z_DNS>(a, tw, s);
-    case 128: return zpass_launch<128, Z_DNS>(a, tw, s);
-    case 144: return zpass_launch<144, Z_DNS>(a, tw, s);
-    case 192: return zpass_launch<192, Z_DNS>(a, tw, s);
-    case 256: return zpass_launch<256, Z_DNS>(a, tw, s);
-    case 288: return zpass_launch<288, Z_DNS>(a, tw, s);
-    case 384: return zpass_launch<384, Z_DNS>(a, tw, s);
-    case 512: return zpass_launch<512, Z_DNS>(a, tw, s);
-    case 576: return zpass_launch<576, Z_DNS>(a, tw, s);
-    case 768: return zpass_launch<768, Z_DNS>(a, tw, s);
+    case 24: return zpass_launch<24, Z_DNS>(a, tw, s, variant);
+    case 32: return zpass_launch<32, Z_DNS>(a, tw, s, variant);
+    case 48: return zpass_launch<48, Z_DNS>(a, tw, s, variant);
+    case 64: return zpass_launch<64, Z_DNS>(a, tw, s, variant);
+    case 72: return zpass_launch<72, Z_DNS>(a, tw, s, variant);
+    case 96: return zpass_launch<96, Z_DNS>(a, tw, s, variant);
+    case 128: return zpass_launch<128, Z_DNS>(a, tw, s, variant);
+    case 144: return zpass_launch<144, Z_DNS>(a, tw, s, variant);
+    case 192: return zpass_launch<192, Z_DNS>(a, tw, s, variant);
+    case 256: return zpass_launch<256, Z_DNS>(a, tw, s, variant);
+    case 288: return zpass_launch<288, Z_DNS>(a, tw, s, variant);
+    case 384: return zpass_launch<384, Z_DNS>(a, tw, s, variant);
+    case 512: return zpass_launch<512, Z_DNS>(a, tw, s, variant);
+    case 576: return zpass_launch<576, Z_DNS>(a, tw, s, variant);
+    case 768: return zpass_launch<768, Z_DNS>(a, tw, s, variant);
     default: return hipErrorInvalidValue;
   }
 }

@@ -19,18 +19,18 @@ int zpass_positions_dt(int L) {
   }
 }
 
-hipError_t launch_zpass_dt(int L, const ZArgs& a, const double2* tw, hipStream_t s) {
+hipError_t launch_zpass_dt(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
   switch (L) {
-    case 16: return zpass_launch<16, Z_DT>(a, tw, s);
-    case 32: return zpass_launch<32, Z_DT>(a, tw, s);
-    case 48: return zpass_launch<48, Z_DT>(a, tw, s);
-    case 64: return zpass_launch<64, Z_DT>(a, tw, s);
-    case 96: return zpass_launch<96, Z_DT>(a, tw, s);
-    case 128: return zpass_launch<128, Z_DT>(a, tw, s);
-    case 192: return zpass_launch<192, Z_DT>(a, tw, s);
-    case 256: return zpass_launch<256, Z_DT>(a, tw, s);
-    case 384: return zpass_launch<384, Z_DT>(a, tw, s);
-    case 512: return zpass_launch<512, Z_DT>(a, tw, s);
+    case 16: return zpass_launch<16, Z_DT>(a, tw, s, variant);
+    case 32: return zpass_launch<32, Z_DT>(a, tw, s, variant);
+    case 48: return zpass_launch<48, Z_DT>(a, tw, s, variant);
+    case 64: return zpass_launch<64, Z_DT>(a, tw, s, variant);
+    case 96: return zpass_launch<96, Z_DT>(a, tw, s, variant);
+    case 128: return zpass_launch<128, Z_DT>(a, tw, s, variant);
+    case 192: return zpass_launch<192, Z_DT>(a, tw, s, variant);
+    case 256: return zpass_launch<256, Z_DT>(a, tw, s, variant);
+    case 384: return zpass_launch<384, Z_DT>(a, tw, s, variant);
+    case 512: return zpass_launch<512, Z_DT>(a, tw, s, variant);
     default: return hipErrorInvalidValue;
   }
 }

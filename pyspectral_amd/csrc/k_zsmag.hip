@@ -17,16 +17,16 @@ int zpass_positions_smag(int L) {
   }
 }
 
-hipError_t launch_zpass_smag(int L, const ZArgs& a, const double2* tw, hipStream_t s) {
+hipError_t launch_zpass_smag(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
   switch (L) {
-    case 24: return zpass_launch<24, Z_SMAG>(a, tw, s);
-    case 48: return zpass_launch<48, Z_SMAG>(a, tw, s);
-    case 72: return zpass_launch<72, Z_SMAG>(a, tw, s);
-    case 96: return zpass_launch<96, Z_SMAG>(a, tw, s);
-    case 144: return zpass_launch<144, Z_SMAG>(a, tw, s);
-    case 192: return zpass_launch<192, Z_SMAG>(a, tw, s);
-    case 288: return zpass_launch<288, Z_SMAG>(a, tw, s);
-    case 384: return zpass_launch<384, Z_SMAG>(a, tw, s);
+    case 24: return zpass_launch<24, Z_SMAG>(a, tw, s, variant);
+    case 48: return zpass_launch<48, Z_SMAG>(a, tw, s, variant);
+    case 72: return zpass_launch<72, Z_SMAG>(a, tw, s, variant);
+    case 96: return zpass_launch<96, Z_SMAG>(a, tw, s, variant);
+    case 144: return zpass_launch<144, Z_SMAG>(a, tw, s, variant);
+    case 192: return zpass_launch<192, Z_SMAG>(a, tw, s, variant);
+    case 288: return zpass_launch<288, Z_SMAG>(a, tw, s, variant);
+    case 384: return zpass_launch<384, Z_SMAG>(a, tw, s, variant);
     default: return hipErrorInvalidValue;
   }
 }

@@ -18,17 +18,17 @@ int zpass_positions_tm2(int L) {
   }
 }
 
-hipError_t launch_zpass_tm2(int L, const ZArgs& a, const double2* tw, hipStream_t s) {
+hipError_t launch_zpass_tm2(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
   switch (L) {
-    case 32: return zpass_launch<32, Z_TM2>(a, tw, s);
-    case 64: return zpass_launch<64, Z_TM2>(a, tw, s);
-    case 96: return zpass_launch<96, Z_TM2>(a, tw, s);
-    case 128: return zpass_launch<128, Z_TM2>(a, tw, s);
-    case 192: return zpass_launch<192, Z_TM2>(a, tw, s);
-    case 256: return zpass_launch<256, Z_TM2>(a, tw, s);
-    case 384: return zpass_launch<384, Z_TM2>(a, tw, s);
-    case 512: return zpass_launch<512, Z_TM2>(a, tw, s);
-    case 768: return zpass_launch<768, Z_TM2>(a, tw, s);
+    case 32: return zpass_launch<32, Z_TM2>(a, tw, s, variant);
+    case 64: return zpass_launch<64, Z_TM2>(a, tw, s, variant);
+    case 96: return zpass_launch<96, Z_TM2>(a, tw, s, variant);
+    case 128: return zpass_launch<128, Z_TM2>(a, tw, s, variant);
+    case 192: return zpass_launch<192, Z_TM2>(a, tw, s, variant);
+    case 256: return zpass_launch<256, Z_TM2>(a, tw, s, variant);
+    case 384: return zpass_launch<384, Z_TM2>(a, tw, s, variant);
+    case 512: return zpass_launch<512, Z_TM2>(a, tw, s, variant);
+    case 768: return zpass_launch<768, Z_TM2>(a, tw, s, variant);
     default: return hipErrorInvalidValue;
   }
 }

@@ -580,6 +580,7 @@ struct EpiArgs {
   long long ms_in, ms_state;
   const double* nu_m;
   int members;  // grid.y (0 or 1: one member)
+  double force;  // linear forcing coefficient (E_DNS, E_SMAG): RHS += force * u
 };
 
 // Accumulate the contribution of transformed field f (value v) to the three RHS
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         u[c] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : uprev_[c * fstride + idx];
-        rhs[c] = EPI == E_TAU2 ? make_double2(0.0, 0.0) : cscale(u[c], -nu_ * (ksq + 1e-50));
+        rhs[c] = EPI == E_TAU2 ? make_double2(0.0, 0.0) : cscale(u[c], a.force - nu_ * (ksq + 1e-50));
         if (EPI == E_TAU2 && w0_) w0_[c * fstride + idx] = make_double2(0.0, 0.0);
         if (EPI == E_SMAG && w0_) w0_[c * fstride + idx] = make_double2(0.0, 0.0);
       }
@@ -794,7 +795,7 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
             }
             rhs[cc] = a.store_only ? make_double2(0.0, 0.0) : cadd(pl_[cc * fstride + idx], w);
           } else {
-            rhs[cc] = csub(acc[it][cc], cscale(u[cc], nu_ * (ksq + 1e-50)));
+            rhs[cc] = cadd(acc[it][cc], cscale(u[cc], a.force - nu_ * (ksq + 1e-50)));
           }
         }
         if constexpr (SGS) {

@@ -34,10 +34,14 @@ template <int MODE> constexpr int z_NT(int L) {
 
 bool length_supported(int L);
 
-hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s);
-hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s);
+// T: tile width (0 = the length's default pencil_T); pencil_tile(L, T) gives the width
+// used, or -1 when the width is not available for that length
+int pencil_tile(int L, int T);
+hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T = 0);
+hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T = 0);
 // mode: ZMode; returns hipErrorInvalidValue for an unsupported (L, mode)
-hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipStream_t s);
+// variant: 0 = the one-wavefront kernel where the length allows it, 1 = the block kernel
+hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipStream_t s, int variant = 0);
 int zpass_positions(int L, int mode);
 // epi: EpiMode; ns1 = spectral length along x
 hipError_t launch_epilogue(int L, int ns1, int epi, bool w0, const EpiArgs& a, const double2* tw, hipStream_t s);

@@ -15,11 +15,11 @@
 #include "small_kernels.hpp"
 
 namespace psx {
-hipError_t launch_zpass_dns(int L, const ZArgs& a, const double2* tw, hipStream_t s);
-hipError_t launch_zpass_smag(int L, const ZArgs& a, const double2* tw, hipStream_t s);
-hipError_t launch_zpass_tm2(int L, const ZArgs& a, const double2* tw, hipStream_t s);
-hipError_t launch_zpass_dt(int L, const ZArgs& a, const double2* tw, hipStream_t s);
-hipError_t launch_zpass_sgs(int L, const ZArgs& a, const double2* tw, hipStream_t s);
+hipError_t launch_zpass_dns(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+hipError_t launch_zpass_smag(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+hipError_t launch_zpass_tm2(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+hipError_t launch_zpass_dt(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+hipError_t launch_zpass_sgs(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
 int zpass_positions_dns(int L);
 int zpass_positions_smag(int L);
 int zpass_positions_tm2(int L);
@@ -37,13 +37,13 @@ int zpass_positions(int L, int mode) {
   return 0;
 }
 
-hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipStream_t s) {
+hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
   switch (mode) {
-    case Z_DNS: return launch_zpass_dns(L, a, tw, s);
-    case Z_SMAG: return launch_zpass_smag(L, a, tw, s);
-    case Z_TM2: return launch_zpass_tm2(L, a, tw, s);
-    case Z_DT: return launch_zpass_dt(L, a, tw, s);
-    case Z_SGS: return launch_zpass_sgs(L, a, tw, s);
+    case Z_DNS: return launch_zpass_dns(L, a, tw, s, variant);
+    case Z_SMAG: return launch_zpass_smag(L, a, tw, s, variant);
+    case Z_TM2: return launch_zpass_tm2(L, a, tw, s, variant);
+    case Z_DT: return launch_zpass_dt(L, a, tw, s, variant);
+    case Z_SGS: return launch_zpass_sgs(L, a, tw, s, variant);
   }
   return hipErrorInvalidValue;
 }
@@ -74,6 +74,9 @@ struct psx_context {
   // state and scratch array, with per-member viscosity nu_m (device) and dt
   int members = 1;
   double* nu_m = nullptr;
+  // kernel shapes (psx_autotune / psx_set_tuning): tile widths of the x-inverse,
+  // y-inverse and y-forward passes of the 3/2 pipeline (0 = default), z-pass variant
+  int tune[4] = {0, 0, 0, 0};
   // t-model family: Y1 (velocity after the 2x y-inverse), PL and PLQL on the base
   // grid, the filtered velocity and the filtered Leonard block (dynamic tau)
   double2* bufC = nullptr;
@@ -235,7 +238,8 @@ void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, in
   set_members(c, a, njobs);
   const double2* tw = twiddles(c, L1);
   TimeScope ts(c, kind);
-  check(c, launch_pencil_inverse(L1, a, tw, njobs * c->members, c->stream), "x_inverse");
+  check(c, launch_pencil_inverse(L1, a, tw, njobs * c->members, c->stream, kind == K_XINV ? c->tune[0] : 0),
+        "x_inverse");
 }
 
 // y-inverse: X[f][L1][N2][C] -> Y[f][L1][L2][C]
@@ -264,7 +268,8 @@ void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int
   set_members(c, a, njobs);
   const double2* tw = twiddles(c, L2);
   TimeScope ts(c, kind);
-  check(c, launch_pencil_inverse(L2, a, tw, njobs * c->members, c->stream), "y_inverse");
+  check(c, launch_pencil_inverse(L2, a, tw, njobs * c->members, c->stream, kind == K_YINV ? c->tune[1] : 0),
+        "y_inverse");
 }
 
 // y-forward with truncation: Z[f][L1][L2][C] -> W[f][L1][N2][C]
@@ -291,7 +296,7 @@ void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int
   set_members(c, a, nf);
   const double2* tw = twiddles(c, L2);
   TimeScope ts(c, K_YFWD);
-  check(c, launch_pencil_forward(L2, a, tw, nf * c->members, c->stream), "y_forward");
+  check(c, launch_pencil_forward(L2, a, tw, nf * c->members, c->stream, c->tune[2]), "y_forward");
 }
 
 // K1-K4 of the 3/2-rule pipeline: the y-forward of the z-pass products of uin lands
@@ -350,7 +355,7 @@ void front_32(psx_context* c, const double2* uin, bool smag, double c2) {
     if (P == 0 || ((M1 * M2) % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
     const double2* tw = twiddles(c, M3);
     TimeScope ts(c, K_ZPASS);
-    check(c, launch_zpass(M3, mode, z, tw, c->stream), "z_pass");
+    check(c, launch_zpass(M3, mode, z, tw, c->stream, c->tune[3]), "z_pass");
   }
   const int nF = smag ? 12 : 6;
   // K4: y-forward + truncation
@@ -396,6 +401,7 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
     e.Om[1] = c->p.Om2;
     e.Om[2] = c->p.Om3;
     e.members = c->members;
+    e.force = c->p.force;
     e.ms_in = member_stride(c, e.in);
     e.ms_state = 3LL * c->N1 * c->N2 * c->Nc;
     e.nu_m = c->nu_m;
@@ -482,6 +488,7 @@ void stage_dsmag(psx_context* c, const double2* uin, const double2* q0, double2*
   a.Om[0] = c->p.Om1;
   a.Om[1] = c->p.Om2;
   a.Om[2] = c->p.Om3;
+  a.force = c->p.force;
   TimeScope ts(c, K_XEPI);
   hipLaunchKernelGGL(dsmag_final, dim3((unsigned)std::min<long long>((total / 3 + 255) / 256, 8192)), dim3(256), 0,
                      c->stream, a);
@@ -749,8 +756,10 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   check(c, hipGetLastError(), "cfl_finalize");
 }
 
-void validate_params(psx_context* c, const psx_params* p) {
+void validate_params(psx_context* c, const psx_params* p, int model) {
   if (!p) fail(c, PSX_E_ARG, "params is NULL");
+  if (p->force != 0.0 && model >= PSX_MODEL_TMODEL && model <= PSX_MODEL_DYNAMIC_TAU)
+    fail(c, PSX_E_UNSUPPORTED, "linear forcing is available for the DNS and Smagorinsky models");
   if (!(p->L1 > 0 && p->L2 > 0 && p->L3 > 0)) fail(c, PSX_E_ARG, "box lengths must be positive");
   if (!(p->dx > 0 && p->dy > 0 && p->dz > 0)) fail(c, PSX_E_ARG, "grid spacings must be positive");
   if (p->rotate != 0 && p->rotate != 1) fail(c, PSX_E_ARG, "rotate must be 0 or 1");
@@ -777,7 +786,7 @@ int psx_create_batch(psx_context** out, int n1, int n2, int n3, int model, const
     if (members < 1) fail(c, PSX_E_ARG, "members must be >= 1");
     if (members > 1 && model != PSX_MODEL_DNS && model != PSX_MODEL_SMAGORINSKY)
       fail(c, PSX_E_UNSUPPORTED, "batched instances are available for the DNS and Smagorinsky models");
-    validate_params(c, params);
+    validate_params(c, params, model);
     c->members = members;
     if (model == PSX_MODEL_DYNAMIC_TAU && params->rotate)
       fail(c, PSX_E_UNSUPPORTED, "the dynamic tau-model is not available with rotation");
@@ -912,7 +921,7 @@ int psx_set_stream(psx_context* c, void* stream) {
 
 int psx_set_params(psx_context* c, const psx_params* params) {
   return guarded(c, [&] {
-    validate_params(c, params);
+    validate_params(c, params, c->model);
     c->p = *params;
   });
 }
@@ -1194,6 +1203,91 @@ int psx_timing_end(psx_context* c, double* ms, int* count) {
     }
     c->events_used = 0;
     c->event_kinds.clear();
+  });
+}
+
+}  // extern "C"
+
+// ---- kernel-shape autotuning -----------------------------------------------------
+extern "C" {
+
+int psx_set_tuning(psx_context* c, const int* tune) {
+  return guarded(c, [&] {
+    if (!tune) fail(c, PSX_E_ARG, "null pointer");
+    const int L[3] = {c->M1, c->M2, c->M2};
+    for (int k = 0; k < 3; ++k)
+      if (tune[k] != 0 && pencil_tile(L[k], tune[k]) < 0) fail(c, PSX_E_ARG, "tile width not available");
+    if (tune[3] != 0 && tune[3] != 1) fail(c, PSX_E_ARG, "z variant must be 0 or 1");
+    for (int k = 0; k < PSX_TUNE_N; ++k) c->tune[k] = tune[k];
+  });
+}
+
+int psx_get_tuning(const psx_context* c, int* tune) {
+  if (!c || !tune) return PSX_E_ARG;
+  for (int k = 0; k < PSX_TUNE_N; ++k) tune[k] = c->tune[k];
+  return PSX_OK;
+}
+
+int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
+  return guarded(c, [&] {
+    if (!uhat) fail(c, PSX_E_ARG, "null pointer");
+    if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
+      fail(c, PSX_E_UNSUPPORTED, "autotuning covers the 3/2-rule pipeline (DNS, Smagorinsky, DSMAG)");
+    if (reps < 1) reps = 1;
+    const size_t state = (size_t)c->members * 3 * c->N1 * c->N2 * c->Nc;
+    double2* tmp = nullptr;
+    check(c, hipMalloc(&tmp, state * sizeof(double2)), "hipMalloc(autotune)");
+    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS};
+    const int L[3] = {c->M1, c->M2, c->M2};
+    const bool was_timing = c->timing;
+    try {
+      for (int knob = 0; knob < PSX_TUNE_N; ++knob) {
+        std::vector<int> cand;
+        if (knob < 3) {
+          for (int T : {4, 8, 16})
+            if (pencil_tile(L[knob], T) == T) cand.push_back(T);
+        } else {
+          cand = {0, 1};
+        }
+        double best = 1e300;
+        int pick = c->tune[knob];
+        for (int v : cand) {
+          c->tune[knob] = v;
+          run_stage(c, (const double2*)uhat, nullptr, tmp, nullptr, 0.0, nullptr, 0, 0.0);  // warm-up
+          c->timing = true;
+          c->events_used = 0;
+          c->event_kinds.clear();
+          for (int r = 0; r < reps; ++r) run_stage(c, (const double2*)uhat, nullptr, tmp, nullptr, 0.0, nullptr, 0, 0.0);
+          check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+          double ms = 0.0;
+          int cnt = 0;
+          for (size_t i = 0; i < c->event_kinds.size(); ++i) {
+            if (c->event_kinds[i] != kinds[knob]) continue;
+            float t = 0.f;
+            check(c, hipEventElapsedTime(&t, c->events[2 * i], c->events[2 * i + 1]), "hipEventElapsedTime");
+            ms += t;
+            ++cnt;
+          }
+          c->timing = false;
+          c->events_used = 0;
+          c->event_kinds.clear();
+          const double avg = cnt ? ms / cnt : 1e300;
+          if (avg < best) {
+            best = avg;
+            pick = v;
+          }
+        }
+        c->tune[knob] = pick;
+      }
+    } catch (...) {
+      c->timing = was_timing;
+      hipFree(tmp);
+      throw;
+    }
+    c->timing = was_timing;
+    check(c, hipFree(tmp), "hipFree(autotune)");
+    if (tune_out)
+      for (int k = 0; k < PSX_TUNE_N; ++k) tune_out[k] = c->tune[k];
   });
 }
 

@@ -381,6 +381,7 @@ struct DsmagArgs {
   int N1, N2, N3, Nc, H;
   int rotate;
   double Om[3];
+  double force;  // linear forcing: RHS += force * u
 };
 
 // RHS_i = -i k_j N_ij - i k_i p - nu k^2 u_i + Cs^2 w0_i(T) (+ Coriolis), with the
@@ -439,7 +440,7 @@ __global__ void dsmag_final(DsmagArgs a) {
       // w0_i = -i k_j tau_ij - k_i (k k tau)/k^2 with tau = Cs^2 T
       const double2 w = cscale(csub(cmul_i<-1>(divT), cscale(kkT, k[i] * kinv)), cs2);
       double2 rhs = cadd(cmul_i<-1>(divN), cmul_i<-1>(cscale(p, k[i])));
-      rhs = csub(rhs, cscale(u[i], a.nu * (ksq + 1e-50)));
+      rhs = cadd(rhs, cscale(u[i], a.force - a.nu * (ksq + 1e-50)));
       rhs = cadd(rhs, w);
       if (a.rotate) rhs = cadd(rhs, cscale(cr[i], 2.0));
       if (a.w0) a.w0[i * fs + e] = w;

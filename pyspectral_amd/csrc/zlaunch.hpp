@@ -11,14 +11,16 @@
 namespace psx {
 
 template <int L, int MODE>
-hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s) {
+hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
   constexpr int P = z_P<MODE>(L);
   const dim3 grid((unsigned)(a.npos / P));
   if constexpr (PSX_ZWAVE != 0 && ZWave<L>::ok) {
-    hipLaunchKernelGGL((zwave<L, P, MODE>), grid, dim3(64), 0, s, a, tw);
-  } else {
-    hipLaunchKernelGGL((zpass<L, P, MODE, z_NT<MODE>(L)>), grid, dim3(z_NT<MODE>(L)), 0, s, a, tw);
+    if (variant == 0) {
+      hipLaunchKernelGGL((zwave<L, P, MODE>), grid, dim3(64), 0, s, a, tw);
+      return hipGetLastError();
+    }
   }
+  hipLaunchKernelGGL((zpass<L, P, MODE, z_NT<MODE>(L)>), grid, dim3(z_NT<MODE>(L)), 0, s, a, tw);
   return hipGetLastError();
 }
 

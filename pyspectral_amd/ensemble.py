@@ -23,7 +23,7 @@ class Ensemble:
     is member i's (3, N1, N2, N3/2+1) view.  ``dt`` tensors hold one value per member,
     indexed by ``slot(i)``."""
 
-    def __init__(self, dims, nu, model=1, *, L=(2 * math.pi,) * 3, cfl=0.5, Cs=0.16, device=None):
+    def __init__(self, dims, nu, model=1, *, L=(2 * math.pi,) * 3, cfl=0.5, Cs=0.16, force=0.0, device=None):
         if not torch.cuda.is_available():
             raise _lib.PsxError("Ensemble needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -37,7 +37,7 @@ class Ensemble:
         L1, L2, L3 = (float(x) for x in L)
         self.params = _lib.psx_params(nu=self.nu[0], L1=L1, L2=L2, L3=L3, dx=1.0, dy=1.0, dz=1.0, cfl=float(cfl),
                                       Cs=float(Cs), Ct=0.1, tau_a=-10.0, tau_b=-10.0, kc=1, rotate=0,
-                                      Om1=0.0, Om2=0.0, Om3=0.0)
+                                      Om1=0.0, Om2=0.0, Om3=0.0, force=float(force))
         flat = (C.c_int * (3 * self.count))(*[n for d in self.dims for n in d])
         nus = (C.c_double * self.count)(*self.nu)
         ens = C.c_void_p()

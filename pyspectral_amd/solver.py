@@ -31,12 +31,13 @@ class SpectralSolver:
     ``nu``, ``L1..L3``, ``dx..dz`` (case grid spacings, used for Delta and the
     CFL limit), ``cfl`` (<= 0 means fixed dt = -cfl), ``Cs`` (0.16 in
     ``src/RHSfunctions.py:197``), ``Ct``, ``tau_a``/``tau_b`` (``dt0``/``dt1``),
-    ``kc``, ``rotate`` and ``Om``.
+    ``kc``, ``rotate`` and ``Om``; ``force`` is a linear forcing coefficient A (RHS += A u,
+    no counterpart in the reference; DNS and Smagorinsky models).
     """
 
     def __init__(self, N1, N2, N3, model=0, *, nu, L=(2 * math.pi,) * 3, dx=None, dy=None, dz=None,
                  cfl=-0.01, Cs=0.16, Ct=0.1, tau_a=-10.0, tau_b=-10.0, kc=None, rotate=0,
-                 Om=(0.0, 0.0, 0.0), device=None):
+                 Om=(0.0, 0.0, 0.0), force=0.0, device=None):
         if not torch.cuda.is_available():
             raise _lib.PsxError("SpectralSolver needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -50,7 +51,7 @@ class SpectralSolver:
             dx=float(L1 / N1 if dx is None else dx), dy=float(L2 / N2 if dy is None else dy),
             dz=float(L3 / N3 if dz is None else dz), cfl=float(cfl), Cs=float(Cs), Ct=float(Ct),
             tau_a=float(tau_a), tau_b=float(tau_b), kc=int(N1 // 2 if kc is None else kc),
-            rotate=int(rotate), Om1=float(Om[0]), Om2=float(Om[1]), Om3=float(Om[2]))
+            rotate=int(rotate), Om1=float(Om[0]), Om2=float(Om[1]), Om3=float(Om[2]), force=float(force))
         ctx = C.c_void_p()
         with torch.cuda.device(self.device):
             stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -152,6 +153,25 @@ class SpectralSolver:
         self._sync_stream()
         _lib.check(self.ctx, self.lib.psx_step_graph(self.ctx, _ptr(q), _ptr(qs), _ptr(dt), _ptr(t)),
                    "psx_step_graph")
+
+    TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant")
+
+    def tuning(self) -> dict:
+        t = (C.c_int * 4)()
+        _lib.check(self.ctx, self.lib.psx_get_tuning(self.ctx, t), "psx_get_tuning")
+        return dict(zip(self.TUNING_KEYS, list(t)))
+
+    def set_tuning(self, tuning: dict) -> None:
+        t = (C.c_int * 4)(*[int(tuning.get(k, 0)) for k in self.TUNING_KEYS])
+        _lib.check(self.ctx, self.lib.psx_set_tuning(self.ctx, t), "psx_set_tuning")
+
+    def autotune(self, uhat: torch.Tensor, reps: int = 5) -> dict:
+        """Time every tile width / z-pass variant on ``reps`` RHS evaluations of ``uhat``
+        and keep the fastest of each (DNS, Smagorinsky, DSMAG)."""
+        self._sync_stream()
+        t = (C.c_int * 4)()
+        _lib.check(self.ctx, self.lib.psx_autotune(self.ctx, _ptr(uhat), int(reps), t), "psx_autotune")
+        return dict(zip(self.TUNING_KEYS, list(t)))
 
     _FILTERS = {"sharp": _lib.PSX_FILTER_SHARP, "box": _lib.PSX_FILTER_BOX}
 

@@ -44,4 +44,4 @@ def test_null_context_is_rejected():
 def test_params_struct_layout():
     import ctypes as C
     # 12 doubles (nu, L1-3, dx-dz, cfl, Cs, Ct, tau_a, tau_b), int kc, int rotate, 3 doubles
-    assert C.sizeof(_lib.psx_params) == 12 * 8 + 2 * 4 + 3 * 8
+    assert C.sizeof(_lib.psx_params) == 12 * 8 + 2 * 4 + 4 * 8

@@ -129,6 +129,17 @@ def test_rhs_long_z_matches_oracle(shape, models):
         s2.close()
 
 
+@pytest.mark.parametrize("model", [0, 1, 7])
+def test_rhs_with_linear_forcing(model):
+    N, nu, a = 16, 0.012, 0.1
+    u = _field(N, 31, nyquist_noise=True)
+    st = so.State(so.Grid(N, N, N), *u, nu=nu, turb_model=model, force=a)
+    so.compute_rhs(st, FFT)
+    s = _solver(N, model, nu, force=a)
+    r = _host(s.rhs(_dev(u), 0.0))
+    assert _rel(r, st.compact()) < 1e-12
+
+
 def test_rhs_tgv_closed_form():
     N, nu = 32, 1.0 / 400
     u, v, w, _ = so.taylor_green(N, N, N)

@@ -183,3 +183,29 @@ def test_smagorinsky_shear_closed_form():
     wu, wv, ww = so.sgs_divergence(st.grid, tau)
     assert np.allclose(wu, -1j * st.grid.k2 * tau[3], atol=1e-15)
     assert np.abs(wv).max() < 1e-14 and np.abs(ww).max() < 1e-14
+
+
+def test_linear_forcing_amplification():
+    """With linear forcing f = a u the Kolmogorov mode (no nonlinear term) grows or
+    decays as R((a - nu k^2) dt)^n exactly, and for every model with forcing
+    RHS(force) - RHS(0) = force * u."""
+    N, nu, dt, a, A = 8, 0.05, 0.1, 0.3, 0.7
+    x = np.arange(N) * 2 * np.pi / N
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    s = np.sqrt(N ** 3)
+    uh = [np.fft.rfftn(A * np.sin(2 * Y)) / s, np.zeros((N, N, N // 2 + 1), complex),
+          np.zeros((N, N, N // 2 + 1), complex)]
+    st = _state(uh, N=N, nu=nu, dt=dt, force=a)
+    for _ in range(4):
+        so.advance_rk4(st, FFT)
+    z = (a - nu * 4) * dt
+    R = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    assert np.abs(st.compact()[0] - uh[0] * R ** 4).max() < 1e-14 * np.abs(uh[0]).max()
+    u = _rand_div_free(N, 21)
+    for model in (0, 1, 7):
+        s0 = _state(u, model=model, N=N)
+        s1 = _state(u, model=model, N=N, force=a)
+        so.compute_rhs(s0, FFT)
+        so.compute_rhs(s1, FFT)
+        expect = a * np.stack([s0.uhat, s0.vhat, s0.what])
+        assert np.abs(s1.compact() - s0.compact() - expect).max() < 1e-13
