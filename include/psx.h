@@ -159,11 +159,15 @@ int psx_tau(psx_context* ctx, double* out_host8);
 /* Kernel shapes (no counterpart in the reference): tune[0..2] = tile widths (k3 columns
  * per block: 4, 8 or 16; 0 = the built-in default) of the x-inverse, y-inverse and
  * y-forward passes, tune[3] = z-pass variant (0: one wavefront per position where the
- * z length is a multiple of 64, 1: the block-wide Stockham kernel).  psx_autotune times
- * every candidate of each knob on reps RHS evaluations of uhat with HIP events, keeps
- * the fastest (knob by knob), and returns the choice in tune_out (may be NULL); it
- * allocates one temporary state.  DNS, Smagorinsky and DSMAG contexts. */
-#define PSX_TUNE_N 4
+ * z length is a multiple of 64, 1: the block-wide Stockham kernel), tune[4] = number of
+ * x-row chunks (0 or 1: none; at most 8) in which the y-inverse, z and y-forward passes
+ * run alternately on the context stream and a second library stream, so that their
+ * HBM- and compute-bound phases overlap.  psx_autotune times every candidate of each
+ * knob on reps RHS evaluations of uhat with HIP events, keeps the fastest (knob by
+ * knob), and returns the choice in tune_out (may be NULL); it allocates one temporary
+ * state, and the chunked pipeline's buffers when it tries chunks, as psx_set_tuning
+ * does when it sets them.  DNS, Smagorinsky and DSMAG contexts. */
+#define PSX_TUNE_N 5
 int psx_set_tuning(psx_context* ctx, const int* tune);
 int psx_get_tuning(const psx_context* ctx, int* tune);
 int psx_autotune(psx_context* ctx, const double* uhat, int reps, int* tune_out);

@@ -76,7 +76,15 @@ struct psx_context {
   double* nu_m = nullptr;
   // kernel shapes (psx_autotune / psx_set_tuning): tile widths of the x-inverse,
   // y-inverse and y-forward passes of the 3/2 pipeline (0 = default), z-pass variant
-  int tune[4] = {0, 0, 0, 0};
+  int tune[5] = {0, 0, 0, 0, 0};
+  // x-chunk pipelining of the y-inverse / z / y-forward passes over two streams
+  // (tune[4] chunks): separate X and W buffers so that chunks never alias, the
+  // second stream and its fork/stagger/join events
+  double2* bufX = nullptr;
+  double2* bufW = nullptr;
+  size_t nX = 0, nW = 0;  // per member
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_stagger = nullptr, ev_join = nullptr;
   // t-model family: Y1 (velocity after the 2x y-inverse), PL and PLQL on the base
   // grid, the filtered velocity and the filtered Leonard block (dynamic tau)
   double2* bufC = nullptr;
@@ -205,6 +213,8 @@ PencilArgs pencil_base() {
 long long member_stride(const psx_context* c, const void* p) {
   if (p == c->bufA) return (long long)c->nA;
   if (p == c->bufB) return (long long)c->nB;
+  if (p == c->bufX) return (long long)c->nX;
+  if (p == c->bufW) return (long long)c->nW;
   return 3LL * c->N1 * c->N2 * c->Nc;
 }
 void set_members(const psx_context* c, PencilArgs& a, int njobs) {
@@ -244,8 +254,10 @@ void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, in
 
 // y-inverse: X[f][L1][N2][C] -> Y[f][L1][L2][C]
 void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int C, int keep_nyq, int njobs,
-               const int* jin, const int* jplain, const int* jderiv, int kind = K_YINV, int ns2 = 0) {
+               const int* jin, const int* jplain, const int* jderiv, int kind = K_YINV, int ns2 = 0, int x0 = 0,
+               int x1 = -1) {
   if (ns2 == 0) ns2 = c->N2;
+  if (x1 < 0) x1 = L1;
   PencilArgs a = pencil_base();
   a.in = X;
   a.out = Y;
@@ -255,7 +267,7 @@ void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int
   a.out_sf = (long long)L1 * L2 * C;
   a.out_sa = (long long)L2 * C;
   a.out_sl = C;
-  a.A = L1;
+  a.A = x1 - x0;
   a.C = C;
   a.Nspec = ns2;
   a.keep_nyq = keep_nyq;
@@ -266,6 +278,8 @@ void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int
     a.job_deriv[j] = jderiv ? jderiv[j] : -1;
   }
   set_members(c, a, njobs);
+  a.in += x0 * a.in_sa;
+  a.out += x0 * a.out_sa;
   const double2* tw = twiddles(c, L2);
   TimeScope ts(c, kind);
   check(c, launch_pencil_inverse(L2, a, tw, njobs * c->members, c->stream, kind == K_YINV ? c->tune[1] : 0),
@@ -274,8 +288,9 @@ void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int
 
 // y-forward with truncation: Z[f][L1][L2][C] -> W[f][L1][N2][C]
 void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int C, int keep_nyq, int nf,
-               int ns2 = 0) {
+               int ns2 = 0, int x0 = 0, int x1 = -1) {
   if (ns2 == 0) ns2 = c->N2;
+  if (x1 < 0) x1 = L1;
   PencilArgs a = pencil_base();
   a.in = Z;
   a.out = W;
@@ -285,7 +300,7 @@ void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int
   a.out_sf = (long long)L1 * ns2 * C;
   a.out_sa = (long long)ns2 * C;
   a.out_sl = C;
-  a.A = L1;
+  a.A = x1 - x0;
   a.C = C;
   a.Nspec = ns2;
   a.keep_nyq = keep_nyq;
@@ -294,72 +309,104 @@ void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int
     a.job_plain[j] = j;
   }
   set_members(c, a, nf);
+  a.in += x0 * a.in_sa;
+  a.out += x0 * a.out_sa;
   const double2* tw = twiddles(c, L2);
   TimeScope ts(c, K_YFWD);
   check(c, launch_pencil_forward(L2, a, tw, nf * c->members, c->stream, c->tune[2]), "y_forward");
 }
 
-// K1-K4 of the 3/2-rule pipeline: the y-forward of the z-pass products of uin lands
-// in bufB (W layout).  smag adds the strain-rate pencils and the stress
-// -2 c2 |S| S_ij (c2 = (Cs Delta)^2 for the static closure).
-void front_32(psx_context* c, const double2* uin, bool smag, double c2) {
+// K1-K4 of the 3/2-rule pipeline; returns the buffer holding W (the y-forward of the
+// z-pass products of uin).  smag adds the strain-rate pencils and the stress
+// -2 c2 |S| S_ij (c2 = (Cs Delta)^2 for the static closure).  With tune[4] = n > 1
+// the y-inverse, z and y-forward passes run in n chunks of x rows, alternating between
+// the context stream and a second stream (the second starting one y-inverse later), so
+// an HBM-bound y pass of one chunk overlaps the z pass of another; X and W then live
+// in their own buffers so that no chunk reads what another writes.
+double2* front_32(psx_context* c, const double2* uin, bool smag, double c2) {
   const int M1 = c->M1, M2 = c->M2, M3 = c->M3, H = c->H;
-  double2* A = c->bufA;
-  double2* B = c->bufB;
+  const int nch = c->tune[4] > 1 ? c->tune[4] : 1;
+  double2* X = nch > 1 ? c->bufX : c->bufA;
+  double2* Y = c->bufB;
+  double2* Z = c->bufA;
+  double2* W = nch > 1 ? c->bufW : c->bufB;
   // K1: x-inverse (+ d/dx for the closure)
   {
     const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2}, jd[3] = {3, 4, 5};
-    x_inverse(c, uin, A, M1, H, 0, 3, jin, jp, smag ? jd : nullptr);
+    x_inverse(c, uin, X, M1, H, 0, 3, jin, jp, smag ? jd : nullptr);
   }
-  // K2: y-inverse (+ d/dy of the velocity for the closure)
   const int nY = smag ? 9 : 3;
-  {
-    const int jin[6] = {0, 1, 2, 3, 4, 5}, jp[6] = {0, 1, 2, 3, 4, 5}, jd[6] = {6, 7, 8, -1, -1, -1};
-    y_inverse(c, A, B, M1, M2, H, 0, smag ? 6 : 3, jin, jp, smag ? jd : nullptr);
-  }
-  // K3: z c2r -> products (+ Smagorinsky stress) -> z r2c
-  {
-    ZArgs z;
-    std::memset(&z, 0, sizeof(z));
-    z.in0 = B;
-    z.in_sf0 = (long long)M1 * M2 * H;
-    z.Hin0 = H;
-    z.nin0 = nY;
-    z.in1 = nullptr;
-    z.out = A;
-    z.out_sf = (long long)M1 * M2 * H;
-    z.Hout = H;
-    z.npos = M1 * M2 * c->members;
-    z.npos_m = c->members > 1 ? M1 * M2 : 0;
-    z.ms_in0 = member_stride(c, z.in0);
-    z.ms_out = member_stride(c, z.out);
-    z.kzscale = 2.0 * kPi / c->p.L3;
-    const double Nt = (double)c->N1 * c->N2 * c->N3;
-    z.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
-    z.c2 = c2;
-    auto one = [](Recipe& r, int f) { r.nterm = 1; r.f[0] = f; r.coef[0] = 1.0; r.dz[0] = 0; };
-    one(z.rec[0], 0);
-    one(z.rec[1], 1);
-    one(z.rec[2], 2);
-    if (smag) {
-      // Y fields: u v w ux vx wx uy vy wy; pencils u v w S11 S22 S33 S12 S13 S23
-      one(z.rec[3], 3);
-      one(z.rec[4], 7);
-      z.rec[5] = Recipe{1, {2, 0}, {1.0, 0.0}, {1, 0}};
-      z.rec[6] = Recipe{2, {6, 4}, {0.5, 0.5}, {0, 0}};
-      z.rec[7] = Recipe{2, {0, 5}, {0.5, 0.5}, {1, 0}};
-      z.rec[8] = Recipe{2, {1, 8}, {0.5, 0.5}, {1, 0}};
-    }
-    const int mode = smag ? Z_SMAG : Z_DNS;
-    const int P = zpass_positions(M3, mode);
-    if (P == 0 || ((M1 * M2) % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
-    const double2* tw = twiddles(c, M3);
-    TimeScope ts(c, K_ZPASS);
-    check(c, launch_zpass(M3, mode, z, tw, c->stream, c->tune[3]), "z_pass");
-  }
   const int nF = smag ? 12 : 6;
-  // K4: y-forward + truncation
-  y_forward(c, A, B, M1, M2, H, 0, nF);
+  ZArgs z;
+  std::memset(&z, 0, sizeof(z));
+  z.in_sf0 = (long long)M1 * M2 * H;
+  z.Hin0 = H;
+  z.nin0 = nY;
+  z.in1 = nullptr;
+  z.out_sf = (long long)M1 * M2 * H;
+  z.Hout = H;
+  z.ms_in0 = member_stride(c, Y);
+  z.ms_out = member_stride(c, Z);
+  z.kzscale = 2.0 * kPi / c->p.L3;
+  const double Nt = (double)c->N1 * c->N2 * c->N3;
+  z.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
+  z.c2 = c2;
+  auto one = [](Recipe& r, int f) { r.nterm = 1; r.f[0] = f; r.coef[0] = 1.0; r.dz[0] = 0; };
+  one(z.rec[0], 0);
+  one(z.rec[1], 1);
+  one(z.rec[2], 2);
+  if (smag) {
+    // Y fields: u v w ux vx wx uy vy wy; pencils u v w S11 S22 S33 S12 S13 S23
+    one(z.rec[3], 3);
+    one(z.rec[4], 7);
+    z.rec[5] = Recipe{1, {2, 0}, {1.0, 0.0}, {1, 0}};
+    z.rec[6] = Recipe{2, {6, 4}, {0.5, 0.5}, {0, 0}};
+    z.rec[7] = Recipe{2, {0, 5}, {0.5, 0.5}, {1, 0}};
+    z.rec[8] = Recipe{2, {1, 8}, {0.5, 0.5}, {1, 0}};
+  }
+  const int mode = smag ? Z_SMAG : Z_DNS;
+  const int P = zpass_positions(M3, mode);
+  if (P == 0 || (M2 % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
+  const double2* tw3 = twiddles(c, M3);
+  const hipStream_t main = c->stream;
+  struct Restore {  // the context stream is switched per chunk; put it back on any exit
+    psx_context* c;
+    hipStream_t s;
+    ~Restore() { c->stream = s; }
+  } restore{c, main};
+  if (nch > 1) {
+    check(c, hipEventRecord(c->ev_fork, main), "hipEventRecord");
+    check(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0), "hipStreamWaitEvent");
+  }
+  for (int ch = 0; ch < nch; ++ch) {
+    const int x0 = (int)((long long)M1 * ch / nch), x1 = (int)((long long)M1 * (ch + 1) / nch);
+    c->stream = (ch & 1) ? c->stream2 : main;
+    if (ch == 1) check(c, hipStreamWaitEvent(c->stream, c->ev_stagger, 0), "hipStreamWaitEvent");
+    // K2: y-inverse (+ d/dy of the velocity for the closure)
+    {
+      const int jin[6] = {0, 1, 2, 3, 4, 5}, jp[6] = {0, 1, 2, 3, 4, 5}, jd[6] = {6, 7, 8, -1, -1, -1};
+      y_inverse(c, X, Y, M1, M2, H, 0, smag ? 6 : 3, jin, jp, smag ? jd : nullptr, K_YINV, 0, x0, x1);
+    }
+    if (ch == 0 && nch > 1) check(c, hipEventRecord(c->ev_stagger, c->stream), "hipEventRecord");
+    // K3: z c2r -> products (+ Smagorinsky stress) -> z r2c, on positions [x0 M2, x1 M2)
+    {
+      ZArgs zc = z;
+      zc.in0 = Y + (long long)x0 * M2 * H;
+      zc.out = Z + (long long)x0 * M2 * H;
+      zc.npos = (x1 - x0) * M2 * c->members;
+      zc.npos_m = c->members > 1 ? (x1 - x0) * M2 : 0;
+      TimeScope ts(c, K_ZPASS);
+      check(c, launch_zpass(M3, mode, zc, tw3, c->stream, c->tune[3]), "z_pass");
+    }
+    // K4: y-forward + truncation
+    y_forward(c, Z, W, M1, M2, H, 0, nF, 0, x0, x1);
+  }
+  c->stream = main;
+  if (nch > 1) {
+    check(c, hipEventRecord(c->ev_join, c->stream2), "hipEventRecord");
+    check(c, hipStreamWaitEvent(main, c->ev_join, 0), "hipStreamWaitEvent");
+  }
+  return W;
 }
 
 double delta_grid(const psx_context* c) { return std::cbrt(c->p.dx * c->p.dy * c->p.dz); }
@@ -369,9 +416,8 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
               double2* w0, int zero_nyq_out) {
   const bool smag = c->model == PSX_MODEL_SMAGORINSKY;
   const int M1 = c->M1, H = c->H;
-  double2* B = c->bufB;
   const double cd = c->p.Cs * delta_grid(c);
-  front_32(c, uin, smag, cd * cd);
+  double2* B = front_32(c, uin, smag, cd * cd);
   // K5: x-forward + truncation + projection/closure/viscous + stage update
   {
     EpiArgs e;
@@ -419,10 +465,10 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
 // whole stage stays on the device (graph-capturable).
 constexpr double kDsmagAlpha = 2.0;
 
-void store_x_forward(psx_context* c, int first) {
+void store_x_forward(psx_context* c, const double2* W, int first) {
   const int M1 = c->M1, H = c->H;
   PencilArgs a = pencil_base();
-  a.in = c->bufB;
+  a.in = W;
   a.out = c->dsD;
   a.in_sf = (long long)M1 * c->N2 * H;
   a.in_sa = H;
@@ -452,10 +498,8 @@ void stage_dsmag(psx_context* c, const double2* uin, const double2* q0, double2*
   hipLaunchKernelGGL(apply_filter, dim3(eb), dim3(256), 0, c->stream, uin, c->uf, 3, c->N1, c->N2, c->Nc, F_SHARP,
                      kL, 0.0, 0.0, 0.0, 0.0);
   check(c, hipGetLastError(), "apply_filter");
-  front_32(c, uin, true, d * d);
-  store_x_forward(c, 0);
-  front_32(c, c->uf, true, kDsmagAlpha * kDsmagAlpha * d * d);
-  store_x_forward(c, 12);
+  store_x_forward(c, front_32(c, uin, true, d * d), 0);
+  store_x_forward(c, front_32(c, c->uf, true, kDsmagAlpha * kDsmagAlpha * d * d), 12);
   {
     TimeScope ts(c, K_OTHER);
     hipLaunchKernelGGL(dsmag_partials, dim3(c->ndsPart), dim3(NT), 0, c->stream, (const double2*)c->dsD, c->dsPart,
@@ -889,6 +933,12 @@ int psx_destroy(psx_context* c) {
   if (!c) return PSX_E_ARG;
   hipFree(c->bufA);
   hipFree(c->bufB);
+  hipFree(c->bufX);
+  hipFree(c->bufW);
+  if (c->stream2) hipStreamDestroy(c->stream2);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_stagger) hipEventDestroy(c->ev_stagger);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
   hipFree(c->nu_m);
   hipFree(c->cfl_part);
   hipFree(c->scal);
@@ -1208,6 +1258,23 @@ int psx_timing_end(psx_context* c, double* ms, int* count) {
 
 }  // extern "C"
 
+// Buffers, stream and events of the chunked y/z pipeline (allocated outside any capture).
+void ensure_pipeline(psx_context* c) {
+  if (c->bufX) return;
+  const bool smag = c->model == PSX_MODEL_SMAGORINSKY || c->model == PSX_MODEL_DSMAG;
+  const size_t MNH = (size_t)c->M1 * c->N2 * c->H;
+  c->nX = (smag ? 6 : 3) * MNH;
+  c->nW = (smag ? 12 : 6) * MNH;
+  const size_t mb = (size_t)c->members;
+  check(c, hipMalloc(&c->bufX, mb * c->nX * sizeof(double2)), "hipMalloc(bufX)");
+  check(c, hipMalloc(&c->bufW, mb * c->nW * sizeof(double2)), "hipMalloc(bufW)");
+  c->scratch_bytes += (long long)(mb * (c->nX + c->nW) * sizeof(double2));
+  check(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking), "hipStreamCreate");
+  check(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
+  check(c, hipEventCreateWithFlags(&c->ev_stagger, hipEventDisableTiming), "hipEventCreate");
+  check(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
+}
+
 // ---- kernel-shape autotuning -----------------------------------------------------
 extern "C" {
 
@@ -1218,6 +1285,12 @@ int psx_set_tuning(psx_context* c, const int* tune) {
     for (int k = 0; k < 3; ++k)
       if (tune[k] != 0 && pencil_tile(L[k], tune[k]) < 0) fail(c, PSX_E_ARG, "tile width not available");
     if (tune[3] != 0 && tune[3] != 1) fail(c, PSX_E_ARG, "z variant must be 0 or 1");
+    if (tune[4] < 0 || tune[4] > 8 || (tune[4] > 1 && c->M1 < 2 * tune[4])) fail(c, PSX_E_ARG, "bad chunk count");
+    if (tune[4] > 1) {
+      if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
+        fail(c, PSX_E_UNSUPPORTED, "chunked pipeline: 3/2-rule models only");
+      ensure_pipeline(c);
+    }
     for (int k = 0; k < PSX_TUNE_N; ++k) c->tune[k] = tune[k];
   });
 }
@@ -1237,7 +1310,10 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     const size_t state = (size_t)c->members * 3 * c->N1 * c->N2 * c->Nc;
     double2* tmp = nullptr;
     check(c, hipMalloc(&tmp, state * sizeof(double2)), "hipMalloc(autotune)");
-    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS};
+    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1};  // -1: the whole stage
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    check(c, hipEventCreate(&t0), "hipEventCreate");
+    check(c, hipEventCreate(&t1), "hipEventCreate");
     const int L[3] = {c->M1, c->M2, c->M2};
     const bool was_timing = c->timing;
     try {
@@ -1246,27 +1322,43 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
         if (knob < 3) {
           for (int T : {4, 8, 16})
             if (pencil_tile(L[knob], T) == T) cand.push_back(T);
-        } else {
+        } else if (knob == 3) {
           cand = {0, 1};
+        } else {
+          cand = {1, 2, 4};
+          ensure_pipeline(c);
         }
         double best = 1e300;
         int pick = c->tune[knob];
         for (int v : cand) {
           c->tune[knob] = v;
           run_stage(c, (const double2*)uhat, nullptr, tmp, nullptr, 0.0, nullptr, 0, 0.0);  // warm-up
-          c->timing = true;
-          c->events_used = 0;
-          c->event_kinds.clear();
-          for (int r = 0; r < reps; ++r) run_stage(c, (const double2*)uhat, nullptr, tmp, nullptr, 0.0, nullptr, 0, 0.0);
-          check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
           double ms = 0.0;
           int cnt = 0;
-          for (size_t i = 0; i < c->event_kinds.size(); ++i) {
-            if (c->event_kinds[i] != kinds[knob]) continue;
+          if (kinds[knob] < 0) {
+            check(c, hipEventRecord(t0, c->stream), "hipEventRecord");
+            for (int r = 0; r < reps; ++r)
+              run_stage(c, (const double2*)uhat, nullptr, tmp, nullptr, 0.0, nullptr, 0, 0.0);
+            check(c, hipEventRecord(t1, c->stream), "hipEventRecord");
+            check(c, hipEventSynchronize(t1), "hipEventSynchronize");
             float t = 0.f;
-            check(c, hipEventElapsedTime(&t, c->events[2 * i], c->events[2 * i + 1]), "hipEventElapsedTime");
-            ms += t;
-            ++cnt;
+            check(c, hipEventElapsedTime(&t, t0, t1), "hipEventElapsedTime");
+            ms = t;
+            cnt = reps;
+          } else {
+            c->timing = true;
+            c->events_used = 0;
+            c->event_kinds.clear();
+            for (int r = 0; r < reps; ++r)
+              run_stage(c, (const double2*)uhat, nullptr, tmp, nullptr, 0.0, nullptr, 0, 0.0);
+            check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            for (size_t i = 0; i < c->event_kinds.size(); ++i) {
+              if (c->event_kinds[i] != kinds[knob]) continue;
+              float t = 0.f;
+              check(c, hipEventElapsedTime(&t, c->events[2 * i], c->events[2 * i + 1]), "hipEventElapsedTime");
+              ms += t;
+              ++cnt;
+            }
           }
           c->timing = false;
           c->events_used = 0;
@@ -1282,9 +1374,13 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     } catch (...) {
       c->timing = was_timing;
       hipFree(tmp);
+      hipEventDestroy(t0);
+      hipEventDestroy(t1);
       throw;
     }
     c->timing = was_timing;
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
     check(c, hipFree(tmp), "hipFree(autotune)");
     if (tune_out)
       for (int k = 0; k < PSX_TUNE_N; ++k) tune_out[k] = c->tune[k];

@@ -154,22 +154,22 @@ class SpectralSolver:
         _lib.check(self.ctx, self.lib.psx_step_graph(self.ctx, _ptr(q), _ptr(qs), _ptr(dt), _ptr(t)),
                    "psx_step_graph")
 
-    TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant")
+    TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant", "yz_chunks")
 
     def tuning(self) -> dict:
-        t = (C.c_int * 4)()
+        t = (C.c_int * 5)()
         _lib.check(self.ctx, self.lib.psx_get_tuning(self.ctx, t), "psx_get_tuning")
         return dict(zip(self.TUNING_KEYS, list(t)))
 
     def set_tuning(self, tuning: dict) -> None:
-        t = (C.c_int * 4)(*[int(tuning.get(k, 0)) for k in self.TUNING_KEYS])
+        t = (C.c_int * 5)(*[int(tuning.get(k, 0)) for k in self.TUNING_KEYS])
         _lib.check(self.ctx, self.lib.psx_set_tuning(self.ctx, t), "psx_set_tuning")
 
     def autotune(self, uhat: torch.Tensor, reps: int = 5) -> dict:
         """Time every tile width / z-pass variant on ``reps`` RHS evaluations of ``uhat``
         and keep the fastest of each (DNS, Smagorinsky, DSMAG)."""
         self._sync_stream()
-        t = (C.c_int * 4)()
+        t = (C.c_int * 5)()
         _lib.check(self.ctx, self.lib.psx_autotune(self.ctx, _ptr(uhat), int(reps), t), "psx_autotune")
         return dict(zip(self.TUNING_KEYS, list(t)))
 
