@@ -172,6 +172,31 @@ int psx_set_tuning(psx_context* ctx, const int* tune);
 int psx_get_tuning(const psx_context* ctx, int* tune);
 int psx_autotune(psx_context* ctx, const double* uhat, int reps, int* tune_out);
 
+/* Learned subgrid closure network (no counterpart in the reference): a pointwise MLP
+ * 9 -> 64 -> 64 -> 64 -> 6 with tanh hidden layers mapping filtered velocity-gradient
+ * components to the six SGS stresses, BF16 operands with FP32 accumulation on the matrix
+ * cores, FP32 master weights.  Parameters (PSX_MLP_NPARAM floats, row-major):
+ * W1[64][9] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[6][64] b4[6].  x: (n, 9) and
+ * y, target: (n, 6) float32 device arrays; inputs and targets are normalised inside
+ * with psx_mlp_set_norm's means and scales.  psx_mlp_grad computes the MSE loss (into
+ * loss_dev, may be NULL) and the parameter gradient (psx_mlp_gradient copies it out);
+ * psx_mlp_train_step adds one Adam step (beta 0.9/0.999, eps 1e-8).  Training batches
+ * are multiples of 16 points. */
+#define PSX_MLP_NPARAM 9350
+typedef struct psx_mlp psx_mlp;
+int psx_mlp_create(psx_mlp** net, int device, void* stream, unsigned long long seed);
+int psx_mlp_destroy(psx_mlp* net);
+const char* psx_mlp_last_error(const psx_mlp* net);
+int psx_mlp_set_stream(psx_mlp* net, void* stream);
+int psx_mlp_get_params(psx_mlp* net, float* host_params);
+int psx_mlp_set_params(psx_mlp* net, const float* host_params);
+int psx_mlp_set_norm(psx_mlp* net, const float* x_mean, const float* x_scale, const float* y_mean,
+                     const float* y_scale);
+int psx_mlp_forward(psx_mlp* net, const float* x, float* y, long long n);
+int psx_mlp_grad(psx_mlp* net, const float* x, const float* target, long long n, float* loss_dev);
+int psx_mlp_gradient(psx_mlp* net, float* grad_dev);
+int psx_mlp_train_step(psx_mlp* net, const float* x, const float* target, long long n, float lr, float* loss_dev);
+
 /* Instrumentation (no counterpart in the reference, which only prints wall time,
  * src/PySpec_3d.py:136-145): bracket every kernel launch with HIP events on the
  * context stream between begin and end.  psx_timing_end synchronizes the

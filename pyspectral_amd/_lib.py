@@ -30,7 +30,12 @@ EXPORTS = (
     "psx_ensemble_create", "psx_ensemble_destroy", "psx_ensemble_last_error", "psx_ensemble_size",
     "psx_ensemble_offset", "psx_ensemble_slot", "psx_ensemble_set_stream", "psx_ensemble_step",
     "psx_ensemble_step_graph", "psx_ensemble_stats", "psx_set_tuning", "psx_get_tuning", "psx_autotune",
+    "psx_mlp_create", "psx_mlp_destroy", "psx_mlp_last_error", "psx_mlp_set_stream", "psx_mlp_get_params",
+    "psx_mlp_set_params", "psx_mlp_set_norm", "psx_mlp_forward", "psx_mlp_grad", "psx_mlp_gradient",
+    "psx_mlp_train_step",
 )
+
+MLP_NPARAM = 9350
 
 PSX_FILTER_SHARP, PSX_FILTER_BOX = 0, 1
 
@@ -106,6 +111,17 @@ def load() -> C.CDLL:
         "psx_set_tuning": (i, [vp, C.POINTER(C.c_int)]),
         "psx_get_tuning": (i, [vp, C.POINTER(C.c_int)]),
         "psx_autotune": (i, [vp, dp, i, C.POINTER(C.c_int)]),
+        "psx_mlp_create": (i, [C.POINTER(vp), i, vp, C.c_ulonglong]),
+        "psx_mlp_destroy": (i, [vp]),
+        "psx_mlp_last_error": (C.c_char_p, [vp]),
+        "psx_mlp_set_stream": (i, [vp, vp]),
+        "psx_mlp_get_params": (i, [vp, C.POINTER(C.c_float)]),
+        "psx_mlp_set_params": (i, [vp, C.POINTER(C.c_float)]),
+        "psx_mlp_set_norm": (i, [vp] + [C.POINTER(C.c_float)] * 4),
+        "psx_mlp_forward": (i, [vp, dp, dp, C.c_longlong]),
+        "psx_mlp_grad": (i, [vp, dp, dp, C.c_longlong, dp]),
+        "psx_mlp_gradient": (i, [vp, dp]),
+        "psx_mlp_train_step": (i, [vp, dp, dp, C.c_longlong, C.c_float, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

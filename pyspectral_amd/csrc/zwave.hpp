@@ -19,6 +19,10 @@ namespace psx {
 #ifndef PSX_ZW_PREFETCH
 #define PSX_ZW_PREFETCH 1
 #endif
+// minimum waves per SIMD the compiler must allow (register cap)
+#ifndef PSX_ZW_MINW
+#define PSX_ZW_MINW 1
+#endif
 
 template <int L> struct ZWave {
   static constexpr int R0 = L / 64;
@@ -185,7 +189,7 @@ __device__ __forceinline__ void zw_sub64(double2* buf, const double2* __restrict
 }
 
 template <int L, int P, int MODE>
-__global__ __launch_bounds__(64) void zwave(ZArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2* __restrict__ tw) {
   using W = ZWave<L>;
   static_assert(W::ok, "zwave: L must be 64 * {1,2,3,4,6,8,12}");
   constexpr int R0 = W::R0, G = W::G, SUB = W::SUB;

@@ -1,0 +1,106 @@
+"""The learned subgrid closure's network: a pointwise MLP 9 -> 64 -> 64 -> 64 -> 6
+(tanh) from filtered velocity-gradient components to the six SGS stresses, run with
+BF16 operands and FP32 accumulation on the matrix cores (libpsx ``psx_mlp_*``).
+No counterpart in the reference."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+NPARAM = _lib.MLP_NPARAM
+# parameter blocks (offset, shape), row-major
+LAYOUT = (("W1", 0, (64, 9)), ("b1", 576, (64,)), ("W2", 640, (64, 64)), ("b2", 4736, (64,)),
+          ("W3", 4800, (64, 64)), ("b3", 8896, (64,)), ("W4", 8960, (6, 64)), ("b4", 9344, (6,)))
+
+
+def _f32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    def __init__(self, seed: int = 0, device=None):
+        if not torch.cuda.is_available():
+            raise _lib.PsxError("MLP needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.lib = _lib.load()
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
+        net = C.c_void_p()
+        with torch.cuda.device(self.device):
+            s = torch.cuda.current_stream(self.device).cuda_stream
+            rc = self.lib.psx_mlp_create(C.byref(net), self.device.index, C.c_void_p(s), C.c_ulonglong(seed))
+        if rc != 0:
+            raise _lib.PsxError(f"psx_mlp_create failed ({rc})")
+        self.net = net
+        self.norm = (np.zeros(9, np.float32), np.ones(9, np.float32), np.zeros(6, np.float32), np.ones(6, np.float32))
+
+    def close(self):
+        if getattr(self, "net", None):
+            self.lib.psx_mlp_destroy(self.net)
+            self.net = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise _lib.PsxError(f"{what} failed ({rc}): {self.lib.psx_mlp_last_error(self.net).decode()}")
+
+    def _stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        self._check(self.lib.psx_mlp_set_stream(self.net, C.c_void_p(s)), "psx_mlp_set_stream")
+
+    @staticmethod
+    def _arg(t, cols):
+        if t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != cols or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError(f"expected a contiguous float32 (n, {cols}) device tensor")
+        return C.c_void_p(t.data_ptr())
+
+    # ---------------------------------------------------------------- parameters
+    def params(self) -> np.ndarray:
+        out = np.empty(NPARAM, np.float32)
+        self._stream()
+        self._check(self.lib.psx_mlp_get_params(self.net, _f32p(out)), "psx_mlp_get_params")
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != NPARAM:
+            raise ValueError(f"{NPARAM} parameters expected")
+        self._stream()
+        self._check(self.lib.psx_mlp_set_params(self.net, _f32p(p)), "psx_mlp_set_params")
+
+    def set_norm(self, x_mean, x_scale, y_mean, y_scale) -> None:
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (x_mean, x_scale, y_mean, y_scale)]
+        self._check(self.lib.psx_mlp_set_norm(self.net, *[_f32p(a) for a in arrs]), "psx_mlp_set_norm")
+        self.norm = tuple(arrs)
+
+    # ---------------------------------------------------------------- compute
+    def forward(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        n = x.shape[0]
+        out = torch.empty((n, 6), dtype=torch.float32, device=self.device) if out is None else out
+        self._stream()
+        self._check(self.lib.psx_mlp_forward(self.net, self._arg(x, 9), self._arg(out, 6), n), "psx_mlp_forward")
+        return out
+
+    def grad(self, x: torch.Tensor, target: torch.Tensor):
+        """(loss, gradient) of the normalised MSE on one batch (n a multiple of 16)."""
+        loss = torch.zeros(1, dtype=torch.float32, device=self.device)
+        g = torch.empty(NPARAM, dtype=torch.float32, device=self.device)
+        self._stream()
+        self._check(self.lib.psx_mlp_grad(self.net, self._arg(x, 9), self._arg(target, 6), x.shape[0],
+                                          C.c_void_p(loss.data_ptr())), "psx_mlp_grad")
+        self._check(self.lib.psx_mlp_gradient(self.net, C.c_void_p(g.data_ptr())), "psx_mlp_gradient")
+        return loss, g
+
+    def train_step(self, x: torch.Tensor, target: torch.Tensor, lr: float = 1e-3) -> torch.Tensor:
+        loss = torch.zeros(1, dtype=torch.float32, device=self.device)
+        self._stream()
+        self._check(self.lib.psx_mlp_train_step(self.net, self._arg(x, 9), self._arg(target, 6), x.shape[0],
+                                                C.c_float(lr), C.c_void_p(loss.data_ptr())), "psx_mlp_train_step")
+        return loss
