@@ -230,8 +230,8 @@ def main():
     _lib.check(solver.ctx, lib.psx_timing_begin(solver.ctx), "psx_timing_begin")
     for _ in range(args.event_steps):
         solver.step(q, qs)
-    ms = (C.c_double * 9)()
-    cnt = (C.c_int * 9)()
+    ms = (C.c_double * len(_lib.KERNEL_NAMES))()
+    cnt = (C.c_int * len(_lib.KERNEL_NAMES))()
     _lib.check(solver.ctx, lib.psx_timing_end(solver.ctx, ms, cnt), "psx_timing_end")
     kb = kernel_bytes(N, args.closure)
     kernels = {}

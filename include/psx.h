@@ -34,6 +34,9 @@ extern "C" {
 #define PSX_MODEL_STATIC_TAU 3   /* computeRHS_staumodel (src/RHSfunctions.py:366-497) */
 #define PSX_MODEL_DYNAMIC_TAU 4  /* computeRHS_Dtaumodel (src/RHSfunctions.py:592-878) */
 #define PSX_MODEL_DSMAG 5        /* computeRHS_DSMAG, turb_model 7 (src/RHSfunctions.py:1663-1862) */
+#define PSX_MODEL_LEARNED 6      /* learned closure (no counterpart in the reference): the
+                                    Smagorinsky-form RHS with the stress of an attached
+                                    network, see psx_set_closure                        */
 
 typedef struct psx_params {
   double nu;              /* viscosity (main.nu)                                     */
@@ -197,6 +200,38 @@ int psx_mlp_grad(psx_mlp* net, const float* x, const float* target, long long n,
 int psx_mlp_gradient(psx_mlp* net, float* grad_dev);
 int psx_mlp_train_step(psx_mlp* net, const float* x, const float* target, long long n, float lr, float* loss_dev);
 
+/* Learned-closure LES (no counterpart in the reference).  psx_set_closure attaches a
+ * network to a PSX_MODEL_LEARNED context (the context does not own it; it must outlive
+ * the context's use).  Each RHS evaluation forms the nine velocity gradients
+ * du_i/dx_j (row-major (i, j), physical units of u = irfftn(uhat) sqrt(Nt)) at every
+ * point of the 3/2-padded grid, maps them through the network to the six stresses
+ * (order 11 22 33 12 13 23) and uses them in place of the Smagorinsky stress of
+ * computeRHS_SMAG (src/RHSfunctions.py:146-237): w0 and the RHS follow
+ * src/RHSfunctions.py:215-237.  A point whose normalised inputs leave [-bound, bound]
+ * takes the Smagorinsky stress -2 (Cs Delta)^2 |S| S_ij of its gradients (bound =
+ * INFINITY: never; a negative bound: everywhere).  Re-attach after changing the
+ * network's normalisation: a captured step (psx_step_graph) holds it by value. */
+int psx_set_closure(psx_context* ctx, psx_mlp* net, double bound);
+
+/* Grid-space fields (no counterpart as one call in the reference, which calls
+ * myFFT.ifftT_obj, src/Classes.py:596-607): out (nfields, N1, N2, N3) =
+ * irfftn(fhat) sqrt(N1 N2 N3) of nfields spectral fields fhat (nfields, N1, N2, N3/2+1),
+ * the inverse of uhat = rfftn(u)/sqrt(Nt).  Single-member contexts of the DNS,
+ * Smagorinsky, DSMAG and learned models. */
+int psx_physical(psx_context* ctx, const double* fhat, int nfields, double* out);
+
+/* The nine velocity gradients du_i/dx_j = irfftn(i k_j uhat_i) sqrt(Nt) at the grid
+ * points, as float32 out[(x N2 + y) N3 + z][3 i + j] (the learned closure's input
+ * layout).  Single-member contexts, as psx_physical. */
+int psx_gradients(psx_context* ctx, const double* uhat, float* out);
+
+/* Spectral restriction from a finer grid (s1, s2, s3) to the context's grid: the
+ * signed modes |k1| < N1/2, |k2| < N2/2, 0 <= k3 < N3/2 of nfields fields
+ * src (nfields, s1, s2, s3/2+1), times sqrt(N1 N2 N3 / (s1 s2 s3)) so that grid values
+ * are kept; the context grid's Nyquist entries are zero.  For filtered-DNS training
+ * data on an LES grid. */
+int psx_restrict(psx_context* ctx, const double* src, int s1, int s2, int s3, int nfields, double* out);
+
 /* Instrumentation (no counterpart in the reference, which only prints wall time,
  * src/PySpec_3d.py:136-145): bracket every kernel launch with HIP events on the
  * context stream between begin and end.  psx_timing_end synchronizes the
@@ -211,7 +246,8 @@ int psx_mlp_train_step(psx_mlp* net, const float* x, const float* target, long l
 #define PSX_KERNEL_CFL_Y 6
 #define PSX_KERNEL_CFL_Z 7
 #define PSX_KERNEL_OTHER 8
-#define PSX_NKERNEL 9
+#define PSX_KERNEL_CLOSURE 9  /* the learned closure's network */
+#define PSX_NKERNEL 10
 int psx_timing_begin(psx_context* ctx);
 int psx_timing_end(psx_context* ctx, double* ms, int* count);
 

@@ -640,6 +640,96 @@ def sgs_dns(st: State, fft: FFTBackend, kind: str = "sharp", delta: float | None
     return list(sgs_divergence(g, tau)), tau
 
 
+# --------------------------------------------------------------------------
+# Grid-space fields and the learned closure (no counterpart in the reference)
+# --------------------------------------------------------------------------
+
+
+def physical(g: Grid, fhat) -> np.ndarray:
+    """Grid values irfftn(fhat) sqrt(Nt) of a spectral field (the inverse of
+    uhat = rfftn(u)/sqrt(Nt), ``run/TGV_Validate100/inputfile.py:42-44``)."""
+    return np.fft.irfftn(fhat, s=(g.N1, g.N2, g.N3), axes=(0, 1, 2)) * np.sqrt(g.Nt)
+
+
+def gradients(g: Grid, uhat, vhat, what) -> np.ndarray:
+    """du_i/dx_j at the grid points, (N1 N2 N3, 9) with column 3i + j; spectral
+    derivatives i k_j on the signed FFT frequencies."""
+    ks = (g.k1, g.k2, g.k3)
+    cols = [physical(g, 1j * ks[j] * f).ravel() for f in (uhat, vhat, what) for j in range(3)]
+    return np.stack(cols, axis=1)
+
+
+def restrict(fhat, dst: Grid) -> np.ndarray:
+    """Modes |k1| < n1/2, |k2| < n2/2, 0 <= k3 < n3/2 of a finer grid's spectrum on the
+    grid ``dst`` (its Nyquist entries zero), times sqrt(nt/Nt) so grid values are kept."""
+    S1, S2, Sc = fhat.shape
+    S3 = 2 * (Sc - 1)
+    n1, n2, n3 = dst.N1, dst.N2, dst.N3
+    out = np.zeros((n1, n2, n3 // 2 + 1), dtype=complex)
+    i1 = np.r_[0:n1 // 2, S1 - n1 // 2 + 1:S1]
+    o1 = np.r_[0:n1 // 2, n1 // 2 + 1:n1]
+    i2 = np.r_[0:n2 // 2, S2 - n2 // 2 + 1:S2]
+    o2 = np.r_[0:n2 // 2, n2 // 2 + 1:n2]
+    out[np.ix_(o1, o2, np.arange(n3 // 2))] = fhat[np.ix_(i1, i2, np.arange(n3 // 2))]
+    return out * np.sqrt((n1 * n2 * n3) / (S1 * S2 * S3))
+
+
+def padded_gradients(st: State, fft: FFTBackend, uhat, vhat, what) -> np.ndarray:
+    """du_i/dx_j at the points of the 3/2-padded grid, (M1 M2 M3, 9), column 3i + j,
+    physical units: the padded c2r of ``src/RHSfunctions.py:174-193`` without its
+    scale, times M1 M2 M3 / sqrt(Nt)."""
+    g = st.grid
+    M = ((3 * g.N1) // 2, (3 * g.N2) // 2, (3 * g.N3) // 2)
+    f = np.prod(M) / np.sqrt(g.Nt)
+    ks = (g.k1, g.k2, g.k3)
+    cols = [(fft.c2r(pad(1j * ks[j] * a), M) * f).ravel() for a in (uhat, vhat, what) for j in range(3)]
+    return np.stack(cols, axis=1)
+
+
+def stress_hat_from_physical(g: Grid, fft: FFTBackend, tau_phys) -> list:
+    """Spectra of stresses given in physical units on the padded grid, (M1 M2 M3, 6),
+    in the reference's units: its padded fields are (3/2)^(-3/2) Nt^(-1/4) times
+    physical (``src/Classes.py:593``), so a stress, quadratic in them, carries
+    (3/2)^(-3) Nt^(-1/2) before the unnormalised r2c and unpad
+    (``src/RHSfunctions.py:208-212``)."""
+    M = ((3 * g.N1) // 2, (3 * g.N2) // 2, (3 * g.N3) // 2)
+    conv = (2.0 / 3.0) ** 3 / np.sqrt(g.Nt)
+    return [unpad(fft.r2c(np.asarray(tau_phys[:, c], np.float64).reshape(M) * conv)) for c in range(6)]
+
+
+def rhs_learned(st: State, fft: FFTBackend, closure) -> np.ndarray:
+    """``computeRHS_SMAG`` (``src/RHSfunctions.py:146-237``) with the Smagorinsky stress
+    replaced by ``closure(gradients)`` evaluated pointwise on the padded grid:
+    ``closure`` maps (n, 9) gradients to (n, 6) stresses (order 11 22 33 12 13 23,
+    physical units) and may return (stresses, mask).  Returns what the closure
+    returned besides the stresses (or None)."""
+    g = st.grid
+    st.Q2U()
+    uu, vv, ww, uv, uw, vw = _products_32(st, fft, st.uhat, st.vhat, st.what)
+    k1, k2, k3 = g.k1, g.k2, g.k3
+    ki = g.ksqr_i
+    phat = (-k1 * k1 * ki * uu - k2 * k2 * ki * vv - k3 * k3 * ki * ww - 2.0 * k1 * k2 * ki * uv
+            - 2.0 * k1 * k3 * ki * uw - 2.0 * k2 * k3 * ki * vw)
+    if st.rotate == 1:
+        phat = phat - _coriolis_p(st, g, st.uhat, st.vhat, st.what, k1, k2, k3, ki)
+    res = closure(padded_gradients(st, fft, st.uhat, st.vhat, st.what))
+    extra = None
+    if isinstance(res, tuple):
+        res, extra = res
+    tau = stress_hat_from_physical(g, fft, res)
+    wu, wv, ww_ = sgs_divergence(g, tau)
+    st.w0 = (wu, wv, ww_)
+    ru = -1j * k1 * uu - 1j * k2 * uv - 1j * k3 * uw - 1j * k1 * phat - st.nu * g.ksqr * st.uhat + wu
+    rv = -1j * k1 * uv - 1j * k2 * vv - 1j * k3 * vw - 1j * k2 * phat - st.nu * g.ksqr * st.vhat + wv
+    rw = -1j * k1 * uw - 1j * k2 * vw - 1j * k3 * ww - 1j * k3 * phat - st.nu * g.ksqr * st.what + ww_
+    if st.rotate == 1:
+        ru, rv, rw = _coriolis_add(st, ru, rv, rw)
+    if st.force != 0.0:
+        ru, rv, rw = ru + st.force * st.uhat, rv + st.force * st.vhat, rw + st.force * st.what
+    st.set_rhs(ru, rv, rw)
+    return extra
+
+
 def _hermitian_inner(a, b, n3h, Nt):
     """sum over the half spectrum with weight 2 for 0 < k3 < N3/2 and 1 for k3 = 0
     (the weighting of ``src/RHSfunctions.py:822-852``)."""
@@ -961,6 +1051,16 @@ PARITY = {
                      "three reductions, each checked by brute force at N=8 (tests/test_oracle_tmodel.py)",
     "advance_rk4": "pinned: linear mode amplification R(z)^n (tests/test_oracle_rhs.py)",
     "compute_dt": "pinned: closed form from max|u| (tests/test_oracle_rhs.py)",
+    "rhs_dsmag": "pinned: Cs^2 as physical-space sums (Parseval), Leonard stress by brute force, "
+                 "reduces to rhs_smag at the dynamic coefficient (tests/test_oracle_dsmag.py)",
+    "sgs_dns/box_transfer": "pinned: filtered convolution differences by brute force, box transfer as "
+                            "the top-hat average (tests/test_oracle_sgs.py)",
+    "physical/gradients/restrict/padded_gradients": "pinned: analytic band-limited fields and their "
+                                                    "derivatives on both grids (tests/test_oracle_learned.py)",
+    "rhs_learned/stress_hat_from_physical": "pinned: with the pointwise Smagorinsky closure it equals "
+                                            "rhs_smag (tests/test_oracle_learned.py)",
+    "mlp_oracle.forward/grad/bf16": "pinned: torch bfloat16 conversion, torch.nn forward and autograd "
+                                    "(tests/test_oracle_mlp.py)",
     "stats": "pinned: Parseval against physical-space sums (tests/test_oracle_rhs.py); "
              "E(t) against the shipped TGV fixture (tests/test_oracle_fixture.py)",
 }

@@ -19,6 +19,7 @@ MODEL_IDS = {
     "Static tau-model": 3, "static_tau": 3,
     "Dynamic tau-model": 4, "dynamic_tau": 4,
     7: 5, "dsmag": 5,
+    "learned": 6,
 }
 
 # every symbol include/psx.h declares
@@ -32,7 +33,7 @@ EXPORTS = (
     "psx_ensemble_step_graph", "psx_ensemble_stats", "psx_set_tuning", "psx_get_tuning", "psx_autotune",
     "psx_mlp_create", "psx_mlp_destroy", "psx_mlp_last_error", "psx_mlp_set_stream", "psx_mlp_get_params",
     "psx_mlp_set_params", "psx_mlp_set_norm", "psx_mlp_forward", "psx_mlp_grad", "psx_mlp_gradient",
-    "psx_mlp_train_step",
+    "psx_mlp_train_step", "psx_set_closure", "psx_physical", "psx_gradients", "psx_restrict",
 )
 
 MLP_NPARAM = 9350
@@ -40,7 +41,7 @@ MLP_NPARAM = 9350
 PSX_FILTER_SHARP, PSX_FILTER_BOX = 0, 1
 
 KERNEL_NAMES = ("x_inverse", "y_inverse", "z_pass", "y_forward", "x_forward_epilogue",
-                "cfl_x", "cfl_y", "cfl_z", "other")
+                "cfl_x", "cfl_y", "cfl_z", "other", "closure_network")
 
 
 class psx_params(C.Structure):
@@ -122,6 +123,10 @@ def load() -> C.CDLL:
         "psx_mlp_grad": (i, [vp, dp, dp, C.c_longlong, dp]),
         "psx_mlp_gradient": (i, [vp, dp]),
         "psx_mlp_train_step": (i, [vp, dp, dp, C.c_longlong, C.c_float, dp]),
+        "psx_set_closure": (i, [vp, vp, d]),
+        "psx_physical": (i, [vp, dp, i, dp]),
+        "psx_gradients": (i, [vp, dp, dp]),
+        "psx_restrict": (i, [vp, dp, i, i, i, i, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

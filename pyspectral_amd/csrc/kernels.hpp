@@ -215,7 +215,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
 }
 
 // ------------------------------------------------------------------ fused z pass
-enum ZMode { Z_DNS = 0, Z_SMAG = 1, Z_TM2 = 2, Z_DT = 3, Z_SGS = 4 };
+enum ZMode { Z_DNS = 0, Z_SMAG = 1, Z_TM2 = 2, Z_DT = 3, Z_SGS = 4, Z_PHYS = 5, Z_GRAD = 6, Z_LRN = 7 };
 
 template <int MODE> struct ZShape;
 template <> struct ZShape<Z_DNS> { static constexpr int nin = 3, nout = 6; };
@@ -223,6 +223,10 @@ template <> struct ZShape<Z_SMAG> { static constexpr int nin = 9, nout = 12; };
 template <> struct ZShape<Z_TM2> { static constexpr int nin = 6, nout = 9; };
 template <> struct ZShape<Z_DT> { static constexpr int nin = 3, nout = 0; };
 template <> struct ZShape<Z_SGS> { static constexpr int nin = 6, nout = 12; };
+// physical values of 3 fields; the 9 velocity gradients; products + a given stress
+template <> struct ZShape<Z_PHYS> { static constexpr int nin = 3, nout = 0; };
+template <> struct ZShape<Z_GRAD> { static constexpr int nin = 9, nout = 0; };
+template <> struct ZShape<Z_LRN> { static constexpr int nin = 3, nout = 12; };
 
 // A real z-pencil's half spectrum is built as sum_t coef_t (i k3)^{dz_t} field_{f_t}.
 struct Recipe {
@@ -251,6 +255,14 @@ struct ZArgs {
   // in1 + m*ms_in1, out + m*ms_out (npos_m = 0: one member)
   int npos_m;
   long long ms_in0, ms_in1, ms_out;
+  // Z_PHYS: real output phys[f*out_sf + pos*L + z] (scaled by out_scale; one member);
+  // Z_GRAD: the 9 gradients as float, g[(gpos*L + z)*9 + c] (scaled by out_scale);
+  // Z_LRN: the stress tau[(gpos*L + z)*6 + c] times c2 joins the products.  gpos is the
+  // position over all members, so the point arrays of the members follow one another.
+  double* phys;
+  float* gout;
+  const float* tin;
+  int nphys;  // Z_PHYS: fields written (<= 3)
 };
 
 // A z-pass block's member and its data pointers
@@ -291,8 +303,8 @@ __device__ __forceinline__ double2 zspec(const ZArgs& a, const ZMember& zm, int 
 }
 
 template <int MODE>
-__device__ __forceinline__ void zpoint(const ZArgs& a, const double* in, double* out) {
-  if constexpr (MODE == Z_DNS) {
+__device__ __forceinline__ void zpoint(const ZArgs& a, const double* in, double* out, long long pt = 0) {
+  if constexpr (MODE == Z_DNS || MODE == Z_LRN) {
     const double u = in[0], v = in[1], w = in[2], s = a.out_scale;
     out[0] = s * (u * u);
     out[1] = s * (v * v);
@@ -300,6 +312,10 @@ __device__ __forceinline__ void zpoint(const ZArgs& a, const double* in, double*
     out[3] = s * (u * v);
     out[4] = s * (u * w);
     out[5] = s * (v * w);
+    if constexpr (MODE == Z_LRN) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) out[6 + c] = a.c2 * (double)a.tin[pt * 6 + c];
+    }
   } else if constexpr (MODE == Z_SMAG) {
     // src/RHSfunctions.py:195-212: |S| = sqrt(2 S_ij S_ij), nu_t = (Cs Delta)^2 |S|,
     // tau_ij = -2 nu_t S_ij; the stress is quadratic in the velocity, so it takes
@@ -443,7 +459,28 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
   __syncthreads();
   fft_batch<L, NCI, NTZ, LS, +1, PADZ>(buf, twp, tid);
 
-  if constexpr (MODE == Z_DT) {
+  if constexpr (MODE == Z_PHYS || MODE == Z_GRAD) {
+    for (int z = tid; z < L; z += NTZ) {
+      double2 w[NCI];
+#pragma unroll
+      for (int q = 0; q < NCI; ++q) w[q] = buf[q * LS + ZRW(z)];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const long long pos = pos0 + p;
+#pragma unroll
+        for (int f = 0; f < NIN; ++f) {
+          const int r = f * P + p;
+          const double v = ((r & 1) ? w[r >> 1].y : w[r >> 1].x) * a.out_scale;
+          if constexpr (MODE == Z_PHYS) {
+            if (f < a.nphys) a.phys[(long long)f * a.out_sf + pos * L + z] = v;
+          }
+          else
+            a.gout[(((long long)blockIdx.x * P + p) * L + z) * 9 + f] = (float)v;
+        }
+      }
+    }
+    return;
+  } else if constexpr (MODE == Z_DT) {
     double m = 0.0;
     for (int z = tid; z < L; z += NTZ) {
       double2 w[NCI];
@@ -493,7 +530,7 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
             const int r = f * P + p;
             in[f] = (r & 1) ? v[r >> 1].y : v[r >> 1].x;
           }
-          zpoint<MODE>(a, in, o[p]);
+          zpoint<MODE>(a, in, o[p], ((long long)blockIdx.x * P + p) * L + z);
         }
 #pragma unroll
         for (int q = 0; q < NCO; ++q) {

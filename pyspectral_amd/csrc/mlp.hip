@@ -72,6 +72,16 @@ struct Norm {
   float mx[NIN], sx[NIN], my[NOUT], sy[NOUT];  // x_n = (x - mx)/sx, y = y_n sy + my
 };
 
+// Smagorinsky stress -2 c2 |S| S_ij from the gradients g[3i + j] = du_i/dx_j, with
+// |S| = sqrt(2 S_ij S_ij); components o0 .. o0+3 of the order 11 22 33 12 13 23
+__device__ __forceinline__ void smag_stress(const float* g, float c2, int o0, float (&t)[4]) {
+  const float S[6] = {g[0], g[4], g[8], 0.5f * (g[1] + g[3]), 0.5f * (g[2] + g[6]), 0.5f * (g[5] + g[7])};
+  const float Smag = sqrtf(2.f * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] +
+                                  2.f * (S[3] * S[3] + S[4] * S[4] + S[5] * S[5])));
+#pragma unroll
+  for (int j = 0; j < 4; ++j) t[j] = (o0 + j < NOUT) ? -2.f * c2 * Smag * S[o0 + j < NOUT ? o0 + j : 0] : 0.f;
+}
+
 // Pack the FP32 parameters into MFMA A-operand tiles (one thread per tile value).
 __global__ void pack_weights(const float* __restrict__ w, short* __restrict__ pk) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;  // over NPACK * 64 * 4 values
@@ -118,7 +128,8 @@ template <bool TRAIN>
 __global__ __launch_bounds__(64) void mlp_tiles(const float* __restrict__ x, const float* __restrict__ target,
                                                 float* __restrict__ y, long long n, const float* __restrict__ w,
                                                 const short* __restrict__ pk, Norm nm, short* __restrict__ act,
-                                                float* __restrict__ loss_part, float gscale) {
+                                                float* __restrict__ loss_part, float gscale, float fb_c2,
+                                                float fb_bound) {
   const int lane = threadIdx.x, g = lane >> 4, r = lane & 15;
   // inference keeps the forward tiles in registers; training (which also holds the
   // activations) reads them from the packed array, which stays in cache
@@ -195,10 +206,23 @@ __global__ __launch_bounds__(64) void mlp_tiles(const float* __restrict__ x, con
 #pragma unroll
     for (int k = 0; k < 4; ++k) out = mfma(Af(T1 + T2 + T3 + k), B3[k], out);
     if constexpr (!TRAIN) {
+      // range guard: a point whose largest |normalised input| exceeds fb_bound takes
+      // the Smagorinsky stress of its own gradients instead of the network's output
+      float xm = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xm = fmaxf(xm, fabsf(x0[j]));
+      xm = fmaxf(xm, __shfl_xor(xm, 16));
+      xm = fmaxf(xm, __shfl_xor(xm, 32));
+      const bool fallback = !(xm <= fb_bound);
+      float sm[4] = {0.f, 0.f, 0.f, 0.f};
+      if (fallback && valid) {
+        const float* gp = x + p * NIN;
+        smag_stress(gp, fb_c2, 4 * g, sm);
+      }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int o = 4 * g + j;
-        if (valid && o < NOUT) y[p * NOUT + o] = out[j] * nm.sy[o] + nm.my[o];
+        if (valid && o < NOUT) y[p * NOUT + o] = fallback ? sm[j] : out[j] * nm.sy[o] + nm.my[o];
       }
     } else {
       // MSE on normalised outputs: d4 = gscale (y - t), gscale = 2 / (6 n)
@@ -533,7 +557,8 @@ int psx_mlp_forward(psx_mlp* m, const float* x, float* y, long long n) {
   const long long tiles = (n + 15) / 16;
   const unsigned blocks = (unsigned)(tiles < kFwdBlocks ? tiles : kFwdBlocks);
   hipLaunchKernelGGL((mlp_tiles<false>), dim3(blocks), dim3(64), 0, m->stream, x, (const float*)nullptr, y, n,
-                     (const float*)m->w, (const short*)m->pk, m->nm, (short*)nullptr, (float*)nullptr, 0.f);
+                     (const float*)m->w, (const short*)m->pk, m->nm, (short*)nullptr, (float*)nullptr, 0.f, 0.f,
+                     INFINITY);
   return hip_ok(m, hipGetLastError(), "mlp_tiles");
 }
 
@@ -555,7 +580,7 @@ int psx_mlp_grad(psx_mlp* m, const float* x, const float* target, long long n, f
   const unsigned blocks = (unsigned)(tiles < kFwdBlocks ? tiles : kFwdBlocks);
   const float gscale = 2.f / (6.f * (float)n);
   hipLaunchKernelGGL((mlp_tiles<true>), dim3(blocks), dim3(64), 0, m->stream, x, target, (float*)nullptr, n,
-                     (const float*)m->w, (const short*)m->pk, m->nm, m->act, m->lpart, gscale);
+                     (const float*)m->w, (const short*)m->pk, m->nm, m->act, m->lpart, gscale, 0.f, INFINITY);
   rc = hip_ok(m, hipGetLastError(), "mlp_tiles(train)");
   if (rc != PSX_OK) return rc;
   const int kpb = (int)((tiles + kSlices - 1) / kSlices);
@@ -598,3 +623,19 @@ int psx_mlp_train_step(psx_mlp* m, const float* x, const float* target, long lon
 }
 
 }  // extern "C"
+
+namespace psx {
+// The network as the learned closure of an LES right-hand side: stresses tau[n][6] from
+// the gradients g[n][9] on stream s; points outside the normalised input range
+// [-bound, bound] take the Smagorinsky stress with c2 = (Cs Delta)^2.
+hipError_t mlp_closure(const psx_mlp* m, const float* g, float* tau, long long n, float c2, float bound,
+                       hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long tiles = (n + 15) / 16;
+  const unsigned blocks = (unsigned)(tiles < kFwdBlocks ? tiles : kFwdBlocks);
+  hipLaunchKernelGGL((mlp_tiles<false>), dim3(blocks), dim3(64), 0, s, g, (const float*)nullptr, tau, n,
+                     (const float*)m->w, (const short*)m->pk, m->nm, (short*)nullptr, (float*)nullptr, 0.f, c2,
+                     bound);
+  return hipGetLastError();
+}
+}  // namespace psx

@@ -20,11 +20,19 @@ hipError_t launch_zpass_smag(int L, const ZArgs& a, const double2* tw, hipStream
 hipError_t launch_zpass_tm2(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
 hipError_t launch_zpass_dt(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
 hipError_t launch_zpass_sgs(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+hipError_t launch_zpass_phys(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+hipError_t launch_zpass_grad(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+hipError_t launch_zpass_lrn(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+hipError_t mlp_closure(const psx_mlp* m, const float* g, float* tau, long long n, float c2, float bound,
+                       hipStream_t s);
 int zpass_positions_dns(int L);
 int zpass_positions_smag(int L);
 int zpass_positions_tm2(int L);
 int zpass_positions_dt(int L);
 int zpass_positions_sgs(int L);
+int zpass_positions_phys(int L);
+int zpass_positions_grad(int L);
+int zpass_positions_lrn(int L);
 
 int zpass_positions(int L, int mode) {
   switch (mode) {
@@ -33,6 +41,9 @@ int zpass_positions(int L, int mode) {
     case Z_TM2: return zpass_positions_tm2(L);
     case Z_DT: return zpass_positions_dt(L);
     case Z_SGS: return zpass_positions_sgs(L);
+    case Z_PHYS: return zpass_positions_phys(L);
+    case Z_GRAD: return zpass_positions_grad(L);
+    case Z_LRN: return zpass_positions_lrn(L);
   }
   return 0;
 }
@@ -44,6 +55,9 @@ hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipS
     case Z_TM2: return launch_zpass_tm2(L, a, tw, s, variant);
     case Z_DT: return launch_zpass_dt(L, a, tw, s, variant);
     case Z_SGS: return launch_zpass_sgs(L, a, tw, s, variant);
+    case Z_PHYS: return launch_zpass_phys(L, a, tw, s, variant);
+    case Z_GRAD: return launch_zpass_grad(L, a, tw, s, variant);
+    case Z_LRN: return launch_zpass_lrn(L, a, tw, s, variant);
   }
   return hipErrorInvalidValue;
 }
@@ -106,6 +120,12 @@ struct psx_context {
   double* partA = nullptr;
   double* partB = nullptr;
   int npartT = 0;
+  // learned closure: the attached network (not owned), its input range bound, and
+  // the FP32 gradient / stress arrays of the padded grid (9 and 6 values per point)
+  psx_mlp* net = nullptr;
+  double net_bound = 0.0;
+  float* lgrad = nullptr;
+  float* ltau = nullptr;
   double* cfl_part = nullptr;  // per-block CFL maxima (N1*N2/P entries)
   double* scal = nullptr;  // device scalars: [0] dt (internal), [1..4] tau family
   double* part = nullptr;
@@ -153,10 +173,12 @@ int guarded(psx_context* c, F&& f) {
   }
 }
 
-bool truncating(int model) { return model != PSX_MODEL_SMAGORINSKY && model != PSX_MODEL_DSMAG; }
+bool truncating(int model) {
+  return model != PSX_MODEL_SMAGORINSKY && model != PSX_MODEL_DSMAG && model != PSX_MODEL_LEARNED;
+}
 
 // Kernel classes reported by psx_timing_end (order of PSX_KERNEL_* in psx.h).
-enum Kind { K_XINV = 0, K_YINV, K_ZPASS, K_YFWD, K_XEPI, K_CFL_X, K_CFL_Y, K_CFL_Z, K_OTHER, K_NKIND };
+enum Kind { K_XINV = 0, K_YINV, K_ZPASS, K_YFWD, K_XEPI, K_CFL_X, K_CFL_Y, K_CFL_Z, K_OTHER, K_CLOSURE, K_NKIND };
 
 hipEvent_t next_event(psx_context* c) {
   if (c->events_used == c->events.size()) {
@@ -411,51 +433,128 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2) {
 
 double delta_grid(const psx_context* c) { return std::cbrt(c->p.dx * c->p.dy * c->p.dz); }
 
+// K5 of the 3/2-rule models: x-forward + truncation + projection/closure/viscous +
+// stage update, from the y-forwarded products in B (with the stress fields when smag)
+void epilogue_32(psx_context* c, const double2* B, const double2* uin, const double2* q0, double2* out,
+                 const double* dt, double coef, double2* w0, int zero_nyq_out, bool smag) {
+  const int M1 = c->M1, H = c->H;
+  EpiArgs e;
+  std::memset(&e, 0, sizeof(e));
+  e.in = B;
+  e.in_sf = (long long)M1 * c->N2 * H;
+  e.N1 = c->N1;
+  e.N2 = c->N2;
+  e.N3 = c->N3;
+  e.Nc = c->Nc;
+  e.H = H;
+  e.keep_nyq = 0;
+  e.uprev = uin;
+  e.q0 = q0;
+  e.out = out;
+  e.w0 = w0;
+  e.dt = dt;
+  e.c = coef;
+  e.nu = c->p.nu;
+  e.ks[0] = 2.0 * kPi / c->p.L1;
+  e.ks[1] = 2.0 * kPi / c->p.L2;
+  e.ks[2] = 2.0 * kPi / c->p.L3;
+  e.trunc_u = truncating(c->model) ? 1 : 0;
+  e.zero_nyq_out = zero_nyq_out;
+  e.rotate = c->p.rotate;
+  e.Om[0] = c->p.Om1;
+  e.Om[1] = c->p.Om2;
+  e.Om[2] = c->p.Om3;
+  e.members = c->members;
+  e.force = c->p.force;
+  e.ms_in = member_stride(c, e.in);
+  e.ms_state = 3LL * c->N1 * c->N2 * c->Nc;
+  e.nu_m = c->nu_m;
+  const double2* tw = twiddles(c, M1);
+  TimeScope ts(c, K_XEPI);
+  check(c, launch_epilogue(M1, c->N1, smag ? E_SMAG : E_DNS, smag && w0 != nullptr, e, tw, c->stream),
+        "x_forward_epilogue");
+}
+
 // RHS / stage for the 3/2-rule models (DNS, Smagorinsky)
 void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt, double coef,
               double2* w0, int zero_nyq_out) {
   const bool smag = c->model == PSX_MODEL_SMAGORINSKY;
-  const int M1 = c->M1, H = c->H;
   const double cd = c->p.Cs * delta_grid(c);
   double2* B = front_32(c, uin, smag, cd * cd);
-  // K5: x-forward + truncation + projection/closure/viscous + stage update
+  epilogue_32(c, B, uin, q0, out, dt, coef, w0, zero_nyq_out, smag);
+}
+
+// Learned closure: the 3/2-rule pipeline with the stress of an attached network.
+// The z pass first writes the nine velocity gradients du_i/dx_j of every padded grid
+// point (FP32, point-major); the network maps them to the six stresses (order 11 22 33
+// 12 13 23, Smagorinsky outside its input range); a second z pass forms the products
+// and joins the stresses; then the y-forward and the Smagorinsky-form epilogue
+// (w0_i = -i k_j tau_ij - k_i k_m k_j tau_mj / k^2, as src/RHSfunctions.py:215-222).
+void stage_learned(psx_context* c, const double2* uin, const double2* q0, double2* out, const double* dt,
+                   double coef, double2* w0, int zero_nyq_out) {
+  if (!c->net) fail(c, PSX_E_ARG, "learned closure: no network attached (psx_set_closure)");
+  const int M1 = c->M1, M2 = c->M2, M3 = c->M3, H = c->H;
+  double2* X = c->bufA;
+  double2* Y = c->bufB;
+  double2* Z = c->bufA;
+  double2* W = c->bufB;
   {
-    EpiArgs e;
-    std::memset(&e, 0, sizeof(e));
-    e.in = B;
-    e.in_sf = (long long)M1 * c->N2 * H;
-    e.N1 = c->N1;
-    e.N2 = c->N2;
-    e.N3 = c->N3;
-    e.Nc = c->Nc;
-    e.H = H;
-    e.keep_nyq = 0;
-    e.uprev = uin;
-    e.q0 = q0;
-    e.out = out;
-    e.w0 = w0;
-    e.dt = dt;
-    e.c = coef;
-    e.nu = c->p.nu;
-    e.ks[0] = 2.0 * kPi / c->p.L1;
-    e.ks[1] = 2.0 * kPi / c->p.L2;
-    e.ks[2] = 2.0 * kPi / c->p.L3;
-    e.trunc_u = truncating(c->model) ? 1 : 0;
-    e.zero_nyq_out = zero_nyq_out;
-    e.rotate = c->p.rotate;
-    e.Om[0] = c->p.Om1;
-    e.Om[1] = c->p.Om2;
-    e.Om[2] = c->p.Om3;
-    e.members = c->members;
-    e.force = c->p.force;
-    e.ms_in = member_stride(c, e.in);
-    e.ms_state = 3LL * c->N1 * c->N2 * c->Nc;
-    e.nu_m = c->nu_m;
-    const double2* tw = twiddles(c, M1);
-    TimeScope ts(c, K_XEPI);
-    check(c, launch_epilogue(M1, c->N1, smag ? E_SMAG : E_DNS, smag && w0 != nullptr, e, tw, c->stream),
-          "x_forward_epilogue");
+    const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2}, jd[3] = {3, 4, 5};
+    x_inverse(c, uin, X, M1, H, 0, 3, jin, jp, jd);
   }
+  {
+    const int jin[6] = {0, 1, 2, 3, 4, 5}, jp[6] = {0, 1, 2, 3, 4, 5}, jd[6] = {6, 7, 8, -1, -1, -1};
+    y_inverse(c, X, Y, M1, M2, H, 0, 6, jin, jp, jd);
+  }
+  const double Nt = (double)c->N1 * c->N2 * c->N3;
+  const double2* tw3 = twiddles(c, M3);
+  ZArgs z;
+  std::memset(&z, 0, sizeof(z));
+  z.in0 = Y;
+  z.in_sf0 = (long long)M1 * M2 * H;
+  z.Hin0 = H;
+  z.nin0 = 9;
+  z.ms_in0 = member_stride(c, Y);
+  z.npos = M1 * M2 * c->members;
+  z.npos_m = c->members > 1 ? M1 * M2 : 0;
+  z.kzscale = 2.0 * kPi / c->p.L3;
+  {
+    // Y fields u v w ux vx wx uy vy wy -> du_i/dx_j, (i, j) row-major
+    ZArgs g = z;
+    const int src[9] = {3, 6, 0, 4, 7, 1, 5, 8, 2};
+    for (int r = 0; r < 9; ++r) g.rec[r] = Recipe{1, {src[r], 0}, {1.0, 0.0}, {r % 3 == 2 ? 1 : 0, 0}};
+    g.out_scale = 1.0 / std::sqrt(Nt);  // pencil values are sqrt(Nt) times physical
+    g.gout = c->lgrad;
+    const int P = zpass_positions(M3, Z_GRAD);
+    if (P == 0 || (M2 % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
+    TimeScope ts(c, K_ZPASS);
+    check(c, launch_zpass(M3, Z_GRAD, g, tw3, c->stream, 0), "z_pass(gradients)");
+  }
+  {
+    const double cd = c->p.Cs * delta_grid(c);
+    const long long npts = (long long)M1 * M2 * M3 * c->members;
+    TimeScope ts(c, K_CLOSURE);
+    check(c, mlp_closure(c->net, c->lgrad, c->ltau, npts, (float)(cd * cd), (float)c->net_bound, c->stream),
+          "closure network");
+  }
+  {
+    ZArgs q = z;
+    q.out = Z;
+    q.out_sf = (long long)M1 * M2 * H;
+    q.Hout = H;
+    q.ms_out = member_stride(c, Z);
+    for (int r = 0; r < 3; ++r) q.rec[r] = Recipe{1, {r, 0}, {1.0, 0.0}, {0, 0}};
+    q.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
+    // physical stress to the products' units: pencil products are Nt times physical
+    q.c2 = q.out_scale * Nt;
+    q.tin = c->ltau;
+    const int P = zpass_positions(M3, Z_LRN);
+    if (P == 0 || (M2 % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
+    TimeScope ts(c, K_ZPASS);
+    check(c, launch_zpass(M3, Z_LRN, q, tw3, c->stream, 0), "z_pass(learned)");
+  }
+  y_forward(c, Z, W, M1, M2, H, 0, 12);
+  epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true);
 }
 
 // computeRHS_DSMAG (src/RHSfunctions.py:1663-1862): the Smagorinsky pipeline on the
@@ -754,6 +853,9 @@ void run_stage(psx_context* c, const double2* uin, const double2* q0, double2* o
     case PSX_MODEL_DSMAG:
       stage_dsmag(c, uin, q0, out, dt, coef, w0);
       break;
+    case PSX_MODEL_LEARNED:
+      stage_learned(c, uin, q0, out, dt, coef, w0, zero_nyq_out);
+      break;
     default:
       fail(c, PSX_E_UNSUPPORTED, "unknown model");
   }
@@ -826,10 +928,10 @@ int psx_create_batch(psx_context** out, int n1, int n2, int n3, int model, const
   int rc = guarded(c, [&] {
     if (n1 < 4 || n2 < 4 || n3 < 4 || (n1 % 2) || (n2 % 2) || (n3 % 4))
       fail(c, PSX_E_ARG, "grid sizes must be even (N3 a multiple of 4)");
-    if (model < PSX_MODEL_DNS || model > PSX_MODEL_DSMAG) fail(c, PSX_E_ARG, "unknown model");
+    if (model < PSX_MODEL_DNS || model > PSX_MODEL_LEARNED) fail(c, PSX_E_ARG, "unknown model");
     if (members < 1) fail(c, PSX_E_ARG, "members must be >= 1");
-    if (members > 1 && model != PSX_MODEL_DNS && model != PSX_MODEL_SMAGORINSKY)
-      fail(c, PSX_E_UNSUPPORTED, "batched instances are available for the DNS and Smagorinsky models");
+    if (members > 1 && model != PSX_MODEL_DNS && model != PSX_MODEL_SMAGORINSKY && model != PSX_MODEL_LEARNED)
+      fail(c, PSX_E_UNSUPPORTED, "batched instances are available for the DNS, Smagorinsky and learned models");
     validate_params(c, params, model);
     c->members = members;
     if (model == PSX_MODEL_DYNAMIC_TAU && params->rotate)
@@ -856,7 +958,7 @@ int psx_create_batch(psx_context** out, int n1, int n2, int n3, int model, const
     }
     for (int L : lens)
       if (!length_supported(L)) fail(c, PSX_E_UNSUPPORTED, "grid size not supported: length " + std::to_string(L));
-    const bool smag = model == PSX_MODEL_SMAGORINSKY || model == PSX_MODEL_DSMAG;
+    const bool smag = model == PSX_MODEL_SMAGORINSKY || model == PSX_MODEL_DSMAG || model == PSX_MODEL_LEARNED;
     const size_t state = (size_t)3 * n1 * n2 * c->Nc;
     if (!tau) {
       const size_t MMH = (size_t)c->M1 * c->M2 * c->H;
@@ -874,6 +976,12 @@ int psx_create_batch(psx_context** out, int n1, int n2, int n3, int model, const
         check(c, hipMalloc(&c->uf, state * sizeof(double2)), "hipMalloc(uf)");
         check(c, hipMalloc(&c->dsPart, 2 * sizeof(double) * c->ndsPart), "hipMalloc(dsPart)");
         c->scratch_bytes += (long long)(nD + state) * sizeof(double2) + 16LL * c->ndsPart;
+      }
+      if (model == PSX_MODEL_LEARNED) {
+        const size_t npts = (size_t)c->M1 * c->M2 * c->M3 * members;
+        check(c, hipMalloc(&c->lgrad, 9 * npts * sizeof(float)), "hipMalloc(lgrad)");
+        check(c, hipMalloc(&c->ltau, 6 * npts * sizeof(float)), "hipMalloc(ltau)");
+        c->scratch_bytes += (long long)(15 * npts * sizeof(float));
       }
     } else {
       const size_t P1 = 2 * (size_t)n1, P2 = 2 * (size_t)n2;
@@ -957,6 +1065,8 @@ int psx_destroy(psx_context* c) {
   hipFree(c->partLE);
   hipFree(c->partA);
   hipFree(c->partB);
+  hipFree(c->lgrad);
+  hipFree(c->ltau);
   for (auto& kv : c->tw) hipFree(kv.second);
   for (auto e : c->events) hipEventDestroy(e);
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
@@ -1195,6 +1305,114 @@ int psx_sgs_dns(psx_context* c, const double* uhat, double* w0_out, double* tau_
                          c->N1, c->N2, c->Nc, 2.0 * kPi / c->p.L1, 2.0 * kPi / c->p.L2, 2.0 * kPi / c->p.L3);
       check(c, hipGetLastError(), "sgs_w0");
     }
+  });
+}
+
+namespace {
+// x- and y-inverse of nj spectral fields onto the N-grid (Nyquist rows kept, so the
+// grid values are those of numpy's irfftn), optionally with d/dx and d/dy outputs
+// (field order of the y-inverse: the fields, their d/dx, their d/dy).
+void ngrid_xy(psx_context* c, const double2* in, int nj, bool deriv) {
+  const int N1 = c->N1, N2 = c->N2, Nc = c->Nc;
+  const size_t field = (size_t)N1 * N2 * Nc;
+  const size_t nx = (deriv ? 2 : 1) * nj * field, ny = (deriv ? 3 : 1) * nj * field;
+  if (nx > c->nA || ny > c->nB) fail(c, PSX_E_UNSUPPORTED, "grid-space output: work buffers too small for this model");
+  int jin[MAXJOBS], jp[MAXJOBS], jd[MAXJOBS];
+  for (int j = 0; j < nj; ++j) {
+    jin[j] = j;
+    jp[j] = j;
+    jd[j] = nj + j;
+  }
+  x_inverse(c, in, c->bufA, N1, Nc, 1, nj, jin, jp, deriv ? jd : nullptr, K_OTHER);
+  const int ny_jobs = deriv ? 2 * nj : nj;
+  for (int j = 0; j < ny_jobs; ++j) {
+    jin[j] = j;
+    jp[j] = j;
+    jd[j] = j < nj ? 2 * nj + j : -1;
+  }
+  y_inverse(c, c->bufA, c->bufB, N1, N2, Nc, 1, ny_jobs, jin, jp, deriv ? jd : nullptr, K_OTHER);
+}
+
+ZArgs ngrid_z(psx_context* c, int nin) {
+  ZArgs z;
+  std::memset(&z, 0, sizeof(z));
+  z.in0 = c->bufB;
+  z.in_sf0 = (long long)c->N1 * c->N2 * c->Nc;
+  z.Hin0 = c->Nc;
+  z.nin0 = nin;
+  z.npos = c->N1 * c->N2;
+  z.kzscale = 2.0 * kPi / c->p.L3;
+  z.out_scale = 1.0 / std::sqrt((double)c->N1 * c->N2 * c->N3);
+  return z;
+}
+}  // namespace
+
+int psx_physical(psx_context* c, const double* fhat, int nfields, double* out) {
+  return guarded(c, [&] {
+    if (c->members > 1) fail(c, PSX_E_UNSUPPORTED, "one instance per call: use a single-member context");
+    if (!fhat || !out || nfields < 1) fail(c, PSX_E_ARG, "null pointer or no fields");
+    const long long field = (long long)c->N1 * c->N2 * c->Nc, grid = (long long)c->N1 * c->N2 * c->N3;
+    const int P = zpass_positions(c->N3, Z_PHYS);
+    if (P == 0 || (c->N2 % P) != 0) fail(c, PSX_E_UNSUPPORTED, "grid-space output: unsupported length");
+    const double2* tw = twiddles(c, c->N3);
+    for (int f0 = 0; f0 < nfields; f0 += 3) {
+      const int nj = std::min(3, nfields - f0);
+      ngrid_xy(c, (const double2*)fhat + f0 * field, nj, false);
+      ZArgs z = ngrid_z(c, 3);
+      for (int r = 0; r < 3; ++r) z.rec[r] = Recipe{1, {std::min(r, nj - 1), 0}, {1.0, 0.0}, {0, 0}};
+      z.phys = out + f0 * grid;
+      z.out_sf = grid;
+      z.nphys = nj;
+      TimeScope ts(c, K_OTHER);
+      check(c, launch_zpass(c->N3, Z_PHYS, z, tw, c->stream, 0), "z_pass(physical)");
+    }
+  });
+}
+
+int psx_gradients(psx_context* c, const double* uhat, float* out) {
+  return guarded(c, [&] {
+    if (c->members > 1) fail(c, PSX_E_UNSUPPORTED, "one instance per call: use a single-member context");
+    if (!uhat || !out) fail(c, PSX_E_ARG, "null pointer");
+    const int P = zpass_positions(c->N3, Z_GRAD);
+    if (P == 0 || (c->N2 % P) != 0) fail(c, PSX_E_UNSUPPORTED, "gradients: unsupported length");
+    ngrid_xy(c, (const double2*)uhat, 3, true);
+    ZArgs z = ngrid_z(c, 9);
+    // y-inverse fields u v w ux vx wx uy vy wy -> du_i/dx_j, (i, j) row-major
+    const int src[9] = {3, 6, 0, 4, 7, 1, 5, 8, 2};
+    for (int r = 0; r < 9; ++r) z.rec[r] = Recipe{1, {src[r], 0}, {1.0, 0.0}, {r % 3 == 2 ? 1 : 0, 0}};
+    z.gout = out;
+    const double2* tw = twiddles(c, c->N3);
+    TimeScope ts(c, K_OTHER);
+    check(c, launch_zpass(c->N3, Z_GRAD, z, tw, c->stream, 0), "z_pass(gradients)");
+  });
+}
+
+int psx_restrict(psx_context* c, const double* src, int s1, int s2, int s3, int nfields, double* out) {
+  return guarded(c, [&] {
+    if (!src || !out || nfields < 1) fail(c, PSX_E_ARG, "null pointer or no fields");
+    if (s1 < c->N1 || s2 < c->N2 || s3 < c->N3 || (s1 % 2) || (s2 % 2) || (s3 % 2))
+      fail(c, PSX_E_ARG, "the source grid must be even and at least as fine as the context grid");
+    const long long total = (long long)nfields * c->N1 * c->N2 * c->Nc;
+    const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 8192);
+    const double scale = std::sqrt(((double)c->N1 * c->N2 * c->N3) / ((double)s1 * s2 * s3));
+    hipLaunchKernelGGL(restrict_modes, dim3(blocks), dim3(256), 0, c->stream, (const double2*)src, (double2*)out,
+                       nfields, c->N1, c->N2, c->Nc, s1, s2, s3 / 2 + 1, scale);
+    check(c, hipGetLastError(), "restrict_modes");
+  });
+}
+
+int psx_set_closure(psx_context* c, psx_mlp* net, double bound) {
+  return guarded(c, [&] {
+    if (c->model != PSX_MODEL_LEARNED) fail(c, PSX_E_ARG, "psx_set_closure needs a PSX_MODEL_LEARNED context");
+    if (std::isnan(bound)) fail(c, PSX_E_ARG, "bound is NaN");
+    c->net = net;
+    c->net_bound = bound;
+    // a captured step holds the network's normalisation and the bound by value
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    for (auto& k : c->graph_key) k = nullptr;
   });
 }
 

@@ -449,4 +449,28 @@ __global__ void dsmag_final(DsmagArgs a) {
   }
 }
 
+// Restriction of nf spectral fields of a finer grid (S1, S2, Sc) to this grid
+// (N1, N2, Nc): the signed modes |k1| < N1/2, |k2| < N2/2, 0 <= k3 < Nc - 1 are copied
+// times scale; this grid's Nyquist rows and plane are zero.
+__global__ void restrict_modes(const double2* __restrict__ src, double2* __restrict__ dst, int nf, int N1, int N2,
+                               int Nc, int S1, int S2, int Sc, double scale) {
+  const long long per = (long long)N1 * N2 * Nc, total = per * nf;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(e / per);
+    long long r = e - f * per;
+    const int i1 = (int)(r / ((long long)N2 * Nc));
+    r -= (long long)i1 * N2 * Nc;
+    const int i2 = (int)(r / Nc), k3 = (int)(r - (long long)i2 * Nc);
+    const int k1 = i1 < N1 / 2 ? i1 : i1 - N1, k2 = i2 < N2 / 2 ? i2 : i2 - N2;
+    double2 v = make_double2(0.0, 0.0);
+    if (k1 != -N1 / 2 && k2 != -N2 / 2 && k3 != Nc - 1) {
+      const int s1 = k1 >= 0 ? k1 : k1 + S1, s2 = k2 >= 0 ? k2 : k2 + S2;
+      const double2 w = src[(((long long)f * S1 + s1) * S2 + s2) * Sc + k3];
+      v = make_double2(scale * w.x, scale * w.y);
+    }
+    dst[e] = v;
+  }
+}
+
 }  // namespace psx

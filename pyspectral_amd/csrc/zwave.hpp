@@ -65,9 +65,15 @@ __device__ __forceinline__ double2 zw_in(const ZArgs& a, const ZMember& zm, int 
 // 7 v_y, 8 w_y; pencils u v w S11 S22 S33 S12 S13 S23 (src/RHSfunctions.py:174-179).
 // Every other mode's pencil f is input field f.
 template <int MODE> struct ZFields {
-  static constexpr int n = MODE == Z_SMAG ? 9 : ((MODE == Z_TM2 || MODE == Z_SGS) ? 6 : 3);
+  static constexpr int n = (MODE == Z_SMAG || MODE == Z_GRAD) ? 9 : ((MODE == Z_TM2 || MODE == Z_SGS) ? 6 : 3);
 };
 template <int MODE> __host__ __device__ constexpr bool zw_uses(int fr, int fi) {
+  if (MODE == Z_GRAD) {
+    // gradient pencils A_ij = du_i/dx_j, row-major: ux uy uz vx vy vz wx wy wz from the
+    // y-pass fields 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y, 7 v_y, 8 w_y
+    constexpr int src[9] = {3, 6, 0, 4, 7, 1, 5, 8, 2};
+    return src[fr] == fi;
+  }
   if (MODE != Z_SMAG) return fr == fi;
   switch (fr) {
     case 0: return fi == 0;
@@ -107,6 +113,18 @@ __device__ __forceinline__ double2 zw_recipe(int fr, const double2* raw, double 
       case 6: return cscale(cadd(raw[6], raw[4]), 0.5);
       case 7: return cscale(cadd(cscale(cmul_i<1>(raw[0]), kz), raw[5]), 0.5);
       default: return cscale(cadd(cscale(cmul_i<1>(raw[1]), kz), raw[8]), 0.5);
+    }
+  } else if constexpr (MODE == Z_GRAD) {
+    switch (fr) {
+      case 0: return raw[3];
+      case 1: return raw[6];
+      case 2: return cscale(cmul_i<1>(raw[0]), kz);
+      case 3: return raw[4];
+      case 4: return raw[7];
+      case 5: return cscale(cmul_i<1>(raw[1]), kz);
+      case 6: return raw[5];
+      case 7: return raw[8];
+      default: return cscale(cmul_i<1>(raw[2]), kz);
     }
   } else {
     return raw[fr];
@@ -261,7 +279,27 @@ __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2*
     wave_sync();
   }
 
-  if constexpr (MODE == Z_DT) {
+  if constexpr (MODE == Z_PHYS || MODE == Z_GRAD) {
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      const int zz = l + 64 * m;
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const long long pos = pos0 + p;
+#pragma unroll
+        for (int f = 0; f < NIN; ++f) {
+          const int r = f * P + p;
+          const double v = ((r & 1) ? x[r >> 1][m].y : x[r >> 1][m].x) * a.out_scale;
+          if constexpr (MODE == Z_PHYS) {
+            if (f < a.nphys) a.phys[(long long)f * a.out_sf + pos * L + zz] = v;
+          }
+          else
+            a.gout[(((long long)blockIdx.x * P + p) * L + zz) * 9 + f] = (float)v;
+        }
+      }
+    }
+    return;
+  } else if constexpr (MODE == Z_DT) {
     double mx = 0.0;
 #pragma unroll
     for (int m = 0; m < R0; ++m) {
@@ -293,7 +331,7 @@ __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2*
           const int r = f * P + p;
           in[f] = (r & 1) ? x[r >> 1][m].y : x[r >> 1][m].x;
         }
-        zpoint<MODE>(a, in, o[p]);
+        zpoint<MODE>(a, in, o[p], ((long long)blockIdx.x * P + p) * L + l + 64 * m);
       }
 #pragma unroll
       for (int q = 0; q < NCO; ++q) {

@@ -104,3 +104,72 @@ class MLP:
         self._check(self.lib.psx_mlp_train_step(self.net, self._arg(x, 9), self._arg(target, 6), x.shape[0],
                                                 C.c_float(lr), C.c_void_p(loss.data_ptr())), "psx_mlp_train_step")
         return loss
+
+
+# -------------------------------------------------------------------- training data
+def training_pairs(dns, uhat_dns: torch.Tensor, les, delta: float | None = None, kind: str = "box"):
+    """Filtered-DNS training pairs on an LES grid.
+
+    ``dns`` and ``les`` are ``SpectralSolver`` contexts of the DNS and LES grids (the
+    same box).  The DNS velocity is filtered (``kind``, width ``delta``; default twice
+    the LES grid spacing), its SGS stress tau_ij = G(u_i u_j) - G(u_i) G(u_j) formed
+    with 3/2-rule products (``computeSGS_DNS``, src/Classes.py:861-929), and both are
+    restricted to the LES modes.  Returns (x, t): float32 device tensors (n, 9) of the
+    filtered velocity gradients du_i/dx_j (column 3i+j) and (n, 6) of the stresses
+    (order 11 22 33 12 13 23) at the n = N1 N2 N3 LES grid points."""
+    if delta is None:
+        delta = 2.0 * les.params.dx
+    dims = (dns.N1, dns.N2, dns.N3)
+    ubar = dns.filter(uhat_dns, kind, delta)
+    _, tau = dns.sgs_dns(uhat_dns, kind, delta, w0=None, tau=True)
+    u_les = les.restrict(ubar, dims)
+    tau_les = les.restrict(tau, dims)
+    x = les.gradients(u_les)
+    t = les.physical(tau_les).reshape(6, -1).t().contiguous().to(torch.float32)
+    return x, t
+
+
+def normalisation(x: torch.Tensor, t: torch.Tensor):
+    """Per-column means and standard deviations of inputs and targets (host float32)."""
+    xs = x.double().cpu().numpy()
+    ts = t.double().cpu().numpy()
+    sx = xs.std(0)
+    sy = ts.std(0)
+    return (xs.mean(0).astype(np.float32), np.where(sx > 0, sx, 1.0).astype(np.float32),
+            ts.mean(0).astype(np.float32), np.where(sy > 0, sy, 1.0).astype(np.float32))
+
+
+def input_range(net: MLP, x: torch.Tensor) -> float:
+    """Largest |normalised input| over x: the closure's range bound for inputs like x."""
+    mx, sx = net.norm[0], net.norm[1]
+    xs = x.cpu().numpy()
+    return float(np.abs((xs - mx) / sx).max())
+
+
+def fit(net: MLP, x: torch.Tensor, t: torch.Tensor, *, steps: int, batch: int = 65536, lr: float = 1e-3,
+        lr_final: float | None = None, seed: int = 0, set_norm: bool = True):
+    """Train ``net`` on (x, t) with Adam: ``steps`` steps of ``batch`` points drawn from
+    one shuffle per pass over the data; the learning rate decays geometrically from
+    ``lr`` to ``lr_final`` (default lr/10).  Returns the per-step losses (host floats)."""
+    if set_norm:
+        net.set_norm(*normalisation(x, t))
+    n = x.shape[0]
+    batch = min(batch, n) // 16 * 16
+    if batch <= 0:
+        raise ValueError("need at least 16 training points")
+    lr_final = lr / 10 if lr_final is None else lr_final
+    g = torch.Generator(device=x.device)
+    g.manual_seed(seed)
+    losses = []
+    pos, perm = n, None
+    for k in range(steps):
+        if pos + batch > n:
+            perm = torch.randperm(n, generator=g, device=x.device)
+            pos = 0
+        idx = perm[pos:pos + batch]
+        pos += batch
+        xb = x.index_select(0, idx).contiguous()
+        tb = t.index_select(0, idx).contiguous()
+        rate = lr * (lr_final / lr) ** (k / max(steps - 1, 1))
+        losses.append(net.train_step(xb, tb, rate))
+    return [float(v) for v in torch.cat(losses).cpu()]

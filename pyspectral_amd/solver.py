@@ -200,6 +200,44 @@ class SpectralSolver:
                                                   float(delta)), "psx_sgs_dns")
         return w, t
 
+    def physical(self, fhat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """irfftn(fhat) * sqrt(Nt) of (nf, N1, N2, N3/2+1) spectral fields: (nf, N1, N2, N3) float64."""
+        nf = fhat.shape[0]
+        if fhat.dtype != torch.complex128 or tuple(fhat.shape[1:]) != self.shape[1:] or not fhat.is_contiguous():
+            raise ValueError(f"fhat must be a contiguous complex128 (nf,) + {self.shape[1:]} tensor")
+        out = torch.empty((nf, self.N1, self.N2, self.N3), dtype=torch.float64, device=self.device) \
+            if out is None else out
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_physical(self.ctx, _ptr(fhat), int(nf), _ptr(out)), "psx_physical")
+        return out
+
+    def gradients(self, uhat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """du_i/dx_j at the grid points: float32 (N1*N2*N3, 9), column 3i+j."""
+        self._state_arg(uhat, "uhat")
+        out = torch.empty((self.N1 * self.N2 * self.N3, 9), dtype=torch.float32, device=self.device) \
+            if out is None else out
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_gradients(self.ctx, _ptr(uhat), _ptr(out)), "psx_gradients")
+        return out
+
+    def restrict(self, src: torch.Tensor, src_dims, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Spectral fields (nf, S1, S2, S3/2+1) of a finer grid restricted to this grid."""
+        s1, s2, s3 = (int(x) for x in src_dims)
+        nf = src.shape[0]
+        if src.dtype != torch.complex128 or tuple(src.shape[1:]) != (s1, s2, s3 // 2 + 1) or not src.is_contiguous():
+            raise ValueError("src must be a contiguous complex128 (nf, S1, S2, S3/2+1) tensor")
+        out = torch.empty((nf,) + self.shape[1:], dtype=torch.complex128, device=self.device) if out is None else out
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_restrict(self.ctx, _ptr(src), s1, s2, s3, int(nf), _ptr(out)),
+                   "psx_restrict")
+        return out
+
+    def set_closure(self, net, bound: float = float("inf")) -> None:
+        """Attach a learned-closure network (``learned.MLP``) to a ``"learned"`` solver;
+        points whose normalised inputs leave [-bound, bound] take the Smagorinsky stress."""
+        self._closure = net  # keep the network alive while attached
+        _lib.check(self.ctx, self.lib.psx_set_closure(self.ctx, net.net, float(bound)), "psx_set_closure")
+
     def stats(self, uhat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """(E, Omega, E_resolved) on the device."""
         self._state_arg(uhat, "uhat")

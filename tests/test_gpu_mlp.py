@@ -6,50 +6,11 @@ import pytest
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
+from oracle import mlp_oracle  # noqa: E402
+from oracle.mlp_oracle import unpack  # noqa: E402
 
-def bf16(a):
-    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
-    return u.astype(np.uint32).view(np.float32).astype(np.float64)
-
-
-def unpack(p):
-    from pyspectral_amd.learned import LAYOUT
-
-    return {k: p[o:o + int(np.prod(s))].reshape(s).astype(np.float64) for k, o, s in LAYOUT}
-
-
-def ref_forward(P, x, norm):
-    mx, sx, my, sy = (np.asarray(a, np.float64) for a in norm)
-    a0 = bf16(((x - mx) / sx).astype(np.float32))
-    z1 = a0 @ bf16(P["W1"]).T + P["b1"]
-    a1 = np.tanh(z1)
-    z2 = bf16(a1) @ bf16(P["W2"]).T + P["b2"]
-    a2 = np.tanh(z2)
-    z3 = bf16(a2) @ bf16(P["W3"]).T + P["b3"]
-    a3 = np.tanh(z3)
-    yn = bf16(a3) @ bf16(P["W4"]).T + P["b4"]
-    return a0, a1, a2, a3, yn, yn * sy + my
-
-
-def ref_grad(P, x, t, norm):
-    mx, sx, my, sy = (np.asarray(a, np.float64) for a in norm)
-    a0, a1, a2, a3, yn, _ = ref_forward(P, x, norm)
-    n = x.shape[0]
-    e = yn - (t - my) / sy
-    loss = (e ** 2).sum() / (6 * n)
-    d4 = bf16(2 * e / (6 * n))
-    d3 = (d4 @ bf16(P["W4"])) * (1 - a3 ** 2)
-    d2 = (bf16(d3) @ bf16(P["W3"])) * (1 - a2 ** 2)
-    d1 = (bf16(d2) @ bf16(P["W2"])) * (1 - a1 ** 2)
-    G = {"W4": d4.T @ bf16(a3), "b4": d4.sum(0), "W3": bf16(d3).T @ bf16(a2), "b3": bf16(d3).sum(0),
-         "W2": bf16(d2).T @ bf16(a1), "b2": bf16(d2).sum(0), "W1": bf16(d1).T @ a0, "b1": bf16(d1).sum(0)}
-    from pyspectral_amd.learned import LAYOUT
-
-    g = np.zeros(9350)
-    for k, o, s in LAYOUT:
-        g[o:o + int(np.prod(s))] = G[k].ravel()
-    return loss, g
+ref_forward = mlp_oracle.forward
+ref_grad = mlp_oracle.grad
 
 
 def _net(seed=3):
