@@ -24,31 +24,47 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
 C16 = 16               # bytes per complex128
 
 
+PREFIX = {"smagorinsky": "smag", "dns": "dns", "learned": "learned"}
+MLP_FLOPS = 2 * (9 * 64 + 64 * 64 + 64 * 64 + 64 * 6)  # per point, algorithmic
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+
 def kernel_bytes(N, closure):
     """Compulsory (algorithmic) bytes per launch of each kernel class at N^3, 3/2 rule:
-    every input field read once, every output field written once."""
+    every input field read once, every output field written once.  The learned closure
+    has two z passes per stage (gradients out as float32, then products with the
+    network's float32 stresses in); its z_pass figure is their mean."""
     M, H, Nc = 3 * N // 2, N // 2, N // 2 + 1
-    smag = closure == "smagorinsky"
+    smag = closure in ("smagorinsky", "learned")
     nX, nY, nZ = (6, 9, 12) if smag else (3, 3, 6)
     state = N * N * Nc * C16
+    MMH, pts = M * M * H * C16, M ** 3
+    z = nY * MMH + nZ * MMH
+    if closure == "learned":
+        z = ((9 * MMH + 36 * pts) + (3 * MMH + 24 * pts + 12 * MMH)) / 2
     return {
         "x_inverse": 3 * N * N * H * C16 + nX * M * N * H * C16,
-        "y_inverse": nX * M * N * H * C16 + nY * M * M * H * C16,
-        "z_pass": nY * M * M * H * C16 + nZ * M * M * H * C16,
-        "y_forward": nZ * M * M * H * C16 + nZ * M * N * H * C16,
+        "y_inverse": nX * M * N * H * C16 + nY * MMH,
+        "z_pass": int(z),
+        "y_forward": nZ * MMH + nZ * M * N * H * C16,
         "x_forward_epilogue": nZ * M * N * H * C16 + 3 * 3 * state,
         "cfl_x": 6 * state,
         "cfl_y": 6 * state,
         "cfl_z": 3 * state,
+        "closure_network": 60 * pts,
     }
 
 
 def survey_bytes_per_step(N, closure):
     """The 3-pass pencil model's algorithmic bytes per RK4 step (4 stages):
-    DNS 16[h(3N^2+18MN+18M^2)+9N^2Nc], Smagorinsky 16[h(3N^2+24MN+30M^2)+9N^2Nc]."""
+    DNS 16[h(3N^2+18MN+18M^2)+9N^2Nc], Smagorinsky 16[h(3N^2+24MN+30M^2)+9N^2Nc];
+    the learned closure adds a second read of 3 velocity pencils, the float32
+    gradients (36 B) and stresses (24 B) written and read: 16 h 3M^2 + 120 M^3."""
     M, h, Nc = 1.5 * N, N / 2, N // 2 + 1
-    if closure == "smagorinsky":
+    if closure in ("smagorinsky", "learned"):
         b = 16 * (h * (3 * N * N + 24 * M * N + 30 * M * M) + 9 * N * N * Nc)
+        if closure == "learned":
+            b += 16 * h * 3 * M * M + 120 * M ** 3
     else:
         b = 16 * (h * (3 * N * N + 18 * M * N + 18 * M * M) + 9 * N * N * Nc)
     return 4 * b
@@ -58,7 +74,7 @@ def measured_traffic(kernel, N, closure):
     """HBM bytes per launch of `kernel` as rocprofv3 FETCH_SIZE/WRITE_SIZE measured them (copy-calibrated),
     read from the committed counter summary for this workload; None when there is none."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"{'smag' if closure == 'smagorinsky' else 'dns'}{N}_hbm_traffic.md")
+                        f"{PREFIX[closure]}{N}_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -164,19 +180,254 @@ def bench_ensemble(args):
     }
 
 
+def _log(msg):
+    print(f"[closure] {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
+
+
+def _advance_to(solver, q, qs, t, T, on_step=None):
+    """Step until the time T is reached exactly: the last step's dt is shortened to
+    land on T (a fixed-dt step).  Returns (t, steps)."""
+    import torch
+
+    n = 0
+    while t < T - 1e-12:
+        dt = float(solver.compute_dt(qs).item())
+        if t + dt >= T - 1e-12:
+            cfl = solver.params.cfl
+            solver.set_params(cfl=-(T - t))
+            solver.step(q, qs, t)
+            solver.set_params(cfl=cfl)
+            t = T
+        else:
+            solver.step(q, qs, t)
+            t += dt
+        n += 1
+        if on_step is not None:
+            on_step(n, t)
+    torch.cuda.synchronize()
+    return t, n
+
+
+def _shell_energy(les, uhat):
+    """E(k) per integer shell on the LES grid (sum of the three components)."""
+    return les.spectrum(uhat).sum(dim=1).cpu().numpy()
+
+
+def bench_closure(args):
+    """The learned closure against filtered DNS.  Training data: forced isotropic DNS
+    (linear forcing A = 0.1, nu = 5e-4, CFL 0.5) at --train-N^3 for --train-steps steps
+    from the seeded field, box-filtered (width 2 dx_LES) to the LES grid every 20 steps;
+    the network is fitted with Adam (lr 1e-3, batch 65536) on all but the last tenth of
+    the snapshots, which are held out for the a-priori stress errors.  Judge: forced DNS
+    at --judge-N^3 from the same seeded field to the times --times; LES at --les-N^3
+    (learned closure with the Smagorinsky fallback, Smagorinsky, t-model) from the
+    box-filtered initial field, each timed on the GPU; E(k) errors against the
+    box-filtered DNS up to the cutoff."""
+    import numpy as np
+    import torch
+
+    from pyspectral_amd import cases
+    from pyspectral_amd.learned import MLP, LAYOUT, fit, input_range, training_pairs
+    from pyspectral_amd.solver import SpectralSolver
+
+    nu, A, cfl = 5e-4, 0.1, 0.5
+    NL, NT, NJ = args.les_N, args.train_N, args.judge_N
+    delta = 2.0 * (2.0 * np.pi / NL)
+    times = [float(x) for x in args.times.split(",")]
+    u0 = cases.decaying_isotropic(NL, seed=args.seed)  # the seeded field, fully resolved at NL
+    out = {"metric": "learned-LES speedup over DNS at <=5% E(k) error", "unit": "x (DNS GPU time / LES GPU time)",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f64 (network bf16/fp32)", "data": "synthetic",
+           "config": {"workload": f"forced isotropic turbulence, A={A}, nu={nu}, CFL={cfl}; training DNS {NT}^3 "
+                                  f"x {args.train_steps} steps, box filter width 2 dx_LES; judge DNS {NJ}^3; "
+                                  f"LES {NL}^3 to t = {times}", "parallelism": "single GPU"}}
+
+    # ---- training data from the forced DNS
+    t0 = time.perf_counter()
+    les = SpectralSolver(NL, NL, NL, "learned", nu=nu, cfl=cfl, force=A)
+    dns = SpectralSolver(NT, NT, NT, 0, nu=nu, cfl=cfl, force=A)
+    q = torch.from_numpy(cases.prolong(u0, NT)).to("cuda")
+    qs = q.clone()
+    xs, ts = [], []
+    t = 0.0
+    for step in range(args.train_steps + 1):
+        if step % 20 == 0:
+            x, tt = training_pairs(dns, q, les, delta)
+            xs.append(x)
+            ts.append(tt)
+        if step == args.train_steps:
+            break
+        t += float(dns.step(q, qs, t).item())
+        if step % 200 == 0:
+            _log(f"training DNS step {step}, t = {t:.3f}")
+    torch.cuda.synchronize()
+    dns.close()
+    del q, qs
+    t_data = time.perf_counter() - t0
+    out["training_dns"] = {"grid": NT, "steps": args.train_steps, "t_end": round(t, 4), "snapshots": len(xs)}
+    n_hold = max(1, len(xs) // 10)
+    x_tr, t_tr = torch.cat(xs[:-n_hold]), torch.cat(ts[:-n_hold])
+    x_te, t_te = torch.cat(xs[-n_hold:]), torch.cat(ts[-n_hold:])
+    del xs, ts
+    _log(f"{x_tr.shape[0]} training and {x_te.shape[0]} held-out points in {t_data:.1f} s")
+
+    # ---- fit
+    t0 = time.perf_counter()
+    net = MLP(seed=args.seed)
+    losses = fit(net, x_tr, t_tr, steps=args.fit_steps, batch=65536, lr=1e-3, lr_final=1e-4, seed=args.seed)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    bound = input_range(net, x_tr)
+    _log(f"fit: loss {np.mean(losses[:20]):.4f} -> {np.mean(losses[-20:]):.4f} in {t_fit:.1f} s, bound {bound:.2f}")
+
+    # ---- a-priori stress errors on the held-out snapshots
+    y = net.forward(x_te).double()
+    p = torch.from_numpy(net.params().astype(np.float64)).cuda()
+    P = {k: p[o:o + int(np.prod(sh))].reshape(sh) for k, o, sh in LAYOUT}
+    mx, sx, my, sy = (torch.from_numpy(np.asarray(a, np.float64)).cuda() for a in net.norm)
+    h = (x_te.double() - mx) / sx
+    for i in (1, 2, 3):
+        h = torch.tanh(h @ P[f"W{i}"].T + P[f"b{i}"])
+    y64 = (h @ P["W4"].T + P["b4"]) * sy + my
+    g = x_te.double()
+    S = torch.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
+                     0.5 * (g[:, 5] + g[:, 7])], 1)
+    Smag = torch.sqrt(2 * (S[:, :3] ** 2).sum(1) + 4 * (S[:, 3:] ** 2).sum(1))
+    tau_sm = -2 * (0.16 * (2 * np.pi / NL)) ** 2 * Smag[:, None] * S
+    T = t_te.double()
+    Td = T - T[:, :3].mean(1, keepdim=True) * torch.tensor([1, 1, 1, 0, 0, 0.], device=T.device, dtype=T.dtype)
+
+    def rel(a, b):
+        return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+    def corr(a, b):
+        a, b = a - a.mean(0), b - b.mean(0)
+        return [round(float(v), 3) for v in (a * b).sum(0) / torch.sqrt((a * a).sum(0) * (b * b).sum(0))]
+
+    out["a_priori"] = {
+        "held_out_points": int(x_te.shape[0]),
+        "bf16_kernel_vs_fp64_network_rel_L2": round(rel(y, y64), 5),
+        "network_vs_filtered_dns_rel_L2": round(rel(y, T), 4),
+        "network_corr_per_component": corr(y, T),
+        "smagorinsky_vs_filtered_dns_deviatoric_rel_L2": round(rel(tau_sm, Td), 4),
+        "smagorinsky_corr_per_component": corr(tau_sm, T),
+        "fallback_fraction": round(float((np.abs((x_te.cpu().numpy() - net.norm[0]) / net.norm[1]).max(1) > bound)
+                                         .mean()), 5),
+        "fit": {"points": int(x_tr.shape[0]), "steps": args.fit_steps, "loss_first": round(float(np.mean(losses[:20])), 4),
+                "loss_last": round(float(np.mean(losses[-20:])), 4), "seconds": round(t_fit, 2),
+                "data_seconds": round(t_data, 2), "bound": round(bound, 3)},
+    }
+    del x_tr, t_tr, x_te, t_te, y, y64, g, S, T, Td
+    torch.cuda.empty_cache()
+
+    # ---- judge DNS
+    ref_box, ref_sharp = {}, {}
+    judge = SpectralSolver(NJ, NJ, NJ, 0, nu=nu, cfl=cfl, force=A)
+    qj = torch.from_numpy(cases.prolong(u0, NJ)).to("cuda")
+    qjs = qj.clone()
+    t, wall, nsteps = 0.0, 0.0, 0
+    for T_ in times:
+        s0 = time.perf_counter()
+        t, n = _advance_to(judge, qj, qjs, t, T_,
+                           on_step=lambda n, tt: _log(f"judge DNS t = {tt:.3f}") if n % 200 == 0 else None)
+        wall += time.perf_counter() - s0
+        nsteps += n
+        fb = judge.filter(qj, "box", delta)
+        ref_box[T_] = _shell_energy(les, les.restrict(fb, (NJ, NJ, NJ)))
+        ref_sharp[T_] = _shell_energy(les, les.restrict(qj, (NJ, NJ, NJ)))
+        del fb
+        _log(f"judge DNS reached t = {T_} ({nsteps} steps, {wall:.1f} s)")
+    judge.close()
+    del qj, qjs
+    torch.cuda.empty_cache()
+    out["judge_dns"] = {"grid": NJ, "steps": nsteps, "gpu_seconds": round(wall, 2)}
+
+    # ---- LES runs
+    kmax = NL // 2 - 1
+    runs = {}
+    ub = None
+    for name, model in (("learned", "learned"), ("smagorinsky", 1), ("tmodel", 2)):
+        s = les if name == "learned" else SpectralSolver(NL, NL, NL, model, nu=nu, cfl=cfl, force=A, Ct=0.1)
+        if name == "learned":
+            s.set_closure(net, bound)
+        if ub is None:
+            ub = s.filter(torch.from_numpy(u0).to("cuda"), "box", delta)
+        q = ub.clone()
+        qs = q.clone()
+        s.step(q.clone(), qs.clone(), 0.0)  # warm-up (first-call setup)
+        torch.cuda.synchronize()
+        t, wall, nsteps, errs = 0.0, 0.0, 0, {}
+        for T_ in times:
+            s0 = time.perf_counter()
+            t, n = _advance_to(s, q, qs, t, T_)
+            wall += time.perf_counter() - s0
+            nsteps += n
+            E = _shell_energy(les, q)
+            kk = np.arange(1, kmax + 1)
+            eb = np.abs(E[kk] / ref_box[T_][kk] - 1)
+            es = np.abs(E[kk] / ref_sharp[T_][kk] - 1)
+            errs[str(T_)] = {"max_rel_err_box_k<=%d" % kmax: round(float(eb.max()), 4),
+                             "max_rel_err_box_k<=%d" % (NL // 4): round(float(eb[:NL // 4].max()), 4),
+                             "max_rel_err_sharp_k<=%d" % (NL // 4): round(float(es[:NL // 4].max()), 4),
+                             "E_les": float(E[1:kmax + 1].sum()), "E_dns_box": float(ref_box[T_][1:kmax + 1].sum())}
+        runs[name] = {"steps": nsteps, "gpu_seconds": round(wall, 3), "errors": errs}
+        _log(f"LES {name}: {nsteps} steps in {wall:.2f} s; {errs[str(times[-1])]}")
+        if s is not les:
+            s.close()
+    out["les"] = runs
+    worst = max(v[f"max_rel_err_box_k<={NL // 4}"] for v in runs["learned"]["errors"].values())
+    out["value"] = round(out["judge_dns"]["gpu_seconds"] / runs["learned"]["gpu_seconds"], 1)
+    out["learned_within_5pct_k<=%d" % (NL // 4)] = bool(worst <= 0.05)
+    return out
+
+
+def attach_trained_closure(solver, q, seed, steps=300):
+    """Train the learned closure on box-filtered (width 2 dx) data of the same seeded
+    field at twice the resolution (filtered-DNS stresses restricted to this grid), and
+    attach it with the Smagorinsky fallback outside the training inputs' range."""
+    import numpy as np
+    import torch
+
+    from pyspectral_amd import cases
+    from pyspectral_amd.learned import MLP, fit, input_range, training_pairs
+    from pyspectral_amd.solver import SpectralSolver
+
+    N = solver.N1
+    D = 2 * N
+    dns = SpectralSolver(D, D, D, 0, nu=solver.params.nu)
+    ud = torch.from_numpy(cases.decaying_isotropic(D, seed=seed)).to("cuda")
+    x, t = training_pairs(dns, ud, solver)
+    dns.close()
+    del ud
+    net = MLP(seed=seed)
+    losses = fit(net, x, t, steps=steps, batch=65536, lr=3e-3)
+    bound = input_range(net, x)
+    solver.set_closure(net, bound)
+    solver._bench_net = net
+    return {"training": f"{x.shape[0]} points of box-filtered {D}^3 data of the seeded field, {steps} Adam steps",
+            "loss_first": round(float(np.mean(losses[:5])), 4), "loss_last": round(float(np.mean(losses[-5:])), 4),
+            "bound": round(bound, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=128)
-    ap.add_argument("--closure", default="smagorinsky", choices=["smagorinsky", "dns"])
+    ap.add_argument("--closure", default="smagorinsky", choices=["smagorinsky", "dns", "learned"])
     ap.add_argument("--event-steps", type=int, default=50, help="steps timed per kernel with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel shapes")
-    ap.add_argument("--workload", default="les128", choices=["les128", "ensemble"],
-                    help="les128: the 128^3 LES metric (default); ensemble: 96 members of 32^3/48^3/64^3")
+    ap.add_argument("--workload", default="les128", choices=["les128", "ensemble", "closure"],
+                    help="les128: the 128^3 LES metric (default); ensemble: 96 members of 32^3/48^3/64^3; "
+                         "closure: the learned closure trained on and judged against forced DNS")
+    ap.add_argument("--les-N", type=int, default=64)
+    ap.add_argument("--train-N", type=int, default=256)
+    ap.add_argument("--judge-N", type=int, default=512)
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--fit-steps", type=int, default=3000)
+    ap.add_argument("--times", default="0.5,1.0,2.0")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size dispatch "
                          "that calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -197,6 +448,9 @@ def main():
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(args)))
         return
+    if args.workload == "closure":
+        print(json.dumps(bench_closure(args)))
+        return
 
     if args.copy_mib > 0:
         n = args.copy_mib * (1 << 20) // 8
@@ -207,11 +461,14 @@ def main():
         del a, b
 
     N, nu, cfl = args.N, 1e-3, 0.5
-    model = 1 if args.closure == "smagorinsky" else 0
+    model = {"smagorinsky": 1, "dns": 0, "learned": "learned"}[args.closure]
     uh = cases.decaying_isotropic(N, seed=args.seed)
     solver = SpectralSolver(N, N, N, model, nu=nu, cfl=cfl, Cs=0.16)
     q = torch.from_numpy(uh).to("cuda")
     qs = q.clone()
+    closure_info = None
+    if args.closure == "learned":
+        closure_info = attach_trained_closure(solver, q, args.seed)
     tuning = solver.tuning() if args.no_autotune else solver.autotune(q, reps=5)
     for _ in range(args.warmup):
         solver.step(q, qs)
@@ -243,9 +500,19 @@ def main():
                  "share": round(ms[i] / max(sum(ms), 1e-30), 4)}
         if name in kb:
             entry["GBs"] = round(kb[name] / (avg_ms * 1e-3) / 1e9, 1)
+        if name == "closure_network":
+            entry["TFLOPs"] = round(MLP_FLOPS * (3 * N // 2) ** 3 / (avg_ms * 1e-3) / 1e12, 2)
         kernels[name] = entry
     dom = max(kernels, key=lambda k: kernels[k]["share"])
     ach = kb[dom] / (kernels[dom]["avg_us"] * 1e-6) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, N, args.closure),
+            "bytes_per_launch": kb[dom]}
+    if dom == "closure_network":
+        tf = kernels[dom]["TFLOPs"]
+        roof = {"bound": "mfma", "kernel": dom, "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / BF16_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, N, args.closure),
+                "flops_per_launch": MLP_FLOPS * (3 * N // 2) ** 3}
     E = solver.stats(qs).cpu().tolist()
     finite = all(map(lambda x: x == x and abs(x) < 1e300, E))
 
@@ -266,17 +533,16 @@ def main():
         "config": {"workload": f"decaying isotropic turbulence LES, {args.closure}, {N}^3 modes "
                                f"({3 * N // 2}^3 padded), nu={nu}, CFL={cfl}, RK4 (low-storage 4-stage)",
                    "grid": N, "closure": args.closure, "cfl": cfl, "nu": nu, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom, N, args.closure),
-                     "bytes_per_launch": kb[dom]},
+        "roofline": roof,
         "step_roofline": {"model_bytes_per_step": int(sb), "achieved": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
                           "unit": "GB/s", "frac": round(sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
         "tuning": tuning,
         "stats_after": {"E": E[0], "Omega": E[1], "finite": finite},
     }
-    if args.cpu_steps > 0:
+    if closure_info is not None:
+        out["config"]["learned_closure"] = closure_info
+    if args.cpu_steps > 0 and args.closure != "learned":
         out["cpu_baseline"] = cpu_baseline(N, args.closure, nu, cfl, args.cpu_steps, args.seed)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out))

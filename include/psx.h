@@ -49,8 +49,8 @@ typedef struct psx_params {
   int kc;                 /* sharp cutoff of Gf (src/Classes.py:584-588)              */
   int rotate;             /* Coriolis terms on/off (src/RHSfunctions.py:33-50)        */
   double Om1, Om2, Om3;   /* rotation vector                                         */
-  double force;           /* linear forcing f = force * u (no counterpart in the reference;
-                             DNS, Smagorinsky and dynamic Smagorinsky)                */
+  double force;           /* linear forcing f = force * u of the velocity as the model
+                             uses it (no counterpart in the reference; every model)   */
 } psx_params;
 
 typedef struct psx_context psx_context;
@@ -169,7 +169,8 @@ int psx_tau(psx_context* ctx, double* out_host8);
  * knob on reps RHS evaluations of uhat with HIP events, keeps the fastest (knob by
  * knob), and returns the choice in tune_out (may be NULL); it allocates one temporary
  * state, and the chunked pipeline's buffers when it tries chunks, as psx_set_tuning
- * does when it sets them.  DNS, Smagorinsky and DSMAG contexts. */
+ * does when it sets them.  DNS, Smagorinsky, DSMAG and learned-closure contexts (the
+ * learned stage does not chunk). */
 #define PSX_TUNE_N 5
 int psx_set_tuning(psx_context* ctx, const int* tune);
 int psx_get_tuning(const psx_context* ctx, int* tune);
@@ -181,7 +182,8 @@ int psx_autotune(psx_context* ctx, const double* uhat, int reps, int* tune_out);
  * cores, FP32 master weights.  Parameters (PSX_MLP_NPARAM floats, row-major):
  * W1[64][9] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[6][64] b4[6].  x: (n, 9) and
  * y, target: (n, 6) float32 device arrays; inputs and targets are normalised inside
- * with psx_mlp_set_norm's means and scales.  psx_mlp_grad computes the MSE loss (into
+ * with psx_mlp_set_norm's means and scales (inputs as (x - mean) * (1/scale), the
+ * reciprocal rounded to float32).  psx_mlp_grad computes the MSE loss (into
  * loss_dev, may be NULL) and the parameter gradient (psx_mlp_gradient copies it out);
  * psx_mlp_train_step adds one Adam step (beta 0.9/0.999, eps 1e-8).  Training batches
  * are multiples of 16 points. */

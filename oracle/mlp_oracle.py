@@ -46,9 +46,11 @@ def _r(a, rounding):
 
 
 def normalise_inputs(x, norm):
-    """(x - mean) / scale in float32, as the kernel forms it."""
+    """(x - mean) * (1 / scale) in float32, as the kernel forms it (the reciprocal
+    rounded to float32 once)."""
     mx, sx = (np.asarray(a, np.float32) for a in norm[:2])
-    return (np.asarray(x, np.float32) - mx) / sx
+    isx = np.float32(1.0) / sx
+    return (np.asarray(x, np.float32) - mx) * isx
 
 
 def forward(P, x, norm, rounding=True):

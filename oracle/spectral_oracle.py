@@ -561,6 +561,8 @@ def _tau_family_finish(st, PL, w0):
     rw = unpad_2x(PL[2]) + ww
     if st.rotate == 1:
         ru, rv, rw = _coriolis_add(st, ru, rv, rw)
+    if st.force != 0.0:
+        ru, rv, rw = ru + st.force * st.uhat, rv + st.force * st.vhat, rw + st.force * st.what
     st.set_rhs(ru, rv, rw)
 
 

@@ -36,9 +36,15 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
+# per-source flags: the network kernels keep matrix-core results in ordinary vector
+# registers (no accumulation-register reads between the layers)
+EXTRA = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: str, hipcc: str, build_dir: str, defines) -> str:
     obj = os.path.join(build_dir, os.path.basename(src)[:-4] + ".o")
-    cmd = [hipcc, *FLAGS, *[f"-D{d}" for d in defines], "-c", src, "-o", obj]
+    extra = EXTRA.get(os.path.basename(src), [])
+    cmd = [hipcc, *FLAGS, *extra, *[f"-D{d}" for d in defines], "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

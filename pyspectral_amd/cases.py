@@ -73,3 +73,27 @@ def decaying_isotropic(N: int, seed: int = 0, k0: float = 4.0, sigma: float = 4.
     u = np.stack(out)
     e = (np.sum(np.abs(u[:, :, :, 1:N // 2]) ** 2) * 2 + np.sum(np.abs(u[:, :, :, 0]) ** 2)) / float(N) ** 3
     return u * (urms * np.sqrt(3.0 / e))
+
+
+def prolong(u: np.ndarray, N1: int, N2: int | None = None, N3: int | None = None) -> np.ndarray:
+    """Spectral interpolation of a state (3, n1, n2, n3/2+1) onto a finer grid
+    (3, N1, N2, N3/2+1): the signed modes |k_i| < n_i/2 are copied (the coarse grid's
+    Nyquist entries dropped), scaled by sqrt(N1 N2 N3 / (n1 n2 n3)) so that grid values
+    of the band-limited field are kept.  Used to start grids of several resolutions from
+    one seeded field."""
+    N2 = N1 if N2 is None else N2
+    N3 = N1 if N3 is None else N3
+    _, n1, n2, nc = u.shape
+    n3 = 2 * (nc - 1)
+    if N1 < n1 or N2 < n2 or N3 < n3:
+        raise ValueError("the target grid must be at least as fine as the source grid")
+    out = np.zeros((u.shape[0], N1, N2, N3 // 2 + 1), dtype=complex)
+    s1 = np.r_[0:n1 // 2, n1 // 2 + 1:n1]
+    d1 = np.r_[0:n1 // 2, N1 - n1 // 2 + 1:N1]
+    s2 = np.r_[0:n2 // 2, n2 // 2 + 1:n2]
+    d2 = np.r_[0:n2 // 2, N2 - n2 // 2 + 1:N2]
+    k3 = np.arange(n3 // 2)
+    scale = np.sqrt((N1 * N2 * N3) / (n1 * n2 * n3))
+    for c in range(u.shape[0]):
+        out[c][np.ix_(d1, d2, k3)] = u[c][np.ix_(s1, s2, k3)] * scale
+    return out

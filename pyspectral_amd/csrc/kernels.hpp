@@ -617,7 +617,7 @@ struct EpiArgs {
   long long ms_in, ms_state;
   const double* nu_m;
   int members;  // grid.y (0 or 1: one member)
-  double force;  // linear forcing coefficient (E_DNS, E_SMAG): RHS += force * u
+  double force;  // linear forcing coefficient: RHS += force * u (u truncated where trunc_u)
 };
 
 // Accumulate the contribution of transformed field f (value v) to the three RHS
@@ -830,7 +830,8 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
               red.x += wz * (w.x * r.x + w.y * r.y);
               red.y += wz * (w.y * r.x - w.x * r.y);
             }
-            rhs[cc] = a.store_only ? make_double2(0.0, 0.0) : cadd(pl_[cc * fstride + idx], w);
+            rhs[cc] = a.store_only ? make_double2(0.0, 0.0)
+                                   : cadd(cadd(pl_[cc * fstride + idx], w), cscale(u[cc], a.force));
           } else {
             rhs[cc] = cadd(acc[it][cc], cscale(u[cc], a.force - nu_ * (ksq + 1e-50)));
           }

@@ -528,7 +528,7 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
     const int P = zpass_positions(M3, Z_GRAD);
     if (P == 0 || (M2 % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
     TimeScope ts(c, K_ZPASS);
-    check(c, launch_zpass(M3, Z_GRAD, g, tw3, c->stream, 0), "z_pass(gradients)");
+    check(c, launch_zpass(M3, Z_GRAD, g, tw3, c->stream, c->tune[3]), "z_pass(gradients)");
   }
   {
     const double cd = c->p.Cs * delta_grid(c);
@@ -551,7 +551,7 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
     const int P = zpass_positions(M3, Z_LRN);
     if (P == 0 || (M2 % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
     TimeScope ts(c, K_ZPASS);
-    check(c, launch_zpass(M3, Z_LRN, q, tw3, c->stream, 0), "z_pass(learned)");
+    check(c, launch_zpass(M3, Z_LRN, q, tw3, c->stream, c->tune[3]), "z_pass(learned)");
   }
   y_forward(c, Z, W, M1, M2, H, 0, 12);
   epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true);
@@ -769,6 +769,7 @@ void tau_pass(psx_context* c, const double2* u, bool filtered, const double2* ui
     e.ks[2] = 2.0 * kPi / c->p.L3;
     e.trunc_u = 1;
     e.zero_nyq_out = zero_nyq_out;
+    e.force = c->p.force;
     e.rotate = c->p.rotate;
     e.Om[0] = c->p.Om1;
     e.Om[1] = c->p.Om2;
@@ -834,7 +835,7 @@ void stage_tau(psx_context* c, const double2* uin, const double2* q0, double2* o
                      1.0 / ((double)N1 * N2 * N3), c->scal + 1);
   check(c, hipGetLastError(), "tau_finalize");
   hipLaunchKernelGGL(tau_update, dim3(eb), dim3(256), 0, c->stream, c->pln, c->plql, c->scal + 2, uin, q0, out, w0, dt,
-                     stage_c, N1, N2, N3, Nc, ks0, ks1, ks2, zero_nyq_out, c->p.rotate, c->p.Om1, c->p.Om2, c->p.Om3);
+                     stage_c, N1, N2, N3, Nc, ks0, ks1, ks2, zero_nyq_out, c->p.rotate, c->p.Om1, c->p.Om2, c->p.Om3, c->p.force);
   check(c, hipGetLastError(), "tau_update");
 }
 
@@ -904,8 +905,6 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
 
 void validate_params(psx_context* c, const psx_params* p, int model) {
   if (!p) fail(c, PSX_E_ARG, "params is NULL");
-  if (p->force != 0.0 && model >= PSX_MODEL_TMODEL && model <= PSX_MODEL_DYNAMIC_TAU)
-    fail(c, PSX_E_UNSUPPORTED, "linear forcing is available for the DNS and Smagorinsky models");
   if (!(p->L1 > 0 && p->L2 > 0 && p->L3 > 0)) fail(c, PSX_E_ARG, "box lengths must be positive");
   if (!(p->dx > 0 && p->dy > 0 && p->dz > 0)) fail(c, PSX_E_ARG, "grid spacings must be positive");
   if (p->rotate != 0 && p->rotate != 1) fail(c, PSX_E_ARG, "rotate must be 0 or 1");
@@ -1506,7 +1505,7 @@ int psx_set_tuning(psx_context* c, const int* tune) {
     if (tune[4] < 0 || tune[4] > 8 || (tune[4] > 1 && c->M1 < 2 * tune[4])) fail(c, PSX_E_ARG, "bad chunk count");
     if (tune[4] > 1) {
       if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
-        fail(c, PSX_E_UNSUPPORTED, "chunked pipeline: 3/2-rule models only");
+        fail(c, PSX_E_UNSUPPORTED, "chunked pipeline: DNS, Smagorinsky and DSMAG");
       ensure_pipeline(c);
     }
     for (int k = 0; k < PSX_TUNE_N; ++k) c->tune[k] = tune[k];
@@ -1522,8 +1521,9 @@ int psx_get_tuning(const psx_context* c, int* tune) {
 int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
   return guarded(c, [&] {
     if (!uhat) fail(c, PSX_E_ARG, "null pointer");
-    if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
-      fail(c, PSX_E_UNSUPPORTED, "autotuning covers the 3/2-rule pipeline (DNS, Smagorinsky, DSMAG)");
+    const bool learned = c->model == PSX_MODEL_LEARNED;
+    if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG && !learned)
+      fail(c, PSX_E_UNSUPPORTED, "autotuning covers the 3/2-rule pipeline (DNS, Smagorinsky, DSMAG, learned)");
     if (reps < 1) reps = 1;
     const size_t state = (size_t)c->members * 3 * c->N1 * c->N2 * c->Nc;
     double2* tmp = nullptr;
@@ -1536,6 +1536,7 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     const bool was_timing = c->timing;
     try {
       for (int knob = 0; knob < PSX_TUNE_N; ++knob) {
+        if (learned && knob == 4) continue;  // the learned stage is not chunked
         std::vector<int> cand;
         if (knob < 3) {
           for (int T : {4, 8, 16})

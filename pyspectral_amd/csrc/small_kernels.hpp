@@ -199,7 +199,8 @@ __global__ void tau_update(const double2* __restrict__ pl, const double2* __rest
                            const double* __restrict__ tau, const double2* __restrict__ uin,
                            const double2* __restrict__ q0, double2* __restrict__ out, double2* __restrict__ w0,
                            const double* __restrict__ dt, double c, int N1, int N2, int N3, int Nc, double ks0,
-                           double ks1, double ks2, int zero_nyq_out, int rotate, double om0, double om1, double om2) {
+                           double ks1, double ks2, int zero_nyq_out, int rotate, double om0, double om1, double om2,
+                           double force) {
   const long long fs = (long long)N1 * N2 * Nc;
   const double t = *tau;
   const double dtc = q0 ? (*dt) * c : 0.0;
@@ -231,6 +232,11 @@ __global__ void tau_update(const double2* __restrict__ pl, const double2* __rest
       const double2 X = cadd(cadd(cscale(cr[0], k[0]), cscale(cr[1], k[1])), cscale(cr[2], k[2]));
 #pragma unroll
       for (int i = 0; i < 3; ++i) rhs[i] = cadd(rhs[i], csub(cscale(cr[i], 2.0), cscale(X, 2.0 * k[i] * kinv)));
+    }
+    // linear forcing A u of the truncated velocity (zero at the Nyquist entries)
+    if (force != 0.0 && !nyq) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) rhs[i] = cadd(rhs[i], cscale(uin[i * fs + e], force));
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {

@@ -32,7 +32,7 @@ class SpectralSolver:
     CFL limit), ``cfl`` (<= 0 means fixed dt = -cfl), ``Cs`` (0.16 in
     ``src/RHSfunctions.py:197``), ``Ct``, ``tau_a``/``tau_b`` (``dt0``/``dt1``),
     ``kc``, ``rotate`` and ``Om``; ``force`` is a linear forcing coefficient A (RHS += A u,
-    no counterpart in the reference; DNS and Smagorinsky models).
+    no counterpart in the reference; every model).
     """
 
     def __init__(self, N1, N2, N3, model=0, *, nu, L=(2 * math.pi,) * 3, dx=None, dy=None, dz=None,

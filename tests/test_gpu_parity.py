@@ -129,14 +129,14 @@ def test_rhs_long_z_matches_oracle(shape, models):
         s2.close()
 
 
-@pytest.mark.parametrize("model", [0, 1, 7])
+@pytest.mark.parametrize("model", MODELS)
 def test_rhs_with_linear_forcing(model):
-    N, nu, a = 16, 0.012, 0.1
+    N, nu, a, t = 16, 0.012, 0.1, 0.37
     u = _field(N, 31, nyquist_noise=True)
-    st = so.State(so.Grid(N, N, N), *u, nu=nu, turb_model=model, force=a)
+    st = so.State(so.Grid(N, N, N), *u, nu=nu, turb_model=model, force=a, t=t, **TAU_KW)
     so.compute_rhs(st, FFT)
-    s = _solver(N, model, nu, force=a)
-    r = _host(s.rhs(_dev(u), 0.0))
+    s = _solver(N, model, nu, force=a, Ct=0.2, tau_a=0.05, tau_b=0.5)
+    r = _host(s.rhs(_dev(u), t))
     assert _rel(r, st.compact()) < 1e-12
 
 

@@ -202,7 +202,7 @@ def test_linear_forcing_amplification():
     R = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
     assert np.abs(st.compact()[0] - uh[0] * R ** 4).max() < 1e-14 * np.abs(uh[0]).max()
     u = _rand_div_free(N, 21)
-    for model in (0, 1, 7):
+    for model in (0, 1, 2, "Static tau-model", "Dynamic tau-model", 7):
         s0 = _state(u, model=model, N=N)
         s1 = _state(u, model=model, N=N, force=a)
         so.compute_rhs(s0, FFT)
