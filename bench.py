@@ -184,14 +184,20 @@ def _log(msg):
     print(f"[closure] {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
 
 
-def _advance_to(solver, q, qs, t, T, on_step=None):
+def _advance_to(solver, q, qs, t, T, on_step=None, max_steps=20000):
     """Step until the time T is reached exactly: the last step's dt is shortened to
-    land on T (a fixed-dt step).  Returns (t, steps)."""
+    land on T (a fixed-dt step).  Returns (t, steps, ok); ok is False when the run left
+    finite values or needed more than max_steps steps (a blow-up)."""
+    import math
+
     import torch
 
     n = 0
     while t < T - 1e-12:
         dt = float(solver.compute_dt(qs).item())
+        if not math.isfinite(dt) or dt <= 0.0 or n >= max_steps:
+            torch.cuda.synchronize()
+            return t, n, False
         if t + dt >= T - 1e-12:
             cfl = solver.params.cfl
             solver.set_params(cfl=-(T - t))
@@ -205,7 +211,7 @@ def _advance_to(solver, q, qs, t, T, on_step=None):
         if on_step is not None:
             on_step(n, t)
     torch.cuda.synchronize()
-    return t, n
+    return t, n, True
 
 
 def _shell_energy(les, uhat):
@@ -213,16 +219,35 @@ def _shell_energy(les, uhat):
     return les.spectrum(uhat).sum(dim=1).cpu().numpy()
 
 
+def _spectral_errors(E, ref, NL):
+    """Relative E(k) errors against a reference spectrum: the largest over k <= kmax for
+    kmax = N/2 - 1 (the cutoff), N/3 and N/4, and the energy-weighted relative L2 over
+    k <= N/2 - 1."""
+    import numpy as np
+
+    out = {}
+    for km in (NL // 2 - 1, NL // 3, NL // 4):
+        kk = np.arange(1, km + 1)
+        e = np.abs(E[kk] / ref[kk] - 1)
+        out[f"max_k<={km}"] = round(float(e.max()), 4) if np.isfinite(e).all() else None
+    kk = np.arange(1, NL // 2)
+    l2 = float(np.sqrt(((E[kk] - ref[kk]) ** 2).sum() / (ref[kk] ** 2).sum()))
+    out[f"rel_L2_k<={NL // 2 - 1}"] = round(l2, 4) if np.isfinite(l2) else None
+    return out
+
+
 def bench_closure(args):
     """The learned closure against filtered DNS.  Training data: forced isotropic DNS
     (linear forcing A = 0.1, nu = 5e-4, CFL 0.5) at --train-N^3 for --train-steps steps
-    from the seeded field, box-filtered (width 2 dx_LES) to the LES grid every 20 steps;
-    the network is fitted with Adam (lr 1e-3, batch 65536) on all but the last tenth of
-    the snapshots, which are held out for the a-priori stress errors.  Judge: forced DNS
-    at --judge-N^3 from the same seeded field to the times --times; LES at --les-N^3
-    (learned closure with the Smagorinsky fallback, Smagorinsky, t-model) from the
-    box-filtered initial field, each timed on the GPU; E(k) errors against the
-    box-filtered DNS up to the cutoff."""
+    from the seeded field, filtered and restricted to the LES grid every 20 steps with
+    two filters: the box of width 2 dx_LES and the sharp cutoff at the LES grid's
+    Nyquist (kc = N_LES/2).  One network is fitted per filter with Adam (lr 1e-3 decaying
+    to 1e-4, batch 65536) on all but the last tenth of the snapshots, which are held out
+    for the a-priori stress errors.  Judge: forced DNS at --judge-N^3 from the same
+    seeded field to the times --times.  LES at --les-N^3 from the filtered initial field:
+    each network with and without the backscatter clip, Smagorinsky, dynamic
+    Smagorinsky and the t-model, each timed on the GPU; E(k) errors against the
+    box-filtered and the sharp-cut DNS spectra."""
     import numpy as np
     import torch
 
@@ -233,27 +258,31 @@ def bench_closure(args):
     nu, A, cfl = 5e-4, 0.1, 0.5
     NL, NT, NJ = args.les_N, args.train_N, args.judge_N
     delta = 2.0 * (2.0 * np.pi / NL)
+    c2_les = (0.16 * 2.0 * np.pi / NL) ** 2
     times = [float(x) for x in args.times.split(",")]
+    filters = ("box", "sharp")
     u0 = cases.decaying_isotropic(NL, seed=args.seed)  # the seeded field, fully resolved at NL
     out = {"metric": "learned-LES speedup over DNS at <=5% E(k) error", "unit": "x (DNS GPU time / LES GPU time)",
            "n_gpus": 1, "higher_is_better": True, "dtype": "f64 (network bf16/fp32)", "data": "synthetic",
            "config": {"workload": f"forced isotropic turbulence, A={A}, nu={nu}, CFL={cfl}; training DNS {NT}^3 "
-                                  f"x {args.train_steps} steps, box filter width 2 dx_LES; judge DNS {NJ}^3; "
-                                  f"LES {NL}^3 to t = {times}", "parallelism": "single GPU"}}
+                                  f"x {args.train_steps} steps, filters: box width 2 dx_LES and sharp cutoff at "
+                                  f"k = {NL // 2}; judge DNS {NJ}^3; LES {NL}^3 to t = {times}",
+                      "parallelism": "single GPU"}}
 
-    # ---- training data from the forced DNS
+    # ---- training data from the forced DNS (both filters from the same snapshots)
     t0 = time.perf_counter()
     les = SpectralSolver(NL, NL, NL, "learned", nu=nu, cfl=cfl, force=A)
-    dns = SpectralSolver(NT, NT, NT, 0, nu=nu, cfl=cfl, force=A)
+    dns = SpectralSolver(NT, NT, NT, 0, nu=nu, cfl=cfl, force=A, kc=NL // 2)
     q = torch.from_numpy(cases.prolong(u0, NT)).to("cuda")
     qs = q.clone()
-    xs, ts = [], []
+    pairs = {f: ([], []) for f in filters}
     t = 0.0
     for step in range(args.train_steps + 1):
         if step % 20 == 0:
-            x, tt = training_pairs(dns, q, les, delta)
-            xs.append(x)
-            ts.append(tt)
+            for f in filters:
+                x, tt = training_pairs(dns, q, les, delta, f)
+                pairs[f][0].append(x)
+                pairs[f][1].append(tt)
         if step == args.train_steps:
             break
         t += float(dns.step(q, qs, t).item())
@@ -263,38 +292,10 @@ def bench_closure(args):
     dns.close()
     del q, qs
     t_data = time.perf_counter() - t0
-    out["training_dns"] = {"grid": NT, "steps": args.train_steps, "t_end": round(t, 4), "snapshots": len(xs)}
-    n_hold = max(1, len(xs) // 10)
-    x_tr, t_tr = torch.cat(xs[:-n_hold]), torch.cat(ts[:-n_hold])
-    x_te, t_te = torch.cat(xs[-n_hold:]), torch.cat(ts[-n_hold:])
-    del xs, ts
-    _log(f"{x_tr.shape[0]} training and {x_te.shape[0]} held-out points in {t_data:.1f} s")
-
-    # ---- fit
-    t0 = time.perf_counter()
-    net = MLP(seed=args.seed)
-    losses = fit(net, x_tr, t_tr, steps=args.fit_steps, batch=65536, lr=1e-3, lr_final=1e-4, seed=args.seed)
-    torch.cuda.synchronize()
-    t_fit = time.perf_counter() - t0
-    bound = input_range(net, x_tr)
-    _log(f"fit: loss {np.mean(losses[:20]):.4f} -> {np.mean(losses[-20:]):.4f} in {t_fit:.1f} s, bound {bound:.2f}")
-
-    # ---- a-priori stress errors on the held-out snapshots
-    y = net.forward(x_te).double()
-    p = torch.from_numpy(net.params().astype(np.float64)).cuda()
-    P = {k: p[o:o + int(np.prod(sh))].reshape(sh) for k, o, sh in LAYOUT}
-    mx, sx, my, sy = (torch.from_numpy(np.asarray(a, np.float64)).cuda() for a in net.norm)
-    h = (x_te.double() - mx) / sx
-    for i in (1, 2, 3):
-        h = torch.tanh(h @ P[f"W{i}"].T + P[f"b{i}"])
-    y64 = (h @ P["W4"].T + P["b4"]) * sy + my
-    g = x_te.double()
-    S = torch.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
-                     0.5 * (g[:, 5] + g[:, 7])], 1)
-    Smag = torch.sqrt(2 * (S[:, :3] ** 2).sum(1) + 4 * (S[:, 3:] ** 2).sum(1))
-    tau_sm = -2 * (0.16 * (2 * np.pi / NL)) ** 2 * Smag[:, None] * S
-    T = t_te.double()
-    Td = T - T[:, :3].mean(1, keepdim=True) * torch.tensor([1, 1, 1, 0, 0, 0.], device=T.device, dtype=T.dtype)
+    nsnap = len(pairs["box"][0])
+    out["training_dns"] = {"grid": NT, "steps": args.train_steps, "t_end": round(t, 4), "snapshots": nsnap,
+                           "seconds": round(t_data, 2)}
+    n_hold = max(1, nsnap // 10)
 
     def rel(a, b):
         return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
@@ -303,38 +304,72 @@ def bench_closure(args):
         a, b = a - a.mean(0), b - b.mean(0)
         return [round(float(v), 3) for v in (a * b).sum(0) / torch.sqrt((a * a).sum(0) * (b * b).sum(0))]
 
-    out["a_priori"] = {
-        "held_out_points": int(x_te.shape[0]),
-        "bf16_kernel_vs_fp64_network_rel_L2": round(rel(y, y64), 5),
-        "network_vs_filtered_dns_rel_L2": round(rel(y, T), 4),
-        "network_corr_per_component": corr(y, T),
-        "smagorinsky_vs_filtered_dns_deviatoric_rel_L2": round(rel(tau_sm, Td), 4),
-        "smagorinsky_corr_per_component": corr(tau_sm, T),
-        "fallback_fraction": round(float((np.abs((x_te.cpu().numpy() - net.norm[0]) / net.norm[1]).max(1) > bound)
-                                         .mean()), 5),
-        "fit": {"points": int(x_tr.shape[0]), "steps": args.fit_steps, "loss_first": round(float(np.mean(losses[:20])), 4),
-                "loss_last": round(float(np.mean(losses[-20:])), 4), "seconds": round(t_fit, 2),
-                "data_seconds": round(t_data, 2), "bound": round(bound, 3)},
-    }
-    del x_tr, t_tr, x_te, t_te, y, y64, g, S, T, Td
+    nets, bounds, out["a_priori"] = {}, {}, {}
+    for f in filters:
+        xs, ts = pairs[f]
+        x_tr, t_tr = torch.cat(xs[:-n_hold]), torch.cat(ts[:-n_hold])
+        x_te, t_te = torch.cat(xs[-n_hold:]), torch.cat(ts[-n_hold:])
+        s0 = time.perf_counter()
+        net = MLP(seed=args.seed)
+        losses = fit(net, x_tr, t_tr, steps=args.fit_steps, batch=65536, lr=1e-3, lr_final=1e-4, seed=args.seed)
+        torch.cuda.synchronize()
+        t_fit = time.perf_counter() - s0
+        bound = input_range(net, x_tr)
+        nets[f], bounds[f] = net, bound
+        _log(f"fit ({f}): loss {np.mean(losses[:20]):.4f} -> {np.mean(losses[-20:]):.4f} in {t_fit:.1f} s")
+        # a-priori stress errors on the held-out snapshots
+        y = net.forward(x_te).double()
+        yc = net.closure(x_te, c2_les, bound, True).double()
+        p = torch.from_numpy(net.params().astype(np.float64)).cuda()
+        P = {k: p[o:o + int(np.prod(sh))].reshape(sh) for k, o, sh in LAYOUT}
+        mx, sx, my, sy = (torch.from_numpy(np.asarray(a, np.float64)).cuda() for a in net.norm)
+        h = (x_te.double() - mx) / sx
+        for i in (1, 2, 3):
+            h = torch.tanh(h @ P[f"W{i}"].T + P[f"b{i}"])
+        y64 = (h @ P["W4"].T + P["b4"]) * sy + my
+        g = x_te.double()
+        S = torch.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
+                         0.5 * (g[:, 5] + g[:, 7])], 1)
+        Smag = torch.sqrt(2 * (S[:, :3] ** 2).sum(1) + 4 * (S[:, 3:] ** 2).sum(1))
+        tau_sm = -2 * c2_les * Smag[:, None] * S
+        T = t_te.double()
+        w6 = torch.tensor([1, 1, 1, 2, 2, 2.], device=T.device, dtype=T.dtype)
+        diss = lambda tau: float(-(tau * S * w6).sum(1).mean())  # noqa: E731  mean SGS dissipation
+        out["a_priori"][f] = {
+            "held_out_points": int(x_te.shape[0]),
+            "bf16_kernel_vs_fp64_network_rel_L2": round(rel(y, y64), 5),
+            "network_vs_filtered_dns_rel_L2": round(rel(y, T), 4),
+            "network_corr_per_component": corr(y, T),
+            "clipped_closure_vs_filtered_dns_rel_L2": round(rel(yc, T), 4),
+            "smagorinsky_corr_per_component": corr(tau_sm, T),
+            "mean_dissipation": {"filtered_dns": round(diss(T), 5), "network": round(diss(y), 5),
+                                 "clipped_closure": round(diss(yc), 5), "smagorinsky": round(diss(tau_sm), 5)},
+            "backscatter_fraction_network": round(float(((y * S * w6).sum(1) > 0).double().mean()), 4),
+            "fit": {"points": int(x_tr.shape[0]), "steps": args.fit_steps,
+                    "loss_first": round(float(np.mean(losses[:20])), 4),
+                    "loss_last": round(float(np.mean(losses[-20:])), 4), "seconds": round(t_fit, 2),
+                    "bound": round(bound, 3)},
+        }
+        del x_tr, t_tr, x_te, t_te, y, yc, y64, g, S, T
+    del pairs
     torch.cuda.empty_cache()
 
     # ---- judge DNS
-    ref_box, ref_sharp = {}, {}
+    refs = {"box": {}, "sharp": {}}
     judge = SpectralSolver(NJ, NJ, NJ, 0, nu=nu, cfl=cfl, force=A)
     qj = torch.from_numpy(cases.prolong(u0, NJ)).to("cuda")
     qjs = qj.clone()
     t, wall, nsteps = 0.0, 0.0, 0
     for T_ in times:
         s0 = time.perf_counter()
-        t, n = _advance_to(judge, qj, qjs, t, T_,
-                           on_step=lambda n, tt: _log(f"judge DNS t = {tt:.3f}") if n % 200 == 0 else None)
+        t, n, _ = _advance_to(judge, qj, qjs, t, T_,
+                              on_step=lambda n, tt: _log(f"judge DNS t = {tt:.3f}") if n % 200 == 0 else None)
         wall += time.perf_counter() - s0
         nsteps += n
         fb = judge.filter(qj, "box", delta)
-        ref_box[T_] = _shell_energy(les, les.restrict(fb, (NJ, NJ, NJ)))
-        ref_sharp[T_] = _shell_energy(les, les.restrict(qj, (NJ, NJ, NJ)))
+        refs["box"][T_] = _shell_energy(les, les.restrict(fb, (NJ, NJ, NJ)))
         del fb
+        refs["sharp"][T_] = _shell_energy(les, les.restrict(qj, (NJ, NJ, NJ)))
         _log(f"judge DNS reached t = {T_} ({nsteps} steps, {wall:.1f} s)")
     judge.close()
     del qj, qjs
@@ -342,41 +377,55 @@ def bench_closure(args):
     out["judge_dns"] = {"grid": NJ, "steps": nsteps, "gpu_seconds": round(wall, 2)}
 
     # ---- LES runs
-    kmax = NL // 2 - 1
+    ud = torch.from_numpy(u0).to("cuda")
+    init = {"box": les.filter(ud, "box", delta), "sharp": ud}
     runs = {}
-    ub = None
-    for name, model in (("learned", "learned"), ("smagorinsky", 1), ("tmodel", 2)):
-        s = les if name == "learned" else SpectralSolver(NL, NL, NL, model, nu=nu, cfl=cfl, force=A, Ct=0.1)
-        if name == "learned":
-            s.set_closure(net, bound)
-        if ub is None:
-            ub = s.filter(torch.from_numpy(u0).to("cuda"), "box", delta)
-        q = ub.clone()
+    variants = [(f"learned_{f}" + ("_clip" if clip else ""), "learned", f, clip)
+                for f in filters for clip in (True, False)]
+    variants += [("smagorinsky", 1, None, False), ("dsmag", 7, None, False), ("tmodel", 2, None, False)]
+    for name, model, f, clip in variants:
+        s = SpectralSolver(NL, NL, NL, model, nu=nu, cfl=cfl, force=A, Ct=0.1)
+        if model == "learned":
+            s.set_closure(nets[f], bounds[f], clip_backscatter=clip)
+        ref_kind = f or "box"
+        q = init[ref_kind].clone()
         qs = q.clone()
         s.step(q.clone(), qs.clone(), 0.0)  # warm-up (first-call setup)
         torch.cuda.synchronize()
-        t, wall, nsteps, errs = 0.0, 0.0, 0, {}
+        t, wall, nsteps, errs, ok = 0.0, 0.0, 0, {}, True
         for T_ in times:
             s0 = time.perf_counter()
-            t, n = _advance_to(s, q, qs, t, T_)
+            t, n, ok = _advance_to(s, q, qs, t, T_)
             wall += time.perf_counter() - s0
             nsteps += n
+            if not ok:
+                errs[str(T_)] = "blew up"
+                break
             E = _shell_energy(les, q)
-            kk = np.arange(1, kmax + 1)
-            eb = np.abs(E[kk] / ref_box[T_][kk] - 1)
-            es = np.abs(E[kk] / ref_sharp[T_][kk] - 1)
-            errs[str(T_)] = {"max_rel_err_box_k<=%d" % kmax: round(float(eb.max()), 4),
-                             "max_rel_err_box_k<=%d" % (NL // 4): round(float(eb[:NL // 4].max()), 4),
-                             "max_rel_err_sharp_k<=%d" % (NL // 4): round(float(es[:NL // 4].max()), 4),
-                             "E_les": float(E[1:kmax + 1].sum()), "E_dns_box": float(ref_box[T_][1:kmax + 1].sum())}
-        runs[name] = {"steps": nsteps, "gpu_seconds": round(wall, 3), "errors": errs}
-        _log(f"LES {name}: {nsteps} steps in {wall:.2f} s; {errs[str(times[-1])]}")
-        if s is not les:
-            s.close()
+            errs[str(T_)] = {"vs_box": _spectral_errors(E, refs["box"][T_], NL),
+                             "vs_sharp": _spectral_errors(E, refs["sharp"][T_], NL),
+                             "E_les": round(float(E[1:NL // 2].sum()), 5),
+                             "E_dns_box": round(float(refs["box"][T_][1:NL // 2].sum()), 5),
+                             "E_dns_sharp": round(float(refs["sharp"][T_][1:NL // 2].sum()), 5)}
+        runs[name] = {"initial": f"{ref_kind}-filtered", "steps": nsteps, "gpu_seconds": round(wall, 3),
+                      "stable": ok, "errors": errs}
+        _log(f"LES {name}: {nsteps} steps in {wall:.2f} s; last {errs.get(str(times[-1]), errs)}")
+        s.close()
     out["les"] = runs
-    worst = max(v[f"max_rel_err_box_k<={NL // 4}"] for v in runs["learned"]["errors"].values())
-    out["value"] = round(out["judge_dns"]["gpu_seconds"] / runs["learned"]["gpu_seconds"], 1)
-    out["learned_within_5pct_k<=%d" % (NL // 4)] = bool(worst <= 0.05)
+    best = None
+    for name, r in runs.items():
+        if not name.startswith("learned") or not r["stable"]:
+            continue
+        kind = "vs_box" if "_box" in name else "vs_sharp"
+        worst = max(e[kind][f"max_k<={NL // 2 - 1}"] or 1e9 for e in r["errors"].values())
+        if best is None or worst < best[1]:
+            best = (name, worst)
+    out["best_learned"] = {"run": best[0], "worst_max_rel_err_to_cutoff": best[1]} if best else None
+    if best:
+        out["value"] = round(out["judge_dns"]["gpu_seconds"] / runs[best[0]]["gpu_seconds"], 1)
+        out["within_5pct_to_cutoff"] = bool(best[1] <= 0.05)
+    else:
+        out["value"] = None
     return out
 
 
@@ -491,6 +540,10 @@ def main():
     cnt = (C.c_int * len(_lib.KERNEL_NAMES))()
     _lib.check(solver.ctx, lib.psx_timing_end(solver.ctx, ms, cnt), "psx_timing_end")
     kb = kernel_bytes(N, args.closure)
+    # the chunked y/z pipeline splits each of those passes into yz_chunks launches
+    nch = max(1, int(tuning.get("yz_chunks", 0)))
+    for name in ("y_inverse", "z_pass", "y_forward"):
+        kb[name] = kb[name] // nch
     kernels = {}
     for i, name in enumerate(_lib.KERNEL_NAMES):
         if cnt[i] == 0:
@@ -506,7 +559,8 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["share"])
     ach = kb[dom] / (kernels[dom]["avg_us"] * 1e-6) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, N, args.closure),
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom, N, args.closure) if nch == 1 else None,
             "bytes_per_launch": kb[dom]}
     if dom == "closure_network":
         tf = kernels[dom]["TFLOPs"]

@@ -10,9 +10,11 @@
  * that view onto this compact one.
  *
  * Every call is asynchronous on the context's HIP stream and returns 0 on
- * success or a negative PSX_E* code; psx_last_error() gives the message.  No
- * call allocates device memory except psx_create.  A context is not
- * thread-safe; separate contexts are independent.
+ * success or a negative PSX_E* code; psx_last_error() gives the message.
+ * Device memory is allocated by the create calls, by psx_sgs_dns (first call),
+ * psx_set_tuning / psx_autotune (chunked-pipeline buffers, a temporary state) and
+ * psx_mlp_grad (activation store, when the batch size changes); by nothing else.
+ * A context is not thread-safe; separate contexts are independent.
  */
 #ifndef PSX_H
 #define PSX_H
@@ -54,6 +56,7 @@ typedef struct psx_params {
 } psx_params;
 
 typedef struct psx_context psx_context;
+typedef struct psx_mlp psx_mlp; /* the learned closure's network, see psx_mlp_create */
 
 /* Replaces FFTclass + gridclass + the turb_model dispatch of variables
  * (src/Classes.py:590-625, 548-588, 28-229): plans, twiddle tables and pipeline
@@ -66,7 +69,7 @@ int psx_create(psx_context** ctx, int n1, int n2, int n3, int model, const psx_p
  * Every state argument then holds the members one after another,
  * (members, 3, N1, N2, N3/2+1); dt_dev arguments hold one dt per member; nu gives
  * each member's viscosity (NULL: params->nu for all).  Each kernel launch covers all
- * members.  DNS and Smagorinsky only; psx_spectrum, psx_filter and psx_sgs_dns take
+ * members.  DNS, Smagorinsky and learned closure; psx_spectrum, psx_filter and psx_sgs_dns take
  * single-member contexts; psx_stats writes 3 values per member. */
 int psx_create_batch(psx_context** ctx, int n1, int n2, int n3, int model, const psx_params* params, int members,
                      const double* nu, int device, void* stream);
@@ -91,6 +94,8 @@ int psx_ensemble_set_stream(psx_ensemble* ens, void* stream);
 int psx_ensemble_step(psx_ensemble* ens, double* q, double* qs, double* dt_dev, double t);
 int psx_ensemble_step_graph(psx_ensemble* ens, double* q, double* qs, double* dt_dev);
 int psx_ensemble_stats(psx_ensemble* ens, const double* q, double* out_dev);
+/* psx_set_closure for every member of a PSX_MODEL_LEARNED ensemble (one network). */
+int psx_ensemble_set_closure(psx_ensemble* ens, psx_mlp* net, double bound, int flags);
 int psx_destroy(psx_context* ctx);
 int psx_set_stream(psx_context* ctx, void* stream);
 int psx_set_params(psx_context* ctx, const psx_params* params);
@@ -165,13 +170,15 @@ int psx_tau(psx_context* ctx, double* out_host8);
  * z length is a multiple of 64, 1: the block-wide Stockham kernel), tune[4] = number of
  * x-row chunks (0 or 1: none; at most 8) in which the y-inverse, z and y-forward passes
  * run alternately on the context stream and a second library stream, so that their
- * HBM- and compute-bound phases overlap.  psx_autotune times every candidate of each
+ * HBM- and compute-bound phases overlap; tune[5] = 1 runs the chunks in order on the
+ * context stream instead (0 or 2: two streams), so that a chunk's intermediates can be
+ * re-read from the last-level cache.  psx_autotune times every candidate of each
  * knob on reps RHS evaluations of uhat with HIP events, keeps the fastest (knob by
  * knob), and returns the choice in tune_out (may be NULL); it allocates one temporary
  * state, and the chunked pipeline's buffers when it tries chunks, as psx_set_tuning
  * does when it sets them.  DNS, Smagorinsky, DSMAG and learned-closure contexts (the
  * learned stage does not chunk). */
-#define PSX_TUNE_N 5
+#define PSX_TUNE_N 6
 int psx_set_tuning(psx_context* ctx, const int* tune);
 int psx_get_tuning(const psx_context* ctx, int* tune);
 int psx_autotune(psx_context* ctx, const double* uhat, int reps, int* tune_out);
@@ -188,7 +195,6 @@ int psx_autotune(psx_context* ctx, const double* uhat, int reps, int* tune_out);
  * psx_mlp_train_step adds one Adam step (beta 0.9/0.999, eps 1e-8).  Training batches
  * are multiples of 16 points. */
 #define PSX_MLP_NPARAM 9350
-typedef struct psx_mlp psx_mlp;
 int psx_mlp_create(psx_mlp** net, int device, void* stream, unsigned long long seed);
 int psx_mlp_destroy(psx_mlp* net);
 const char* psx_mlp_last_error(const psx_mlp* net);
@@ -198,6 +204,11 @@ int psx_mlp_set_params(psx_mlp* net, const float* host_params);
 int psx_mlp_set_norm(psx_mlp* net, const float* x_mean, const float* x_scale, const float* y_mean,
                      const float* y_scale);
 int psx_mlp_forward(psx_mlp* net, const float* x, float* y, long long n);
+/* The closure as the learned LES evaluates it: stresses tau (n, 6) from gradients g (n, 9)
+ * (du_i/dx_j at column 3i+j), the Smagorinsky stress -2 c2 |S| S_ij outside the input
+ * range [-bound, bound] and, with PSX_CLOSURE_CLIP_BACKSCATTER in flags, where the
+ * network's stress is backscatter (see psx_set_closure). */
+int psx_mlp_closure(psx_mlp* net, const float* g, float* tau, long long n, double c2, double bound, int flags);
 int psx_mlp_grad(psx_mlp* net, const float* x, const float* target, long long n, float* loss_dev);
 int psx_mlp_gradient(psx_mlp* net, float* grad_dev);
 int psx_mlp_train_step(psx_mlp* net, const float* x, const float* target, long long n, float lr, float* loss_dev);
@@ -211,9 +222,12 @@ int psx_mlp_train_step(psx_mlp* net, const float* x, const float* target, long l
  * computeRHS_SMAG (src/RHSfunctions.py:146-237): w0 and the RHS follow
  * src/RHSfunctions.py:215-237.  A point whose normalised inputs leave [-bound, bound]
  * takes the Smagorinsky stress -2 (Cs Delta)^2 |S| S_ij of its gradients (bound =
- * INFINITY: never; a negative bound: everywhere).  Re-attach after changing the
+ * INFINITY: never; a negative bound: everywhere).  With flags PSX_CLOSURE_CLIP_BACKSCATTER
+ * a point where the network's stress is backscatter (tau_ij S_ij > 0: negative subgrid
+ * dissipation) takes the Smagorinsky stress too.  Re-attach after changing the
  * network's normalisation: a captured step (psx_step_graph) holds it by value. */
-int psx_set_closure(psx_context* ctx, psx_mlp* net, double bound);
+#define PSX_CLOSURE_CLIP_BACKSCATTER 1
+int psx_set_closure(psx_context* ctx, psx_mlp* net, double bound, int flags);
 
 /* Grid-space fields (no counterpart as one call in the reference, which calls
  * myFFT.ifftT_obj, src/Classes.py:596-607): out (nfields, N1, N2, N3) =

@@ -96,18 +96,31 @@ def smag_pointwise(g, c2):
     return -2.0 * c2 * Smag[:, None] * S
 
 
-def closure(P, norm, bound, c2):
+def backscatter(tau, g):
+    """tau_ij S_ij > 0 (negative subgrid dissipation -tau_ij S_ij) per point, for stresses
+    (n, 6) in the order 11 22 33 12 13 23 and gradients (n, 9)."""
+    g = np.asarray(g, np.float64)
+    S = np.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
+                  0.5 * (g[:, 5] + g[:, 7])], axis=1)
+    w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])
+    return (np.asarray(tau, np.float64) * S * w).sum(axis=1)
+
+
+def closure(P, norm, bound, c2, clip=False):
     """The kernel's closure as a function of float64 gradients (n, 9) -> stresses (n, 6):
-    inputs cast to float32, the network where every |normalised input| <= bound,
-    the Smagorinsky stress elsewhere; outputs rounded to float32 as the kernel stores them."""
+    inputs cast to float32, the network where every |normalised input| <= bound (and,
+    with ``clip``, where its stress is not backscatter), the Smagorinsky stress
+    elsewhere; outputs rounded to float32 as the kernel stores them."""
 
     def f(g):
         gf = np.asarray(g, np.float32)
         xn = normalise_inputs(gf, norm)
         use_net = np.abs(xn).max(axis=1) <= bound
         y = smag_pointwise(gf, c2)
-        if use_net.any():
-            y[use_net] = forward(P, gf[use_net], norm)[-1]
+        net = forward(P, gf, norm)[-1].astype(np.float32).astype(np.float64)
+        if clip:
+            use_net &= backscatter(net, gf) <= 0.0
+        y[use_net] = net[use_net]
         return y.astype(np.float32).astype(np.float64), use_net
 
     return f

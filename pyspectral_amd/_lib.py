@@ -34,11 +34,13 @@ EXPORTS = (
     "psx_mlp_create", "psx_mlp_destroy", "psx_mlp_last_error", "psx_mlp_set_stream", "psx_mlp_get_params",
     "psx_mlp_set_params", "psx_mlp_set_norm", "psx_mlp_forward", "psx_mlp_grad", "psx_mlp_gradient",
     "psx_mlp_train_step", "psx_set_closure", "psx_physical", "psx_gradients", "psx_restrict",
+    "psx_ensemble_set_closure", "psx_mlp_closure",
 )
 
 MLP_NPARAM = 9350
 
 PSX_FILTER_SHARP, PSX_FILTER_BOX = 0, 1
+PSX_CLOSURE_CLIP_BACKSCATTER = 1
 
 KERNEL_NAMES = ("x_inverse", "y_inverse", "z_pass", "y_forward", "x_forward_epilogue",
                 "cfl_x", "cfl_y", "cfl_z", "other", "closure_network")
@@ -123,10 +125,12 @@ def load() -> C.CDLL:
         "psx_mlp_grad": (i, [vp, dp, dp, C.c_longlong, dp]),
         "psx_mlp_gradient": (i, [vp, dp]),
         "psx_mlp_train_step": (i, [vp, dp, dp, C.c_longlong, C.c_float, dp]),
-        "psx_set_closure": (i, [vp, vp, d]),
+        "psx_set_closure": (i, [vp, vp, d, i]),
         "psx_physical": (i, [vp, dp, i, dp]),
         "psx_gradients": (i, [vp, dp, dp]),
         "psx_restrict": (i, [vp, dp, i, i, i, i, dp]),
+        "psx_ensemble_set_closure": (i, [vp, vp, d, i]),
+        "psx_mlp_closure": (i, [vp, dp, dp, C.c_longlong, d, d, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

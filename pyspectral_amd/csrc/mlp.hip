@@ -85,10 +85,21 @@ struct Norm {
 
 // Smagorinsky stress -2 c2 |S| S_ij from the gradients g[3i + j] = du_i/dx_j, with
 // |S| = sqrt(2 S_ij S_ij); components o0 .. o0+3 of the order 11 22 33 12 13 23
+__device__ __forceinline__ void strain(const float* g, float (&S)[6]) {
+  S[0] = g[0];
+  S[1] = g[4];
+  S[2] = g[8];
+  S[3] = 0.5f * (g[1] + g[3]);
+  S[4] = 0.5f * (g[2] + g[6]);
+  S[5] = 0.5f * (g[5] + g[7]);
+}
+__device__ __forceinline__ float strain_norm(const float (&S)[6]) {
+  return sqrtf(2.f * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + 2.f * (S[3] * S[3] + S[4] * S[4] + S[5] * S[5])));
+}
 __device__ __forceinline__ void smag_stress(const float* g, float c2, int o0, float (&t)[4]) {
-  const float S[6] = {g[0], g[4], g[8], 0.5f * (g[1] + g[3]), 0.5f * (g[2] + g[6]), 0.5f * (g[5] + g[7])};
-  const float Smag = sqrtf(2.f * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] +
-                                  2.f * (S[3] * S[3] + S[4] * S[4] + S[5] * S[5])));
+  float S[6];
+  strain(g, S);
+  const float Smag = strain_norm(S);
 #pragma unroll
   for (int j = 0; j < 4; ++j) t[j] = (o0 + j < NOUT) ? -2.f * c2 * Smag * S[o0 + j < NOUT ? o0 + j : 0] : 0.f;
 }
@@ -304,6 +315,150 @@ __global__ __launch_bounds__(64) void mlp_tiles(const float* __restrict__ x, con
   }
 }
 
+// Inference over TP tiles of 16 points per wavefront at a time: the TP tiles' matrix
+// products and tanh evaluations are independent, so one tile's MFMAs overlap another's
+// VALU work inside the wavefront.  Outputs y (denormalised); with the range guard a
+// point whose largest |normalised input| exceeds fb_bound takes the Smagorinsky stress
+// of its own gradients.
+template <int TP>
+__global__ __launch_bounds__(64) void mlp_infer(const float* __restrict__ x, float* __restrict__ y, long long n,
+                                                const float* __restrict__ w, const short* __restrict__ pk, Norm nm,
+                                                float fb_c2, float fb_bound, int clip) {
+  const int lane = threadIdx.x, g = lane >> 4, r = lane & 15;
+  constexpr int NT_ = T1 + T2 + T3 + T4;
+  bf4 A[NT_];
+#pragma unroll
+  for (int t = 0; t < NT_; ++t) A[t] = tile_of(pk, t, lane);
+  f4 b1[4], b2[4], b3[4], b4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      b1[t][j] = w[OB1 + 16 * t + 4 * g + j];
+      b2[t][j] = w[OB2 + 16 * t + 4 * g + j];
+      b3[t][j] = w[OB3 + 16 * t + 4 * g + j];
+    }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b4[j] = (4 * g + j < NOUT) ? w[OB4 + 4 * g + j] : 0.f;
+  // this lane's normalisation constants (features and outputs 4g .. 4g+3)
+  float lmx[4], lis[4], lmy[4], lsy[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    lmx[j] = lis[j] = lmy[j] = lsy[j] = 0.f;
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const int f = 4 * gg + j;
+      if (g == gg && f < NIN) {
+        lmx[j] = nm.mx[f];
+        lis[j] = nm.isx[f];
+      }
+      if (g == gg && f < NOUT) {
+        lmy[j] = nm.my[f];
+        lsy[j] = nm.sy[f];
+      }
+    }
+  }
+  const long long ntiles = (n + 15) / 16;
+  for (long long t0 = (long long)blockIdx.x * TP; t0 < ntiles; t0 += (long long)gridDim.x * TP) {
+    long long p[TP];
+    bool valid[TP];
+    f4 xn[TP];
+    bf4 B[TP][4];
+#pragma unroll
+    for (int q = 0; q < TP; ++q) {
+      p[q] = (t0 + q) * 16 + r;
+      valid[q] = p[q] < n;
+      // branch-free: clamped (in-bounds) loads, then a select
+      const long long pc = valid[q] ? p[q] : n - 1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int f = 4 * g + j, fc = f < NIN ? f : NIN - 1;
+        const float v = (x[pc * NIN + fc] - lmx[j]) * lis[j];
+        xn[q][j] = (valid[q] && f < NIN) ? v : 0.f;
+      }
+      B[q][0] = pack4(xn[q]);
+    }
+    // layer 1 (K = 16, inputs zero-padded): B[q][0] -> B[q][0..3]
+    {
+      f4 h[TP][4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < TP; ++q) h[q][t] = mfma(A[t], B[q][0], b1[t]);
+#pragma unroll
+      for (int q = 0; q < TP; ++q)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) h[q][t][j] = tanh_f(h[q][t][j]);
+          B[q][t] = pack4(h[q][t]);
+        }
+    }
+    // layers 2 and 3 (K = 64)
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+      const int base = layer == 0 ? T1 : T1 + T2;
+      f4 h[TP][4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < TP; ++q) {
+          f4 z = layer == 0 ? b2[t] : b3[t];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) z = mfma(A[base + 4 * t + k], B[q][k], z);
+          h[q][t] = z;
+        }
+#pragma unroll
+      for (int q = 0; q < TP; ++q)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) h[q][t][j] = tanh_f(h[q][t][j]);
+          B[q][t] = pack4(h[q][t]);
+        }
+    }
+    // output layer (M = 6 of 16) and the range guard
+#pragma unroll
+    for (int q = 0; q < TP; ++q) {
+      f4 out = b4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) out = mfma(A[T1 + T2 + T3 + k], B[q][k], out);
+      float xm = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xm = fmaxf(xm, fabsf(xn[q][j]));
+      xm = fmaxf(xm, __shfl_xor(xm, 16));
+      xm = fmaxf(xm, __shfl_xor(xm, 32));
+      bool fallback = !(xm <= fb_bound);
+      float tv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) tv[j] = out[j] * lsy[j] + lmy[j];
+      if (clip) {
+        // backscatter clip: where the network's stress would move energy from the
+        // subgrid scales to the resolved ones (tau_ij S_ij > 0, i.e. a negative
+        // dissipation -tau_ij S_ij), the point takes the Smagorinsky stress
+        float S[6];
+        strain(x + (valid[q] ? p[q] : n - 1) * NIN, S);
+        float part = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int o = 4 * g + j;
+          if (o < NOUT) part += (o < 3 ? 1.f : 2.f) * tv[j] * S[o < NOUT ? o : 0];
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        fallback = fallback || part > 0.f;
+      }
+      float sm[4] = {0.f, 0.f, 0.f, 0.f};
+      if (fallback && valid[q]) smag_stress(x + p[q] * NIN, fb_c2, 4 * g, sm);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int o = 4 * g + j;
+        if (valid[q] && o < NOUT) y[p[q] * NOUT + o] = fallback ? sm[j] : tv[j];
+      }
+    }
+  }
+}
+
 // Weight gradients dW[o][i] = sum_p d[o][p] a[i][p] for one (layer, 16x16 output tile)
 // per blockIdx.y and a slice of points per blockIdx.x; partials [slice][tile][256].
 struct GradTile {
@@ -445,6 +600,18 @@ int repack(psx_mlp* m) {
   return hip_ok(m, hipGetLastError(), "pack_weights");
 }
 
+// inference: two tiles per wavefront per iteration, at most kFwdBlocks one-wave blocks
+constexpr int kInferTiles = 2;
+hipError_t launch_infer(const psx_mlp* m, const float* x, float* y, long long n, float c2, float bound, int clip,
+                        hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long groups = ((n + 15) / 16 + kInferTiles - 1) / kInferTiles;
+  const unsigned blocks = (unsigned)(groups < kFwdBlocks ? groups : kFwdBlocks);
+  hipLaunchKernelGGL((mlp_infer<kInferTiles>), dim3(blocks), dim3(64), 0, s, x, y, n, (const float*)m->w,
+                     (const short*)m->pk, m->nm, c2, bound, clip);
+  return hipGetLastError();
+}
+
 int upload_tiles() {
   GradTile t[kGradTiles];
   int k = 0;
@@ -567,10 +734,17 @@ int psx_mlp_forward(psx_mlp* m, const float* x, float* y, long long n) {
   if (n == 0) return PSX_OK;
   const long long tiles = (n + 15) / 16;
   const unsigned blocks = (unsigned)(tiles < kFwdBlocks ? tiles : kFwdBlocks);
-  hipLaunchKernelGGL((mlp_tiles<false>), dim3(blocks), dim3(64), 0, m->stream, x, (const float*)nullptr, y, n,
-                     (const float*)m->w, (const short*)m->pk, m->nm, (short*)nullptr, (float*)nullptr, 0.f, 0.f,
-                     INFINITY);
-  return hip_ok(m, hipGetLastError(), "mlp_tiles");
+  return hip_ok(m, launch_infer(m, x, y, n, 0.f, INFINITY, 0, m->stream), "mlp_infer");
+}
+
+int psx_mlp_closure(psx_mlp* m, const float* g, float* tau, long long n, double c2, double bound, int flags) {
+  if (!m || !g || !tau || n < 0 || std::isnan(bound) || (flags & ~PSX_CLOSURE_CLIP_BACKSCATTER)) {
+    if (m) m->err = "bad closure arguments";
+    return PSX_E_ARG;
+  }
+  return hip_ok(m, launch_infer(m, g, tau, n, (float)c2, (float)bound, (flags & PSX_CLOSURE_CLIP_BACKSCATTER) ? 1 : 0,
+                                m->stream),
+                "mlp_infer(closure)");
 }
 
 int psx_mlp_grad(psx_mlp* m, const float* x, const float* target, long long n, float* loss_dev) {
@@ -638,15 +812,10 @@ int psx_mlp_train_step(psx_mlp* m, const float* x, const float* target, long lon
 namespace psx {
 // The network as the learned closure of an LES right-hand side: stresses tau[n][6] from
 // the gradients g[n][9] on stream s; points outside the normalised input range
-// [-bound, bound] take the Smagorinsky stress with c2 = (Cs Delta)^2.
-hipError_t mlp_closure(const psx_mlp* m, const float* g, float* tau, long long n, float c2, float bound,
+// [-bound, bound], and with clip points where the network's stress is backscatter, take
+// the Smagorinsky stress with c2 = (Cs Delta)^2.
+hipError_t mlp_closure(const psx_mlp* m, const float* g, float* tau, long long n, float c2, float bound, int clip,
                        hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const long long tiles = (n + 15) / 16;
-  const unsigned blocks = (unsigned)(tiles < kFwdBlocks ? tiles : kFwdBlocks);
-  hipLaunchKernelGGL((mlp_tiles<false>), dim3(blocks), dim3(64), 0, s, g, (const float*)nullptr, tau, n,
-                     (const float*)m->w, (const short*)m->pk, m->nm, (short*)nullptr, (float*)nullptr, 0.f, c2,
-                     bound);
-  return hipGetLastError();
+  return launch_infer(m, g, tau, n, c2, bound, clip, s);
 }
 }  // namespace psx

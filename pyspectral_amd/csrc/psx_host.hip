@@ -23,7 +23,7 @@ hipError_t launch_zpass_sgs(int L, const ZArgs& a, const double2* tw, hipStream_
 hipError_t launch_zpass_phys(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
 hipError_t launch_zpass_grad(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
 hipError_t launch_zpass_lrn(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
-hipError_t mlp_closure(const psx_mlp* m, const float* g, float* tau, long long n, float c2, float bound,
+hipError_t mlp_closure(const psx_mlp* m, const float* g, float* tau, long long n, float c2, float bound, int clip,
                        hipStream_t s);
 int zpass_positions_dns(int L);
 int zpass_positions_smag(int L);
@@ -90,7 +90,7 @@ struct psx_context {
   double* nu_m = nullptr;
   // kernel shapes (psx_autotune / psx_set_tuning): tile widths of the x-inverse,
   // y-inverse and y-forward passes of the 3/2 pipeline (0 = default), z-pass variant
-  int tune[5] = {0, 0, 0, 0, 0};
+  int tune[PSX_TUNE_N] = {0, 0, 0, 0, 0, 0};
   // x-chunk pipelining of the y-inverse / z / y-forward passes over two streams
   // (tune[4] chunks): separate X and W buffers so that chunks never alias, the
   // second stream and its fork/stagger/join events
@@ -124,6 +124,7 @@ struct psx_context {
   // the FP32 gradient / stress arrays of the padded grid (9 and 6 values per point)
   psx_mlp* net = nullptr;
   double net_bound = 0.0;
+  int net_flags = 0;
   float* lgrad = nullptr;
   float* ltau = nullptr;
   double* cfl_part = nullptr;  // per-block CFL maxima (N1*N2/P entries)
@@ -396,20 +397,23 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2) {
     hipStream_t s;
     ~Restore() { c->stream = s; }
   } restore{c, main};
-  if (nch > 1) {
+  // tune[5] = 1: the chunks run in order on the context stream (each chunk's Y and Z
+  // stay small enough to be re-read from the last-level cache); else two streams
+  const bool two = nch > 1 && c->tune[5] != 1;
+  if (two) {
     check(c, hipEventRecord(c->ev_fork, main), "hipEventRecord");
     check(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0), "hipStreamWaitEvent");
   }
   for (int ch = 0; ch < nch; ++ch) {
     const int x0 = (int)((long long)M1 * ch / nch), x1 = (int)((long long)M1 * (ch + 1) / nch);
-    c->stream = (ch & 1) ? c->stream2 : main;
-    if (ch == 1) check(c, hipStreamWaitEvent(c->stream, c->ev_stagger, 0), "hipStreamWaitEvent");
+    c->stream = (two && (ch & 1)) ? c->stream2 : main;
+    if (two && ch == 1) check(c, hipStreamWaitEvent(c->stream, c->ev_stagger, 0), "hipStreamWaitEvent");
     // K2: y-inverse (+ d/dy of the velocity for the closure)
     {
       const int jin[6] = {0, 1, 2, 3, 4, 5}, jp[6] = {0, 1, 2, 3, 4, 5}, jd[6] = {6, 7, 8, -1, -1, -1};
       y_inverse(c, X, Y, M1, M2, H, 0, smag ? 6 : 3, jin, jp, smag ? jd : nullptr, K_YINV, 0, x0, x1);
     }
-    if (ch == 0 && nch > 1) check(c, hipEventRecord(c->ev_stagger, c->stream), "hipEventRecord");
+    if (ch == 0 && two) check(c, hipEventRecord(c->ev_stagger, c->stream), "hipEventRecord");
     // K3: z c2r -> products (+ Smagorinsky stress) -> z r2c, on positions [x0 M2, x1 M2)
     {
       ZArgs zc = z;
@@ -424,7 +428,7 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2) {
     y_forward(c, Z, W, M1, M2, H, 0, nF, 0, x0, x1);
   }
   c->stream = main;
-  if (nch > 1) {
+  if (two) {
     check(c, hipEventRecord(c->ev_join, c->stream2), "hipEventRecord");
     check(c, hipStreamWaitEvent(main, c->ev_join, 0), "hipStreamWaitEvent");
   }
@@ -534,7 +538,8 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
     const double cd = c->p.Cs * delta_grid(c);
     const long long npts = (long long)M1 * M2 * M3 * c->members;
     TimeScope ts(c, K_CLOSURE);
-    check(c, mlp_closure(c->net, c->lgrad, c->ltau, npts, (float)(cd * cd), (float)c->net_bound, c->stream),
+    check(c, mlp_closure(c->net, c->lgrad, c->ltau, npts, (float)(cd * cd), (float)c->net_bound,
+                         (c->net_flags & PSX_CLOSURE_CLIP_BACKSCATTER) ? 1 : 0, c->stream),
           "closure network");
   }
   {
@@ -1400,12 +1405,14 @@ int psx_restrict(psx_context* c, const double* src, int s1, int s2, int s3, int 
   });
 }
 
-int psx_set_closure(psx_context* c, psx_mlp* net, double bound) {
+int psx_set_closure(psx_context* c, psx_mlp* net, double bound, int flags) {
   return guarded(c, [&] {
     if (c->model != PSX_MODEL_LEARNED) fail(c, PSX_E_ARG, "psx_set_closure needs a PSX_MODEL_LEARNED context");
     if (std::isnan(bound)) fail(c, PSX_E_ARG, "bound is NaN");
+    if (flags & ~PSX_CLOSURE_CLIP_BACKSCATTER) fail(c, PSX_E_ARG, "unknown closure flags");
     c->net = net;
     c->net_bound = bound;
+    c->net_flags = flags;
     // a captured step holds the network's normalisation and the bound by value
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
@@ -1503,6 +1510,7 @@ int psx_set_tuning(psx_context* c, const int* tune) {
       if (tune[k] != 0 && pencil_tile(L[k], tune[k]) < 0) fail(c, PSX_E_ARG, "tile width not available");
     if (tune[3] != 0 && tune[3] != 1) fail(c, PSX_E_ARG, "z variant must be 0 or 1");
     if (tune[4] < 0 || tune[4] > 8 || (tune[4] > 1 && c->M1 < 2 * tune[4])) fail(c, PSX_E_ARG, "bad chunk count");
+    if (tune[5] < 0 || tune[5] > 2) fail(c, PSX_E_ARG, "chunk streams must be 0, 1 or 2");
     if (tune[4] > 1) {
       if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
         fail(c, PSX_E_UNSUPPORTED, "chunked pipeline: DNS, Smagorinsky and DSMAG");
@@ -1528,7 +1536,7 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     const size_t state = (size_t)c->members * 3 * c->N1 * c->N2 * c->Nc;
     double2* tmp = nullptr;
     check(c, hipMalloc(&tmp, state * sizeof(double2)), "hipMalloc(autotune)");
-    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1};  // -1: the whole stage
+    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1, -1};  // -1: the whole stage
     hipEvent_t t0 = nullptr, t1 = nullptr;
     check(c, hipEventCreate(&t0), "hipEventCreate");
     check(c, hipEventCreate(&t1), "hipEventCreate");
@@ -1536,7 +1544,9 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     const bool was_timing = c->timing;
     try {
       for (int knob = 0; knob < PSX_TUNE_N; ++knob) {
-        if (learned && knob == 4) continue;  // the learned stage is not chunked
+        if (learned && knob >= 4) continue;  // the learned stage is not chunked
+        // knob 4: chunk count and streams together (v = chunks + 16 * streams)
+        if (knob == 5) continue;
         std::vector<int> cand;
         if (knob < 3) {
           for (int T : {4, 8, 16})
@@ -1544,13 +1554,19 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
         } else if (knob == 3) {
           cand = {0, 1};
         } else {
-          cand = {1, 2, 4};
+          cand = {1 + 16 * 2, 2 + 16 * 2, 4 + 16 * 2, 2 + 16 * 1, 4 + 16 * 1, 8 + 16 * 1};
           ensure_pipeline(c);
         }
         double best = 1e300;
-        int pick = c->tune[knob];
+        int pick = knob == 4 ? c->tune[4] + 16 * c->tune[5] : c->tune[knob];
         for (int v : cand) {
-          c->tune[knob] = v;
+          if (knob == 4) {
+            if ((v & 15) > 1 && c->M1 < 2 * (v & 15)) continue;
+            c->tune[4] = v & 15;
+            c->tune[5] = v >> 4;
+          } else {
+            c->tune[knob] = v;
+          }
           run_stage(c, (const double2*)uhat, nullptr, tmp, nullptr, 0.0, nullptr, 0, 0.0);  // warm-up
           double ms = 0.0;
           int cnt = 0;
@@ -1583,12 +1599,20 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
           c->events_used = 0;
           c->event_kinds.clear();
           const double avg = cnt ? ms / cnt : 1e300;
-          if (avg < best) {
+          // whole-stage timings (chunking) differ by a few percent between candidates:
+          // a chunked pipeline must beat the unchunked stage by 2% to be picked
+          const double margin = (knob == 4 && best < 1e300 && (v & 15) > 1) ? 0.98 : 1.0;
+          if (avg < best * margin) {
             best = avg;
             pick = v;
           }
         }
-        c->tune[knob] = pick;
+        if (knob == 4) {
+          c->tune[4] = pick & 15;
+          c->tune[5] = pick >> 4;
+        } else {
+          c->tune[knob] = pick;
+        }
       }
     } catch (...) {
       c->timing = was_timing;
@@ -1695,6 +1719,24 @@ int psx_ensemble_destroy(psx_ensemble* e) {
 }
 
 const char* psx_ensemble_last_error(const psx_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int psx_ensemble_set_closure(psx_ensemble* e, psx_mlp* net, double bound, int flags) {
+  if (!e) return PSX_E_ARG;
+  for (psx_context* c : e->cls) {
+    const int rc = psx_set_closure(c, net, bound, flags);
+    if (rc != PSX_OK) {
+      e->err = c->err;
+      return rc;
+    }
+  }
+  // the captured ensemble step holds the closure's normalisation and bound by value
+  if (e->exec) hipGraphExecDestroy(e->exec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  for (auto& k : e->key) k = nullptr;
+  return PSX_OK;
+}
 long long psx_ensemble_size(const psx_ensemble* e) { return e ? e->total : PSX_E_ARG; }
 long long psx_ensemble_offset(const psx_ensemble* e, int member) {
   return (e && member >= 0 && member < e->count) ? e->offset[member] : PSX_E_ARG;

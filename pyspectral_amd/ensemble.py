@@ -103,6 +103,13 @@ class Ensemble:
         a = self._args(q, qs, dt)
         self._check(self.lib.psx_ensemble_step_graph(self.ens, *a), "psx_ensemble_step_graph")
 
+    def set_closure(self, net, bound: float = float("inf"), clip_backscatter: bool = False) -> None:
+        """Attach a learned-closure network to every member of a ``"learned"`` ensemble."""
+        self._closure = net
+        flags = _lib.PSX_CLOSURE_CLIP_BACKSCATTER if clip_backscatter else 0
+        self._check(self.lib.psx_ensemble_set_closure(self.ens, net.net, float(bound), flags),
+                    "psx_ensemble_set_closure")
+
     def stats(self, q) -> torch.Tensor:
         """(E, Omega, E_resolved) per member, rows in member order."""
         out = torch.empty(3 * self.count, dtype=torch.float64, device=self.device)

@@ -88,6 +88,19 @@ class MLP:
         self._check(self.lib.psx_mlp_forward(self.net, self._arg(x, 9), self._arg(out, 6), n), "psx_mlp_forward")
         return out
 
+    def closure(self, g: torch.Tensor, c2: float, bound: float = float("inf"), clip_backscatter: bool = False,
+                out: torch.Tensor | None = None) -> torch.Tensor:
+        """The stresses the learned LES uses for gradients g (n, 9): the network, or the
+        Smagorinsky stress -2 c2 |S| S_ij outside [-bound, bound] (and at backscatter
+        points with ``clip_backscatter``)."""
+        n = g.shape[0]
+        out = torch.empty((n, 6), dtype=torch.float32, device=self.device) if out is None else out
+        self._stream()
+        flags = _lib.PSX_CLOSURE_CLIP_BACKSCATTER if clip_backscatter else 0
+        self._check(self.lib.psx_mlp_closure(self.net, self._arg(g, 9), self._arg(out, 6), n, float(c2), float(bound),
+                                             flags), "psx_mlp_closure")
+        return out
+
     def grad(self, x: torch.Tensor, target: torch.Tensor):
         """(loss, gradient) of the normalised MSE on one batch (n a multiple of 16)."""
         loss = torch.zeros(1, dtype=torch.float32, device=self.device)

@@ -154,22 +154,22 @@ class SpectralSolver:
         _lib.check(self.ctx, self.lib.psx_step_graph(self.ctx, _ptr(q), _ptr(qs), _ptr(dt), _ptr(t)),
                    "psx_step_graph")
 
-    TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant", "yz_chunks")
+    TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant", "yz_chunks", "yz_streams")
 
     def tuning(self) -> dict:
-        t = (C.c_int * 5)()
+        t = (C.c_int * len(self.TUNING_KEYS))()
         _lib.check(self.ctx, self.lib.psx_get_tuning(self.ctx, t), "psx_get_tuning")
         return dict(zip(self.TUNING_KEYS, list(t)))
 
     def set_tuning(self, tuning: dict) -> None:
-        t = (C.c_int * 5)(*[int(tuning.get(k, 0)) for k in self.TUNING_KEYS])
+        t = (C.c_int * len(self.TUNING_KEYS))(*[int(tuning.get(k, 0)) for k in self.TUNING_KEYS])
         _lib.check(self.ctx, self.lib.psx_set_tuning(self.ctx, t), "psx_set_tuning")
 
     def autotune(self, uhat: torch.Tensor, reps: int = 5) -> dict:
-        """Time every tile width / z-pass variant on ``reps`` RHS evaluations of ``uhat``
-        and keep the fastest of each (DNS, Smagorinsky, DSMAG)."""
+        """Time every tile width / z-pass variant / chunking on ``reps`` RHS evaluations of
+        ``uhat`` and keep the fastest of each (DNS, Smagorinsky, DSMAG, learned)."""
         self._sync_stream()
-        t = (C.c_int * 5)()
+        t = (C.c_int * len(self.TUNING_KEYS))()
         _lib.check(self.ctx, self.lib.psx_autotune(self.ctx, _ptr(uhat), int(reps), t), "psx_autotune")
         return dict(zip(self.TUNING_KEYS, list(t)))
 
@@ -232,11 +232,14 @@ class SpectralSolver:
                    "psx_restrict")
         return out
 
-    def set_closure(self, net, bound: float = float("inf")) -> None:
+    def set_closure(self, net, bound: float = float("inf"), clip_backscatter: bool = False) -> None:
         """Attach a learned-closure network (``learned.MLP``) to a ``"learned"`` solver;
-        points whose normalised inputs leave [-bound, bound] take the Smagorinsky stress."""
+        points whose normalised inputs leave [-bound, bound] (and, with
+        ``clip_backscatter``, points where the network's stress is backscatter) take the
+        Smagorinsky stress."""
         self._closure = net  # keep the network alive while attached
-        _lib.check(self.ctx, self.lib.psx_set_closure(self.ctx, net.net, float(bound)), "psx_set_closure")
+        flags = _lib.PSX_CLOSURE_CLIP_BACKSCATTER if clip_backscatter else 0
+        _lib.check(self.ctx, self.lib.psx_set_closure(self.ctx, net.net, float(bound), flags), "psx_set_closure")
 
     def stats(self, uhat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """(E, Omega, E_resolved) on the device."""

@@ -9,7 +9,15 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("model", [1, 0])
+def _closure_net():
+    from pyspectral_amd.learned import MLP
+
+    net = MLP(seed=7)
+    net.set_norm(np.zeros(9), np.full(9, 2.0), np.zeros(6), np.full(6, 0.02))
+    return net
+
+
+@pytest.mark.parametrize("model", [1, 0, "learned"])
 def test_ensemble_members_match_single_solvers(model):
     from pyspectral_amd.ensemble import Ensemble
     from pyspectral_amd.solver import SpectralSolver
@@ -18,6 +26,9 @@ def test_ensemble_members_match_single_solvers(model):
     nus = [2e-3, 5e-3, 8e-4, 1e-3, 3e-3, 6e-4]
     print(f"\nensemble model={model} sizes={sizes}", flush=True)
     ens = Ensemble([(n, n, n) for n in sizes], nus, model=model, cfl=0.5)
+    net = _closure_net() if model == "learned" else None
+    if net is not None:
+        ens.set_closure(net, 3.0)
     q = ens.new_state()
     fields = [so.random_isotropic(n, seed=10 + i, kp=3.0 + i % 3) for i, n in enumerate(sizes)]
     for i, f in enumerate(fields):
@@ -25,6 +36,9 @@ def test_ensemble_members_match_single_solvers(model):
     qs = q.clone()
     dt = ens.new_dt()
     singles = [SpectralSolver(n, n, n, model, nu=nu, cfl=0.5) for n, nu in zip(sizes, nus)]
+    if net is not None:
+        for s in singles:
+            s.set_closure(net, 3.0)
     sq = [torch.from_numpy(np.stack(f)).cuda() for f in fields]
     sqs = [x.clone() for x in sq]
     for step in range(3):

@@ -163,6 +163,62 @@ def test_learned_network_rhs_matches_oracle(shape, bound):
     net.close()
 
 
+@pytest.mark.parametrize("clip", [False, True])
+def test_closure_kernel_matches_oracle(clip):
+    """The closure as the LES evaluates it (network, range guard, backscatter clip)
+    against the oracle's: decisions agree wherever the oracle's is clear, network points
+    to the BF16 emulation's accuracy, Smagorinsky points to float32 rounding."""
+    rng = np.random.default_rng(17)
+    n = 20000
+    G = rng.normal(0, 1.0, (n, 9)) * rng.uniform(0.2, 2.0, 9)
+    net = _trained_like_net(G, seed=5)
+    c2, bound = 0.03, 2.5
+    G32 = G.astype(np.float32)
+    y = _host(net.closure(_dev(G32), c2, bound, clip))
+    P = mo.unpack(net.params())
+    ref, use = mo.closure(P, net.norm, bound, c2, clip)(G)
+    tn = mo.forward(P, G32, net.norm)[-1]
+    sm = mo.smag_pointwise(G32, c2)
+    scale = np.abs(tn).max()
+    clear = np.ones(n, bool)
+    if clip:
+        Pi = mo.backscatter(tn, G32)
+        clear = np.abs(Pi) > 0.02 * np.abs(tn).sum(1) * np.abs(G32).sum(1)
+    print(f"\nnetwork at {use.mean():.3f} of the points, {clear.mean():.3f} clear")
+    a, b = use & clear, ~use & clear
+    assert a.sum() > 100 and b.sum() > 100
+    assert np.abs(y[a] - tn[a]).max() < 2e-2 * scale
+    assert np.abs(y[b] - sm[b]).max() < 1e-5 * np.abs(sm[b]).max()
+    net.close()
+
+
+def test_learned_rhs_with_clip_matches_oracle():
+    shape = (16, 16, 16)
+    nu, t = 0.013, 0.1
+    u = _field(*shape, 33, nyquist=False)
+    g = so.Grid(*shape)
+    st = so.State(g, *u, nu=nu, turb_model=1, t=t)
+    G = so.padded_gradients(st, FFT, *u)
+    net = _trained_like_net(G)
+    c2 = (0.16 * g.Delta) ** 2
+    tau = _host(net.closure(_dev(G.astype(np.float32)), c2, 3.0, True)).astype(np.float64)
+    so.rhs_learned(st, FFT, lambda _G: tau)
+    ref, w_ref = st.compact(), np.stack(st.w0)
+    from pyspectral_amd.solver import SpectralSolver
+
+    s = SpectralSolver(*shape, "learned", nu=nu)
+    s.set_closure(net, 3.0, clip_backscatter=True)
+    ud = _dev(u)
+    w0 = torch.empty_like(ud)
+    r = _host(s.rhs(ud, t, w0=w0))
+    scale = np.linalg.norm(w_ref)
+    err = np.linalg.norm(r - ref) / scale
+    print(f"\nrhs error / |w0| = {err:.2e}")
+    assert err < 1e-6 and np.linalg.norm(_host(w0) - w_ref) / scale < 1e-6
+    s.close()
+    net.close()
+
+
 def test_learned_graph_replays_match_eager():
     from pyspectral_amd import cases
 

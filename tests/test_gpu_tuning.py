@@ -60,10 +60,10 @@ def test_chunked_pipeline_is_bitwise_identical(model):
     q0, s0 = u.clone(), u.clone()
     for _ in range(2):
         ref.step(q0, s0)
-    for n in (2, 3, 4):
-        print(f"\nmodel={model} chunks={n}", flush=True)
+    for n, streams in ((2, 2), (3, 2), (4, 2), (2, 1), (4, 1)):
+        print(f"\nmodel={model} chunks={n} streams={streams}", flush=True)
         s = SpectralSolver(*shape, model, nu=0.01, cfl=0.5)
-        s.set_tuning({"yz_chunks": n})
+        s.set_tuning({"yz_chunks": n, "yz_streams": streams})
         assert torch.equal(s.rhs(u, 0.0), r0)
         q, qs = u.clone(), u.clone()
         for _ in range(2):

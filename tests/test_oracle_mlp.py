@@ -83,3 +83,16 @@ def test_smag_pointwise_matches_the_rhs_oracle_form():
     full = -2 * c2 * Smag[:, None, None] * S
     idx = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
     assert np.abs(tau - np.stack([full[:, i, j] for i, j in idx], 1)).max() < 1e-14
+
+
+def test_backscatter_is_the_full_contraction_and_smagorinsky_never_backscatters():
+    rng = np.random.default_rng(6)
+    g = rng.normal(0, 1, (40, 9))
+    tau = rng.normal(0, 1, (40, 6))
+    A = g.reshape(40, 3, 3)
+    S = 0.5 * (A + A.transpose(0, 2, 1))
+    T = np.zeros((40, 3, 3))
+    for c, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
+        T[:, i, j] = T[:, j, i] = tau[:, c]
+    assert np.abs(mo.backscatter(tau, g) - (T * S).sum(axis=(1, 2))).max() < 1e-13
+    assert (mo.backscatter(mo.smag_pointwise(g, 0.7), g) <= 0).all()
