@@ -252,7 +252,7 @@ def bench_closure(args):
     import torch
 
     from pyspectral_amd import cases
-    from pyspectral_amd.learned import MLP, LAYOUT, fit, input_range, training_pairs
+    from pyspectral_amd.learned import MLP, LAYOUT, calibrate_dissipation, fit, input_range, training_pairs
     from pyspectral_amd.solver import SpectralSolver
 
     nu, A, cfl = 5e-4, 0.1, 0.5
@@ -304,11 +304,19 @@ def bench_closure(args):
         a, b = a - a.mean(0), b - b.mean(0)
         return [round(float(v), 3) for v in (a * b).sum(0) / torch.sqrt((a * a).sum(0) * (b * b).sum(0))]
 
+    def deviatoric(t):
+        d = t.clone()
+        d[:, :3] -= d[:, :3].mean(dim=1, keepdim=True)
+        return d
+
     nets, bounds, out["a_priori"] = {}, {}, {}
-    for f in filters:
-        xs, ts = pairs[f]
+    for f0, dev in ((f, d) for f in filters for d in (False, True)):
+        f = f0 + ("_dev" if dev else "")
+        xs, ts = pairs[f0]
         x_tr, t_tr = torch.cat(xs[:-n_hold]), torch.cat(ts[:-n_hold])
         x_te, t_te = torch.cat(xs[-n_hold:]), torch.cat(ts[-n_hold:])
+        if dev:
+            t_tr, t_te = deviatoric(t_tr), deviatoric(t_te)
         s0 = time.perf_counter()
         net = MLP(seed=args.seed)
         losses = fit(net, x_tr, t_tr, steps=args.fit_steps, batch=65536, lr=1e-3, lr_final=1e-4, seed=args.seed)
@@ -316,6 +324,12 @@ def bench_closure(args):
         t_fit = time.perf_counter() - s0
         bound = input_range(net, x_tr)
         nets[f], bounds[f] = net, bound
+        # a copy whose outputs are scaled to the filtered-DNS mean subgrid dissipation
+        cal = MLP(seed=args.seed)
+        cal.set_params(net.params())
+        cal.set_norm(*net.norm)
+        alpha = calibrate_dissipation(cal, x_tr, t_tr, c2_les, bound, True)
+        nets[f + "_cal"], bounds[f + "_cal"] = cal, bound
         _log(f"fit ({f}): loss {np.mean(losses[:20]):.4f} -> {np.mean(losses[-20:]):.4f} in {t_fit:.1f} s")
         # a-priori stress errors on the held-out snapshots
         y = net.forward(x_te).double()
@@ -345,6 +359,7 @@ def bench_closure(args):
             "mean_dissipation": {"filtered_dns": round(diss(T), 5), "network": round(diss(y), 5),
                                  "clipped_closure": round(diss(yc), 5), "smagorinsky": round(diss(tau_sm), 5)},
             "backscatter_fraction_network": round(float(((y * S * w6).sum(1) > 0).double().mean()), 4),
+            "dissipation_calibration_factor": round(alpha, 4),
             "fit": {"points": int(x_tr.shape[0]), "steps": args.fit_steps,
                     "loss_first": round(float(np.mean(losses[:20])), 4),
                     "loss_last": round(float(np.mean(losses[-20:])), 4), "seconds": round(t_fit, 2),
@@ -354,40 +369,58 @@ def bench_closure(args):
     del pairs
     torch.cuda.empty_cache()
 
-    # ---- judge DNS
+    # ---- judge DNS, or its spectra from an earlier run of the same configuration
     refs = {"box": {}, "sharp": {}}
-    judge = SpectralSolver(NJ, NJ, NJ, 0, nu=nu, cfl=cfl, force=A)
-    qj = torch.from_numpy(cases.prolong(u0, NJ)).to("cuda")
-    qjs = qj.clone()
-    t, wall, nsteps = 0.0, 0.0, 0
-    for T_ in times:
-        s0 = time.perf_counter()
-        t, n, _ = _advance_to(judge, qj, qjs, t, T_,
-                              on_step=lambda n, tt: _log(f"judge DNS t = {tt:.3f}") if n % 200 == 0 else None)
-        wall += time.perf_counter() - s0
-        nsteps += n
-        fb = judge.filter(qj, "box", delta)
-        refs["box"][T_] = _shell_energy(les, les.restrict(fb, (NJ, NJ, NJ)))
-        del fb
-        refs["sharp"][T_] = _shell_energy(les, les.restrict(qj, (NJ, NJ, NJ)))
-        _log(f"judge DNS reached t = {T_} ({nsteps} steps, {wall:.1f} s)")
-    judge.close()
-    del qj, qjs
-    torch.cuda.empty_cache()
-    out["judge_dns"] = {"grid": NJ, "steps": nsteps, "gpu_seconds": round(wall, 2)}
+    key = {"NJ": NJ, "NL": NL, "times": times, "seed": args.seed, "nu": nu, "A": A, "cfl": cfl}
+    cache = None
+    if args.judge_cache and os.path.exists(args.judge_cache):
+        with open(args.judge_cache) as fh:
+            cache = json.load(fh)
+        if cache.get("key") != key:
+            cache = None
+    if cache is not None:
+        for kind in refs:
+            refs[kind] = {float(t_): np.asarray(v) for t_, v in cache[kind].items()}
+        out["judge_dns"] = dict(cache["judge_dns"], from_cache=True)
+        _log("judge DNS spectra from the cache")
+    else:
+        judge = SpectralSolver(NJ, NJ, NJ, 0, nu=nu, cfl=cfl, force=A)
+        qj = torch.from_numpy(cases.prolong(u0, NJ)).to("cuda")
+        qjs = qj.clone()
+        t, wall, nsteps = 0.0, 0.0, 0
+        for T_ in times:
+            s0 = time.perf_counter()
+            t, n, _ = _advance_to(judge, qj, qjs, t, T_,
+                                  on_step=lambda n, tt: _log(f"judge DNS t = {tt:.3f}") if n % 200 == 0 else None)
+            wall += time.perf_counter() - s0
+            nsteps += n
+            fb = judge.filter(qj, "box", delta)
+            refs["box"][T_] = _shell_energy(les, les.restrict(fb, (NJ, NJ, NJ)))
+            del fb
+            refs["sharp"][T_] = _shell_energy(les, les.restrict(qj, (NJ, NJ, NJ)))
+            _log(f"judge DNS reached t = {T_} ({nsteps} steps, {wall:.1f} s)")
+        judge.close()
+        del qj, qjs
+        torch.cuda.empty_cache()
+        out["judge_dns"] = {"grid": NJ, "steps": nsteps, "gpu_seconds": round(wall, 2)}
+        if args.judge_cache:
+            with open(args.judge_cache, "w") as fh:
+                json.dump({"key": key, "judge_dns": out["judge_dns"],
+                           **{kind: {str(t_): v.tolist() for t_, v in refs[kind].items()} for kind in refs}}, fh)
 
     # ---- LES runs
     ud = torch.from_numpy(u0).to("cuda")
     init = {"box": les.filter(ud, "box", delta), "sharp": ud}
     runs = {}
-    variants = [(f"learned_{f}" + ("_clip" if clip else ""), "learned", f, clip)
-                for f in filters for clip in (True, False)]
+    variants = [("learned_box_plain", "learned", "box", False)]
+    variants += [(f"learned_{f}{d}{c}_clip", "learned", f + d + c, True)
+                 for f in filters for d in ("", "_dev") for c in ("", "_cal")]
     variants += [("smagorinsky", 1, None, False), ("dsmag", 7, None, False), ("tmodel", 2, None, False)]
     for name, model, f, clip in variants:
         s = SpectralSolver(NL, NL, NL, model, nu=nu, cfl=cfl, force=A, Ct=0.1)
         if model == "learned":
             s.set_closure(nets[f], bounds[f], clip_backscatter=clip)
-        ref_kind = f or "box"
+        ref_kind = (f or "box").split("_")[0]
         q = init[ref_kind].clone()
         qs = q.clone()
         s.step(q.clone(), qs.clone(), 0.0)  # warm-up (first-call setup)
@@ -416,7 +449,7 @@ def bench_closure(args):
     for name, r in runs.items():
         if not name.startswith("learned") or not r["stable"]:
             continue
-        kind = "vs_box" if "_box" in name else "vs_sharp"
+        kind = "vs_box" if "_box_" in name else "vs_sharp"
         worst = max(e[kind][f"max_k<={NL // 2 - 1}"] or 1e9 for e in r["errors"].values())
         if best is None or worst < best[1]:
             best = (name, worst)
@@ -477,6 +510,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--fit-steps", type=int, default=3000)
     ap.add_argument("--times", default="0.5,1.0,2.0")
+    ap.add_argument("--judge-cache", default="", help="closure: JSON file holding (or receiving) the judge DNS "
+                                                      "spectra of this configuration")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size dispatch "
                          "that calibrates rocprof FETCH_SIZE/WRITE_SIZE)")

@@ -96,12 +96,19 @@ __device__ __forceinline__ void strain(const float* g, float (&S)[6]) {
 __device__ __forceinline__ float strain_norm(const float (&S)[6]) {
   return sqrtf(2.f * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + 2.f * (S[3] * S[3] + S[4] * S[4] + S[5] * S[5])));
 }
-__device__ __forceinline__ void smag_stress(const float* g, float c2, int o0, float (&t)[4]) {
+// component i (0..5, else 0) of a 6-vector, as selects (a run-time index into a
+// per-thread array would place it in scratch memory)
+__device__ __forceinline__ float sel6(int i, const float (&v)[6]) {
+  return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : i == 3 ? v[3] : i == 4 ? v[4] : i == 5 ? v[5] : 0.f;
+}
+__device__ __forceinline__ f4 smag_stress(const float* g, float c2, int o0) {
   float S[6];
   strain(g, S);
   const float Smag = strain_norm(S);
+  f4 t;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) t[j] = (o0 + j < NOUT) ? -2.f * c2 * Smag * S[o0 + j < NOUT ? o0 + j : 0] : 0.f;
+  for (int j = 0; j < 4; ++j) t[j] = -2.f * c2 * Smag * sel6(o0 + j, S);
+  return t;
 }
 
 // Pack the FP32 parameters into MFMA A-operand tiles (one thread per tile value).
@@ -143,32 +150,18 @@ __device__ __forceinline__ bf4 tile_of(const short* __restrict__ pk, int tile, i
   return p[tile * 64 + lane];
 }
 
-// Forward (and, with TRAIN, backward-data) over tiles of 16 points, one wavefront per
-// tile at a time.  Inference writes y (denormalised).  Training writes the loss
-// partial of its block, the activations and deltas (feature-major, BF16).
-template <bool TRAIN>
-__global__ __launch_bounds__(64) void mlp_tiles(const float* __restrict__ x, const float* __restrict__ target,
-                                                float* __restrict__ y, long long n, const float* __restrict__ w,
-                                                const short* __restrict__ pk, Norm nm, short* __restrict__ act,
-                                                float* __restrict__ loss_part, float gscale, float fb_c2,
-                                                float fb_bound) {
+// Training pass over tiles of 16 points, one wavefront per tile at a time: forward,
+// MSE loss, backward-data.  Writes the loss partial of its block and the activations
+// and deltas (feature-major, BF16) for the weight-gradient kernels.  The weight tiles
+// are read from the packed array per tile (it stays in cache): holding them in
+// registers next to the activations would spill.
+__global__ __launch_bounds__(64) void mlp_train_tiles(const float* __restrict__ x, const float* __restrict__ target,
+                                                      long long n, const float* __restrict__ w,
+                                                      const short* __restrict__ pk, Norm nm, short* __restrict__ act,
+                                                      float* __restrict__ loss_part, float gscale) {
   const int lane = threadIdx.x, g = lane >> 4, r = lane & 15;
-  // inference keeps the forward tiles in registers; training (which also holds the
-  // activations) reads them from the packed array, which stays in cache
-  constexpr int NREG = TRAIN ? 1 : T1 + T2 + T3 + T4;
-  bf4 Areg[NREG];
-  if constexpr (!TRAIN) {
-#pragma unroll
-    for (int t = 0; t < NREG; ++t) Areg[t] = tile_of(pk, t, lane);
-  }
   const short* pkt = pk;  // per-tile copy of pk the compiler cannot hoist loads through
-  auto Af = [&](int t) -> bf4 {
-    if constexpr (TRAIN) {
-      return tile_of(pkt, t, lane);
-    } else {
-      return Areg[t];
-    }
-  };
+  auto Af = [&](int t) -> bf4 { return tile_of(pkt, t, lane); };
   f4 b1[4], b2[4], b3[4], b4;
 #pragma unroll
   for (int t = 0; t < 4; ++t)
@@ -183,7 +176,7 @@ __global__ __launch_bounds__(64) void mlp_tiles(const float* __restrict__ x, con
   float lsum = 0.f;
   const long long ntiles = (n + 15) / 16;
   for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    if constexpr (TRAIN) asm volatile("" : "+s"(pkt));  // an empty statement: keeps weight loads per tile
+    asm volatile("" : "+s"(pkt));  // an empty statement: keeps weight loads per tile
     const long long p = tile * 16 + r;
     const bool valid = p < n;
     // inputs: B0[k = 4g + j][point r], normalised, zero beyond the 9 features
@@ -227,26 +220,7 @@ __global__ __launch_bounds__(64) void mlp_tiles(const float* __restrict__ x, con
     f4 out = b4;
 #pragma unroll
     for (int k = 0; k < 4; ++k) out = mfma(Af(T1 + T2 + T3 + k), B3[k], out);
-    if constexpr (!TRAIN) {
-      // range guard: a point whose largest |normalised input| exceeds fb_bound takes
-      // the Smagorinsky stress of its own gradients instead of the network's output
-      float xm = 0.f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) xm = fmaxf(xm, fabsf(x0[j]));
-      xm = fmaxf(xm, __shfl_xor(xm, 16));
-      xm = fmaxf(xm, __shfl_xor(xm, 32));
-      const bool fallback = !(xm <= fb_bound);
-      float sm[4] = {0.f, 0.f, 0.f, 0.f};
-      if (fallback && valid) {
-        const float* gp = x + p * NIN;
-        smag_stress(gp, fb_c2, 4 * g, sm);
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int o = 4 * g + j;
-        if (valid && o < NOUT) y[p * NOUT + o] = fallback ? sm[j] : out[j] * nm.sy[o] + nm.my[o];
-      }
-    } else {
+    {
       // MSE on normalised outputs: d4 = gscale (y - t), gscale = 2 / (6 n)
       f4 d4;
 #pragma unroll
@@ -309,10 +283,8 @@ __global__ __launch_bounds__(64) void mlp_tiles(const float* __restrict__ x, con
       }
     }
   }
-  if constexpr (TRAIN) {
-    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-    if (lane == 0) loss_part[blockIdx.x] = lsum;
-  }
+  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+  if (lane == 0) loss_part[blockIdx.x] = lsum;
 }
 
 // Inference over TP tiles of 16 points per wavefront at a time: the TP tiles' matrix
@@ -429,7 +401,7 @@ __global__ __launch_bounds__(64) void mlp_infer(const float* __restrict__ x, flo
       xm = fmaxf(xm, __shfl_xor(xm, 16));
       xm = fmaxf(xm, __shfl_xor(xm, 32));
       bool fallback = !(xm <= fb_bound);
-      float tv[4];
+      f4 tv;
 #pragma unroll
       for (int j = 0; j < 4; ++j) tv[j] = out[j] * lsy[j] + lmy[j];
       if (clip) {
@@ -442,18 +414,19 @@ __global__ __launch_bounds__(64) void mlp_infer(const float* __restrict__ x, flo
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int o = 4 * g + j;
-          if (o < NOUT) part += (o < 3 ? 1.f : 2.f) * tv[j] * S[o < NOUT ? o : 0];
+          part += (o < 3 ? 1.f : 2.f) * tv[j] * sel6(o, S);
         }
         part += __shfl_xor(part, 16);
         part += __shfl_xor(part, 32);
         fallback = fallback || part > 0.f;
       }
-      float sm[4] = {0.f, 0.f, 0.f, 0.f};
-      if (fallback && valid[q]) smag_stress(x + p[q] * NIN, fb_c2, 4 * g, sm);
+      // vector values, not arrays: a select between two arrays can become a select of
+      // addresses into scratch memory
+      const f4 res = (fallback && valid[q]) ? smag_stress(x + p[q] * NIN, fb_c2, 4 * g) : tv;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int o = 4 * g + j;
-        if (valid[q] && o < NOUT) y[p[q] * NOUT + o] = fallback ? sm[j] : tv[j];
+        if (valid[q] && o < NOUT) y[p[q] * NOUT + o] = res[j];
       }
     }
   }
@@ -764,9 +737,9 @@ int psx_mlp_grad(psx_mlp* m, const float* x, const float* target, long long n, f
   const long long tiles = n / 16;
   const unsigned blocks = (unsigned)(tiles < kFwdBlocks ? tiles : kFwdBlocks);
   const float gscale = 2.f / (6.f * (float)n);
-  hipLaunchKernelGGL((mlp_tiles<true>), dim3(blocks), dim3(64), 0, m->stream, x, target, (float*)nullptr, n,
-                     (const float*)m->w, (const short*)m->pk, m->nm, m->act, m->lpart, gscale, 0.f, INFINITY);
-  rc = hip_ok(m, hipGetLastError(), "mlp_tiles(train)");
+  hipLaunchKernelGGL(mlp_train_tiles, dim3(blocks), dim3(64), 0, m->stream, x, target, n, (const float*)m->w,
+                     (const short*)m->pk, m->nm, m->act, m->lpart, gscale);
+  rc = hip_ok(m, hipGetLastError(), "mlp_train_tiles");
   if (rc != PSX_OK) return rc;
   const int kpb = (int)((tiles + kSlices - 1) / kSlices);
   hipLaunchKernelGGL(mlp_wgrad, dim3(kSlices, kGradTiles), dim3(64), 0, m->stream, (const short*)m->act, n, kpb,

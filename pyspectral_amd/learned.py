@@ -120,7 +120,8 @@ class MLP:
 
 
 # -------------------------------------------------------------------- training data
-def training_pairs(dns, uhat_dns: torch.Tensor, les, delta: float | None = None, kind: str = "box"):
+def training_pairs(dns, uhat_dns: torch.Tensor, les, delta: float | None = None, kind: str = "box",
+                   deviatoric: bool = False):
     """Filtered-DNS training pairs on an LES grid.
 
     ``dns`` and ``les`` are ``SpectralSolver`` contexts of the DNS and LES grids (the
@@ -129,7 +130,9 @@ def training_pairs(dns, uhat_dns: torch.Tensor, les, delta: float | None = None,
     with 3/2-rule products (``computeSGS_DNS``, src/Classes.py:861-929), and both are
     restricted to the LES modes.  Returns (x, t): float32 device tensors (n, 9) of the
     filtered velocity gradients du_i/dx_j (column 3i+j) and (n, 6) of the stresses
-    (order 11 22 33 12 13 23) at the n = N1 N2 N3 LES grid points."""
+    (order 11 22 33 12 13 23) at the n = N1 N2 N3 LES grid points.  With ``deviatoric``
+    the targets are tau_ij - delta_ij tau_kk/3: the isotropic part, which the pressure
+    projection removes from the RHS, is left out of the fit."""
     if delta is None:
         delta = 2.0 * les.params.dx
     dims = (dns.N1, dns.N2, dns.N3)
@@ -138,8 +141,10 @@ def training_pairs(dns, uhat_dns: torch.Tensor, les, delta: float | None = None,
     u_les = les.restrict(ubar, dims)
     tau_les = les.restrict(tau, dims)
     x = les.gradients(u_les)
-    t = les.physical(tau_les).reshape(6, -1).t().contiguous().to(torch.float32)
-    return x, t
+    t = les.physical(tau_les).reshape(6, -1)
+    if deviatoric:
+        t[:3] -= t[:3].mean(dim=0, keepdim=True)
+    return x, t.t().contiguous().to(torch.float32)
 
 
 def normalisation(x: torch.Tensor, t: torch.Tensor):
@@ -186,3 +191,41 @@ def fit(net: MLP, x: torch.Tensor, t: torch.Tensor, *, steps: int, batch: int = 
         rate = lr * (lr_final / lr) ** (k / max(steps - 1, 1))
         losses.append(net.train_step(xb, tb, rate))
     return [float(v) for v in torch.cat(losses).cpu()]
+
+
+def _strain(g: np.ndarray) -> np.ndarray:
+    return np.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
+                     0.5 * (g[:, 5] + g[:, 7])], axis=1)
+
+
+def subgrid_dissipation(tau: np.ndarray, g: np.ndarray) -> float:
+    """Mean subgrid dissipation -<tau_ij S_ij> of stresses (n, 6) (order 11 22 33 12 13 23)
+    for gradients g (n, 9)."""
+    w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])
+    return float(-(np.asarray(tau, np.float64) * _strain(np.asarray(g, np.float64)) * w).sum(1).mean())
+
+
+def calibrate_dissipation(net: MLP, x: torch.Tensor, t: torch.Tensor, c2: float, bound: float = float("inf"),
+                          clip_backscatter: bool = True, max_points: int = 1 << 20) -> float:
+    """Scale the network's outputs (its output mean and scale) by one factor so that the
+    closure's mean subgrid dissipation on (x, t) equals that of the targets.  A network
+    fitted by least squares predicts the conditional mean stress and under-predicts the
+    energy drain to the subgrid scales; this restores the drain.  Returns the factor."""
+    n = x.shape[0]
+    step = max(1, n // max_points)
+    xs, ts = x[::step].contiguous(), t[::step].contiguous()
+    g, tt = xs.cpu().numpy(), ts.cpu().numpy()
+    d_ref = subgrid_dissipation(tt, g)
+    total = 1.0
+    # the Smagorinsky points are not scaled, hence a few fixed-point passes
+    for _ in range(4):
+        d_net = subgrid_dissipation(net.closure(xs, c2, bound, clip_backscatter).cpu().numpy(), g)
+        if not (d_net > 0 and d_ref > 0):
+            break
+        a = d_ref / d_net
+        if abs(a - 1.0) < 1e-3:
+            break
+        mx, sx, my, sy = net.norm
+        net.set_norm(mx, sx, my * a, sy * a)
+        total *= a
+    return total

@@ -308,6 +308,15 @@ def test_fit_learns_the_subgrid_stress():
           f"Smagorinsky (off-diagonal) {np.round(c_sm, 3)}")
     assert last < 0.5 * first
     assert min(c_net[3:]) > max(c_sm) and min(c_net[3:]) > 0.7
+    # dissipation calibration: the clipped closure then drains what the targets drain
+    from pyspectral_amd.learned import calibrate_dissipation, subgrid_dissipation
+
+    c2 = (0.16 * 2 * np.pi / Nl) ** 2
+    a = calibrate_dissipation(net, x, t, c2, 50.0, True)
+    d_cl = subgrid_dissipation(_host(net.closure(x, c2, 50.0, True)), X)
+    d_ref = subgrid_dissipation(tt, X)
+    print(f"calibration factor {a:.3f}; dissipation {d_cl:.4e} vs targets {d_ref:.4e}")
+    assert abs(d_cl / d_ref - 1) < 1e-2
     dns.close()
     les.close()
     net.close()
