@@ -308,15 +308,29 @@ def test_fit_learns_the_subgrid_stress():
           f"Smagorinsky (off-diagonal) {np.round(c_sm, 3)}")
     assert last < 0.5 * first
     assert min(c_net[3:]) > max(c_sm) and min(c_net[3:]) > 0.7
-    # dissipation calibration: the clipped closure then drains what the targets drain
-    from pyspectral_amd.learned import calibrate_dissipation, subgrid_dissipation
+    dns.close()
+    les.close()
+    net.close()
 
-    c2 = (0.16 * 2 * np.pi / Nl) ** 2
+
+def test_dissipation_calibration_matches_the_targets():
+    """Targets with a known positive drain (1.7 x the Smagorinsky stress of the inputs):
+    after calibration the clipped closure's mean subgrid dissipation equals theirs."""
+    from pyspectral_amd.learned import MLP, calibrate_dissipation, fit, subgrid_dissipation
+    from pyspectral_amd.solver import SpectralSolver
+
+    N = 32
+    les = SpectralSolver(N, N, N, "learned", nu=0.01)
+    x = les.gradients(_dev(np.stack(so.random_isotropic(N, seed=4, kp=4.0))))
+    X = _host(x).astype(np.float64)
+    c2 = (0.16 * 2 * np.pi / N) ** 2
+    t = _dev((1.7 * mo.smag_pointwise(X, c2)).astype(np.float32))
+    net = MLP(seed=8)
+    fit(net, x, t, steps=150, batch=4096, lr=3e-3)
     a = calibrate_dissipation(net, x, t, c2, 50.0, True)
     d_cl = subgrid_dissipation(_host(net.closure(x, c2, 50.0, True)), X)
-    d_ref = subgrid_dissipation(tt, X)
-    print(f"calibration factor {a:.3f}; dissipation {d_cl:.4e} vs targets {d_ref:.4e}")
-    assert abs(d_cl / d_ref - 1) < 1e-2
-    dns.close()
+    d_ref = subgrid_dissipation(_host(t), X)
+    print(f"\ncalibration factor {a:.3f}; dissipation {d_cl:.4e} vs targets {d_ref:.4e}")
+    assert d_ref > 0 and abs(d_cl / d_ref - 1) < 1e-2
     les.close()
     net.close()
