@@ -313,7 +313,7 @@ def bench_closure(args):
     for f0, dev in ((f, d) for f in filters for d in (False, True)):
         f = f0 + ("_dev" if dev else "")
         xs, ts = pairs[f0]
-        x_tr, t_tr = torch.cat(xs[:-n_hold]), torch.cat(ts[:-n_hold])
+        x_tr, t_tr = torch.cat(xs[args.skip_snapshots:-n_hold]), torch.cat(ts[args.skip_snapshots:-n_hold])
         x_te, t_te = torch.cat(xs[-n_hold:]), torch.cat(ts[-n_hold:])
         if dev:
             t_tr, t_te = deviatoric(t_tr), deviatoric(t_te)
@@ -510,6 +510,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--fit-steps", type=int, default=3000)
     ap.add_argument("--times", default="0.5,1.0,2.0")
+    ap.add_argument("--skip-snapshots", type=int, default=0,
+                    help="closure: leave the first snapshots (the still-smooth early field) out of the fit")
     ap.add_argument("--judge-cache", default="", help="closure: JSON file holding (or receiving) the judge DNS "
                                                       "spectra of this configuration")
     ap.add_argument("--copy-mib", type=int, default=0,
