@@ -140,6 +140,30 @@ def test_rhs_with_linear_forcing(model):
     assert _rel(r, st.compact()) < 1e-12
 
 
+@pytest.mark.parametrize("model", [0, 1, 2, "Static tau-model", 7])
+def test_rhs_with_rotation(model):
+    """Coriolis terms (rotate = 1, src/RHSfunctions.py:33-50): every model's RHS with a
+    rotation vector against the oracle, at a time where the t-model memory is active."""
+    N, nu, t = 16, 0.012, 0.37
+    Om = (0.3, -0.2, 0.5)
+    u = _field(N, 37, nyquist_noise=True)
+    st = so.State(so.Grid(N, N, N), *u, nu=nu, turb_model=model, t=t, rotate=1, Om=Om, **TAU_KW)
+    so.compute_rhs(st, FFT)
+    s = _solver(N, model, nu, Ct=0.2, tau_a=0.05, tau_b=0.5, rotate=1, Om=Om)
+    r = _host(s.rhs(_dev(u), t))
+    err = _rel(r, st.compact())
+    print(f"\nrotation model={model}: rhs relative L2 = {err:.3e}")
+    assert err < 1e-12, err
+    s.close()
+
+
+def test_rotation_with_the_dynamic_tau_model_is_refused():
+    from pyspectral_amd._lib import PsxError
+
+    with pytest.raises(PsxError):
+        _solver(16, "Dynamic tau-model", 0.01, rotate=1, Om=(0.0, 0.0, 1.0))
+
+
 def test_rhs_tgv_closed_form():
     N, nu = 32, 1.0 / 400
     u, v, w, _ = so.taylor_green(N, N, N)

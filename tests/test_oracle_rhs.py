@@ -209,3 +209,28 @@ def test_linear_forcing_amplification():
         so.compute_rhs(s1, FFT)
         expect = a * np.stack([s0.uhat, s0.vhat, s0.what])
         assert np.abs(s1.compact() - s0.compact() - expect).max() < 1e-13
+
+
+def test_coriolis_terms_are_the_projected_rotation_force():
+    """With rotate = 1 the RHS gains the Leray projection of 2 u x Omega (the Coriolis
+    force with the pressure that keeps it solenoidal), evaluated on the velocity as the
+    model uses it, and it does no work on any mode: Re(conj(u_k) . delta_k) = 0.  At
+    t = 0 (no t-model memory term) this holds for every model except the dynamic
+    tau-model, which the GPU build does not run with rotation."""
+    N = 8
+    Om = (0.3, -0.2, 0.5)
+    uh = _rand_div_free(N, 23)
+    for model in (0, 1, 2, 7):
+        s0 = _state(uh, model=model, N=N)
+        s1 = _state(uh, model=model, N=N, rotate=1, Om=Om)
+        so.compute_rhs(s0, FFT)
+        so.compute_rhs(s1, FFT)
+        d = s1.compact() - s0.compact()
+        g = s0.grid
+        u = np.stack([s0.uhat, s0.vhat, s0.what])
+        f = 2.0 * np.cross(np.moveaxis(u, 0, -1), np.asarray(Om))   # 2 u x Omega per mode
+        f = np.moveaxis(f, -1, 0)
+        k = np.stack([g.k1, g.k2, g.k3])
+        proj = f - k * (k * f).sum(0) / g.ksqr
+        assert np.abs(d - proj).max() < 1e-13 * np.abs(f).max(), model
+        assert np.abs((np.conj(u) * d).sum(0).real).max() < 1e-13 * np.abs(f).max() ** 2
