@@ -209,8 +209,9 @@ def calibrate_dissipation(net: MLP, x: torch.Tensor, t: torch.Tensor, c2: float,
                           clip_backscatter: bool = True, max_points: int = 1 << 20) -> float:
     """Scale the network's outputs (its output mean and scale) by one factor so that the
     closure's mean subgrid dissipation on (x, t) equals that of the targets.  A network
-    fitted by least squares predicts the conditional mean stress and under-predicts the
-    energy drain to the subgrid scales; this restores the drain.  Returns the factor."""
+    fitted by least squares predicts the conditional mean stress, whose energy drain to
+    the subgrid scales need not match the targets'; this matches it.  Returns the
+    factor (1 when either drain is not positive)."""
     n = x.shape[0]
     step = max(1, n // max_points)
     xs, ts = x[::step].contiguous(), t[::step].contiguous()
