@@ -29,18 +29,26 @@ MLP_FLOPS = 2 * (9 * 64 + 64 * 64 + 64 * 64 + 64 * 6)  # per point, algorithmic
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 
 
-def kernel_bytes(N, closure):
+def fused_closure(N, tuning):
+    """Whether the learned closure's network runs inside the z pass: the one-wavefront z
+    lengths 64 * {1,2,3,4,6,8,12}, unless the tuning asks for the separate kernel."""
+    M = 3 * N // 2
+    return M % 64 == 0 and M // 64 in (1, 2, 3, 4, 6, 8, 12) and not tuning.get("closure_separate", 0)
+
+
+def kernel_bytes(N, closure, fused=False):
     """Compulsory (algorithmic) bytes per launch of each kernel class at N^3, 3/2 rule:
     every input field read once, every output field written once.  The learned closure
-    has two z passes per stage (gradients out as float32, then products with the
-    network's float32 stresses in); its z_pass figure is their mean."""
+    fused into the z pass moves what the Smagorinsky z pass moves (9 fields in, 12 out);
+    run as its own kernel it has two z passes per stage (gradients out as float32, then
+    products with the network's float32 stresses in), and the z_pass figure is their mean."""
     M, H, Nc = 3 * N // 2, N // 2, N // 2 + 1
     smag = closure in ("smagorinsky", "learned")
     nX, nY, nZ = (6, 9, 12) if smag else (3, 3, 6)
     state = N * N * Nc * C16
     MMH, pts = M * M * H * C16, M ** 3
     z = nY * MMH + nZ * MMH
-    if closure == "learned":
+    if closure == "learned" and not fused:
         z = ((9 * MMH + 36 * pts) + (3 * MMH + 24 * pts + 12 * MMH)) / 2
     return {
         "x_inverse": 3 * N * N * H * C16 + nX * M * N * H * C16,
@@ -576,7 +584,8 @@ def main():
     ms = (C.c_double * len(_lib.KERNEL_NAMES))()
     cnt = (C.c_int * len(_lib.KERNEL_NAMES))()
     _lib.check(solver.ctx, lib.psx_timing_end(solver.ctx, ms, cnt), "psx_timing_end")
-    kb = kernel_bytes(N, args.closure)
+    fused = args.closure == "learned" and fused_closure(N, tuning)
+    kb = kernel_bytes(N, args.closure, fused)
     # the chunked y/z pipeline splits each of those passes into yz_chunks launches
     nch = max(1, int(tuning.get("yz_chunks", 0)))
     for name in ("y_inverse", "z_pass", "y_forward"):
@@ -590,7 +599,7 @@ def main():
                  "share": round(ms[i] / max(sum(ms), 1e-30), 4)}
         if name in kb:
             entry["GBs"] = round(kb[name] / (avg_ms * 1e-3) / 1e9, 1)
-        if name == "closure_network":
+        if name == "closure_network" or (name == "z_pass" and fused):
             entry["TFLOPs"] = round(MLP_FLOPS * (3 * N // 2) ** 3 / (avg_ms * 1e-3) / 1e12, 2)
         kernels[name] = entry
     dom = max(kernels, key=lambda k: kernels[k]["share"])
@@ -599,7 +608,9 @@ def main():
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom, N, args.closure) if nch == 1 else None,
             "bytes_per_launch": kb[dom]}
-    if dom == "closure_network":
+    if "TFLOPs" in kernels[dom]:
+        # the network's kernel (on its own or inside the z pass): bound by its VALU work,
+        # priced against the dense BF16 matrix peak
         tf = kernels[dom]["TFLOPs"]
         roof = {"bound": "mfma", "kernel": dom, "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tf / BF16_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, N, args.closure),

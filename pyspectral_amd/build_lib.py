@@ -36,9 +36,10 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-# per-source flags: the network kernels keep matrix-core results in ordinary vector
-# registers (no accumulation-register reads between the layers)
-EXTRA = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# per-source flags: the network kernels (and the z pass with the network fused in)
+# keep matrix-core results in ordinary vector registers (no accumulation-register
+# reads between the layers)
+EXTRA = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"], "k_zlearn.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src: str, hipcc: str, build_dir: str, defines) -> str:

@@ -1,6 +1,7 @@
 // Fused z-pass instantiations for the learned closure and physical-space output:
-// Z_PHYS (c2r to the physical grid), Z_GRAD (the 9 velocity gradients) and Z_LRN
-// (products plus a given stress).
+// Z_PHYS (c2r to the physical grid), Z_GRAD (the 9 velocity gradients), Z_LRN
+// (products plus a given stress) and Z_LRNF (products plus the network's stress,
+// evaluated inside the pass).
 #include "zlaunch.hpp"
 
 namespace psx {
@@ -109,6 +110,33 @@ hipError_t launch_zpass_lrn(int L, const ZArgs& a, const double2* tw, hipStream_
     case 288: return zpass_launch<288, Z_LRN>(a, tw, s, variant);
     case 384: return zpass_launch<384, Z_LRN>(a, tw, s, variant);
     case 576: return zpass_launch<576, Z_LRN>(a, tw, s, variant);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// the fused closure (network inside the pass): one-wavefront lengths only
+int zpass_positions_lrnf(int L) {
+  switch (L) {
+    case 64: return z_P<Z_LRNF>(64);
+    case 128: return z_P<Z_LRNF>(128);
+    case 192: return z_P<Z_LRNF>(192);
+    case 256: return z_P<Z_LRNF>(256);
+    case 384: return z_P<Z_LRNF>(384);
+    case 512: return z_P<Z_LRNF>(512);
+    case 768: return z_P<Z_LRNF>(768);
+    default: return 0;
+  }
+}
+
+hipError_t launch_zpass_lrnf(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
+  switch (L) {
+    case 64: return zpass_launch<64, Z_LRNF>(a, tw, s, variant);
+    case 128: return zpass_launch<128, Z_LRNF>(a, tw, s, variant);
+    case 192: return zpass_launch<192, Z_LRNF>(a, tw, s, variant);
+    case 256: return zpass_launch<256, Z_LRNF>(a, tw, s, variant);
+    case 384: return zpass_launch<384, Z_LRNF>(a, tw, s, variant);
+    case 512: return zpass_launch<512, Z_LRNF>(a, tw, s, variant);
+    case 768: return zpass_launch<768, Z_LRNF>(a, tw, s, variant);
     default: return hipErrorInvalidValue;
   }
 }

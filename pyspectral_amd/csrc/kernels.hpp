@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "fft_core.hpp"
+#include "mlp_core.hpp"
 
 namespace psx {
 
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
 }
 
 // ------------------------------------------------------------------ fused z pass
-enum ZMode { Z_DNS = 0, Z_SMAG = 1, Z_TM2 = 2, Z_DT = 3, Z_SGS = 4, Z_PHYS = 5, Z_GRAD = 6, Z_LRN = 7 };
+enum ZMode { Z_DNS = 0, Z_SMAG = 1, Z_TM2 = 2, Z_DT = 3, Z_SGS = 4, Z_PHYS = 5, Z_GRAD = 6, Z_LRN = 7, Z_LRNF = 8 };
 
 template <int MODE> struct ZShape;
 template <> struct ZShape<Z_DNS> { static constexpr int nin = 3, nout = 6; };
@@ -227,6 +228,8 @@ template <> struct ZShape<Z_SGS> { static constexpr int nin = 6, nout = 12; };
 template <> struct ZShape<Z_PHYS> { static constexpr int nin = 3, nout = 0; };
 template <> struct ZShape<Z_GRAD> { static constexpr int nin = 9, nout = 0; };
 template <> struct ZShape<Z_LRN> { static constexpr int nin = 3, nout = 12; };
+// the learned closure fused: velocity and its nine gradients in, products + stress out
+template <> struct ZShape<Z_LRNF> { static constexpr int nin = 12, nout = 12; };
 
 // A real z-pencil's half spectrum is built as sum_t coef_t (i k3)^{dz_t} field_{f_t}.
 struct Recipe {
@@ -263,6 +266,16 @@ struct ZArgs {
   float* gout;
   const float* tin;
   int nphys;  // Z_PHYS: fields written (<= 3)
+  // Z_LRNF: the closure network (mlp_core.hpp) evaluated inside the pass on the
+  // gradients times gscale; c2 then scales its stresses into the products' units as in
+  // Z_LRN, and nc2 / nbound / nclip are the Smagorinsky fallback's (Cs Delta)^2, the
+  // input range bound and the backscatter clip
+  const short* npk;
+  const float* nw;
+  psx_mlp_impl::Norm nnorm;
+  double gscale;
+  float nc2, nbound;
+  int nclip;
 };
 
 // A z-pass block's member and its data pointers

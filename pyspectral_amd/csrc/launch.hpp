@@ -21,7 +21,9 @@ constexpr int epi_T(int L) { return L <= 64 ? 8 : (L <= 192 ? PSX_ET_SMALL : (L 
 // z pass: positions per block (pairing two positions, or two fields when P = 1)
 // and threads per block, sized to the first (radix-8 or -4) stage's butterflies.
 template <int MODE> constexpr int z_P(int L) {
-  return (MODE == Z_SMAG || MODE == Z_TM2 || MODE == Z_SGS || MODE == Z_GRAD || MODE == Z_LRN) ? 1 : 2;
+  return (MODE == Z_SMAG || MODE == Z_TM2 || MODE == Z_SGS || MODE == Z_GRAD || MODE == Z_LRN || MODE == Z_LRNF)
+             ? 1
+             : 2;
 }
 template <int MODE> constexpr int z_NT(int L) {
   constexpr int nin = ZShape<MODE>::nin, nout = ZShape<MODE>::nout;

@@ -25,6 +25,9 @@ hipError_t launch_zpass_grad(int L, const ZArgs& a, const double2* tw, hipStream
 hipError_t launch_zpass_lrn(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
 hipError_t mlp_closure(const psx_mlp* m, const float* g, float* tau, long long n, float c2, float bound, int clip,
                        hipStream_t s);
+void mlp_closure_args(const psx_mlp* m, const short** pk, const float** w, psx_mlp_impl::Norm* nm);
+hipError_t launch_zpass_lrnf(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant);
+int zpass_positions_lrnf(int L);
 int zpass_positions_dns(int L);
 int zpass_positions_smag(int L);
 int zpass_positions_tm2(int L);
@@ -44,6 +47,7 @@ int zpass_positions(int L, int mode) {
     case Z_PHYS: return zpass_positions_phys(L);
     case Z_GRAD: return zpass_positions_grad(L);
     case Z_LRN: return zpass_positions_lrn(L);
+    case Z_LRNF: return zpass_positions_lrnf(L);
   }
   return 0;
 }
@@ -58,6 +62,7 @@ hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipS
     case Z_PHYS: return launch_zpass_phys(L, a, tw, s, variant);
     case Z_GRAD: return launch_zpass_grad(L, a, tw, s, variant);
     case Z_LRN: return launch_zpass_lrn(L, a, tw, s, variant);
+    case Z_LRNF: return launch_zpass_lrnf(L, a, tw, s, variant);
   }
   return hipErrorInvalidValue;
 }
@@ -90,7 +95,7 @@ struct psx_context {
   double* nu_m = nullptr;
   // kernel shapes (psx_autotune / psx_set_tuning): tile widths of the x-inverse,
   // y-inverse and y-forward passes of the 3/2 pipeline (0 = default), z-pass variant
-  int tune[PSX_TUNE_N] = {0, 0, 0, 0, 0, 0};
+  int tune[PSX_TUNE_N] = {0, 0, 0, 0, 0, 0, 0};
   // x-chunk pipelining of the y-inverse / z / y-forward passes over two streams
   // (tune[4] chunks): separate X and W buffers so that chunks never alias, the
   // second stream and its fork/stagger/join events
@@ -522,10 +527,37 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
   z.npos = M1 * M2 * c->members;
   z.npos_m = c->members > 1 ? M1 * M2 : 0;
   z.kzscale = 2.0 * kPi / c->p.L3;
+  const double cd = c->p.Cs * delta_grid(c);
+  const int clip = (c->net_flags & PSX_CLOSURE_CLIP_BACKSCATTER) ? 1 : 0;
+  const int src[9] = {3, 6, 0, 4, 7, 1, 5, 8, 2};
+  if (c->tune[6] == 0 && zpass_positions(M3, Z_LRNF) > 0) {
+    // the network inside the z pass: velocity and gradient pencils in, products and
+    // stresses out, no gradient or stress arrays in HBM
+    ZArgs q = z;
+    q.out = Z;
+    q.out_sf = (long long)M1 * M2 * H;
+    q.Hout = H;
+    q.ms_out = member_stride(c, Z);
+    for (int r = 0; r < 3; ++r) q.rec[r] = Recipe{1, {r, 0}, {1.0, 0.0}, {0, 0}};
+    for (int r = 0; r < 9; ++r) q.rec[3 + r] = Recipe{1, {src[r], 0}, {1.0, 0.0}, {r % 3 == 2 ? 1 : 0, 0}};
+    q.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
+    q.c2 = q.out_scale * Nt;
+    q.gscale = 1.0 / std::sqrt(Nt);
+    mlp_closure_args(c->net, &q.npk, &q.nw, &q.nnorm);
+    q.nc2 = (float)(cd * cd);
+    q.nbound = (float)c->net_bound;
+    q.nclip = clip;
+    {
+      TimeScope ts(c, K_ZPASS);
+      check(c, launch_zpass(M3, Z_LRNF, q, tw3, c->stream, 0), "z_pass(fused closure)");
+    }
+    y_forward(c, Z, W, M1, M2, H, 0, 12);
+    epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true);
+    return;
+  }
   {
     // Y fields u v w ux vx wx uy vy wy -> du_i/dx_j, (i, j) row-major
     ZArgs g = z;
-    const int src[9] = {3, 6, 0, 4, 7, 1, 5, 8, 2};
     for (int r = 0; r < 9; ++r) g.rec[r] = Recipe{1, {src[r], 0}, {1.0, 0.0}, {r % 3 == 2 ? 1 : 0, 0}};
     g.out_scale = 1.0 / std::sqrt(Nt);  // pencil values are sqrt(Nt) times physical
     g.gout = c->lgrad;
@@ -535,11 +567,9 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
     check(c, launch_zpass(M3, Z_GRAD, g, tw3, c->stream, c->tune[3]), "z_pass(gradients)");
   }
   {
-    const double cd = c->p.Cs * delta_grid(c);
     const long long npts = (long long)M1 * M2 * M3 * c->members;
     TimeScope ts(c, K_CLOSURE);
-    check(c, mlp_closure(c->net, c->lgrad, c->ltau, npts, (float)(cd * cd), (float)c->net_bound,
-                         (c->net_flags & PSX_CLOSURE_CLIP_BACKSCATTER) ? 1 : 0, c->stream),
+    check(c, mlp_closure(c->net, c->lgrad, c->ltau, npts, (float)(cd * cd), (float)c->net_bound, clip, c->stream),
           "closure network");
   }
   {
@@ -1511,6 +1541,7 @@ int psx_set_tuning(psx_context* c, const int* tune) {
     if (tune[3] != 0 && tune[3] != 1) fail(c, PSX_E_ARG, "z variant must be 0 or 1");
     if (tune[4] < 0 || tune[4] > 8 || (tune[4] > 1 && c->M1 < 2 * tune[4])) fail(c, PSX_E_ARG, "bad chunk count");
     if (tune[5] < 0 || tune[5] > 2) fail(c, PSX_E_ARG, "chunk streams must be 0, 1 or 2");
+    if (tune[6] != 0 && tune[6] != 1) fail(c, PSX_E_ARG, "closure placement must be 0 or 1");
     if (tune[4] > 1) {
       if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
         fail(c, PSX_E_UNSUPPORTED, "chunked pipeline: DNS, Smagorinsky and DSMAG");
@@ -1536,7 +1567,7 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     const size_t state = (size_t)c->members * 3 * c->N1 * c->N2 * c->Nc;
     double2* tmp = nullptr;
     check(c, hipMalloc(&tmp, state * sizeof(double2)), "hipMalloc(autotune)");
-    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1, -1};  // -1: the whole stage
+    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1, -1, -1};  // -1: the whole stage
     hipEvent_t t0 = nullptr, t1 = nullptr;
     check(c, hipEventCreate(&t0), "hipEventCreate");
     check(c, hipEventCreate(&t1), "hipEventCreate");
@@ -1544,14 +1575,15 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     const bool was_timing = c->timing;
     try {
       for (int knob = 0; knob < PSX_TUNE_N; ++knob) {
-        if (learned && knob >= 4) continue;  // the learned stage is not chunked
+        if (learned && (knob == 4 || knob == 5)) continue;  // the learned stage is not chunked
+        if (!learned && knob == 6) continue;                 // closure placement: learned only
         // knob 4: chunk count and streams together (v = chunks + 16 * streams)
         if (knob == 5) continue;
         std::vector<int> cand;
         if (knob < 3) {
           for (int T : {4, 8, 16})
             if (pencil_tile(L[knob], T) == T) cand.push_back(T);
-        } else if (knob == 3) {
+        } else if (knob == 3 || knob == 6) {
           cand = {0, 1};
         } else {
           cand = {1 + 16 * 2, 2 + 16 * 2, 4 + 16 * 2, 2 + 16 * 1, 4 + 16 * 1, 8 + 16 * 1};

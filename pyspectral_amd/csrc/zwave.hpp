@@ -23,6 +23,10 @@ namespace psx {
 #ifndef PSX_ZW_MINW
 #define PSX_ZW_MINW 1
 #endif
+// 16-point network tiles evaluated together in the fused closure pass
+#ifndef PSX_ZW_FUSED_TILES
+#define PSX_ZW_FUSED_TILES 1
+#endif
 
 template <int L> struct ZWave {
   static constexpr int R0 = L / 64;
@@ -65,15 +69,15 @@ __device__ __forceinline__ double2 zw_in(const ZArgs& a, const ZMember& zm, int 
 // 7 v_y, 8 w_y; pencils u v w S11 S22 S33 S12 S13 S23 (src/RHSfunctions.py:174-179).
 // Every other mode's pencil f is input field f.
 template <int MODE> struct ZFields {
-  static constexpr int n = (MODE == Z_SMAG || MODE == Z_GRAD) ? 9 : ((MODE == Z_TM2 || MODE == Z_SGS) ? 6 : 3);
+  static constexpr int n =
+      (MODE == Z_SMAG || MODE == Z_GRAD || MODE == Z_LRNF) ? 9 : ((MODE == Z_TM2 || MODE == Z_SGS) ? 6 : 3);
 };
 template <int MODE> __host__ __device__ constexpr bool zw_uses(int fr, int fi) {
-  if (MODE == Z_GRAD) {
-    // gradient pencils A_ij = du_i/dx_j, row-major: ux uy uz vx vy vz wx wy wz from the
-    // y-pass fields 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y, 7 v_y, 8 w_y
-    constexpr int src[9] = {3, 6, 0, 4, 7, 1, 5, 8, 2};
-    return src[fr] == fi;
-  }
+  // gradient pencils A_ij = du_i/dx_j, row-major: ux uy uz vx vy vz wx wy wz from the
+  // y-pass fields 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y, 7 v_y, 8 w_y
+  constexpr int src[9] = {3, 6, 0, 4, 7, 1, 5, 8, 2};
+  if (MODE == Z_GRAD) return src[fr] == fi;
+  if (MODE == Z_LRNF) return fr < 3 ? fr == fi : src[fr - 3] == fi;  // u v w, then the gradients
   if (MODE != Z_SMAG) return fr == fi;
   switch (fr) {
     case 0: return fi == 0;
@@ -114,7 +118,11 @@ __device__ __forceinline__ double2 zw_recipe(int fr, const double2* raw, double 
       case 7: return cscale(cadd(cscale(cmul_i<1>(raw[0]), kz), raw[5]), 0.5);
       default: return cscale(cadd(cscale(cmul_i<1>(raw[1]), kz), raw[8]), 0.5);
     }
-  } else if constexpr (MODE == Z_GRAD) {
+  } else if constexpr (MODE == Z_GRAD || MODE == Z_LRNF) {
+    if constexpr (MODE == Z_LRNF) {
+      if (fr < 3) return raw[fr];
+      fr -= 3;
+    }
     switch (fr) {
       case 0: return raw[3];
       case 1: return raw[6];
@@ -318,6 +326,47 @@ __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2*
     if (l == 0) a.red[blockIdx.x] = mx;
     return;
   } else {
+    // ---- Z_LRNF: the closure network on this position's own gradients.  They are
+    // staged through LDS as float point records of stride 16 (the nine gradients at
+    // 0..8, then the six stresses at 9..14), so that 16-point MFMA tiles can be read
+    // with the points along N; the stresses go back into the same records.
+    [[maybe_unused]] float* rec = nullptr;
+    if constexpr (MODE == Z_LRNF) {
+      static_assert(P == 1 && (L / 16) % PSX_ZW_FUSED_TILES == 0, "fused closure: one position, whole tiles");
+      __shared__ float recs[L * 16];
+      rec = recs;
+#pragma unroll
+      for (int m = 0; m < R0; ++m) {
+        const int zz = l + 64 * m;
+#pragma unroll
+        for (int f = 0; f < 9; ++f) {
+          const int r = 3 + f;
+          recs[zz * 16 + f] = (float)(((r & 1) ? x[r >> 1][m].y : x[r >> 1][m].x) * a.gscale);
+        }
+      }
+      wave_sync();
+      const int g = l >> 4, rr = l & 15;
+      psx_mlp_impl::Net R;
+      psx_mlp_impl::net_load(R, a.npk, a.nw, a.nnorm, l);
+      constexpr int TP = PSX_ZW_FUSED_TILES;
+#pragma unroll 1
+      for (int t0 = 0; t0 < L / 16; t0 += TP) {
+        psx_mlp_impl::f4 xn[TP], out[TP];
+#pragma unroll
+        for (int q = 0; q < TP; ++q) xn[q] = psx_mlp_impl::net_input(R, &recs[((t0 + q) * 16 + rr) * 16], true, g);
+        psx_mlp_impl::net_forward<TP>(R, xn, out);
+#pragma unroll
+        for (int q = 0; q < TP; ++q) {
+          float* gp = &recs[((t0 + q) * 16 + rr) * 16];
+          const psx_mlp_impl::f4 res =
+              psx_mlp_impl::net_closure(R, out[q], xn[q], gp, true, g, a.nc2, a.nbound, a.nclip);
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (4 * g + j < 6) gp[9 + 4 * g + j] = res[j];
+        }
+      }
+      wave_sync();
+    }
     // ---- pointwise products / closure on the lane's R0 z points
     double2 xo[NCO][R0];
 #pragma unroll
@@ -331,7 +380,14 @@ __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2*
           const int r = f * P + p;
           in[f] = (r & 1) ? x[r >> 1][m].y : x[r >> 1][m].x;
         }
-        zpoint<MODE>(a, in, o[p], ((long long)blockIdx.x * P + p) * L + l + 64 * m);
+        if constexpr (MODE == Z_LRNF) {
+          // the products, then the stresses joined as Z_LRN does
+          zpoint<Z_DNS>(a, in, o[p]);
+#pragma unroll
+          for (int c = 0; c < 6; ++c) o[p][6 + c] = a.c2 * (double)rec[(l + 64 * m) * 16 + 9 + c];
+        } else {
+          zpoint<MODE>(a, in, o[p], ((long long)blockIdx.x * P + p) * L + l + 64 * m);
+        }
       }
 #pragma unroll
       for (int q = 0; q < NCO; ++q) {

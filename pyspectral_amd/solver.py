@@ -154,7 +154,8 @@ class SpectralSolver:
         _lib.check(self.ctx, self.lib.psx_step_graph(self.ctx, _ptr(q), _ptr(qs), _ptr(dt), _ptr(t)),
                    "psx_step_graph")
 
-    TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant", "yz_chunks", "yz_streams")
+    TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant", "yz_chunks", "yz_streams",
+                   "closure_separate")
 
     def tuning(self) -> dict:
         t = (C.c_int * len(self.TUNING_KEYS))()

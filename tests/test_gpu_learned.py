@@ -219,6 +219,36 @@ def test_learned_rhs_with_clip_matches_oracle():
     net.close()
 
 
+@pytest.mark.parametrize("shape", [(16, 16, 128), (16, 32, 256)])
+@pytest.mark.parametrize("bound,clip", [(float("inf"), False), (1.5, True)])
+def test_fused_closure_pass_matches_separate_network_kernel(shape, bound, clip):
+    """Padded z lengths 192 and 384 run the network inside the z pass; the separate
+    network kernel between a gradient and a products pass gives the same RHS (the two
+    pair their pencils differently, so float casts of the gradients may differ in the
+    last bit at a few points)."""
+    from pyspectral_amd.solver import SpectralSolver
+
+    print(f"\nfused vs separate closure shape={shape} bound={bound} clip={clip}", flush=True)
+    u = _field(*shape, 41, nyquist=False)
+    st = so.State(so.Grid(*shape), *u, nu=0.01, turb_model=1)
+    net = _trained_like_net(so.padded_gradients(st, FFT, *u))
+    out = []
+    for separate in (0, 1):
+        s = SpectralSolver(*shape, "learned", nu=0.01)
+        s.set_closure(net, bound, clip_backscatter=clip)
+        s.set_tuning({"closure_separate": separate})
+        ud = _dev(u)
+        w0 = torch.empty_like(ud)
+        out.append((_host(s.rhs(ud, 0.0, w0=w0)), _host(w0)))
+        s.close()
+    (r0, w0a), (r1, w0b) = out
+    scale = np.linalg.norm(w0b)
+    err = max(np.linalg.norm(r0 - r1), np.linalg.norm(w0a - w0b)) / scale
+    print(f"fused vs separate: error / |w0| = {err:.2e}")
+    assert err < 1e-6
+    net.close()
+
+
 def test_learned_graph_replays_match_eager():
     from pyspectral_amd import cases
 
