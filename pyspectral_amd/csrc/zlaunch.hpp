@@ -8,8 +8,21 @@
 #ifndef PSX_ZWAVE
 #define PSX_ZWAVE 1
 #endif
+// the compile-time half-spectrum instantiations below (0: runtime lengths only)
+#ifndef PSX_ZW_HK
+#define PSX_ZW_HK 1
+#endif
 
 namespace psx {
+
+// The 3/2-rule modes with L/3 half-spectrum entries in and out (the padded length a
+// multiple of 192) run an instantiation with that length fixed at compile time.
+template <int L, int MODE> constexpr bool zw_third_mode() {
+  return PSX_ZW_HK != 0 && ZWave<L>::ok && L % 192 == 0 && (MODE == Z_DNS || MODE == Z_SMAG || MODE == Z_LRNF);
+}
+template <int L, int MODE> bool zw_third(const ZArgs& a) {
+  return a.Hin0 == L / 3 && (a.nin0 >= ZShape<MODE>::nin || a.Hin1 == L / 3) && a.Hout == L / 3;
+}
 
 template <int L, int MODE>
 hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
@@ -18,6 +31,12 @@ hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int va
   if constexpr (MODE == Z_LRNF) {
     // the fused closure exists as the one-wavefront kernel only
     if constexpr (ZWave<L>::ok) {
+      if constexpr (zw_third_mode<L, MODE>()) {
+        if (zw_third<L, MODE>(a)) {
+          hipLaunchKernelGGL((zwave<L, P, MODE, L / 3>), grid, dim3(64), 0, s, a, tw);
+          return hipGetLastError();
+        }
+      }
       hipLaunchKernelGGL((zwave<L, P, MODE>), grid, dim3(64), 0, s, a, tw);
       return hipGetLastError();
     } else {
@@ -26,6 +45,12 @@ hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int va
   } else {
     if constexpr (PSX_ZWAVE != 0 && ZWave<L>::ok) {
       if (variant == 0) {
+        if constexpr (zw_third_mode<L, MODE>()) {
+          if (zw_third<L, MODE>(a)) {
+            hipLaunchKernelGGL((zwave<L, P, MODE, L / 3>), grid, dim3(64), 0, s, a, tw);
+            return hipGetLastError();
+          }
+        }
         hipLaunchKernelGGL((zwave<L, P, MODE>), grid, dim3(64), 0, s, a, tw);
         return hipGetLastError();
       }

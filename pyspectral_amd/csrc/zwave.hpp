@@ -23,6 +23,11 @@ namespace psx {
 #ifndef PSX_ZW_MINW
 #define PSX_ZW_MINW 1
 #endif
+// skip the forward transform's rows that the r2c split never reads (compile-time
+// half spectra only)
+#ifndef PSX_ZW_PRUNE
+#define PSX_ZW_PRUNE 1
+#endif
 // 16-point network tiles evaluated together in the fused closure pass
 #ifndef PSX_ZW_FUSED_TILES
 #define PSX_ZW_FUSED_TILES 1
@@ -53,15 +58,30 @@ __device__ __forceinline__ double2 zw_mask(double2 v, bool keep) {
   return make_double2(__longlong_as_double(__double_as_longlong(v.x) & m),
                       __longlong_as_double(__double_as_longlong(v.y) & m));
 }
+// HK > 0: every input field has HK half-spectrum entries (a compile-time length).
+template <int HK>
 __device__ __forceinline__ double2 zw_in(const ZArgs& a, const ZMember& zm, int f, long long pos, int k) {
   // the field's buffer is chosen by (wave-uniform) selects, not a branch
   const bool first = f < a.nin0;
   const double2* base = first ? zm.in0 : zm.in1;
   const long long sf = first ? a.in_sf0 : a.in_sf1;
-  const int H = first ? a.Hin0 : a.Hin1;
+  const int H = HK > 0 ? HK : (first ? a.Hin0 : a.Hin1);
   const int g = first ? f : f - a.nin0;
   const double2 v = base[(long long)g * sf + pos * H + (k < H ? k : H - 1)];
   return zw_mask(v, k < H);
+}
+
+// Register chunk m (entries k = l + 64 m, l < 64) of a c2r input whose half spectra
+// hold HK entries is zero in every lane when each of its k has min(k, L - k) >= HK:
+// with the 3/2 rule (HK = L/3) that is the middle third of the chunks, the zero pad.
+template <int L, int HK>
+__host__ __device__ constexpr bool zw_zero_chunk(int m) {
+  if (HK <= 0) return false;
+  for (int l = 0; l < 64; ++l) {
+    const int k = l + 64 * m, kk = k > L / 2 ? L - k : k;
+    if (kk < HK) return false;
+  }
+  return true;
 }
 
 // Input fields of each mode, and the recipes that build the real z-pencils from
@@ -152,10 +172,23 @@ __device__ __forceinline__ double2 zw_hermitian(int q, int k, double kz, const d
   return mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
 }
 
+// Row b2 of the forward transform's second radix-8 pass holds x_n for
+// n = a + R0 (b1 + 8 b2), a < R0, b1 < 8, i.e. n in [8 R0 b2, 8 R0 (b2 + 1)).  With
+// HK half-spectrum entries kept, the r2c split reads only n < HK and n > L - HK, so
+// a row whose every n lies in [HK, L - HK] is never read.
+template <int L, int HK>
+__host__ __device__ constexpr bool zw_row_read(int b2) {
+  if (HK <= 0) return true;
+  constexpr int R0 = L / 64;
+  const int lo = R0 * 8 * b2, hi = lo + R0 * 8 - 1;
+  return lo < HK || hi > L - HK;
+}
+
 // The 64-point transforms of `nrows` pencils (R0 sub-transforms each) held in LDS
 // slots f*SUB + l; the result x_{a + R0 b} of pencil g lands in natural order at
-// g*NATS + nat(n).
-template <int L, int SIGN>
+// g*NATS + nat(n) (for the forward transform with HK > 0, only the rows the r2c
+// split reads).
+template <int L, int SIGN, int HK = 0>
 __device__ __forceinline__ void zw_sub64(double2* buf, const double2* __restrict__ tw, int l, int nrows) {
   using W = ZWave<L>;
   constexpr int R0 = W::R0, SUB = W::SUB, NB = W::G * R0 * 8, IT = (NB + 63) / 64;
@@ -207,14 +240,22 @@ __device__ __forceinline__ void zw_sub64(double2* buf, const double2* __restrict
     const int u = l + 64 * it, f = u >> 3, b1 = u & 7;
     if ((NB % 64 == 0 || u < NB) && f < nrows * R0) {
       const int g = f / R0, av = f - g * R0;
+      if constexpr (SIGN < 0 && HK > 0 && PSX_ZW_PRUNE != 0) {
 #pragma unroll
-      for (int b2 = 0; b2 < 8; ++b2) buf[g * W::NATS + W::nat(av + R0 * (b1 + 8 * b2))] = y[it][b2];
+        for (int b2 = 0; b2 < 8; ++b2)
+          if (zw_row_read<L, HK>(b2)) buf[g * W::NATS + W::nat(av + R0 * (b1 + 8 * b2))] = y[it][b2];
+      } else {
+#pragma unroll
+        for (int b2 = 0; b2 < 8; ++b2) buf[g * W::NATS + W::nat(av + R0 * (b1 + 8 * b2))] = y[it][b2];
+      }
     }
   }
   wave_sync();
 }
 
-template <int L, int P, int MODE>
+// HK > 0: the input and output half spectra hold HK entries (the launcher checks
+// a.Hin0, a.Hin1 and a.Hout), so the zero-pad chunks are neither loaded nor held.
+template <int L, int P, int MODE, int HK = 0>
 __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2* __restrict__ tw) {
   using W = ZWave<L>;
   static_assert(W::ok, "zwave: L must be 64 * {1,2,3,4,6,8,12}");
@@ -240,12 +281,13 @@ __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2*
   auto load_group = [&](int g0, double2 (&raw)[R0][P][NF]) {
 #pragma unroll
     for (int m = 0; m < R0; ++m) {
+      if (zw_zero_chunk<L, HK>(m)) continue;
       const int k = l + 64 * m, kk = k > L / 2 ? L - k : k;
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi)
-          if ((use.bits[g0 / G][p] >> fi) & 1u) raw[m][p][fi] = zw_in(a, zm, fi, pos0 + p, kk);
+          if ((use.bits[g0 / G][p] >> fi) & 1u) raw[m][p][fi] = zw_in<HK>(a, zm, fi, pos0 + p, kk);
     }
     // keep the group's loads together: nothing is scheduled across this point
     __builtin_amdgcn_sched_barrier(0);
@@ -265,7 +307,8 @@ __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2*
 #pragma unroll
         for (int m = 0; m < R0; ++m) {
           const int k = l + 64 * m, kk = k > L / 2 ? L - k : k;
-          v[m] = zw_hermitian<L, MODE, P, NRI, NF>(q, k, a.kzscale * kk, raw[cur][m]);
+          v[m] = zw_zero_chunk<L, HK>(m) ? make_double2(0.0, 0.0)
+                                         : zw_hermitian<L, MODE, P, NRI, NF>(q, k, a.kzscale * kk, raw[cur][m]);
         }
         Dft<R0, +1>::run(v);
         buf[(gi * R0) * SUB + l] = v[0];
@@ -414,13 +457,14 @@ __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2*
         }
       }
       wave_sync();
-      zw_sub64<L, -1>(buf, tw, l, NCO - g0 < G ? NCO - g0 : G);
+      zw_sub64<L, -1, HK>(buf, tw, l, NCO - g0 < G ? NCO - g0 : G);
 #pragma unroll
       for (int gi = 0; gi < G; ++gi) {
         const int q = g0 + gi;
         if (q < NCO) {
           const int r0 = 2 * q, r1 = 2 * q + 1;
-          for (int k = l; k < a.Hout; k += 64) {
+          const int Ho = HK > 0 ? HK : a.Hout;
+          for (int k = l; k < Ho; k += 64) {
             const double2 z1 = buf[gi * W::NATS + W::nat(k)];
             const double2 z2 = buf[gi * W::NATS + W::nat(k == 0 ? 0 : L - k)];
             zm.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k] =
