@@ -95,6 +95,26 @@ def measured_traffic(kernel, N, closure):
     return None
 
 
+def copy_rate(torch, mib=1024, reps=10):
+    """GB/s (bytes read + written) of a device-to-device copy of `mib` MiB, timed with
+    HIP events: the bandwidth a plain streaming kernel reaches on this part, next to
+    the published peak."""
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    rate = 2 * n * 8 * reps / (s.elapsed_time(e) * 1e-3) / 1e9
+    del a, b
+    return rate
+
+
 def cpu_baseline(N, closure, nu, cfl, steps, seed):
     """The oracle (numpy restatement of the reference's src/ path, scipy.fft with one
     worker per available core) advancing the same field; bounded sample of `steps`."""
@@ -509,6 +529,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel shapes")
+    ap.add_argument("--tuning", default="",
+                    help="kernel shapes as key=value,... (SpectralSolver.tuning() keys); set instead of autotuning, "
+                         "e.g. to profile the shapes the autotuner picked without its trial launches")
     ap.add_argument("--workload", default="les128", choices=["les128", "ensemble", "closure"],
                     help="les128: the 128^3 LES metric (default); ensemble: 96 members of 32^3/48^3/64^3; "
                          "closure: the learned closure trained on and judged against forced DNS")
@@ -563,7 +586,11 @@ def main():
     closure_info = None
     if args.closure == "learned":
         closure_info = attach_trained_closure(solver, q, args.seed)
-    tuning = solver.tuning() if args.no_autotune else solver.autotune(q, reps=5)
+    if args.tuning:
+        solver.set_tuning({k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
+        tuning = solver.tuning()
+    else:
+        tuning = solver.tuning() if args.no_autotune else solver.autotune(q, reps=5)
     for _ in range(args.warmup):
         solver.step(q, qs)
     torch.cuda.synchronize()
@@ -608,6 +635,9 @@ def main():
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom, N, args.closure) if nch == 1 else None,
             "bytes_per_launch": kb[dom]}
+    copy_gbs = copy_rate(torch)
+    roof["copy_GBs"] = round(copy_gbs, 1)
+    roof["frac_of_copy"] = round(ach / copy_gbs, 4)
     if "TFLOPs" in kernels[dom]:
         # the network's kernel (on its own or inside the z pass): bound by its VALU work,
         # priced against the dense BF16 matrix peak
@@ -637,7 +667,8 @@ def main():
                    "grid": N, "closure": args.closure, "cfl": cfl, "nu": nu, "parallelism": "single GPU"},
         "roofline": roof,
         "step_roofline": {"model_bytes_per_step": int(sb), "achieved": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
-                          "unit": "GB/s", "frac": round(sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                          "unit": "GB/s", "frac": round(sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "frac_of_copy": round(sb / (ms_per_step * 1e-3) / 1e9 / copy_gbs, 4)},
         "kernels": kernels,
         "tuning": tuning,
         "stats_after": {"E": E[0], "Omega": E[1], "finite": finite},

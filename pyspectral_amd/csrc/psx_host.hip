@@ -1631,9 +1631,9 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
           c->events_used = 0;
           c->event_kinds.clear();
           const double avg = cnt ? ms / cnt : 1e300;
-          // whole-stage timings (chunking) differ by a few percent between candidates:
-          // a chunked pipeline must beat the unchunked stage by 2% to be picked
-          const double margin = (knob == 4 && best < 1e300 && (v & 15) > 1) ? 0.98 : 1.0;
+          // whole-stage timings (chunking) vary by a few percent from run to run: a
+          // chunked pipeline must beat the best stage so far by 5% to be picked
+          const double margin = (knob == 4 && best < 1e300 && (v & 15) > 1) ? 0.95 : 1.0;
           if (avg < best * margin) {
             best = avg;
             pick = v;

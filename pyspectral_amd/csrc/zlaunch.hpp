@@ -21,7 +21,7 @@ template <int L, int MODE> constexpr bool zw_third_mode() {
   return PSX_ZW_HK != 0 && ZWave<L>::ok && L % 192 == 0 && (MODE == Z_DNS || MODE == Z_SMAG || MODE == Z_LRNF);
 }
 template <int L, int MODE> bool zw_third(const ZArgs& a) {
-  return a.Hin0 == L / 3 && (a.nin0 >= ZShape<MODE>::nin || a.Hin1 == L / 3) && a.Hout == L / 3;
+  return a.Hin0 == L / 3 && (a.nin0 >= ZFields<MODE>::n || a.Hin1 == L / 3) && a.Hout == L / 3;
 }
 
 template <int L, int MODE>
