@@ -38,9 +38,9 @@ def fused_closure(N, tuning):
 
 def kernel_bytes(N, closure, fused=False):
     """Compulsory (algorithmic) bytes per launch of each kernel class at N^3, 3/2 rule:
-    every input field read once, every output field written once.  The learned closure
-    fused into the z pass moves what the Smagorinsky z pass moves (9 fields in, 12 out);
-    run as its own kernel it has two z passes per stage (gradients out as float32, then
+    every input field read once, every output field written once.  The learned closure fused
+    into the z pass moves what the Smagorinsky z pass moves (9 fields in, 12 out); run
+    as its own kernel it has two z passes per stage (gradients out as float32, then
     products with the network's float32 stresses in), and the z_pass figure is their mean."""
     M, H, Nc = 3 * N // 2, N // 2, N // 2 + 1
     smag = closure in ("smagorinsky", "learned")

@@ -35,11 +35,12 @@ int pencil_tile(int L, int T) {
   }
 }
 
-template <int L, int T, bool INV>
+// kind: 0 forward, 1 inverse
+template <int L, int T, int KIND>
 static hipError_t launch_tile(const PencilArgs& a, const double2* tw, int njobs, hipStream_t s) {
   if constexpr (tile_ok<L, T>()) {
     const dim3 grid((unsigned)(a.A * ((a.C + T - 1) / T)), (unsigned)njobs);
-    if constexpr (INV)
+    if constexpr (KIND == 1)
       hipLaunchKernelGGL((pencil_inverse<L, T>), grid, dim3(NT), 0, s, a, tw);
     else
       hipLaunchKernelGGL((pencil_forward<L, T>), grid, dim3(NT), 0, s, a, tw);
@@ -49,12 +50,12 @@ static hipError_t launch_tile(const PencilArgs& a, const double2* tw, int njobs,
   }
 }
 
-template <int L, bool INV>
+template <int L, int KIND>
 static hipError_t launch_len(int T, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s) {
   switch (T) {
-    case 4: return launch_tile<L, 4, INV>(a, tw, njobs, s);
-    case 8: return launch_tile<L, 8, INV>(a, tw, njobs, s);
-    case 16: return launch_tile<L, 16, INV>(a, tw, njobs, s);
+    case 4: return launch_tile<L, 4, KIND>(a, tw, njobs, s);
+    case 8: return launch_tile<L, 8, KIND>(a, tw, njobs, s);
+    case 16: return launch_tile<L, 16, KIND>(a, tw, njobs, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -64,7 +65,7 @@ hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, 
   switch (L) {
 #define PSX_CASE(LL) \
   case LL:           \
-    return launch_len<LL, true>(t, a, tw, njobs, s);
+    return launch_len<LL, 1>(t, a, tw, njobs, s);
     PSX_LENGTHS(PSX_CASE)
 #undef PSX_CASE
     default:
@@ -77,7 +78,7 @@ hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, 
   switch (L) {
 #define PSX_CASE(LL) \
   case LL:           \
-    return launch_len<LL, false>(t, a, tw, njobs, s);
+    return launch_len<LL, 0>(t, a, tw, njobs, s);
     PSX_LENGTHS(PSX_CASE)
 #undef PSX_CASE
     default:

@@ -276,6 +276,9 @@ struct ZArgs {
   double gscale;
   float nc2, nbound;
   int nclip;
+  // Smagorinsky-form modes: 1 writes D_mn = N_mn + tau_mn in place of the products
+  // (fields 0..5), for the folded epilogue E_SMAGF
+  int fold;
 };
 
 // A z-pass block's member and its data pointers
@@ -371,6 +374,18 @@ __device__ __forceinline__ void zpoint(const ZArgs& a, const double* in, double*
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) out[3 * i + j] = s * (in[j] * in[3 + i]);
+  }
+}
+
+// a.fold: the products leave as D_mn = N_mn + tau_mn (outputs 0..5), next to the
+// stresses (6..11)
+template <int MODE>
+__device__ __forceinline__ void zpoint_fold(const ZArgs& a, double* out) {
+  if constexpr (MODE == Z_SMAG || MODE == Z_LRN || MODE == Z_LRNF) {
+    if (a.fold) {
+#pragma unroll
+      for (int g = 0; g < 6; ++g) out[g] += out[6 + g];
+    }
   }
 }
 
@@ -544,6 +559,7 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
             in[f] = (r & 1) ? v[r >> 1].y : v[r >> 1].x;
           }
           zpoint<MODE>(a, in, o[p], ((long long)blockIdx.x * P + p) * L + z);
+          zpoint_fold<MODE>(a, o[p]);
         }
 #pragma unroll
         for (int q = 0; q < NCO; ++q) {
@@ -593,10 +609,26 @@ __device__ __forceinline__ double2 block_sum2(double2 v, double2* scratch, int t
 }
 
 // ------------------------------------------------------------------ x-forward + epilogue
-enum EpiMode { E_DNS = 0, E_SMAG = 1, E_TAU2 = 2 };
+enum EpiMode { E_DNS = 0, E_SMAG = 1, E_TAU2 = 2, E_SMAGF = 3 };
 template <int EPI> struct EpiShape;
 template <> struct EpiShape<E_DNS> { static constexpr int nf = 6; };
 template <> struct EpiShape<E_SMAG> { static constexpr int nf = 12; };
+// E_SMAGF: the Smagorinsky form with products and stresses folded.  The input holds
+// D_mn = N_mn + tau_mn (fields 0..5, ZArgs::fold) and the stresses tau_mn (6..11, mn =
+// 11 22 33 12 13 23); the epilogue transforms 9 fields: D_mn, A = tau_11,
+// B = 2 k2 tau_12 + 2 k3 tau_13 and C = k2^2 tau_22 + 2 k2 k3 tau_23 + k3^2 tau_33,
+// formed before its x transform (k2 and k3 are constants of a block column), so that
+// sum_mn s_mn k_m k_n tau_mn = k1^2 A + k1 B + C (s = 1 on the diagonal, 2 off it).
+template <> struct EpiShape<E_SMAGF> { static constexpr int nf = 9; };
+// The epilogue reads its input one field per step.  E_SMAGF has 12 steps for its 9
+// transformed fields: steps 0..6 are D_mn and A; steps 7, 8 (tau_12, tau_13) build B
+// and steps 9, 10, 11 (tau_22, tau_23, tau_33) build C in LDS, and only the last step
+// of each transforms.  Other modes: step s is field s.
+template <int EPI> struct EpiSteps { static constexpr int n = EpiShape<EPI>::nf; };
+template <> struct EpiSteps<E_SMAGF> { static constexpr int n = 12; };
+__host__ __device__ constexpr int epi_step_field(int epi, int s) {
+  return epi != E_SMAGF || s < 7 ? s : (s == 7 ? 9 : (s == 8 ? 10 : (s == 9 ? 7 : (s == 10 ? 11 : 8))));
+}
 template <> struct EpiShape<E_TAU2> { static constexpr int nf = 9; };
 
 struct EpiArgs {
@@ -638,8 +670,21 @@ struct EpiArgs {
 template <int EPI>
 __device__ __forceinline__ void epi_accumulate(int f, const double* k, double kinv, double2 v, double2* acc,
                                                double2* acc_sgs) {
-  if constexpr (EPI == E_DNS || EPI == E_SMAG) {
-    // fields 0..5: N_mn = u_m u_n, 6..11: tau_mn; (mn) in 11,22,33,12,13,23
+  if constexpr (EPI == E_SMAGF) {
+    if (f >= 6) {
+      // the folded stress fields A, B, C (xforward_epilogue): with the products'
+      // projection applied to D = N + tau, the stress's own term as written becomes
+      // -(1 + i) k_i (k1^2 A + k1 B + C) / k^2
+      const double w = (f == 6 ? k[0] * k[0] : (f == 7 ? k[0] : 1.0)) * kinv;
+      const double2 t = make_double2(v.x - v.y, v.x + v.y);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc[i] = csub(acc[i], cscale(t, k[i] * w));
+      return;
+    }
+  }
+  if constexpr (EPI == E_DNS || EPI == E_SMAG || EPI == E_SMAGF) {
+    // fields 0..5: N_mn = u_m u_n (E_SMAGF: D_mn = N_mn + tau_mn), 6..11: tau_mn;
+    // (mn) in 11,22,33,12,13,23
     const int g = f % 6;
     const int m = sym_m(g), n = sym_n(g);
     const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
@@ -765,12 +810,15 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   for (int it = 0; it < (SGS ? IT : 1); ++it) accs[it][0] = accs[it][1] = accs[it][2] = make_double2(0.0, 0.0);
 
   const double k2 = a.ks[1] * signed_freq(ky, a.N2);
-#if PSX_EPI_PREFETCH
-  // field f+1 is loaded into registers while field f is transformed
+  // One step: stage input field epi_step_field(s) in LDS (from the registers `pre`,
+  // or straight from memory without prefetch), weighted and added to what the
+  // group's earlier steps staged (E_SMAGF's B and C), refill `pre` with step snext,
+  // and after a group's last step transform and accumulate.  Between steps without a
+  // transform each thread touches only its own staged elements, so no barrier.
   constexpr int LT = (L * T + NT - 1) / NT;
-  double2 pre[LT];
-  auto load_field = [&](int f) {
-    const double2* src = src0 + (long long)f * a.in_sf;
+  constexpr int NSTEP = EpiSteps<EPI>::n;
+  auto load_field = [&](int s, double2 (&pre)[LT]) {
+    const double2* src = src0 + (long long)epi_step_field(EPI, s) * a.in_sf;
 #pragma unroll
     for (int i = 0; i < LT; ++i) {
       const int e = tid + i * NT;
@@ -779,25 +827,43 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
                                                                     : make_double2(0.0, 0.0);
     }
   };
-  load_field(0);
-#endif
-#pragma unroll 1
-  for (int f = 0; f < NF; ++f) {
-#if PSX_EPI_PREFETCH
+  // E_SMAGF step weights: B = 2 k2 tau_12 + 2 k3 tau_13, C = k2^2 tau_22 + 2 k2 k3 tau_23
+  // + k3^2 tau_33 (k3 of the element's column)
+  auto weight = [&](int s, int c) -> double {
+    const double k3 = a.ks[2] * (c0 + c);
+    return s < 7 ? 1.0
+                 : (s == 7 ? 2.0 * k2
+                           : (s == 8 ? 2.0 * k3 : (s == 9 ? k2 * k2 : (s == 10 ? 2.0 * k2 * k3 : k3 * k3))));
+  };
+  auto run_step = [&](int s, double2 (&pre)[LT], int snext) {
+    const bool fold = EPI == E_SMAGF && s >= 7;
+    const bool add = fold && (s == 8 || s == 10 || s == 11);
+    const bool transform = !fold || s == 8 || s == 11;
+    const int f = fold ? (s <= 8 ? 7 : 8) : s;  // the transformed field this step feeds
+    if constexpr (PSX_EPI_PREFETCH != 0) {
 #pragma unroll
-    for (int i = 0; i < LT; ++i) {
-      const int e = tid + i * NT;
-      const int l = e / T, c = e - (e / T) * T;
-      if ((L * T) % NT == 0 || e < L * T) buf[c * LS + RW(l)] = pre[i];
+      for (int i = 0; i < LT; ++i) {
+        const int e = tid + i * NT;
+        const int l = e / T, c = e - (e / T) * T;
+        if ((L * T) % NT == 0 || e < L * T) {
+          double2 v = pre[i];
+          if (fold) v = cscale(v, weight(s, c));
+          if (add) v = cadd(v, buf[c * LS + RW(l)]);
+          buf[c * LS + RW(l)] = v;
+        }
+      }
+      if (snext < NSTEP) load_field(snext, pre);
+    } else {
+      const double2* src = src0 + (long long)epi_step_field(EPI, s) * a.in_sf;
+      for (int e = tid; e < L * T; e += NT) {
+        const int l = e / T, c = e - (e / T) * T;
+        double2 v = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
+        if (fold) v = cscale(v, weight(s, c));
+        if (add) v = cadd(v, buf[c * LS + RW(l)]);
+        buf[c * LS + RW(l)] = v;
+      }
     }
-    if (f + 1 < NF) load_field(f + 1);
-#else
-    const double2* src = src0 + (long long)f * a.in_sf;
-    for (int e = tid; e < L * T; e += NT) {
-      const int l = e / T, c = e - (e / T) * T;
-      buf[c * LS + RW(l)] = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
-    }
-#endif
+    if (!transform) return;
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
 #pragma unroll
@@ -814,7 +880,13 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
       }
     }
     __syncthreads();
-  }
+  };
+  // with PSX_EPI_PREFETCH, step s+1 loads while step s is transformed (a second
+  // buffer, two steps ahead, needs 256 VGPRs: it spilled and ran 12% slower)
+  double2 pa[LT];
+  if (PSX_EPI_PREFETCH != 0) load_field(0, pa);
+#pragma unroll 1
+  for (int s = 0; s < NSTEP; ++s) run_step(s, pa, s + 1);
 
   const double dtc = q0_ ? (*dt_) * a.c : 0.0;
   const double coef = (EPI == E_TAU2 && a.store_only) ? 1.0 : (a.coef_dev ? *a.coef_dev : a.coef);

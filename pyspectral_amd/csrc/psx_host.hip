@@ -346,12 +346,13 @@ void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int
 
 // K1-K4 of the 3/2-rule pipeline; returns the buffer holding W (the y-forward of the
 // z-pass products of uin).  smag adds the strain-rate pencils and the stress
-// -2 c2 |S| S_ij (c2 = (Cs Delta)^2 for the static closure).  With tune[4] = n > 1
+// -2 c2 |S| S_ij (c2 = (Cs Delta)^2 for the static closure); fold (with smag) has the
+// z pass write D_mn = N_mn + tau_mn in place of the products (for E_SMAGF).  With tune[4] = n > 1
 // the y-inverse, z and y-forward passes run in n chunks of x rows, alternating between
 // the context stream and a second stream (the second starting one y-inverse later), so
 // an HBM-bound y pass of one chunk overlaps the z pass of another; X and W then live
 // in their own buffers so that no chunk reads what another writes.
-double2* front_32(psx_context* c, const double2* uin, bool smag, double c2) {
+double2* front_32(psx_context* c, const double2* uin, bool smag, double c2, bool fold = false) {
   const int M1 = c->M1, M2 = c->M2, M3 = c->M3, H = c->H;
   const int nch = c->tune[4] > 1 ? c->tune[4] : 1;
   double2* X = nch > 1 ? c->bufX : c->bufA;
@@ -379,6 +380,7 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2) {
   const double Nt = (double)c->N1 * c->N2 * c->N3;
   z.out_scale = 1.0 / (std::sqrt(Nt) * (double)M1 * M2 * M3);
   z.c2 = c2;
+  z.fold = smag && fold ? 1 : 0;
   auto one = [](Recipe& r, int f) { r.nterm = 1; r.f[0] = f; r.coef[0] = 1.0; r.dz[0] = 0; };
   one(z.rec[0], 0);
   one(z.rec[1], 1);
@@ -443,9 +445,11 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2) {
 double delta_grid(const psx_context* c) { return std::cbrt(c->p.dx * c->p.dy * c->p.dz); }
 
 // K5 of the 3/2-rule models: x-forward + truncation + projection/closure/viscous +
-// stage update, from the y-forwarded products in B (with the stress fields when smag)
+// stage update, from the y-forwarded products in B (with the stress fields when smag;
+// folded: D_mn = N_mn + tau_mn in place of the products, no w0)
 void epilogue_32(psx_context* c, const double2* B, const double2* uin, const double2* q0, double2* out,
-                 const double* dt, double coef, double2* w0, int zero_nyq_out, bool smag) {
+                 const double* dt, double coef, double2* w0, int zero_nyq_out, bool smag, bool folded = false) {
+  if (folded && (!smag || w0)) fail(c, PSX_E_ARG, "epilogue: the folded fields carry no separate stress");
   const int M1 = c->M1, H = c->H;
   EpiArgs e;
   std::memset(&e, 0, sizeof(e));
@@ -480,7 +484,8 @@ void epilogue_32(psx_context* c, const double2* B, const double2* uin, const dou
   e.nu_m = c->nu_m;
   const double2* tw = twiddles(c, M1);
   TimeScope ts(c, K_XEPI);
-  check(c, launch_epilogue(M1, c->N1, smag ? E_SMAG : E_DNS, smag && w0 != nullptr, e, tw, c->stream),
+  check(c, launch_epilogue(M1, c->N1, folded ? E_SMAGF : (smag ? E_SMAG : E_DNS), !folded && smag && w0 != nullptr,
+                           e, tw, c->stream),
         "x_forward_epilogue");
 }
 
@@ -489,8 +494,10 @@ void stage_32(psx_context* c, const double2* uin, const double2* q0, double2* ou
               double2* w0, int zero_nyq_out) {
   const bool smag = c->model == PSX_MODEL_SMAGORINSKY;
   const double cd = c->p.Cs * delta_grid(c);
-  double2* B = front_32(c, uin, smag, cd * cd);
-  epilogue_32(c, B, uin, q0, out, dt, coef, w0, zero_nyq_out, smag);
+  // without a w0 output the products and stresses fold into 9 transformed fields
+  const bool fold = smag && w0 == nullptr;
+  double2* B = front_32(c, uin, smag, cd * cd, fold);
+  epilogue_32(c, B, uin, q0, out, dt, coef, w0, zero_nyq_out, smag, fold);
 }
 
 // Learned closure: the 3/2-rule pipeline with the stress of an attached network.
@@ -547,12 +554,13 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
     q.nc2 = (float)(cd * cd);
     q.nbound = (float)c->net_bound;
     q.nclip = clip;
+    q.fold = w0 == nullptr ? 1 : 0;
     {
       TimeScope ts(c, K_ZPASS);
       check(c, launch_zpass(M3, Z_LRNF, q, tw3, c->stream, 0), "z_pass(fused closure)");
     }
     y_forward(c, Z, W, M1, M2, H, 0, 12);
-    epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true);
+    epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true, w0 == nullptr);
     return;
   }
   {
@@ -583,13 +591,14 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
     // physical stress to the products' units: pencil products are Nt times physical
     q.c2 = q.out_scale * Nt;
     q.tin = c->ltau;
+    q.fold = w0 == nullptr ? 1 : 0;
     const int P = zpass_positions(M3, Z_LRN);
     if (P == 0 || (M2 % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported padded length");
     TimeScope ts(c, K_ZPASS);
     check(c, launch_zpass(M3, Z_LRN, q, tw3, c->stream, c->tune[3]), "z_pass(learned)");
   }
   y_forward(c, Z, W, M1, M2, H, 0, 12);
-  epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true);
+  epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true, w0 == nullptr);
 }
 
 // computeRHS_DSMAG (src/RHSfunctions.py:1663-1862): the Smagorinsky pipeline on the

@@ -431,6 +431,7 @@ __global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2*
         } else {
           zpoint<MODE>(a, in, o[p], ((long long)blockIdx.x * P + p) * L + l + 64 * m);
         }
+        zpoint_fold<MODE>(a, o[p]);
       }
 #pragma unroll
       for (int q = 0; q < NCO; ++q) {

@@ -249,6 +249,29 @@ def test_fused_closure_pass_matches_separate_network_kernel(shape, bound, clip):
     net.close()
 
 
+@pytest.mark.parametrize("shape,separate", [((16, 16, 128), 0), ((16, 16, 128), 1), ((32, 16, 48), 1)])
+def test_learned_rhs_folded_matches_twelve_fields(shape, separate):
+    """Without a w0 output the learned closure's y-forward folds its 12 fields into 9;
+    the RHS equals the 12-field one (which the oracle pins) to rounding."""
+    from pyspectral_amd.solver import SpectralSolver
+
+    print(f"\nfolded learned rhs shape={shape} separate={separate}", flush=True)
+    u = _field(*shape, 43, nyquist=False)
+    st = so.State(so.Grid(*shape), *u, nu=0.01, turb_model=1)
+    net = _trained_like_net(so.padded_gradients(st, FFT, *u))
+    s = SpectralSolver(*shape, "learned", nu=0.01)
+    s.set_closure(net, 1.5, clip_backscatter=True)
+    s.set_tuning({"closure_separate": separate})
+    ud = _dev(u)
+    r_fold = _host(s.rhs(ud, 0.0))
+    r_full = _host(s.rhs(ud, 0.0, w0=torch.empty_like(ud)))
+    err = _rel(r_fold, r_full)
+    print(f"folded vs 12-field: {err:.2e}")
+    assert err < 1e-13
+    s.close()
+    net.close()
+
+
 def test_learned_graph_replays_match_eager():
     from pyspectral_amd import cases
 

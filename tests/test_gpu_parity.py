@@ -129,6 +129,31 @@ def test_rhs_long_z_matches_oracle(shape, models):
         s2.close()
 
 
+# Without a w0 output the Smagorinsky-form y-forward folds products and stresses from
+# 12 fields into 9 (D = N + tau and the k1-polynomial coefficients of the stress's
+# projection term); with one it keeps the 12.  Both agree with the oracle.
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (48, 48, 48), (16, 16, 128), (16, 32, 64),
+                                   (32, 16, 48), (16, 16, 256)])
+def test_smagorinsky_rhs_folded_matches_oracle(shape):
+    N1, N2, N3 = shape
+    nu, t = 0.009, 0.41
+    print(f"\nfolded smagorinsky rhs shape={shape}", flush=True)
+    u = _field(N1, 7 + N1, nyquist_noise=True) if N1 == N2 == N3 else _field_shape(N1, N2, N3, 53 + N3)
+    st = so.State(so.Grid(N1, N2, N3), *u, nu=nu, turb_model=1, t=t)
+    so.compute_rhs(st, FFT)
+    ref = st.compact()
+    from pyspectral_amd.solver import SpectralSolver
+
+    s = SpectralSolver(N1, N2, N3, 1, nu=nu)
+    r_fold = _host(s.rhs(_dev(u), t))
+    w0 = torch.empty_like(_dev(u))
+    r_full = _host(s.rhs(_dev(u), t, w0=w0))
+    e_fold, e_pair = _rel(r_fold, ref), _rel(r_fold, r_full)
+    print(f"folded vs oracle {e_fold:.3e}, folded vs 12-field {e_pair:.3e}", flush=True)
+    assert e_fold < 1e-12 and e_pair < 1e-13, (e_fold, e_pair)
+    s.close()
+
+
 @pytest.mark.parametrize("model", MODELS)
 def test_rhs_with_linear_forcing(model):
     N, nu, a, t = 16, 0.012, 0.1, 0.37
