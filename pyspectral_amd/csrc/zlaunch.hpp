@@ -16,9 +16,11 @@
 namespace psx {
 
 // The 3/2-rule modes with L/3 half-spectrum entries in and out (the padded length a
-// multiple of 192) run an instantiation with that length fixed at compile time.
+// multiple of 192) run an instantiation with that length fixed at compile time (not
+// the fused closure at 768, which spills more that way).
 template <int L, int MODE> constexpr bool zw_third_mode() {
-  return PSX_ZW_HK != 0 && ZWave<L>::ok && L % 192 == 0 && (MODE == Z_DNS || MODE == Z_SMAG || MODE == Z_LRNF);
+  return PSX_ZW_HK != 0 && ZWave<L>::ok && L % 192 == 0 &&
+         (MODE == Z_DNS || MODE == Z_SMAG || (MODE == Z_LRNF && L < 768));
 }
 template <int L, int MODE> bool zw_third(const ZArgs& a) {
   return a.Hin0 == L / 3 && (a.nin0 >= ZFields<MODE>::n || a.Hin1 == L / 3) && a.Hout == L / 3;

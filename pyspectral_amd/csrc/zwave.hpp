@@ -33,6 +33,13 @@ namespace psx {
 #define PSX_ZW_FUSED_TILES 1
 #endif
 
+// Waves per SIMD the register allocation must allow: the Smagorinsky pass at L = 192
+// with 64-entry half spectra fits 4 (128 VGPRs, no scratch); every other pass keeps
+// PSX_ZW_MINW.
+template <int L, int MODE, int HK> constexpr int zw_min_waves() {
+  return (MODE == Z_SMAG && L == 192 && HK == 64 && PSX_ZW_MINW < 4) ? 4 : PSX_ZW_MINW;
+}
+
 template <int L> struct ZWave {
   static constexpr int R0 = L / 64;
   static constexpr bool ok = (L % 64 == 0) && (R0 == 1 || R0 == 2 || R0 == 3 || R0 == 4 || R0 == 6 ||
@@ -201,17 +208,16 @@ __device__ __forceinline__ void zw_sub64(double2* buf, const double2* __restrict
 #pragma unroll
       for (int r = 0; r < 8; ++r) y[it][r] = buf[f * SUB + j + 8 * r];
       Dft<8, SIGN>::run(y[it]);
-      double2 w[8];
-      w[1] = tw[R0 * j];
-      if constexpr (SIGN > 0) w[1].y = -w[1].y;
-      w[2] = cmul(w[1], w[1]);
-      w[3] = cmul(w[1], w[2]);
-      w[4] = cmul(w[2], w[2]);
-      w[5] = cmul(w[2], w[3]);
-      w[6] = cmul(w[3], w[3]);
-      w[7] = cmul(w[3], w[4]);
+      // twiddle w^b1 as a running power of w = w_64^{S j}: two live values instead of
+      // seven (a few ulp, like the table's own rounding)
+      double2 w1 = tw[R0 * j];
+      if constexpr (SIGN > 0) w1.y = -w1.y;
+      double2 wc = w1;
 #pragma unroll
-      for (int b1 = 1; b1 < 8; ++b1) y[it][b1] = cmul(y[it][b1], w[b1]);
+      for (int b1 = 1; b1 < 8; ++b1) {
+        y[it][b1] = cmul(y[it][b1], wc);
+        if (b1 < 7) wc = cmul(wc, w1);
+      }
     }
   }
   wave_sync();
@@ -256,7 +262,7 @@ __device__ __forceinline__ void zw_sub64(double2* buf, const double2* __restrict
 // HK > 0: the input and output half spectra hold HK entries (the launcher checks
 // a.Hin0, a.Hin1 and a.Hout), so the zero-pad chunks are neither loaded nor held.
 template <int L, int P, int MODE, int HK = 0>
-__global__ __launch_bounds__(64, PSX_ZW_MINW) void zwave(ZArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs a, const double2* __restrict__ tw) {
   using W = ZWave<L>;
   static_assert(W::ok, "zwave: L must be 64 * {1,2,3,4,6,8,12}");
   constexpr int R0 = W::R0, G = W::G, SUB = W::SUB;
