@@ -275,15 +275,16 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
         if constexpr (NS > 1) {
           const int k = jj % NS;
 #if PSX_TW_POW
-          // one table load per butterfly; the other twiddles are its powers
-          // (products of depth <= 3, a few ulp)
-          double2 w[R];
-          w[1] = tw[k * TWS];
-          if constexpr (SIGN > 0) w[1].y = -w[1].y;
+          // one table load per butterfly; the other twiddles are its running powers
+          // (at most R - 2 products, a few ulp; two live values instead of R - 1)
+          double2 w1 = tw[k * TWS];
+          if constexpr (SIGN > 0) w1.y = -w1.y;
+          double2 wc = w1;
 #pragma unroll
-          for (int r = 2; r < R; ++r) w[r] = cmul(w[r / 2], w[r - r / 2]);
-#pragma unroll
-          for (int r = 1; r < R; ++r) v[it][r] = cmul(v[it][r], w[r]);
+          for (int r = 1; r < R; ++r) {
+            v[it][r] = cmul(v[it][r], wc);
+            if (r + 1 < R) wc = cmul(wc, w1);
+          }
 #else
 #pragma unroll
           for (int r = 1; r < R; ++r) {
