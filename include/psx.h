@@ -14,7 +14,10 @@
  * Device memory is allocated by the create calls, by psx_sgs_dns (first call),
  * psx_set_tuning / psx_autotune (chunked-pipeline buffers, a temporary state) and
  * psx_mlp_grad (activation store, when the batch size changes); by nothing else.
- * A context is not thread-safe; separate contexts are independent.
+ * A context is not thread-safe; separate contexts are independent.  With cfl > 0 the
+ * DNS, Smagorinsky and learned-closure contexts also hold a second library stream and
+ * two 3-field state-sized buffers: psx_step runs its CFL reduction there, overlapping
+ * the first stage's transforms, and joins it before the first stage update.
  */
 #ifndef PSX_H
 #define PSX_H

@@ -133,6 +133,14 @@ struct psx_context {
   float* lgrad = nullptr;
   float* ltau = nullptr;
   double* cfl_part = nullptr;  // per-block CFL maxima (N1*N2/P entries)
+  // the step's CFL passes on their own stream and buffers (3/2-pipeline models with
+  // cfl > 0), overlapping the first stage's x/y/z passes; the epilogue that first
+  // needs dt waits for ev_cfl_join (cfl_pending)
+  double2* cflA = nullptr;
+  double2* cflB = nullptr;
+  hipStream_t stream_cfl = nullptr;
+  hipEvent_t ev_cfl_fork = nullptr, ev_cfl_join = nullptr;
+  bool cfl_pending = false;
   double* scal = nullptr;  // device scalars: [0] dt (internal), [1..4] tau family
   double* part = nullptr;
   int npart = 0;
@@ -444,12 +452,15 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2, bool
 
 double delta_grid(const psx_context* c) { return std::cbrt(c->p.dx * c->p.dy * c->p.dz); }
 
+void join_cfl(psx_context* c);
+
 // K5 of the 3/2-rule models: x-forward + truncation + projection/closure/viscous +
 // stage update, from the y-forwarded products in B (with the stress fields when smag;
 // folded: D_mn = N_mn + tau_mn in place of the products, no w0)
 void epilogue_32(psx_context* c, const double2* B, const double2* uin, const double2* q0, double2* out,
                  const double* dt, double coef, double2* w0, int zero_nyq_out, bool smag, bool folded = false) {
   if (folded && (!smag || w0)) fail(c, PSX_E_ARG, "epilogue: the folded fields carry no separate stress");
+  join_cfl(c);  // dt of an overlapped CFL reduction
   const int M1 = c->M1, H = c->H;
   EpiArgs e;
   std::memset(&e, 0, sizeof(e));
@@ -906,19 +917,34 @@ void run_stage(psx_context* c, const double2* uin, const double2* q0, double2* o
   }
 }
 
-void compute_dt(psx_context* c, const double2* u, double* dt) {
+// overlap: run the passes on stream_cfl (forked from the context stream) into the CFL
+// buffers and leave the join to the next epilogue_32 (or join_cfl)
+void compute_dt(psx_context* c, const double2* u, double* dt, bool overlap = false) {
   if (c->p.cfl <= 0) {
     hipLaunchKernelGGL(set_f64, dim3(1), dim3(1), 0, c->stream, dt, -c->p.cfl);
     check(c, hipGetLastError(), "set_f64");
     return;
   }
+  overlap = overlap && c->cflA != nullptr;
+  struct Restore {  // the context stream is switched to stream_cfl; put it back on any exit
+    psx_context* c;
+    hipStream_t s;
+    ~Restore() { c->stream = s; }
+  } restore{c, c->stream};
+  if (overlap) {
+    check(c, hipEventRecord(c->ev_cfl_fork, c->stream), "hipEventRecord");
+    check(c, hipStreamWaitEvent(c->stream_cfl, c->ev_cfl_fork, 0), "hipStreamWaitEvent");
+    c->stream = c->stream_cfl;
+  }
+  double2* XA = overlap ? c->cflA : c->bufA;
+  double2* XB = overlap ? c->cflB : c->bufB;
   const int N1 = c->N1, N2 = c->N2, N3 = c->N3, Nc = c->Nc;
   const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2};
-  x_inverse(c, u, c->bufA, N1, Nc, 1, 3, jin, jp, nullptr, K_CFL_X);
-  y_inverse(c, c->bufA, c->bufB, N1, N2, Nc, 1, 3, jin, jp, nullptr, K_CFL_Y);
+  x_inverse(c, u, XA, N1, Nc, 1, 3, jin, jp, nullptr, K_CFL_X);
+  y_inverse(c, XA, XB, N1, N2, Nc, 1, 3, jin, jp, nullptr, K_CFL_Y);
   ZArgs z;
   std::memset(&z, 0, sizeof(z));
-  z.in0 = c->bufB;
+  z.in0 = XB;
   z.in_sf0 = (long long)N1 * N2 * Nc;
   z.Hin0 = Nc;
   z.nin0 = 3;
@@ -945,6 +971,17 @@ void compute_dt(psx_context* c, const double2* u, double* dt) {
   hipLaunchKernelGGL(cfl_finalize, dim3(c->members), dim3(NT), 0, c->stream, c->cfl_part, nparts, dt,
                      1.0 / std::sqrt(Nt), c->p.cfl, c->p.nu, (const double*)c->nu_m, ksum);
   check(c, hipGetLastError(), "cfl_finalize");
+  if (overlap) {
+    check(c, hipEventRecord(c->ev_cfl_join, c->stream_cfl), "hipEventRecord");
+    c->cfl_pending = true;
+  }
+}
+
+// the context stream waits for the overlapped CFL passes (dt is ready after this)
+void join_cfl(psx_context* c) {
+  if (!c->cfl_pending) return;
+  c->cfl_pending = false;
+  check(c, hipStreamWaitEvent(c->stream, c->ev_cfl_join, 0), "hipStreamWaitEvent");
 }
 
 void validate_params(psx_context* c, const psx_params* p, int model) {
@@ -1052,6 +1089,16 @@ int psx_create_batch(psx_context** out, int n1, int n2, int n3, int model, const
     check(c, hipMalloc(&c->bufA, mb * c->nA * sizeof(double2)), "hipMalloc(bufA)");
     check(c, hipMalloc(&c->bufB, mb * c->nB * sizeof(double2)), "hipMalloc(bufB)");
     check(c, hipMalloc(&c->cfl_part, mb * sizeof(double) * ((size_t)n1 * n2 / 2 + 1)), "hipMalloc(cfl_part)");
+    if (params->cfl > 0 &&
+        (model == PSX_MODEL_DNS || model == PSX_MODEL_SMAGORINSKY || model == PSX_MODEL_LEARNED)) {
+      const size_t nst = 3 * (size_t)n1 * n2 * (n3 / 2 + 1);
+      check(c, hipMalloc(&c->cflA, mb * nst * sizeof(double2)), "hipMalloc(cflA)");
+      check(c, hipMalloc(&c->cflB, mb * nst * sizeof(double2)), "hipMalloc(cflB)");
+      c->scratch_bytes += (long long)(2 * mb * nst * sizeof(double2));
+      check(c, hipStreamCreateWithFlags(&c->stream_cfl, hipStreamNonBlocking), "hipStreamCreate");
+      check(c, hipEventCreateWithFlags(&c->ev_cfl_fork, hipEventDisableTiming), "hipEventCreate");
+      check(c, hipEventCreateWithFlags(&c->ev_cfl_join, hipEventDisableTiming), "hipEventCreate");
+    }
     if (members > 1) {
       check(c, hipMalloc(&c->nu_m, mb * sizeof(double)), "hipMalloc(nu_m)");
       std::vector<double> nu(mb, params->nu);
@@ -1092,6 +1139,11 @@ int psx_destroy(psx_context* c) {
   if (c->ev_join) hipEventDestroy(c->ev_join);
   hipFree(c->nu_m);
   hipFree(c->cfl_part);
+  hipFree(c->cflA);
+  hipFree(c->cflB);
+  if (c->stream_cfl) hipStreamDestroy(c->stream_cfl);
+  if (c->ev_cfl_fork) hipEventDestroy(c->ev_cfl_fork);
+  if (c->ev_cfl_join) hipEventDestroy(c->ev_cfl_join);
   hipFree(c->scal);
   hipFree(c->part);
   hipFree(c->bufC);
@@ -1171,10 +1223,17 @@ namespace {
 // The body of advanceQ_RK4 (src/PySpec_3d.py:96-105).  With t_dev the step-start
 // time is read from, and advanced by dt on, the device.
 void step_body(psx_context* c, double2* Q, double2* S, double* dt_dev, double t, double* t_dev) {
-  compute_dt(c, S, dt_dev);
+  // the 3/2-pipeline stages read dt only in their epilogue and write S only there, so
+  // the CFL reduction of S overlaps the first stage's x/y/z passes (in order while
+  // per-kernel timing is on, so that each kernel's time is its own)
+  const bool overlap =
+      !c->timing &&
+      (c->model == PSX_MODEL_DNS || c->model == PSX_MODEL_SMAGORINSKY || c->model == PSX_MODEL_LEARNED);
+  compute_dt(c, S, dt_dev, overlap);
   const int z = truncating(c->model) ? 1 : 0;
   // src/PySpec_3d.py:99-104: Q = Q0 + dt*c_i*RHS(Q), c = (1/4, 1/3, 1/2, 1)
   run_stage(c, Q, Q, S, dt_dev, 0.25, nullptr, z, t, t_dev);
+  join_cfl(c);
   run_stage(c, S, Q, S, dt_dev, 1.0 / 3.0, nullptr, z, t, t_dev);
   run_stage(c, S, Q, S, dt_dev, 0.5, nullptr, z, t, t_dev);
   run_stage(c, S, Q, Q, dt_dev, 1.0, nullptr, 0, t, t_dev);
