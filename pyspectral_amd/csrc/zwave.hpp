@@ -28,6 +28,10 @@ namespace psx {
 #ifndef PSX_ZW_PRUNE
 #define PSX_ZW_PRUNE 1
 #endif
+// the unpadded natural-order row layout at L = 192 (ZWave::FLAT)
+#ifndef PSX_ZW_NAT192
+#define PSX_ZW_NAT192 1
+#endif
 // 16-point network tiles evaluated together in the fused closure pass
 #ifndef PSX_ZW_FUSED_TILES
 #define PSX_ZW_FUSED_TILES 1
@@ -48,9 +52,13 @@ template <int L> struct ZWave {
   static constexpr int G = R0 >= 8 ? 1 : 8 / R0;
   // LDS slot of one 64-point sub-transform: 64 entries, pass-1 output at b1*9 + j
   static constexpr int SUB = 72;
-  // natural-order row: one pad entry per 16
-  static constexpr int NATS = L + L / 16;
-  static __device__ __forceinline__ int nat(int n) { return n + (n >> 4); }
+  // natural-order rows: one pad entry per 16, except at L = 192 (with PSX_ZW_NAT192):
+  // no pad inside the row and a row stride of 202 entries, which a bank model of the
+  // second pass's stride-3 writes and the lanes' contiguous reads finds conflict-free
+  // where the padded layout puts n and n + 16 (or 17) in the same banks
+  static constexpr bool FLAT = L == 192 && PSX_ZW_NAT192 != 0;
+  static constexpr int NATS = FLAT ? 202 : L + L / 16;
+  static __device__ __forceinline__ int nat(int n) { return FLAT ? n : n + (n >> 4); }
 };
 
 // One workgroup = one wavefront.  The compiler drops the s_barrier of a
@@ -265,6 +273,7 @@ template <int L, int P, int MODE, int HK = 0>
 __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs a, const double2* __restrict__ tw) {
   using W = ZWave<L>;
   static_assert(W::ok, "zwave: L must be 64 * {1,2,3,4,6,8,12}");
+  static_assert(W::G * W::NATS <= W::G * W::R0 * W::SUB, "zwave: natural-order rows exceed the LDS buffer");
   constexpr int R0 = W::R0, G = W::G, SUB = W::SUB;
   constexpr int NIN = ZShape<MODE>::nin, NOUT = ZShape<MODE>::nout;
   constexpr int NRI = NIN * P, NRO = NOUT * P;
