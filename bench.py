@@ -45,6 +45,8 @@ def kernel_bytes(N, closure, fused=False):
     M, H, Nc = 3 * N // 2, N // 2, N // 2 + 1
     smag = closure in ("smagorinsky", "learned")
     nX, nY, nZ = (6, 9, 12) if smag else (3, 3, 6)
+    if closure == "smagorinsky":
+        nY = 8  # u v w u_x v_y (u_y + v_x)/2 w_x w_y: the strain needs no separate u_y, v_x
     state = N * N * Nc * C16
     MMH, pts = M * M * H * C16, M ** 3
     z = nY * MMH + nZ * MMH

@@ -111,6 +111,10 @@ struct PencilArgs {
   int job_in[MAXJOBS];
   int job_plain[MAXJOBS];
   int job_deriv[MAXJOBS];
+  // inverse passes: the derivative output is dcoef * (i k in + in2), in2 = field
+  // job_in2 of the same input array (-1: none) -- e.g. (u_y + v_x) / 2 in one transform
+  int job_in2[MAXJOBS];
+  double job_dcoef[MAXJOBS];
   // ensembles: grid.y = jpm * members; member m = blockIdx.y / jpm reads at
   // in + m*in_ms and writes at out + m*out_ms (jpm = 0: one member)
   int jpm;
@@ -145,6 +149,8 @@ __global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a
   const int mem = pencil_member(a);
   const int job = blockIdx.y - mem * a.jpm;
   const double2* src = a.in + mem * a.in_ms + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
+  const int in2 = a.job_in2[job];
+  const double2* src2 = a.in + mem * a.in_ms + (long long)(in2 < 0 ? 0 : in2) * a.in_sf + (long long)aidx * a.in_sa + c0;
   constexpr int IT = (L * T + NT - 1) / NT;
   double2 r[IT];
   const int nload = a.Nspec * T;
@@ -170,8 +176,22 @@ __global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a
         const int p = pad_slot(l, a.Nspec, L, a.keep_nyq);
         if (p >= 0) {
           double2 v = r[it];
-          if (var == 1) v = cscale(cmul_i<1>(v), a.kscale * signed_freq(l, a.Nspec));
+          if (var == 1) v = cscale(cmul_i<1>(v), a.kscale * signed_freq(l, a.Nspec) * a.job_dcoef[job]);
           buf[c * LS + RW(p)] = v;
+        }
+      }
+    }
+    if (var == 1 && in2 >= 0) {
+      // the second input, added by the thread that staged each slot (no barrier), after
+      // the first input's registers are dead
+      const double dc = a.job_dcoef[job];
+#pragma unroll 3
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * NT;
+        if (e < nload) {
+          const int l = e / T, c = e - (e / T) * T;
+          const int p = pad_slot(l, a.Nspec, L, a.keep_nyq);
+          if (p >= 0 && c0 + c < a.C) buf[c * LS + RW(p)] = cadd(buf[c * LS + RW(p)], cscale(src2[(long long)l * a.in_sl + c], dc));
         }
       }
     }
@@ -233,6 +253,17 @@ template <> struct ZShape<Z_GRAD> { static constexpr int nin = 9, nout = 0; };
 template <> struct ZShape<Z_LRN> { static constexpr int nin = 3, nout = 12; };
 // the learned closure fused: velocity and its nine gradients in, products + stress out
 template <> struct ZShape<Z_LRNF> { static constexpr int nin = 12, nout = 12; };
+
+// Input fields of each mode (the pencils above are built from them by recipes).
+// Smagorinsky: y-pass fields 0 u, 1 v, 2 w, 3 u_x, 4 v_y, 5 S12 = (u_y + v_x)/2, 6 w_x,
+// 7 w_y; pencils u v w S11 S22 S33 S12 S13 S23 (src/RHSfunctions.py:174-179).  Gradient
+// modes: 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y, 7 v_y, 8 w_y.  Every other mode's
+// pencil f is input field f.
+template <int MODE> struct ZFields {
+  static constexpr int n = MODE == Z_SMAG ? 8
+                                          : ((MODE == Z_GRAD || MODE == Z_LRNF) ? 9
+                                                                                : ((MODE == Z_TM2 || MODE == Z_SGS) ? 6 : 3));
+};
 
 // A real z-pencil's half spectrum is built as sum_t coef_t (i k3)^{dz_t} field_{f_t}.
 struct Recipe {
@@ -418,12 +449,13 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
 #define ZRW(i) (::psx::Row<L, PADZ>::at(i))
 
   // --- build the inverse spectra
-  constexpr int HS = L2 + 1;  // staged k3 modes per real pencil
-  constexpr bool STAGE = (PSX_ZSTAGE != 0) && (NRI * HS <= NCMAX * LS);
+  constexpr int HS = L2 + 1;  // staged k3 modes per real input field
+  constexpr int NRS = ZFields<MODE>::n * P;  // staged rows: input fields x positions
+  constexpr bool STAGE = (PSX_ZSTAGE != 0) && (NRS * HS <= NCMAX * LS);
   if constexpr (STAGE) {
     // one coalesced pass of independent global loads into LDS, then the recipe
     // combinations from LDS into registers, then the Hermitian pencils
-    for (int e = tid; e < NRI * HS; e += NTZ) {
+    for (int e = tid; e < NRS * HS; e += NTZ) {
       const int r = e / HS, k = e - r * HS;
       buf[e] = zin(a, zm, r / P, pos0 + r % P, k);
     }

@@ -238,7 +238,10 @@ const double2* twiddles(psx_context* c, int L) {
 PencilArgs pencil_base() {
   PencilArgs a;
   std::memset(&a, 0, sizeof(a));
-  for (int j = 0; j < MAXJOBS; ++j) a.job_in[j] = a.job_plain[j] = a.job_deriv[j] = -1;
+  for (int j = 0; j < MAXJOBS; ++j) {
+    a.job_in[j] = a.job_plain[j] = a.job_deriv[j] = a.job_in2[j] = -1;
+    a.job_dcoef[j] = 1.0;
+  }
   a.scale = 1.0;
   return a;
 }
@@ -291,7 +294,7 @@ void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, in
 // y-inverse: X[f][L1][N2][C] -> Y[f][L1][L2][C]
 void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int C, int keep_nyq, int njobs,
                const int* jin, const int* jplain, const int* jderiv, int kind = K_YINV, int ns2 = 0, int x0 = 0,
-               int x1 = -1) {
+               int x1 = -1, const int* jin2 = nullptr, const double* jdcoef = nullptr) {
   if (ns2 == 0) ns2 = c->N2;
   if (x1 < 0) x1 = L1;
   PencilArgs a = pencil_base();
@@ -312,6 +315,8 @@ void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int
     a.job_in[j] = jin[j];
     a.job_plain[j] = jplain[j];
     a.job_deriv[j] = jderiv ? jderiv[j] : -1;
+    if (jin2) a.job_in2[j] = jin2[j];
+    if (jdcoef) a.job_dcoef[j] = jdcoef[j];
   }
   set_members(c, a, njobs);
   a.in += x0 * a.in_sa;
@@ -372,7 +377,7 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2, bool
     const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2}, jd[3] = {3, 4, 5};
     x_inverse(c, uin, X, M1, H, 0, 3, jin, jp, smag ? jd : nullptr);
   }
-  const int nY = smag ? 9 : 3;
+  const int nY = smag ? 8 : 3;
   const int nF = smag ? 12 : 6;
   ZArgs z;
   std::memset(&z, 0, sizeof(z));
@@ -394,13 +399,13 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2, bool
   one(z.rec[1], 1);
   one(z.rec[2], 2);
   if (smag) {
-    // Y fields: u v w ux vx wx uy vy wy; pencils u v w S11 S22 S33 S12 S13 S23
+    // Y fields: u v w ux vy S12 wx wy; pencils u v w S11 S22 S33 S12 S13 S23
     one(z.rec[3], 3);
-    one(z.rec[4], 7);
+    one(z.rec[4], 4);
     z.rec[5] = Recipe{1, {2, 0}, {1.0, 0.0}, {1, 0}};
-    z.rec[6] = Recipe{2, {6, 4}, {0.5, 0.5}, {0, 0}};
-    z.rec[7] = Recipe{2, {0, 5}, {0.5, 0.5}, {1, 0}};
-    z.rec[8] = Recipe{2, {1, 8}, {0.5, 0.5}, {1, 0}};
+    one(z.rec[6], 5);
+    z.rec[7] = Recipe{2, {0, 6}, {0.5, 0.5}, {1, 0}};
+    z.rec[8] = Recipe{2, {1, 7}, {0.5, 0.5}, {1, 0}};
   }
   const int mode = smag ? Z_SMAG : Z_DNS;
   const int P = zpass_positions(M3, mode);
@@ -423,10 +428,17 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2, bool
     const int x0 = (int)((long long)M1 * ch / nch), x1 = (int)((long long)M1 * (ch + 1) / nch);
     c->stream = (two && (ch & 1)) ? c->stream2 : main;
     if (two && ch == 1) check(c, hipStreamWaitEvent(c->stream, c->ev_stagger, 0), "hipStreamWaitEvent");
-    // K2: y-inverse (+ d/dy of the velocity for the closure)
-    {
-      const int jin[6] = {0, 1, 2, 3, 4, 5}, jp[6] = {0, 1, 2, 3, 4, 5}, jd[6] = {6, 7, 8, -1, -1, -1};
-      y_inverse(c, X, Y, M1, M2, H, 0, smag ? 6 : 3, jin, jp, smag ? jd : nullptr, K_YINV, 0, x0, x1);
+    // K2: y-inverse.  With the closure, X holds u v w ux vx wx and Y gets u v w ux vy
+    // S12 = (uy + vx)/2 wx wy: S12 is one transform of (i k2 u + vx)/2, and vx needs
+    // no field of its own
+    if (smag) {
+      const int jin[5] = {0, 1, 2, 3, 5}, jp[5] = {0, 1, 2, 3, 6}, jd[5] = {5, 4, 7, -1, -1};
+      const int jin2[5] = {4, -1, -1, -1, -1};
+      const double dc[5] = {0.5, 1.0, 1.0, 1.0, 1.0};
+      y_inverse(c, X, Y, M1, M2, H, 0, 5, jin, jp, jd, K_YINV, 0, x0, x1, jin2, dc);
+    } else {
+      const int jin[3] = {0, 1, 2}, jp[3] = {0, 1, 2};
+      y_inverse(c, X, Y, M1, M2, H, 0, 3, jin, jp, nullptr, K_YINV, 0, x0, x1);
     }
     if (ch == 0 && two) check(c, hipEventRecord(c->ev_stagger, c->stream), "hipEventRecord");
     // K3: z c2r -> products (+ Smagorinsky stress) -> z r2c, on positions [x0 M2, x1 M2)

@@ -99,14 +99,8 @@ __host__ __device__ constexpr bool zw_zero_chunk(int m) {
   return true;
 }
 
-// Input fields of each mode, and the recipes that build the real z-pencils from
-// them.  Smagorinsky: y-pass fields 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y,
-// 7 v_y, 8 w_y; pencils u v w S11 S22 S33 S12 S13 S23 (src/RHSfunctions.py:174-179).
-// Every other mode's pencil f is input field f.
-template <int MODE> struct ZFields {
-  static constexpr int n =
-      (MODE == Z_SMAG || MODE == Z_GRAD || MODE == Z_LRNF) ? 9 : ((MODE == Z_TM2 || MODE == Z_SGS) ? 6 : 3);
-};
+// The recipes that build the real z-pencils from the input fields (ZFields,
+// kernels.hpp).  Every mode without a recipe here uses input field f as pencil f.
 template <int MODE> __host__ __device__ constexpr bool zw_uses(int fr, int fi) {
   // gradient pencils A_ij = du_i/dx_j, row-major: ux uy uz vx vy vz wx wy wz from the
   // y-pass fields 0 u, 1 v, 2 w, 3 u_x, 4 v_x, 5 w_x, 6 u_y, 7 v_y, 8 w_y
@@ -119,11 +113,11 @@ template <int MODE> __host__ __device__ constexpr bool zw_uses(int fr, int fi) {
     case 1: return fi == 1;
     case 2: return fi == 2;
     case 3: return fi == 3;
-    case 4: return fi == 7;
+    case 4: return fi == 4;
     case 5: return fi == 2;
-    case 6: return fi == 6 || fi == 4;
-    case 7: return fi == 0 || fi == 5;
-    default: return fi == 1 || fi == 8;
+    case 6: return fi == 5;
+    case 7: return fi == 0 || fi == 6;
+    default: return fi == 1 || fi == 7;
   }
 }
 // bit fi of bits[g][p]: input field fi of position p is read by the pencils of
@@ -147,11 +141,11 @@ __device__ __forceinline__ double2 zw_recipe(int fr, const double2* raw, double 
       case 1: return raw[1];
       case 2: return raw[2];
       case 3: return raw[3];
-      case 4: return raw[7];
+      case 4: return raw[4];
       case 5: return cscale(cmul_i<1>(raw[2]), kz);
-      case 6: return cscale(cadd(raw[6], raw[4]), 0.5);
-      case 7: return cscale(cadd(cscale(cmul_i<1>(raw[0]), kz), raw[5]), 0.5);
-      default: return cscale(cadd(cscale(cmul_i<1>(raw[1]), kz), raw[8]), 0.5);
+      case 6: return raw[5];
+      case 7: return cscale(cadd(cscale(cmul_i<1>(raw[0]), kz), raw[6]), 0.5);
+      default: return cscale(cadd(cscale(cmul_i<1>(raw[1]), kz), raw[7]), 0.5);
     }
   } else if constexpr (MODE == Z_GRAD || MODE == Z_LRNF) {
     if constexpr (MODE == Z_LRNF) {
