@@ -185,6 +185,51 @@ def test_smagorinsky_shear_closed_form():
     assert np.abs(wv).max() < 1e-14 and np.abs(ww).max() < 1e-14
 
 
+def test_smagorinsky_stress_spectra_by_explicit_sums():
+    """The Smagorinsky stress spectra of a random field (N = 8) equal explicit sums:
+    the strain synthesised on the 3/2 grid (M = 12) from its Fourier coefficients
+    (Nyquist dropped), -2 (Cs Delta)^2 |S| S_ij formed pointwise, analysed by a direct
+    DFT over the M^3 points and kept at |k1|, |k2| < N/2, 0 <= k3 < N/2.  This pins the
+    aliasing the cubic stress picks up on the 3/2 grid, the truncation and the scaled
+    units ((3/2)^(-3/2) Nt^(-1/4) times physical for the strain, squared for the
+    stress) without the oracle's pad/unpad or FFTs."""
+    N = 8
+    M = 3 * N // 2
+    uh = [a.copy() for a in _rand_div_free(N, 23)]
+    rng = np.random.default_rng(5)
+    for a in uh:  # Nyquist entries, which the 3/2 pad must drop
+        a[N // 2] += 0.3 * rng.standard_normal((N, N // 2 + 1))
+        a[:, N // 2] += 0.3 * rng.standard_normal((N, N // 2 + 1))
+        a[:, :, N // 2] += 0.3 * rng.standard_normal((N, N))
+    st = _state(uh, model=1, N=N)
+    tau = so.smag_stress_hat(st, FFT, *uh)
+    C = [bf.half_to_full(a, N) for a in uh]  # |k| <= N/2 - 1: the Nyquist modes dropped
+    K = N // 2 - 1
+    ks = np.arange(-K, K + 1)
+    kk = np.meshgrid(ks, ks, ks, indexing="ij")
+    dC = [[1j * kk[j] * C[i] for j in range(3)] for i in range(3)]  # coefficients of du_i/dx_j
+    xm = np.arange(M) * 2 * np.pi / M
+    pts = np.stack(np.meshgrid(xm, xm, xm, indexing="ij"), axis=-1)
+    g = [[bf.synth(dC[i][j], pts) for j in range(3)] for i in range(3)]
+    S = [[0.5 * (g[i][j] + g[j][i]) for j in range(3)] for i in range(3)]
+    Smag = np.sqrt(2.0 * sum(S[i][j] ** 2 for i in range(3) for j in range(3)))
+    CsD2 = (so.SMAG_CS * st.grid.Delta) ** 2
+    Nt = float(N) ** 3
+    alpha = 1.5 ** -1.5 * Nt ** -0.25
+    E = np.exp(-1j * np.outer(xm, ks))            # analysis, k1 and k2 signed
+    Ez = np.exp(-1j * np.outer(xm, np.arange(0, K + 1)))
+    for (i, j), t in zip([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)], tau):
+        phys = -2.0 * CsD2 * Smag * S[i][j]
+        T = np.einsum("xyz,xa,yb,zc->abc", phys, E, E, Ez)
+        expect = np.zeros_like(t)
+        for a in range(2 * K + 1):
+            for b in range(2 * K + 1):
+                expect[ks[a] % N, ks[b] % N, :K + 1] = alpha ** 2 * T[a, b]
+        assert np.abs(t - expect).max() < 1e-12 * np.abs(expect).max(), (i, j)
+        # the Nyquist rows and plane carry nothing
+        assert np.abs(t[N // 2]).max() == 0 and np.abs(t[:, N // 2]).max() == 0 and np.abs(t[:, :, N // 2]).max() == 0
+
+
 def test_linear_forcing_amplification():
     """With linear forcing f = a u the Kolmogorov mode (no nonlinear term) grows or
     decays as R((a - nu k^2) dt)^n exactly, and for every model with forcing
