@@ -1049,8 +1049,9 @@ PARITY = {
     "rhs_nosgs": "pinned: TGV closed form, k.RHS=0, energy conservation (tests/test_oracle_rhs.py)",
     "rhs_smag": "pinned: pointwise closed form for a shear mode (tests/test_oracle_rhs.py)",
     "rhs_tmodel/static_tau": "pinned: brute-force convolution restatement at N=8, k.w0 invariants (tests/test_oracle_tmodel.py)",
-    "rhs_dtaumodel": "partially pinned: reduces to the static tau-model form with tau from the "
-                     "three reductions, each checked by brute force at N=8 (tests/test_oracle_tmodel.py)",
+    "rhs_dtaumodel": "pinned: reduces to the static tau-model form with tau from the three "
+                     "reductions LE, PLQLE_filt and filt_PLQLE, each rebuilt from direct convolutions "
+                     "at N=8 (tests/test_oracle_tmodel.py); the Leonard pressure sign is as written",
     "advance_rk4": "pinned: linear mode amplification R(z)^n (tests/test_oracle_rhs.py)",
     "compute_dt": "pinned: closed form from max|u| (tests/test_oracle_rhs.py)",
     "rhs_dsmag": "pinned: Cs^2 as physical-space sums (Parseval), Leonard stress by brute force, "

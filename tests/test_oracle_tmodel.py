@@ -132,6 +132,49 @@ def test_dynamic_tau_reductions_bruteforce():
         assert np.abs(R[c] - expect).max() < 1e-11 * np.abs(expect).max()
 
 
+def _bf_leonard_nl(uh):
+    """-i k_j u_i u_j + i k_i p with p = -k_m k_n u_m u_n / k^2, the form the dynamic
+    tau-model's Leonard term takes as written (src/RHSfunctions.py:814-819), from
+    direct convolutions; centred arrays on |k_i| <= N - 2 and their wavenumbers."""
+    K = N // 2 - 1
+    C = [bf.half_to_full(a, N, K=K) for a in uh]
+    prod = {(i, j): bf.conv(C[i], C[j]) for i in range(3) for j in range(i, 3)}
+    ks = np.arange(-2 * K, 2 * K + 1).astype(float)
+    k1, k2, k3 = np.meshgrid(ks, ks, ks, indexing="ij")
+    ksq = k1 * k1 + k2 * k2 + k3 * k3 + 1e-50
+    uu, vv, ww = prod[(0, 0)], prod[(1, 1)], prod[(2, 2)]
+    uv, uw, vw = prod[(0, 1)], prod[(0, 2)], prod[(1, 2)]
+    p = (-k1 * k1 * uu - k2 * k2 * vv - k3 * k3 * ww - 2 * k1 * k2 * uv - 2 * k1 * k3 * uw
+         - 2 * k2 * k3 * vw) / ksq
+    nl = [-1j * k1 * uu - 1j * k2 * uv - 1j * k3 * uw + 1j * k1 * p,
+          -1j * k1 * uv - 1j * k2 * vv - 1j * k3 * vw + 1j * k2 * p,
+          -1j * k1 * uw - 1j * k2 * vw - 1j * k3 * ww + 1j * k3 * p]
+    return nl, (k1, k2, k3)
+
+
+def test_dynamic_tau_leonard_reduction_bruteforce():
+    """Dynamic tau: the Leonard reduction LE = <G(N(u)) - G(N(G u)), G u> over the half
+    spectrum (weight 2 for 0 < k3 < N/2), with G the sharp test filter |k_i| < kf, rebuilt
+    from direct convolutions: this pins the oracle's 2x padding, products, test filter
+    on the 2x grid and Hermitian weights for LE as for its two companion reductions."""
+    uh = _field(29)
+    g = so.Grid(N, N, N, kc=5)   # kf = 3
+    st = so.State(g, *uh, nu=0.013, turb_model="Dynamic tau-model")
+    kf = st.kf
+    so.compute_rhs(st, FFT)
+    uf = [so.test_filter(g, kf, a) for a in uh]
+    nl, (k1, k2, k3) = _bf_leonard_nl(uh)
+    nlf, _ = _bf_leonard_nl(uf)
+    inside = (np.abs(k1) < kf) & (np.abs(k2) < kf) & (np.abs(k3) < kf)
+    K = N // 2 - 1
+    Cf = [np.pad(bf.half_to_full(a, N, K=K), K) for a in uf]  # same support as nl
+    w = np.where(k3 == 0, 1.0, np.where((k3 > 0) & (k3 < N // 2), 2.0, 0.0))
+    LE = sum(np.sum(w * np.where(inside, a - b, 0) * np.conj(c)) for a, b, c in zip(nl, nlf, Cf))
+    got = st.tau_parts[0]
+    assert abs(LE) > 0
+    assert abs(got - LE) < 1e-12 * abs(LE), (got, LE)
+
+
 def test_separate_modes_per_axis_cutoffs():
     """p holds exactly the modes with |k1| < N1/2, |k2| < N2/2 and 0 <= k3 < N3/2 of the
     2x grid, each axis at its own N (src/padding.py:65-79), stated on integer wavenumbers."""
