@@ -34,6 +34,21 @@ constexpr int NT = 256;       // threads per block
 #else
 #define PSX_STAGE_TW(NTH) const double2* twp = tw;
 #endif
+// The x-forward epilogue keeps the next field's loads in flight across a transform.
+// A twiddle read from global memory inside the transform would wait on the vector
+// memory counter, which retires loads in order, and so on those prefetched loads as
+// well: staged in LDS (1) the twiddle reads wait only on the LDS counter.
+#ifndef PSX_EPI_TW_LDS
+#define PSX_EPI_TW_LDS 1
+#endif
+#if PSX_EPI_TW_LDS && !PSX_TW_LDS
+#define PSX_STAGE_TW_EPI(NTH)                                          \
+  __shared__ double2 tw_lds[L];                                        \
+  for (int i_ = threadIdx.x; i_ < L; i_ += (NTH)) tw_lds[i_] = tw[i_]; \
+  const double2* twp = tw_lds;
+#else
+#define PSX_STAGE_TW_EPI(NTH) PSX_STAGE_TW(NTH)
+#endif
 #ifndef PSX_ZPAD
 #define PSX_ZPAD 0
 #endif
@@ -768,7 +783,7 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   constexpr int IT = (NS1 * T + NT - 1) / NT;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
-  PSX_STAGE_TW(NT)
+  PSX_STAGE_TW_EPI(NT)
   const int ct = (a.H + T - 1) / T;
   const int nmain = a.N2 * ct;
   // PSX_XCD_EPI = 2: the ct column tiles of one k2 row run on one XCD (they share the

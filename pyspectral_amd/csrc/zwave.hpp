@@ -196,9 +196,11 @@ __host__ __device__ constexpr bool zw_row_read(int b2) {
 // The 64-point transforms of `nrows` pencils (R0 sub-transforms each) held in LDS
 // slots f*SUB + l; the result x_{a + R0 b} of pencil g lands in natural order at
 // g*NATS + nat(n) (for the forward transform with HK > 0, only the rows the r2c
-// split reads).
+// split reads).  w8 = w_64^{l mod 8}, the lane's one table twiddle, is read once per
+// kernel by the caller: a table read here would wait on the vector memory counter,
+// which retires loads in order, and so on the next group's prefetched loads too.
 template <int L, int SIGN, int HK = 0>
-__device__ __forceinline__ void zw_sub64(double2* buf, const double2* __restrict__ tw, int l, int nrows) {
+__device__ __forceinline__ void zw_sub64(double2* buf, double2 w8, int l, int nrows) {
   using W = ZWave<L>;
   constexpr int R0 = W::R0, SUB = W::SUB, NB = W::G * R0 * 8, IT = (NB + 63) / 64;
   double2 y[IT][8];
@@ -212,7 +214,10 @@ __device__ __forceinline__ void zw_sub64(double2* buf, const double2* __restrict
       Dft<8, SIGN>::run(y[it]);
       // twiddle w^b1 as a running power of w = w_64^{S j}: two live values instead of
       // seven (a few ulp, like the table's own rounding)
-      double2 w1 = tw[R0 * j];
+      // j = l mod 8; the empty asm hides that w8 is the same every call, so the
+      // compiler keeps computing the powers here instead of holding all seven live
+      double2 w1 = w8;
+      asm volatile("" : "+v"(w1.x), "+v"(w1.y));
       if constexpr (SIGN > 0) w1.y = -w1.y;
       double2 wc = w1;
 #pragma unroll
@@ -281,6 +286,7 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
   double2 t1[R0];
 #pragma unroll
   for (int av = 0; av < R0; ++av) t1[av] = tw[av * l];
+  const double2 w8 = tw[R0 * (l & 7)];  // w_64^{l mod 8}, for every 64-point pass
 
   // ---- inverse transforms; lane l keeps x_n, n = l + 64 m, of every pencil
   constexpr int NF = ZFields<MODE>::n;
@@ -327,7 +333,7 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
     }
     if (PSX_ZW_PREFETCH && g0 + G < NCI) load_group(g0 + G, raw[cur ^ 1]);
     wave_sync();
-    zw_sub64<L, +1>(buf, tw, l, NCI - g0 < G ? NCI - g0 : G);
+    zw_sub64<L, +1>(buf, w8, l, NCI - g0 < G ? NCI - g0 : G);
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
       const int q = g0 + gi;
@@ -467,7 +473,7 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
         }
       }
       wave_sync();
-      zw_sub64<L, -1, HK>(buf, tw, l, NCO - g0 < G ? NCO - g0 : G);
+      zw_sub64<L, -1, HK>(buf, w8, l, NCO - g0 < G ? NCO - g0 : G);
 #pragma unroll
       for (int gi = 0; gi < G; ++gi) {
         const int q = g0 + gi;
