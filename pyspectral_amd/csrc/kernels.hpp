@@ -22,9 +22,14 @@ namespace psx {
 
 constexpr int NT = 256;       // threads per block
 
-// Twiddle tables read from global memory (0) or staged in LDS per block (1).
+// Twiddle tables read from global memory (0) or staged in LDS per block (1).  A table
+// read from global memory inside a transform waits on the vector memory counter,
+// which retires loads and stores in order, so it also waits for loads the block has
+// in flight for its next field (the epilogue's prefetch, the second input of a
+// pencil job) and for the stores of its previous transform; reads from LDS wait
+// only on the LDS counter.
 #ifndef PSX_TW_LDS
-#define PSX_TW_LDS 0
+#define PSX_TW_LDS 1
 #endif
 #if PSX_TW_LDS
 #define PSX_STAGE_TW(NTH)                                              \
@@ -33,21 +38,6 @@ constexpr int NT = 256;       // threads per block
   const double2* twp = tw_lds;
 #else
 #define PSX_STAGE_TW(NTH) const double2* twp = tw;
-#endif
-// The x-forward epilogue keeps the next field's loads in flight across a transform.
-// A twiddle read from global memory inside the transform would wait on the vector
-// memory counter, which retires loads in order, and so on those prefetched loads as
-// well: staged in LDS (1) the twiddle reads wait only on the LDS counter.
-#ifndef PSX_EPI_TW_LDS
-#define PSX_EPI_TW_LDS 1
-#endif
-#if PSX_EPI_TW_LDS && !PSX_TW_LDS
-#define PSX_STAGE_TW_EPI(NTH)                                          \
-  __shared__ double2 tw_lds[L];                                        \
-  for (int i_ = threadIdx.x; i_ < L; i_ += (NTH)) tw_lds[i_] = tw[i_]; \
-  const double2* twp = tw_lds;
-#else
-#define PSX_STAGE_TW_EPI(NTH) PSX_STAGE_TW(NTH)
 #endif
 #ifndef PSX_ZPAD
 #define PSX_ZPAD 0
@@ -150,6 +140,10 @@ __device__ __forceinline__ int pencil_member(const PencilArgs& a) { return a.jpm
 #endif
 #ifndef PSX_EPI_MINW
 #define PSX_EPI_MINW 1
+#endif
+// x-forward epilogue: load the state entries of the final update in one batch (1)
+#ifndef PSX_EPI_STATE_PRELOAD
+#define PSX_EPI_STATE_PRELOAD 1
 #endif
 template <int L, int T>
 __global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a, const double2* __restrict__ tw) {
@@ -783,7 +777,7 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   constexpr int IT = (NS1 * T + NT - 1) / NT;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
-  PSX_STAGE_TW_EPI(NT)
+  PSX_STAGE_TW(NT)
   const int ct = (a.H + T - 1) / T;
   const int nmain = a.N2 * ct;
   // PSX_XCD_EPI = 2: the ct column tiles of one k2 row run on one XCD (they share the
@@ -946,6 +940,33 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   const double dtc = q0_ ? (*dt_) * a.c : 0.0;
   const double coef = (EPI == E_TAU2 && a.store_only) ? 1.0 : (a.coef_dev ? *a.coef_dev : a.coef);
   double2 red = make_double2(0.0, 0.0);
+  // the state entries this thread updates (uprev, and q0 when set), all loaded before
+  // any is used: one memory round trip instead of one per component (not with the
+  // SGS output, whose accumulators leave no room for them)
+  constexpr bool PRE = PSX_EPI_STATE_PRELOAD != 0 && EPI != E_TAU2 && !SGS;
+  double2 su[PRE ? IT : 1][3], sq[PRE ? IT : 1][3];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      const int kx = e / T, c = e - (e / T) * T;
+      const bool ok = e < NS1 * T && c0 + c < a.H;
+      const long long idx = (long long)kx * plane + (long long)ky * a.Nc + c0 + c;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) su[it][cc] = ok ? uprev_[cc * fstride + idx] : make_double2(0.0, 0.0);
+    }
+    if (q0_) {
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * NT;
+        const int kx = e / T, c = e - (e / T) * T;
+        const bool ok = e < NS1 * T && c0 + c < a.H;
+        const long long idx = (long long)kx * plane + (long long)ky * a.Nc + c0 + c;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) sq[it][cc] = ok ? q0_[cc * fstride + idx] : make_double2(0.0, 0.0);
+      }
+    }
+  }
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int e = tid + it * NT;
@@ -960,7 +981,8 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
         double2 u[3], rhs[3];
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
-          u[cc] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0) : uprev_[cc * fstride + idx];
+          u[cc] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0)
+                                     : (PRE ? su[PRE ? it : 0][cc] : uprev_[cc * fstride + idx]);
           if constexpr (EPI == E_TAU2) {
             const double2 w = cscale(acc[it][cc], coef);
             if (w0_) w0_[cc * fstride + idx] = w;
@@ -994,7 +1016,8 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
         if (!(EPI == E_TAU2 && a.store_only)) {
 #pragma unroll
           for (int cc = 0; cc < 3; ++cc) {
-            double2 o = q0_ ? cadd(q0_[cc * fstride + idx], cscale(rhs[cc], dtc)) : rhs[cc];
+            double2 o = q0_ ? cadd(PRE ? sq[PRE ? it : 0][cc] : q0_[cc * fstride + idx], cscale(rhs[cc], dtc))
+                            : rhs[cc];
             if (a.zero_nyq_out && nyq) o = make_double2(0.0, 0.0);
             out_[cc * fstride + idx] = o;
           }
