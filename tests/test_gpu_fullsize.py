@@ -55,15 +55,18 @@ def _half_weights(g):
     return np.where(k3 == 0, 1.0, np.where(k3 < N // 2, 2.0, 0.0))[None, None, :]
 
 
-@pytest.mark.parametrize("model", [0, 1])
+# 0: plain method, 1: Smagorinsky (Cs = 0.16), 2: t-model (Ct = 0.1; its products on
+# the 2x grid, 256^3)
+@pytest.mark.parametrize("model", [0, 1, 2])
 def test_rhs_128_matches_oracle(field, model):
     print(f"\nrhs 128^3 model={model}", flush=True)
     nu, t = 1e-3, 0.0
+    kw = dict(Ct=0.1) if model == 2 else {}
     g = so.Grid(N, N, N)
-    st = so.State(g, *field, nu=nu, turb_model=model, t=t)
+    st = so.State(g, *field, nu=nu, turb_model=model, t=t, **kw)
     so.compute_rhs(st, FFT)
     ref = st.compact()
-    s = _solver(model, nu)
+    s = _solver(model, nu, **kw)
     r = _host(s.rhs(_dev(field), t))  # Smagorinsky: the folded form the step uses
     s.close()
     err = _rel(r, ref)
