@@ -124,6 +124,9 @@ struct PencilArgs {
   // in + m*in_ms and writes at out + m*out_ms (jpm = 0: one member)
   int jpm;
   long long in_ms, out_ms;
+  // block order: 0 gives each XCD a contiguous range of blocks (PSX_XCD), 1 keeps the
+  // dispatch order (which of the two is faster depends on the pass and the job count)
+  int order;
 };
 
 // (member, job) of a pencil block
@@ -152,7 +155,7 @@ __global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int bid = xcd_block<PSX_XCD != 0>();
+  const int bid = a.order ? (int)blockIdx.x : xcd_block<PSX_XCD != 0>();
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
   const int mem = pencil_member(a);
@@ -224,7 +227,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int bid = xcd_block<PSX_XCD != 0>();
+  const int bid = a.order ? (int)blockIdx.x : xcd_block<PSX_XCD != 0>();
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
   const int mem = pencil_member(a);

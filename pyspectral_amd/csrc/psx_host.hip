@@ -95,7 +95,7 @@ struct psx_context {
   double* nu_m = nullptr;
   // kernel shapes (psx_autotune / psx_set_tuning): tile widths of the x-inverse,
   // y-inverse and y-forward passes of the 3/2 pipeline (0 = default), z-pass variant
-  int tune[PSX_TUNE_N] = {0, 0, 0, 0, 0, 0, 0};
+  int tune[PSX_TUNE_N] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   // x-chunk pipelining of the y-inverse / z / y-forward passes over two streams
   // (tune[4] chunks): separate X and W buffers so that chunks never alias, the
   // second stream and its fork/stagger/join events
@@ -286,6 +286,7 @@ void x_inverse(psx_context* c, const double2* uin, double2* X, int L1, int C, in
   }
   set_members(c, a, njobs);
   const double2* tw = twiddles(c, L1);
+  if (kind == K_XINV) a.order = c->tune[7];
   TimeScope ts(c, kind);
   check(c, launch_pencil_inverse(L1, a, tw, njobs * c->members, c->stream, kind == K_XINV ? c->tune[0] : 0),
         "x_inverse");
@@ -322,6 +323,7 @@ void y_inverse(psx_context* c, const double2* X, double2* Y, int L1, int L2, int
   a.in += x0 * a.in_sa;
   a.out += x0 * a.out_sa;
   const double2* tw = twiddles(c, L2);
+  if (kind == K_YINV) a.order = c->tune[8];
   TimeScope ts(c, kind);
   check(c, launch_pencil_inverse(L2, a, tw, njobs * c->members, c->stream, kind == K_YINV ? c->tune[1] : 0),
         "y_inverse");
@@ -353,6 +355,7 @@ void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int
   a.in += x0 * a.in_sa;
   a.out += x0 * a.out_sa;
   const double2* tw = twiddles(c, L2);
+  a.order = c->tune[9];
   TimeScope ts(c, K_YFWD);
   check(c, launch_pencil_forward(L2, a, tw, nf * c->members, c->stream, c->tune[2]), "y_forward");
 }
@@ -1622,6 +1625,8 @@ int psx_set_tuning(psx_context* c, const int* tune) {
     if (tune[4] < 0 || tune[4] > 8 || (tune[4] > 1 && c->M1 < 2 * tune[4])) fail(c, PSX_E_ARG, "bad chunk count");
     if (tune[5] < 0 || tune[5] > 2) fail(c, PSX_E_ARG, "chunk streams must be 0, 1 or 2");
     if (tune[6] != 0 && tune[6] != 1) fail(c, PSX_E_ARG, "closure placement must be 0 or 1");
+    for (int k = 7; k < 10; ++k)
+      if (tune[k] != 0 && tune[k] != 1) fail(c, PSX_E_ARG, "block order must be 0 or 1");
     if (tune[4] > 1) {
       if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
         fail(c, PSX_E_UNSUPPORTED, "chunked pipeline: DNS, Smagorinsky and DSMAG");
@@ -1647,7 +1652,8 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     const size_t state = (size_t)c->members * 3 * c->N1 * c->N2 * c->Nc;
     double2* tmp = nullptr;
     check(c, hipMalloc(&tmp, state * sizeof(double2)), "hipMalloc(autotune)");
-    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1, -1, -1};  // -1: the whole stage
+    const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1, -1, -1,  // -1: the whole stage
+                                   K_XINV, K_YINV, K_YFWD};
     hipEvent_t t0 = nullptr, t1 = nullptr;
     check(c, hipEventCreate(&t0), "hipEventCreate");
     check(c, hipEventCreate(&t1), "hipEventCreate");
@@ -1663,7 +1669,7 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
         if (knob < 3) {
           for (int T : {4, 8, 16})
             if (pencil_tile(L[knob], T) == T) cand.push_back(T);
-        } else if (knob == 3 || knob == 6) {
+        } else if (knob == 3 || knob >= 6) {
           cand = {0, 1};
         } else {
           cand = {1 + 16 * 2, 2 + 16 * 2, 4 + 16 * 2, 2 + 16 * 1, 4 + 16 * 1, 8 + 16 * 1};
