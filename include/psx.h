@@ -11,6 +11,9 @@
  *
  * Every call is asynchronous on the context's HIP stream and returns 0 on
  * success or a negative PSX_E* code; psx_last_error() gives the message.
+ * PSX_E_NONFINITE is a status, not a failure: the call's work was enqueued as
+ * usual (the reference keeps stepping a NaN state, src/PySpec_3d.py:121-141), but
+ * a CFL or diagnostics reduction that has completed met a NaN or an infinite value.
  * Device memory is allocated by the create calls, by psx_sgs_dns (first call),
  * psx_set_tuning / psx_autotune (chunked-pipeline buffers, a temporary state) and
  * psx_mlp_grad (activation store, when the batch size changes); by nothing else.
@@ -31,6 +34,7 @@ extern "C" {
 #define PSX_E_HIP (-2)
 #define PSX_E_OOM (-3)
 #define PSX_E_UNSUPPORTED (-4)
+#define PSX_E_NONFINITE (-5)
 
 /* turb_model values (src/Classes.py:28,77,101,129,192) */
 #define PSX_MODEL_DNS 0          /* computeRHS_NOSGS (src/RHSfunctions.py:3-50)        */
@@ -52,7 +56,7 @@ typedef struct psx_params {
   double Ct;              /* t-model constant (src/Classes.py:103-108)                */
   double tau_a, tau_b;    /* static tau-model tau = tau_a + tau_b t (src/Classes.py:194-195) */
   int kc;                 /* sharp cutoff of Gf (src/Classes.py:584-588)              */
-  int rotate;             /* Coriolis terms on/off (src/RHSfunctions.py:33-50)        */
+  int rotate;             /* Coriolis terms on/off (src/RHSfunctions.py:33-50), every model */
   double Om1, Om2, Om3;   /* rotation vector                                         */
   double force;           /* linear forcing f = force * u of the velocity as the model
                              uses it (no counterpart in the reference; every model)   */
@@ -97,12 +101,28 @@ int psx_ensemble_set_stream(psx_ensemble* ens, void* stream);
 int psx_ensemble_step(psx_ensemble* ens, double* q, double* qs, double* dt_dev, double t);
 int psx_ensemble_step_graph(psx_ensemble* ens, double* q, double* qs, double* dt_dev);
 int psx_ensemble_stats(psx_ensemble* ens, const double* q, double* out_dev);
+/* psx_status over every member: waits for the ensemble's stream. */
+int psx_ensemble_status(psx_ensemble* ens);
 /* psx_set_closure for every member of a PSX_MODEL_LEARNED ensemble (one network). */
 int psx_ensemble_set_closure(psx_ensemble* ens, psx_mlp* net, double bound, int flags);
 int psx_destroy(psx_context* ctx);
 int psx_set_stream(psx_context* ctx, void* stream);
+/* Replaces the parameters; a step captured by psx_step_graph is re-captured by its
+ * next call (as after psx_set_tuning, psx_autotune and psx_set_closure). */
 int psx_set_params(psx_context* ctx, const psx_params* params);
 const char* psx_last_error(const psx_context* ctx);
+
+/* Non-finite detection (no counterpart in the reference, whose driver prints NaN
+ * statistics, src/PySpec_3d.py:129-141).  The CFL reduction of psx_step /
+ * psx_step_graph / psx_compute_dt (whose dt then is NaN for a NaN velocity and 0 for
+ * an infinite one, as numpy's amax and minimum give in src/Classes.py:687-698) and
+ * psx_stats raise a sticky per-context flag when they meet a NaN or an infinite
+ * value.  Entry points that run them return PSX_E_NONFINITE once the flag is set by
+ * work that has completed.  psx_status waits for the context's stream and returns
+ * PSX_E_NONFINITE if the flag is set (PSX_OK otherwise); psx_clear_status waits and
+ * clears it. */
+int psx_status(psx_context* ctx);
+int psx_clear_status(psx_context* ctx);
 /* Bytes of device scratch the context holds. */
 long long psx_scratch_bytes(const psx_context* ctx);
 
@@ -131,7 +151,8 @@ int psx_step(psx_context* ctx, double* q, double* qs, double* dt_dev, double t);
  * step-start time held on the device (*t_dev, advanced by dt), as one hipGraph.
  * The first call for a given (q, qs, dt_dev, t_dev, stream) captures the step on
  * the context's stream, which must not be the default stream; later calls replay
- * it.  No counterpart in the reference (whose loop is host Python); for callers
+ * it until the parameters, tuning or closure change (then the next call captures
+ * again).  No counterpart in the reference (whose loop is host Python); for callers
  * that run many steps without host work in between. */
 int psx_step_graph(psx_context* ctx, double* q, double* qs, double* dt_dev, double* t_dev);
 

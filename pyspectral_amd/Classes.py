@@ -114,17 +114,21 @@ class variables:
         elif turb_model == "Static tau-model":
             self.kf = int(grid.kc / 2.0)
         self.nvars = 3
+        # weave = 1 selects computeRHS_NOSGS_WEAVE for the DNS (src/Classes.py:43-51), whose
+        # fused assembly has no Coriolis terms (src/RHSfunctionsWEAVE.py:44-66)
+        self.weave = int(weave)
+        weave_dns = self.weave != 0 and turb_model == 0
         self.solver = SpectralSolver(
             grid.N1, grid.N2, grid.N3, turb_model, nu=self.nu, L=(grid.L1, grid.L2, grid.L3),
             dx=grid.dx, dy=grid.dy, dz=grid.dz, cfl=self.cfl, Cs=0.16, Ct=self.Ct, tau_a=self.tau_a,
-            tau_b=self.tau_b, kc=grid.kc, rotate=self.rotate, Om=(self.Om1, self.Om2, self.Om3))
+            tau_b=self.tau_b, kc=grid.kc, rotate=0 if weave_dns else self.rotate, Om=(self.Om1, self.Om2, self.Om3))
         if myFFT is not None:
             myFFT.solver = self.solver
         self.Ud = _upload(np.stack([uhat, vhat, what]))
         self.Qd = self.Ud.clone()
         self.w0d = None
         self._dt_dev = torch.zeros(1, dtype=torch.float64, device=self.Ud.device)
-        self.computeRHS = getattr(_rhs, _MODEL_RHS[turb_model])
+        self.computeRHS = _rhs.computeRHS_NOSGS_WEAVE if weave_dns else getattr(_rhs, _MODEL_RHS[turb_model])
         self.iteration = 0
 
     # -- the reference's Q2U / U2Q (src/Classes.py:35-42)

@@ -29,8 +29,19 @@ def computeRHS_NOSGS(main, grid, myFFT, utilities):
     _rhs(main, False)
 
 
-# The scipy.weave variant computes the same arithmetic (src/RHSfunctionsWEAVE.py:5-77).
-computeRHS_NOSGS_WEAVE = computeRHS_NOSGS
+def computeRHS_NOSGS_WEAVE(main, grid, myFFT, utilities):
+    """``src/RHSfunctionsWEAVE.py:5-77``: the arithmetic of ``computeRHS_NOSGS`` with the
+    assembly fused into one loop (``:44-66``), which has no Coriolis terms: rotation is
+    off whatever ``main.rotate`` says."""
+    s = main.solver
+    if s.params.rotate:
+        s.set_params(rotate=0)
+        try:
+            _rhs(main, False)
+        finally:
+            s.set_params(rotate=1)
+    else:
+        _rhs(main, False)
 
 
 def computeRHS_SMAG(main, grid, myFFT, utilities):

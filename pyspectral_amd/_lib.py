@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PSX_LIBRARY") or os.path.join(HERE, "libpsx.so")
 
 PSX_OK = 0
+PSX_E_NONFINITE = -5
 MODEL_IDS = {
     0: 0, "dns": 0,
     1: 1, "smagorinsky": 1,
@@ -34,7 +35,7 @@ EXPORTS = (
     "psx_mlp_create", "psx_mlp_destroy", "psx_mlp_last_error", "psx_mlp_set_stream", "psx_mlp_get_params",
     "psx_mlp_set_params", "psx_mlp_set_norm", "psx_mlp_forward", "psx_mlp_grad", "psx_mlp_gradient",
     "psx_mlp_train_step", "psx_set_closure", "psx_physical", "psx_gradients", "psx_restrict",
-    "psx_ensemble_set_closure", "psx_mlp_closure",
+    "psx_ensemble_set_closure", "psx_mlp_closure", "psx_status", "psx_clear_status", "psx_ensemble_status",
 )
 
 MLP_NPARAM = 9350
@@ -131,6 +132,9 @@ def load() -> C.CDLL:
         "psx_restrict": (i, [vp, dp, i, i, i, i, dp]),
         "psx_ensemble_set_closure": (i, [vp, vp, d, i]),
         "psx_mlp_closure": (i, [vp, dp, dp, C.c_longlong, d, d, i]),
+        "psx_status": (i, [vp]),
+        "psx_clear_status": (i, [vp]),
+        "psx_ensemble_status": (i, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -140,7 +144,11 @@ def load() -> C.CDLL:
     return lib
 
 
-def check(ctx, rc: int, what: str) -> None:
-    if rc != PSX_OK:
+def check(ctx, rc: int, what: str) -> int:
+    """Raise on a failure code.  PSX_E_NONFINITE is a status (the work was enqueued, a
+    reduction met a NaN or an infinite value) and is returned, not raised: the
+    reference keeps stepping a NaN state."""
+    if rc != PSX_OK and rc != PSX_E_NONFINITE:
         msg = load().psx_last_error(ctx).decode() if ctx else ""
         raise PsxError(f"{what} failed ({rc}): {msg}")
+    return rc

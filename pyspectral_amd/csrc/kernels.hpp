@@ -58,6 +58,10 @@ __device__ __forceinline__ int sym_m(int g) { return g == 1 ? 1 : (g == 2 ? 2 : 
 __device__ __forceinline__ int sym_n(int g) { return g == 0 ? 0 : (g == 1 ? 1 : (g == 3 ? 1 : 2)); }
 __device__ __forceinline__ double sel3(int i, double a, double b, double c) { return i == 0 ? a : (i == 1 ? b : c); }
 
+// Maximum that propagates NaN, as numpy's amax does (fmax drops a NaN operand, so a
+// NaN state would give a finite CFL limit where src/Classes.py:693 gives NaN).
+__device__ __forceinline__ double max_nan(double m, double v) { return (v > m || v != v) ? v : m; }
+
 // ------------------------------------------------------------------ index maps
 // Signed frequency of FFT-ordered spectral index i on an n-point axis; index n/2
 // is the -n/2 "oddball" mode, as in the reference's wavenumber arrays
@@ -569,17 +573,17 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
           const int r = f * P + p;
           v[f] = (r & 1) ? w[r >> 1].y : w[r >> 1].x;
         }
-        m = fmax(m, fabs(v[0]) * a.dinv[0] + fabs(v[1]) * a.dinv[1] + fabs(v[2]) * a.dinv[2]);
+        m = max_nan(m, fabs(v[0]) * a.dinv[0] + fabs(v[1]) * a.dinv[1] + fabs(v[2]) * a.dinv[2]);
       }
     }
     // wave reduction, then the block's maximum into its own slot (no atomics)
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off));
     __shared__ double wmax[NTZ / 64];
     if ((tid & 63) == 0) wmax[tid >> 6] = m;
     __syncthreads();
     if (tid == 0) {
       double b = wmax[0];
-      for (int w = 1; w < NTZ / 64; ++w) b = fmax(b, wmax[w]);
+      for (int w = 1; w < NTZ / 64; ++w) b = max_nan(b, wmax[w]);
       a.red[blockIdx.x] = b;
     }
     return;

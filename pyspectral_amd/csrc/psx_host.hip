@@ -147,10 +147,18 @@ struct psx_context {
   std::map<int, double2*> tw;
   long long scratch_bytes = 0;
   std::string err;
-  // captured step (psx_step_graph) and the pointers it was captured for
+  // captured step (psx_step_graph), the pointers it was captured for and the
+  // settings generation it holds by value (gen: bumped by every call that changes
+  // parameters, tuning or the closure)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   const void* graph_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  unsigned long long gen = 1, graph_gen = 0;
+  // non-finite flag: set by the CFL and diagnostics reductions when they meet a NaN or
+  // an infinite value; host-mapped, so the host reads what completed kernels wrote
+  // without a copy (nf_dev is the device view of nf_host)
+  int* nf_host = nullptr;
+  int* nf_dev = nullptr;
   // optional per-kernel timing with HIP events on the context stream
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -186,6 +194,22 @@ int guarded(psx_context* c, F&& f) {
     return PSX_E_HIP;
   }
 }
+
+// A captured step holds parameters, tuning and the closure by value: drop it.
+void drop_graph(psx_context* c) {
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  for (auto& k : c->graph_key) k = nullptr;
+  ++c->gen;
+}
+
+bool nonfinite_seen(const psx_context* c) { return c->nf_host && *(volatile const int*)c->nf_host != 0; }
+
+// PSX_OK, or PSX_E_NONFINITE when a reduction that has completed met a NaN or an
+// infinite value (the call's work is enqueued either way)
+int with_status(const psx_context* c, int rc) { return (rc == PSX_OK && nonfinite_seen(c)) ? PSX_E_NONFINITE : rc; }
 
 bool truncating(int model) {
   return model != PSX_MODEL_SMAGORINSKY && model != PSX_MODEL_DSMAG && model != PSX_MODEL_LEARNED;
@@ -984,7 +1008,7 @@ void compute_dt(psx_context* c, const double2* u, double* dt, bool overlap = fal
   const double dx = c->p.dx, dy = c->p.dy, dz = c->p.dz;
   const double ksum = 1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz);
   hipLaunchKernelGGL(cfl_finalize, dim3(c->members), dim3(NT), 0, c->stream, c->cfl_part, nparts, dt,
-                     1.0 / std::sqrt(Nt), c->p.cfl, c->p.nu, (const double*)c->nu_m, ksum);
+                     1.0 / std::sqrt(Nt), c->p.cfl, c->p.nu, (const double*)c->nu_m, ksum, c->nf_dev);
   check(c, hipGetLastError(), "cfl_finalize");
   if (overlap) {
     check(c, hipEventRecord(c->ev_cfl_join, c->stream_cfl), "hipEventRecord");
@@ -1029,8 +1053,6 @@ int psx_create_batch(psx_context** out, int n1, int n2, int n3, int model, const
       fail(c, PSX_E_UNSUPPORTED, "batched instances are available for the DNS, Smagorinsky and learned models");
     validate_params(c, params, model);
     c->members = members;
-    if (model == PSX_MODEL_DYNAMIC_TAU && params->rotate)
-      fail(c, PSX_E_UNSUPPORTED, "the dynamic tau-model is not available with rotation");
     c->N1 = n1;
     c->N2 = n2;
     c->N3 = n3;
@@ -1122,6 +1144,10 @@ int psx_create_batch(psx_context** out, int n1, int n2, int n3, int model, const
         if (!(v >= 0.0)) fail(c, PSX_E_ARG, "member viscosities must be >= 0");
       check(c, hipMemcpy(c->nu_m, nu.data(), mb * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(nu_m)");
     }
+    check(c, hipHostMalloc((void**)&c->nf_host, 64, hipHostMallocMapped | hipHostMallocCoherent),
+          "hipHostMalloc(flag)");
+    *(volatile int*)c->nf_host = 0;
+    check(c, hipHostGetDevicePointer((void**)&c->nf_dev, c->nf_host, 0), "hipHostGetDevicePointer(flag)");
     check(c, hipMalloc(&c->scal, 16 * sizeof(double)), "hipMalloc(scal)");
     check(c, hipMemset(c->scal, 0, 16 * sizeof(double)), "hipMemset(scal)");
     c->npart = 1024;
@@ -1181,6 +1207,8 @@ int psx_destroy(psx_context* c) {
   for (auto e : c->events) hipEventDestroy(e);
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
   if (c->graph) hipGraphDestroy(c->graph);
+  // hipHostFree waits for the device, as hipFree does for the buffers above
+  if (c->nf_host) hipHostFree(c->nf_host);
   delete c;
   return PSX_OK;
 }
@@ -1193,6 +1221,19 @@ int psx_set_params(psx_context* c, const psx_params* params) {
   return guarded(c, [&] {
     validate_params(c, params, c->model);
     c->p = *params;
+    drop_graph(c);  // the captured step holds the old values
+  });
+}
+
+int psx_status(psx_context* c) {
+  const int rc = guarded(c, [&] { check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"); });
+  return with_status(c, rc);
+}
+
+int psx_clear_status(psx_context* c) {
+  return guarded(c, [&] {
+    check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    *(volatile int*)c->nf_host = 0;
   });
 }
 
@@ -1201,7 +1242,7 @@ const char* psx_last_error(const psx_context* c) { return c ? c->err.c_str() : "
 long long psx_scratch_bytes(const psx_context* c) { return c ? c->scratch_bytes : 0; }
 
 int psx_rhs(psx_context* c, const double* uhat, double* rhs, double* uhat_model, double* w0, double t) {
-  return guarded(c, [&] {
+  return with_status(c, guarded(c, [&] {
     if (!uhat || !rhs) fail(c, PSX_E_ARG, "null pointer");
     if ((const double*)rhs == uhat) fail(c, PSX_E_ARG, "rhs must not alias uhat");
     run_stage(c, (const double2*)uhat, nullptr, (double2*)rhs, nullptr, 0.0, (double2*)w0, 0, t);
@@ -1215,23 +1256,23 @@ int psx_rhs(psx_context* c, const double* uhat, double* rhs, double* uhat_model,
         check(c, hipGetLastError(), "copy_truncate");
       }
     }
-  });
+  }));
 }
 
 int psx_rk_stage(psx_context* c, const double* q0, const double* uin, double* out, const double* dt_dev, double coef,
                  double t, int zero_nyquist_out) {
-  return guarded(c, [&] {
+  return with_status(c, guarded(c, [&] {
     if (!q0 || !uin || !out || !dt_dev) fail(c, PSX_E_ARG, "null pointer");
     run_stage(c, (const double2*)uin, (const double2*)q0, (double2*)out, dt_dev, coef, nullptr,
               zero_nyquist_out ? 1 : 0, t);
-  });
+  }));
 }
 
 int psx_compute_dt(psx_context* c, const double* uhat, double* dt_dev) {
-  return guarded(c, [&] {
+  return with_status(c, guarded(c, [&] {
     if (!uhat || !dt_dev) fail(c, PSX_E_ARG, "null pointer");
     compute_dt(c, (const double2*)uhat, dt_dev);
-  });
+  }));
 }
 
 namespace {
@@ -1260,20 +1301,20 @@ void step_body(psx_context* c, double2* Q, double2* S, double* dt_dev, double t,
 }  // namespace
 
 int psx_step(psx_context* c, double* q, double* qs, double* dt_dev, double t) {
-  return guarded(c, [&] {
+  return with_status(c, guarded(c, [&] {
     if (!q || !qs || !dt_dev) fail(c, PSX_E_ARG, "null pointer");
     if (q == qs) fail(c, PSX_E_ARG, "q and qs must be distinct buffers");
     step_body(c, (double2*)q, (double2*)qs, dt_dev, t, nullptr);
-  });
+  }));
 }
 
 int psx_step_graph(psx_context* c, double* q, double* qs, double* dt_dev, double* t_dev) {
-  return guarded(c, [&] {
+  return with_status(c, guarded(c, [&] {
     if (!q || !qs || !dt_dev || !t_dev) fail(c, PSX_E_ARG, "null pointer");
     if (q == qs) fail(c, PSX_E_ARG, "q and qs must be distinct buffers");
     if (c->stream == nullptr) fail(c, PSX_E_ARG, "graph capture needs a non-default stream (psx_set_stream)");
     const void* key[5] = {q, qs, dt_dev, t_dev, c->stream};
-    bool same = c->graph_exec != nullptr;
+    bool same = c->graph_exec != nullptr && c->graph_gen == c->gen;
     for (int i = 0; i < 5 && same; ++i) same = c->graph_key[i] == key[i];
     if (!same) {
       if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
@@ -1296,13 +1337,14 @@ int psx_step_graph(psx_context* c, double* q, double* qs, double* dt_dev, double
       check(c, hipStreamEndCapture(c->stream, &c->graph), "hipStreamEndCapture");
       check(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
       for (int i = 0; i < 5; ++i) c->graph_key[i] = key[i];
+      c->graph_gen = c->gen;
     }
     check(c, hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
-  });
+  }));
 }
 
 int psx_stats(psx_context* c, const double* uhat, double* out_dev) {
-  return guarded(c, [&] {
+  return with_status(c, guarded(c, [&] {
     if (!uhat || !out_dev) fail(c, PSX_E_ARG, "null pointer");
     const long long total = (long long)c->N1 * c->N2 * c->Nc;
     int nblk = (int)std::min<long long>((total + NT - 1) / NT, c->npart);
@@ -1312,10 +1354,11 @@ int psx_stats(psx_context* c, const double* uhat, double* out_dev) {
       hipLaunchKernelGGL(stats_partial, dim3(nblk), dim3(NT), 0, c->stream, u, c->part, c->N1, c->N2, c->N3, c->Nc,
                          2.0 * kPi / c->p.L1, 2.0 * kPi / c->p.L2, 2.0 * kPi / c->p.L3, c->p.kc);
       check(c, hipGetLastError(), "stats_partial");
-      hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, c->stream, c->part, nblk, out_dev + 3 * m, 1.0 / Nt);
+      hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, c->stream, c->part, nblk, out_dev + 3 * m, 1.0 / Nt,
+                         c->nf_dev);
       check(c, hipGetLastError(), "stats_final");
     }
-  });
+  }));
 }
 
 namespace {
@@ -1526,12 +1569,7 @@ int psx_set_closure(psx_context* c, psx_mlp* net, double bound, int flags) {
     c->net = net;
     c->net_bound = bound;
     c->net_flags = flags;
-    // a captured step holds the network's normalisation and the bound by value
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
-    c->graph = nullptr;
-    for (auto& k : c->graph_key) k = nullptr;
+    drop_graph(c);  // a captured step holds the network's normalisation and the bound by value
   });
 }
 
@@ -1633,6 +1671,7 @@ int psx_set_tuning(psx_context* c, const int* tune) {
       ensure_pipeline(c);
     }
     for (int k = 0; k < PSX_TUNE_N; ++k) c->tune[k] = tune[k];
+    drop_graph(c);  // the captured step was built with the old shapes
   });
 }
 
@@ -1745,6 +1784,7 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     check(c, hipFree(tmp), "hipFree(autotune)");
     if (tune_out)
       for (int k = 0; k < PSX_TUNE_N; ++k) tune_out[k] = c->tune[k];
+    drop_graph(c);
   });
 }
 
@@ -1767,8 +1807,17 @@ struct psx_ensemble {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<unsigned long long> gens;  // each class context's settings generation at capture
   std::string err;
 };
+
+namespace {
+// PSX_E_NONFINITE when any class context's non-finite flag is set
+int ensemble_status(const psx_ensemble* e, int rc) {
+  for (const psx_context* c : e->cls) rc = with_status(c, rc);
+  return rc;
+}
+}  // namespace
 
 
 extern "C" {
@@ -1847,12 +1896,7 @@ int psx_ensemble_set_closure(psx_ensemble* e, psx_mlp* net, double bound, int fl
       return rc;
     }
   }
-  // the captured ensemble step holds the closure's normalisation and bound by value
-  if (e->exec) hipGraphExecDestroy(e->exec);
-  if (e->graph) hipGraphDestroy(e->graph);
-  e->exec = nullptr;
-  e->graph = nullptr;
-  for (auto& k : e->key) k = nullptr;
+  // the captured ensemble step is rebuilt: psx_set_closure advanced every class's generation
   return PSX_OK;
 }
 long long psx_ensemble_size(const psx_ensemble* e) { return e ? e->total : PSX_E_ARG; }
@@ -1884,7 +1928,16 @@ int psx_ensemble_step(psx_ensemble* e, double* q, double* qs, double* dt_dev, do
       return rc;
     }
   }
-  return PSX_OK;
+  return ensemble_status(e, PSX_OK);
+}
+
+int psx_ensemble_status(psx_ensemble* e) {
+  if (!e) return PSX_E_ARG;
+  if (hipStreamSynchronize(e->stream) != hipSuccess) {
+    e->err = "hipStreamSynchronize failed";
+    return PSX_E_HIP;
+  }
+  return ensemble_status(e, PSX_OK);
 }
 
 int psx_ensemble_step_graph(psx_ensemble* e, double* q, double* qs, double* dt_dev) {
@@ -1895,8 +1948,9 @@ int psx_ensemble_step_graph(psx_ensemble* e, double* q, double* qs, double* dt_d
     return PSX_E_ARG;
   }
   const void* key[4] = {q, qs, dt_dev, e->stream};
-  bool same = e->exec != nullptr;
+  bool same = e->exec != nullptr && e->gens.size() == e->cls.size();
   for (int i = 0; i < 4 && same; ++i) same = e->key[i] == key[i];
+  for (size_t k = 0; k < e->cls.size() && same; ++k) same = e->gens[k] == e->cls[k]->gen;
   if (!same) {
     if (e->exec) hipGraphExecDestroy(e->exec);
     if (e->graph) hipGraphDestroy(e->graph);
@@ -1931,12 +1985,14 @@ int psx_ensemble_step_graph(psx_ensemble* e, double* q, double* qs, double* dt_d
       return PSX_E_HIP;
     }
     for (int i = 0; i < 4; ++i) e->key[i] = key[i];
+    e->gens.clear();
+    for (const psx_context* c : e->cls) e->gens.push_back(c->gen);
   }
   if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) {
     e->err = "hipGraphLaunch failed";
     return PSX_E_HIP;
   }
-  return PSX_OK;
+  return ensemble_status(e, PSX_OK);
 }
 
 int psx_ensemble_stats(psx_ensemble* e, const double* q, double* out_dev) {
@@ -1944,12 +2000,12 @@ int psx_ensemble_stats(psx_ensemble* e, const double* q, double* out_dev) {
   for (size_t k = 0; k < e->cls.size(); ++k) {
     const int rc = psx_stats(e->cls[k], (const double*)((const double2*)q + e->cls_offset[k]),
                              out_dev + 3 * e->cls_first_slot[k]);
-    if (rc != PSX_OK) {
+    if (rc != PSX_OK && rc != PSX_E_NONFINITE) {
       e->err = e->cls[k]->err;
       return rc;
     }
   }
-  return PSX_OK;
+  return ensemble_status(e, PSX_OK);
 }
 
 }  // extern "C"

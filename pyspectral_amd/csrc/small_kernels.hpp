@@ -8,26 +8,29 @@ namespace psx {
 // dt from the CFL maximum (src/Classes.py:687-698).  part holds per-block maxima of
 // |u|/dx+|v|/dy+|w|/dz in raw (unnormalised inverse) units; one block of NT threads
 // per ensemble member (block m reads part + m*nparts, writes dt[m], viscosity
-// nu_m[m] when nu_m is set).  ksum = 1/dx^2 + 1/dy^2 + 1/dz^2.
+// nu_m[m] when nu_m is set).  ksum = 1/dx^2 + 1/dy^2 + 1/dz^2.  As numpy's amax and
+// minimum do, a NaN maximum gives dt = NaN and an infinite one dt = 0; either raises
+// the context's non-finite flag (*flag = 1, never cleared here).
 __global__ __launch_bounds__(NT) void cfl_finalize(const double* __restrict__ part, int nparts, double* dt,
                                                    double inv_sqrt_nt, double cfl, double nu_all,
-                                                   const double* __restrict__ nu_m, double ksum) {
+                                                   const double* __restrict__ nu_m, double ksum, int* flag) {
   __shared__ double sh[NT];
   part += (long long)blockIdx.x * nparts;
   const double nu = nu_m ? nu_m[blockIdx.x] : nu_all;
   double m = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += NT) m = fmax(m, part[i]);
+  for (int i = threadIdx.x; i < nparts; i += NT) m = max_nan(m, part[i]);
   sh[threadIdx.x] = m;
   __syncthreads();
   for (int s = NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = max_nan(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     const double mv = sh[0] * inv_sqrt_nt;
     double d = cfl / (mv + 1e-10);
-    if (nu > 0) d = fmin(d, 0.634 / ksum / nu * cfl / 1.35);
+    if (nu > 0 && d == d) d = fmin(d, 0.634 / ksum / nu * cfl / 1.35);
     dt[blockIdx.x] = d;
+    if (!isfinite(mv)) *flag = 1;
   }
 }
 
@@ -110,7 +113,9 @@ __global__ __launch_bounds__(NT) void stats_partial(const double2* __restrict__ 
   }
 }
 
-__global__ void stats_final(const double* __restrict__ part, int nblk, double* __restrict__ out, double scale) {
+// A NaN or infinite energy or enstrophy raises the context's non-finite flag.
+__global__ void stats_final(const double* __restrict__ part, int nblk, double* __restrict__ out, double scale,
+                            int* flag) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double e = 0.0, o = 0.0, er = 0.0;
     for (int b = 0; b < nblk; ++b) {
@@ -121,6 +126,7 @@ __global__ void stats_final(const double* __restrict__ part, int nblk, double* _
     out[0] = 0.5 * e * scale;
     out[1] = 0.5 * o * scale;
     out[2] = 0.5 * er * scale;
+    if (!isfinite(e) || !isfinite(o)) *flag = 1;
   }
 }
 

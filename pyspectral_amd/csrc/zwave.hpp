@@ -377,10 +377,10 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
           const int r = f * P + p;
           v[f] = (r & 1) ? x[r >> 1][m].y : x[r >> 1][m].x;
         }
-        mx = fmax(mx, fabs(v[0]) * a.dinv[0] + fabs(v[1]) * a.dinv[1] + fabs(v[2]) * a.dinv[2]);
+        mx = max_nan(mx, fabs(v[0]) * a.dinv[0] + fabs(v[1]) * a.dinv[1] + fabs(v[2]) * a.dinv[2]);
       }
     }
-    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    for (int off = 32; off > 0; off >>= 1) mx = max_nan(mx, __shfl_xor(mx, off));
     if (l == 0) a.red[blockIdx.x] = mx;
     return;
   } else {

@@ -80,8 +80,15 @@ class Ensemble:
 
     # ---------------------------------------------------------------- stepping
     def _check(self, rc, what):
-        if rc != 0:
+        # PSX_E_NONFINITE is a status (work enqueued; a reduction met NaN or Inf)
+        if rc != 0 and rc != _lib.PSX_E_NONFINITE:
             raise _lib.PsxError(f"{what} failed ({rc}): {self.lib.psx_ensemble_last_error(self.ens).decode()}")
+        return rc
+
+    def finite(self) -> bool:
+        """Wait for the ensemble's stream; False if any member's CFL or diagnostics
+        reduction met a NaN or an infinite value."""
+        return self._check(self.lib.psx_ensemble_status(self.ens), "psx_ensemble_status") == 0
 
     def _args(self, q, qs, dt):
         for name, x, n, dtype in (("q", q, self.size, torch.complex128), ("qs", qs, self.size, torch.complex128),

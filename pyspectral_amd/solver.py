@@ -265,6 +265,15 @@ class SpectralSolver:
         _lib.check(self.ctx, self.lib.psx_tau(self.ctx, out), "psx_tau")
         return list(out)
 
+    def finite(self) -> bool:
+        """Wait for the context's stream; False if a CFL or diagnostics reduction since
+        creation (or the last ``clear_status``) met a NaN or an infinite value."""
+        rc = self.lib.psx_status(self.ctx)
+        return _lib.check(self.ctx, rc, "psx_status") == _lib.PSX_OK
+
+    def clear_status(self) -> None:
+        _lib.check(self.ctx, self.lib.psx_clear_status(self.ctx), "psx_clear_status")
+
     def scratch_bytes(self) -> int:
         return int(self.lib.psx_scratch_bytes(self.ctx))
 

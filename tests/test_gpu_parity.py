@@ -165,10 +165,11 @@ def test_rhs_with_linear_forcing(model):
     assert _rel(r, st.compact()) < 1e-12
 
 
-@pytest.mark.parametrize("model", [0, 1, 2, "Static tau-model", 7])
+@pytest.mark.parametrize("model", MODELS)
 def test_rhs_with_rotation(model):
-    """Coriolis terms (rotate = 1, src/RHSfunctions.py:33-50): every model's RHS with a
-    rotation vector against the oracle, at a time where the t-model memory is active."""
+    """Coriolis terms (rotate = 1, src/RHSfunctions.py:33-50,653-656,760-763,875-878):
+    every model's RHS with a rotation vector against the oracle, at a time where the
+    t-model memory is active."""
     N, nu, t = 16, 0.012, 0.37
     Om = (0.3, -0.2, 0.5)
     u = _field(N, 37, nyquist_noise=True)
@@ -182,11 +183,26 @@ def test_rhs_with_rotation(model):
     s.close()
 
 
-def test_rotation_with_the_dynamic_tau_model_is_refused():
-    from pyspectral_amd._lib import PsxError
-
-    with pytest.raises(PsxError):
-        _solver(16, "Dynamic tau-model", 0.01, rotate=1, Om=(0.0, 0.0, 1.0))
+def test_dynamic_tau_model_with_rotation_steps_and_set_params():
+    """The dynamic tau-model with rotation: its dynamic coefficient and two RK4 steps
+    against the oracle, then rotation switched on by psx_set_params on a context created
+    without it (src/RHSfunctions.py:653-656,760-763,875-878)."""
+    N, nu, Om = 16, 0.01, (0.4, 0.1, -0.6)
+    u = _field(N, 41, nyquist_noise=True)
+    st = so.State(so.Grid(N, N, N), *u, nu=nu, turb_model="Dynamic tau-model", cfl=0.5, rotate=1, Om=Om,
+                  **TAU_KW)
+    s = _solver(N, "Dynamic tau-model", nu, cfl=0.5, Ct=0.2, tau_a=0.05, tau_b=0.5)
+    s.set_params(rotate=1, Om=Om)
+    q = _dev(u)
+    qs = q.clone()
+    t = 0.1
+    for _ in range(2):
+        st.t = t
+        so.advance_rk4(st, FFT)
+        s.step(q, qs, t)
+        t += st.dt
+    assert _rel(_host(q), st.compact()) < 1e-12
+    assert abs(s.tau_values()[0] - st.tau) < 1e-10 * max(abs(st.tau), 1e-30)
 
 
 def test_rhs_tgv_closed_form():

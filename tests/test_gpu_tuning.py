@@ -34,11 +34,13 @@ def test_every_kernel_shape_matches_oracle(model):
     ud = torch.from_numpy(u).cuda()
     for T in (4, 8, 16):
         for zv in (0, 1):
-            print(f"\nmodel={model} tile={T} z={zv}", flush=True)
-            s.set_tuning({"x_inverse_tile": T, "y_inverse_tile": T, "y_forward_tile": T, "z_variant": zv})
-            r = s.rhs(ud, 0.0).cpu().numpy()
-            err = np.linalg.norm(r - ref) / np.linalg.norm(ref)
-            assert err < 1e-12, (T, zv, err)
+            for order in (0, 1):
+                print(f"\nmodel={model} tile={T} z={zv} order={order}", flush=True)
+                s.set_tuning({"x_inverse_tile": T, "y_inverse_tile": T, "y_forward_tile": T, "z_variant": zv,
+                              "x_inverse_order": order, "y_inverse_order": order, "y_forward_order": order})
+                r = s.rhs(ud, 0.0).cpu().numpy()
+                err = np.linalg.norm(r - ref) / np.linalg.norm(ref)
+                assert err < 1e-12, (T, zv, order, err)
     tuned = s.autotune(ud, reps=2)
     assert set(tuned) == set(s.TUNING_KEYS)
     assert all(tuned[k] in (4, 8, 16) for k in s.TUNING_KEYS[:3]) and tuned["z_variant"] in (0, 1)
@@ -60,10 +62,11 @@ def test_chunked_pipeline_is_bitwise_identical(model):
     q0, s0 = u.clone(), u.clone()
     for _ in range(2):
         ref.step(q0, s0)
-    for n, streams in ((2, 2), (3, 2), (4, 2), (2, 1), (4, 1)):
-        print(f"\nmodel={model} chunks={n} streams={streams}", flush=True)
+    for n, streams, order in ((2, 2, 0), (3, 2, 0), (4, 2, 0), (2, 1, 0), (4, 1, 0), (2, 2, 1), (1, 0, 1)):
+        print(f"\nmodel={model} chunks={n} streams={streams} order={order}", flush=True)
         s = SpectralSolver(*shape, model, nu=0.01, cfl=0.5)
-        s.set_tuning({"yz_chunks": n, "yz_streams": streams})
+        s.set_tuning({"yz_chunks": n, "yz_streams": streams, "x_inverse_order": order, "y_inverse_order": order,
+                      "y_forward_order": order})
         assert torch.equal(s.rhs(u, 0.0), r0)
         q, qs = u.clone(), u.clone()
         for _ in range(2):
