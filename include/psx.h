@@ -160,6 +160,21 @@ int psx_step_graph(psx_context* ctx, double* q, double* qs, double* dt_dev, doub
  * (src/Classes.py:652-676,733-744): out_dev[0..2] = E, Omega, E_resolved. */
 int psx_stats(psx_context* ctx, const double* uhat, double* out_dev);
 
+/* No counterpart in the reference: psx_stats for callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops, harnesses that forbid hipMalloc),
+ * without a context.  uhat holds `members` states (members, 3, N1, N2, N3/2+1) of an
+ * N1 x N2 x N3 grid with box lengths L1..L3 and sharp cutoff kc; out_dev[3m..3m+2]
+ * receives member m's E, Omega, E_resolved, bitwise as psx_stats gives them.  Runs on
+ * `stream` (a hipStream_t, NULL = default) of the current device and calls no allocator:
+ * its reduction partials live in a static device array of 1024 x 3 doubles (24 KiB),
+ * which each GPU holds until the process ends.  Any grid size is accepted (the partials
+ * are capped at 1024 blocks; every block writes its own slot before the final sum reads
+ * it, so nothing needs clearing).  Calls that overlap on one GPU share that array and
+ * race: order them on one stream.  Returns PSX_OK, PSX_E_ARG or PSX_E_HIP; there is no
+ * context, hence no message and no non-finite flag (a NaN state gives NaN values). */
+int psx_stats_noalloc(const double* uhat, int members, int n1, int n2, int n3, double L1, double L2, double L3,
+                      int kc, double* out_dev, void* stream);
+
 /* computeSpectrum / computeSpectrum_resolved (src/Classes.py:762-797,825-844):
  * spec_dev[(kmax+1) x 3], kmax from psx_spectrum_bins(). */
 int psx_spectrum_bins(const psx_context* ctx);

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <array>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1359,6 +1360,62 @@ int psx_stats(psx_context* c, const double* uhat, double* out_dev) {
       check(c, hipGetLastError(), "stats_final");
     }
   }));
+}
+
+// ---- diagnostics without allocation ------------------------------------------------
+// The reduction partials of psx_stats_noalloc live in this file-scope device array, one
+// copy per GPU for the life of the process.  Host code never passes `scratch` itself (in
+// host code the name is a stand-in, not a device address): the address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+namespace {
+constexpr int kNoallocParts = 1024;  // partial sums per reduction (3 doubles each)
+constexpr int kNoallocMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) double scratch[3 * kNoallocParts];
+
+namespace {
+double* noalloc_ptr[kNoallocMaxDevices] = {};
+std::mutex noalloc_mu;
+
+// The device address of this GPU's scratch array (nullptr on failure).
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kNoallocMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_ptr[device]) {
+    double* p = nullptr;
+    if (hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    noalloc_ptr[device] = p;
+  }
+  return noalloc_ptr[device];
+}
+}  // namespace
+
+extern "C" int psx_stats_noalloc(const double* uhat, int members, int n1, int n2, int n3, double L1, double L2,
+                                 double L3, int kc, double* out_dev, void* stream) {
+  if (!uhat || !out_dev || members < 1) return PSX_E_ARG;
+  if (n1 < 2 || n2 < 2 || n3 < 2 || (n1 % 2) || (n2 % 2) || (n3 % 2)) return PSX_E_ARG;
+  if (!(L1 > 0 && L2 > 0 && L3 > 0)) return PSX_E_ARG;
+  const int nc = n3 / 2 + 1;
+  const long long total = (long long)n1 * n2 * nc;
+  // the grid-stride reduction covers any grid with at most kNoallocParts blocks, so the
+  // partials never exceed the array
+  const int nblk = (int)std::min<long long>((total + NT - 1) / NT, kNoallocParts);
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return PSX_E_HIP;
+  double* part = noalloc_scratch(device);
+  if (!part) return PSX_E_HIP;
+  const hipStream_t s = (hipStream_t)stream;
+  const double Nt = (double)n1 * n2 * n3;
+  for (int m = 0; m < members; ++m) {
+    const double2* u = (const double2*)uhat + (long long)m * 3 * total;
+    hipLaunchKernelGGL(stats_partial, dim3(nblk), dim3(NT), 0, s, u, part, n1, n2, n3, nc, 2.0 * kPi / L1,
+                       2.0 * kPi / L2, 2.0 * kPi / L3, kc);
+    if (hipGetLastError() != hipSuccess) return PSX_E_HIP;
+    hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, s, (const double*)part, nblk, out_dev + 3 * m, 1.0 / Nt,
+                       (int*)nullptr);
+    if (hipGetLastError() != hipSuccess) return PSX_E_HIP;
+  }
+  return PSX_OK;
 }
 
 namespace {

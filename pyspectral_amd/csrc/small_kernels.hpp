@@ -113,7 +113,8 @@ __global__ __launch_bounds__(NT) void stats_partial(const double2* __restrict__ 
   }
 }
 
-// A NaN or infinite energy or enstrophy raises the context's non-finite flag.
+// A NaN or infinite energy or enstrophy raises the context's non-finite flag (when
+// flag is set).
 __global__ void stats_final(const double* __restrict__ part, int nblk, double* __restrict__ out, double scale,
                             int* flag) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -126,7 +127,7 @@ __global__ void stats_final(const double* __restrict__ part, int nblk, double* _
     out[0] = 0.5 * e * scale;
     out[1] = 0.5 * o * scale;
     out[2] = 0.5 * er * scale;
-    if (!isfinite(e) || !isfinite(o)) *flag = 1;
+    if (flag && (!isfinite(e) || !isfinite(o))) *flag = 1;
   }
 }
 
