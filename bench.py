@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
 C16 = 16               # bytes per complex128
 
 
-PREFIX = {"smagorinsky": "smag", "dns": "dns", "learned": "learned"}
+PREFIX = {"smagorinsky": "smag", "dns": "dns", "learned": "learned", "tmodel": "tmodel"}
 MLP_FLOPS = 2 * (9 * 64 + 64 * 64 + 64 * 64 + 64 * 6)  # per point, algorithmic
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 
@@ -117,27 +117,45 @@ def copy_rate(torch, mib=1024, reps=10):
     return rate
 
 
-def cpu_baseline(N, closure, nu, cfl, steps, seed):
-    """The oracle (numpy restatement of the reference's src/ path, scipy.fft with one
-    worker per available core) advancing the same field; bounded sample of `steps`."""
-    import numpy as np
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(N, closure, nu, cfl, steps, seed, warmup=1):
+    """The oracle (numpy restatement of the reference's src/ path, scipy.fft with one
+    worker per available core) advancing the same field: `warmup` untimed steps, then
+    `steps` timed ones (time.perf_counter)."""
     from oracle import spectral_oracle as so
     from pyspectral_amd import cases
 
     cores = so.default_workers()
     uh = cases.decaying_isotropic(N, seed=seed)
     g = so.Grid(N, N, N)
-    st = so.State(g, *uh, nu=nu, turb_model=1 if closure == "smagorinsky" else 0, cfl=cfl)
+    model = {"smagorinsky": 1, "dns": 0, "tmodel": 2}[closure]
+    st = so.State(g, *uh, nu=nu, turb_model=model, cfl=cfl, Ct=0.1)
     fft = so.FFTBackend(workers=cores)
+    t = 0.0
+    for _ in range(warmup):
+        st.t = t
+        so.advance_rk4(st, fft)
+        t += st.dt
     t0 = time.perf_counter()
     for _ in range(steps):
+        st.t = t
         so.advance_rk4(st, fft)
+        t += st.dt
     el = time.perf_counter() - t0
-    del np
-    return {"value": N ** 3 * steps / el / 1e6, "unit": "Mpts*RK4-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} RK4 steps of the same {N}^3 {closure} case (oracle, {fft.name}), "
-                      f"{el:.1f} s wall", "seconds": el}
+    return {"value": round(N ** 3 * steps / el / 1e6, 4), "unit": "Mpts*RK4-steps/s", "cores": cores,
+            "kind": "port", "cpu": cpu_model(), "fft": fft.name,
+            "sample": f"{warmup} warm-up + {steps} timed RK4 steps of the same {N}^3 {closure} case "
+                      f"(the oracle, {fft.name} with {cores} workers), {el:.1f} s timed",
+            "seconds": round(el, 2)}
 
 
 def bench_ensemble(args):
@@ -175,22 +193,29 @@ def bench_ensemble(args):
     pts = sum(n ** 3 for n in sizes)
     value = pts * args.steps / el / 1e6
     E = ens.stats(q).cpu().numpy()
+    finite = ens.finite()
+    ens.close()
     # the same members, one solver each, eager launches
-    singles = [SpectralSolver(n, n, n, 1, nu=nu, cfl=0.5) for n, nu in zip(sizes, nus)]
-    uq = [ens.member(q, i).clone() for i in range(len(sizes))]
-    us = [x.clone() for x in uq]
-    ns = 10
-    for i, s in enumerate(singles):
-        s.step(uq[i], us[i])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(ns):
+    ns = getattr(args, "unbatched_steps", 10)
+    unbatched = None
+    if ns > 0:
+        singles = [SpectralSolver(n, n, n, 1, nu=nu, cfl=0.5) for n, nu in zip(sizes, nus)]
+        uq = [torch.from_numpy(cases.decaying_isotropic(n, seed=args.seed + i, k0=k0s[i])).cuda()
+              for i, n in enumerate(sizes)]
+        us = [x.clone() for x in uq]
         for i, s in enumerate(singles):
             s.step(uq[i], us[i])
-    torch.cuda.synchronize()
-    el1 = time.perf_counter() - t1
-    for s in singles:
-        s.close()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(ns):
+            for i, s in enumerate(singles):
+                s.step(uq[i], us[i])
+        torch.cuda.synchronize()
+        el1 = time.perf_counter() - t1
+        for s in singles:
+            s.close()
+        unbatched = {"value": round(pts * ns / el1 / 1e6, 2), "ms_per_step": round(el1 / ns * 1e3, 3),
+                     "how": "the same 96 members, one SpectralSolver.step each, eager launches"}
     bytes_step = sum(survey_bytes_per_step(n, "smagorinsky") for n in sizes)
     return {
         "metric": "Mpts*RK4-steps/s, ensemble of 96 LES members (32x 32^3, 48^3, 64^3)",
@@ -204,9 +229,8 @@ def bench_ensemble(args):
         "step_roofline": {"model_bytes_per_step": int(bytes_step),
                           "achieved": round(bytes_step / (el / args.steps) / 1e9, 1), "unit": "GB/s",
                           "frac": round(bytes_step / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)},
-        "unbatched": {"value": round(pts * ns / el1 / 1e6, 2), "ms_per_step": round(el1 / ns * 1e3, 3),
-                      "how": "the same 96 members, one SpectralSolver.step each, eager launches"},
-        "stats_after": {"E_mean": float(E[:, 0].mean()), "finite": bool(np.isfinite(E).all())},
+        "unbatched": unbatched,
+        "stats_after": {"E_mean": float(E[:, 0].mean()), "finite": bool(finite and np.isfinite(E).all())},
     }
 
 
@@ -520,15 +544,142 @@ def attach_trained_closure(solver, q, seed, steps=300):
             "bound": round(bound, 3)}
 
 
+def kernel_bytes_tmodel(N):
+    """Algorithmic bytes of one t-model stage (src/RHSfunctions.py:240-363) per kernel
+    class, summed over that class's launches in the stage, on the 2x grid P = 2N with
+    H = N/2 carried k3 modes (16 B per complex value, every input read once and every
+    output written once): x-inverse 3 state fields -> X (3 P N H); y-inverse X -> Y1
+    (3 P P H) and, in round 2, PL_q (3 P P N -> 3 P P N); z pass 3 P P H -> 6 P P N
+    products (round 1) and Y1 + PL_q -> 9 P P H (round 2); y-forward 6 P P N -> 6 P P N
+    and 9 P P H -> 9 P N H; the PL pass reads the 6 products and u, writes PL (base grid)
+    and the x-inverse of PL_q (3 P P N); the epilogue reads 9 P N H and PL, u, q0 and
+    writes the state.  Returns {class: (launches, bytes)}."""
+    P, H, Nc = 2 * N, N // 2, N // 2 + 1
+    st = N * N * Nc
+    c = C16
+    return {
+        "x_inverse": (1, c * (3 * N * N * H + 3 * P * N * H)),
+        "y_inverse": (2, c * (3 * P * N * H + 3 * P * P * H + 6 * P * P * N)),
+        "z_pass": (2, c * (3 * P * P * H + 6 * P * P * N + 3 * P * P * H + 3 * P * P * N + 9 * P * P * H)),
+        "y_forward": (2, c * (12 * P * P * N + 9 * P * P * H + 9 * P * N * H)),
+        "other": (1, c * (6 * P * P * N + 3 * P * P * N + 6 * st)),
+        "x_forward_epilogue": (1, c * (9 * P * N * H + 12 * st)),
+        # the CFL passes run once per step (a quarter launch per stage)
+        "cfl_x": (0.25, 0.25 * 6 * st * c),
+        "cfl_y": (0.25, 0.25 * 6 * st * c),
+        "cfl_z": (0.25, 0.25 * 3 * st * c),
+    }
+
+
+# the step model of the t-model (two 2x-grid rounds per stage): 3698 N^3 bytes per
+# stage in the 3-pass pencil convention, 14 792 bytes per point and RK4 step
+TMODEL_BYTES_PER_POINT_STEP = 14792
+
+
+def kernel_table(solver, q, qs, steps, bytes_of, t=0.0):
+    """Per-kernel-class launch counts, mean durations (HIP events on the library's
+    stream, every kernel in order) and GB/s of algorithmic bytes.  bytes_of(name, count)
+    gives the bytes moved by `count` launches of a class."""
+    import ctypes as C
+
+    from pyspectral_amd import _lib
+
+    lib = solver.lib
+    _lib.check(solver.ctx, lib.psx_timing_begin(solver.ctx), "psx_timing_begin")
+    for _ in range(steps):
+        solver.step(q, qs, t)
+    ms = (C.c_double * len(_lib.KERNEL_NAMES))()
+    cnt = (C.c_int * len(_lib.KERNEL_NAMES))()
+    _lib.check(solver.ctx, lib.psx_timing_end(solver.ctx, ms, cnt), "psx_timing_end")
+    kernels = {}
+    for i, name in enumerate(_lib.KERNEL_NAMES):
+        if cnt[i] == 0:
+            continue
+        entry = {"launches": cnt[i], "avg_us": round(ms[i] / cnt[i] * 1e3, 2),
+                 "share": round(ms[i] / max(sum(ms), 1e-30), 4)}
+        b = bytes_of(name, cnt[i])
+        if b:
+            entry["GBs"] = round(b / (ms[i] * 1e-3) / 1e9, 1)
+            entry["bytes_per_launch"] = int(b / cnt[i])
+        kernels[name] = entry
+    return kernels
+
+
+TRAFFIC_NOTE = ("HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary "
+                "(copy-calibrated), not measured in this run")
+
+
+def bench_tmodel(args, uh, copy_gbs):
+    """The t-model run of the 128^3 decaying LES (Ct = 0.1, 2x padding to 256^3):
+    throughput, kernel table and roofline (a sub-object of the JSON line)."""
+    import torch
+
+    from pyspectral_amd.solver import SpectralSolver
+
+    N, nu, cfl = args.N, 1e-3, 0.5
+    s = SpectralSolver(N, N, N, 2, nu=nu, cfl=cfl, Ct=0.1)
+    q = torch.from_numpy(uh).to("cuda")
+    qs = q.clone()
+    t = 0.0
+    for _ in range(args.tmodel_warmup):
+        t += float(s.step(q, qs, t).item())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.tmodel_steps):
+        s.step(q, qs, t)  # the step-start time enters only the closure coefficient Ct t
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_step = el / args.tmodel_steps * 1e3
+    kb = kernel_bytes_tmodel(N)
+    stages = 4 * args.event_steps_tmodel
+
+    def bytes_of(name, cnt):
+        if name not in kb:
+            return None
+        per, b = kb[name]
+        return b * cnt / per
+
+    kernels = kernel_table(s, q, qs, args.event_steps_tmodel, bytes_of, t)
+    dom = max(kernels, key=lambda k: kernels[k]["share"])
+    ach = kernels[dom]["GBs"]
+    sb = TMODEL_BYTES_PER_POINT_STEP * N ** 3
+    E = s.stats(qs).cpu().tolist()
+    s.close()
+    out = {"value": round(N ** 3 * args.tmodel_steps / el / 1e6, 2), "unit": "Mpts*RK4-steps/s",
+           "steps": args.tmodel_steps, "warmup": args.tmodel_warmup, "ms_per_step": round(ms_step, 4),
+           "config": {"workload": f"decaying isotropic turbulence LES, t-model Ct=0.1, {N}^3 modes ({2 * N}^3 "
+                                  f"2x-padded), nu={nu}, CFL={cfl}", "grid": N, "closure": "tmodel"},
+           "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ach / HBM_PEAK_GBS, 4), "frac_of_copy": round(ach / copy_gbs, 4),
+                        "traffic": measured_traffic(dom, N, "tmodel"), "traffic_source": TRAFFIC_NOTE,
+                        "bytes_per_launch": kernels[dom].get("bytes_per_launch")},
+           "step_roofline": {"model_bytes_per_step": int(sb), "bytes_per_point_step": TMODEL_BYTES_PER_POINT_STEP,
+                             "achieved": round(sb / (ms_step * 1e-3) / 1e9, 1), "unit": "GB/s",
+                             "frac": round(sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "kernels": kernels, "stats_after": {"E": E[0], "Omega": E[1]},
+           "stages_timed_per_kernel": stages}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=128)
-    ap.add_argument("--closure", default="smagorinsky", choices=["smagorinsky", "dns", "learned"])
+    ap.add_argument("--closure", default="smagorinsky", choices=["smagorinsky", "dns", "learned", "tmodel"],
+                    help="tmodel: only the t-model run (its own line, e.g. for profiling)")
     ap.add_argument("--event-steps", type=int, default=50, help="steps timed per kernel with HIP events")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="timed oracle steps for the CPU baseline, after one warm-up step (0: skip)")
+    ap.add_argument("--tmodel-steps", type=int, default=100,
+                    help="timed steps of the t-model run reported beside the Smagorinsky one (0: skip)")
+    ap.add_argument("--tmodel-warmup", type=int, default=5)
+    ap.add_argument("--event-steps-tmodel", type=int, default=10)
+    ap.add_argument("--tmodel-cpu-steps", type=int, default=5,
+                    help="timed oracle t-model steps for its CPU baseline, after one warm-up step (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=50,
+                    help="timed steps of the 96-member ensemble reported beside the LES (0: skip)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel shapes")
     ap.add_argument("--tuning", default="",
@@ -557,11 +708,9 @@ def main():
               "of it (see DESIGN.md for how the grid would shard).", file=sys.stderr)
         sys.exit(2)
 
-    import ctypes as C
-
     import torch
 
-    from pyspectral_amd import _lib, cases
+    from pyspectral_amd import cases
     from pyspectral_amd.solver import SpectralSolver
 
     if args.workload == "ensemble":
@@ -580,8 +729,17 @@ def main():
         del a, b
 
     N, nu, cfl = args.N, 1e-3, 0.5
-    model = {"smagorinsky": 1, "dns": 0, "learned": "learned"}[args.closure]
     uh = cases.decaying_isotropic(N, seed=args.seed)
+    if args.closure == "tmodel":
+        out = bench_tmodel(args, uh, copy_rate(torch))
+        out.update({"metric": "Mpts*RK4-steps/s at 128^3 FP64 LES (t-model)", "n_gpus": 1,
+                    "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                    "data": "synthetic (seeded random-phase isotropic field)"})
+        if args.tmodel_cpu_steps > 0:
+            out["cpu_baseline"] = cpu_baseline(N, "tmodel", nu, cfl, args.tmodel_cpu_steps, args.seed)
+        print(json.dumps(out))
+        return
+    model = {"smagorinsky": 1, "dns": 0, "learned": "learned"}[args.closure]
     solver = SpectralSolver(N, N, N, model, nu=nu, cfl=cfl, Cs=0.16)
     q = torch.from_numpy(uh).to("cuda")
     qs = q.clone()
@@ -606,37 +764,22 @@ def main():
     value = N ** 3 * args.steps / el / 1e6
 
     # per-kernel durations from HIP events on the library's stream
-    lib = solver.lib
-    _lib.check(solver.ctx, lib.psx_timing_begin(solver.ctx), "psx_timing_begin")
-    for _ in range(args.event_steps):
-        solver.step(q, qs)
-    ms = (C.c_double * len(_lib.KERNEL_NAMES))()
-    cnt = (C.c_int * len(_lib.KERNEL_NAMES))()
-    _lib.check(solver.ctx, lib.psx_timing_end(solver.ctx, ms, cnt), "psx_timing_end")
     fused = args.closure == "learned" and fused_closure(N, tuning)
     kb = kernel_bytes(N, args.closure, fused)
     # the chunked y/z pipeline splits each of those passes into yz_chunks launches
     nch = max(1, int(tuning.get("yz_chunks", 0)))
     for name in ("y_inverse", "z_pass", "y_forward"):
         kb[name] = kb[name] // nch
-    kernels = {}
-    for i, name in enumerate(_lib.KERNEL_NAMES):
-        if cnt[i] == 0:
-            continue
-        avg_ms = ms[i] / cnt[i]
-        entry = {"launches": cnt[i], "avg_us": round(avg_ms * 1e3, 2),
-                 "share": round(ms[i] / max(sum(ms), 1e-30), 4)}
-        if name in kb:
-            entry["GBs"] = round(kb[name] / (avg_ms * 1e-3) / 1e9, 1)
+    kernels = kernel_table(solver, q, qs, args.event_steps, lambda name, cnt: kb[name] * cnt if name in kb else None)
+    for name, entry in kernels.items():
         if name == "closure_network" or (name == "z_pass" and fused):
-            entry["TFLOPs"] = round(MLP_FLOPS * (3 * N // 2) ** 3 / (avg_ms * 1e-3) / 1e12, 2)
-        kernels[name] = entry
+            entry["TFLOPs"] = round(MLP_FLOPS * (3 * N // 2) ** 3 / (entry["avg_us"] * 1e-6) / 1e12, 2)
     dom = max(kernels, key=lambda k: kernels[k]["share"])
     ach = kb[dom] / (kernels[dom]["avg_us"] * 1e-6) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom, N, args.closure) if nch == 1 else None,
-            "bytes_per_launch": kb[dom]}
+            "traffic_source": TRAFFIC_NOTE, "bytes_per_launch": kb[dom]}
     copy_gbs = copy_rate(torch)
     roof["copy_GBs"] = round(copy_gbs, 1)
     roof["frac_of_copy"] = round(ach / copy_gbs, 4)
@@ -646,9 +789,11 @@ def main():
         tf = kernels[dom]["TFLOPs"]
         roof = {"bound": "mfma", "kernel": dom, "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tf / BF16_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, N, args.closure),
-                "flops_per_launch": MLP_FLOPS * (3 * N // 2) ** 3}
+                "traffic_source": TRAFFIC_NOTE, "flops_per_launch": MLP_FLOPS * (3 * N // 2) ** 3}
     E = solver.stats(qs).cpu().tolist()
-    finite = all(map(lambda x: x == x and abs(x) < 1e300, E))
+    finite = solver.finite()
+    solver.close()
+    del q, qs
 
     sb = survey_bytes_per_step(N, args.closure)
     out = {
@@ -677,9 +822,20 @@ def main():
     }
     if closure_info is not None:
         out["config"]["learned_closure"] = closure_info
+    if args.closure == "smagorinsky" and args.tmodel_steps > 0:
+        out["tmodel"] = bench_tmodel(args, uh, copy_gbs)
+    if args.closure == "smagorinsky" and args.ensemble_steps > 0:
+        ea = argparse.Namespace(seed=args.seed, steps=args.ensemble_steps, warmup=5, unbatched_steps=0)
+        ens = bench_ensemble(ea)
+        out["ensemble"] = {k: ens[k] for k in ("value", "unit", "steps", "warmup", "ms_per_step", "config",
+                                              "step_roofline", "stats_after")}
     if args.cpu_steps > 0 and args.closure != "learned":
         out["cpu_baseline"] = cpu_baseline(N, args.closure, nu, cfl, args.cpu_steps, args.seed)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    if "tmodel" in out and args.tmodel_cpu_steps > 0:
+        tcb = cpu_baseline(N, "tmodel", nu, cfl, args.tmodel_cpu_steps, args.seed)
+        out["tmodel"]["cpu_baseline"] = tcb
+        out["tmodel"]["speedup_vs_cpu"] = round(out["tmodel"]["value"] / tcb["value"], 1)
     print(json.dumps(out))
 
 

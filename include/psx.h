@@ -279,6 +279,13 @@ int psx_set_closure(psx_context* ctx, psx_mlp* net, double bound, int flags);
  * Smagorinsky, DSMAG and learned models. */
 int psx_physical(psx_context* ctx, const double* fhat, int nfields, double* out);
 
+/* The inverse of psx_physical (no counterpart as one call in the reference, which calls
+ * myFFT.fft_obj on padded grids, src/Classes.py:608-613, and FFTFromFile's plans in
+ * post-processing, src/postProcessingFunctions.py:67-108): fhat (nfields, N1, N2, N3/2+1)
+ * = rfftn(phys) / sqrt(N1 N2 N3) of nfields real grid fields phys (nfields, N1, N2, N3),
+ * every spectral entry kept.  Single-member contexts of any model. */
+int psx_spectral(psx_context* ctx, const double* phys, int nfields, double* fhat);
+
 /* The nine velocity gradients du_i/dx_j = irfftn(i k_j uhat_i) sqrt(Nt) at the grid
  * points, as float32 out[(x N2 + y) N3 + z][3 i + j] (the learned closure's input
  * layout).  Single-member contexts, as psx_physical. */

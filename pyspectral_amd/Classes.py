@@ -103,6 +103,7 @@ class variables:
         self.turb_model = turb_model
         self.rotate, self.Om1, self.Om2, self.Om3 = int(rotate), float(Om1), float(Om2), float(Om3)
         self.t = float(t)
+        self.t_state = self.t  # the time of the state in Qd (advanced by each step)
         self.kc = grid.N1 // 2 - 1  # np.amax(grid.k1) for L1 = 2 pi (src/Classes.py:16)
         self.dt = float(dt)
         self.nu = float(nu)

@@ -14,6 +14,7 @@ import sys
 import time
 
 import numpy as np
+import torch
 
 from .Classes import FFTclass, gridclass, utilitiesClass, variables
 
@@ -26,6 +27,7 @@ def advanceQ_RK4(main, grid, myFFT, utilities):
     utilities.preAdvanceQ_hook(main, grid, myFFT)
     dt = main.solver.step(main.Qd, main.Ud, main.t, main._dt_dev)
     main.dt = float(dt.item())
+    main.t_state = main.t + main.dt  # the time of the new state in main.Qd
     utilities.postAdvanceQ_hook(main, grid, myFFT)
 
 
@@ -34,10 +36,14 @@ _DEFAULTS = dict(weave=0, dim=3, rotate=0, Om1=0, Om2=0, Om3=0, Ct=-10, dt0=-10,
                  L3=2.0 * np.pi)
 
 
-def run_case(g: dict, outdir: str = "3DSolution", verbose: bool = True, max_steps: int | None = None):
+def run_case(g: dict, outdir: str = "3DSolution", verbose: bool = True, max_steps: int | None = None,
+             resume: str | None = None):
     """Run a reference-style case from its globals ``g`` (needs N1, N2, N3, kc, nu, t,
     et, dt, save_freq, turb_model, x, y, z, uhat, vhat, what; optional savehook,
-    cfl, nthreads and the defaults above).  Returns the stats dict written to
+    cfl, nthreads and the defaults above).  ``max_steps`` bounds the steps of this call.
+    ``resume``: an ``npsol`` file written by ``postProcessingFunctions.save_npsol``
+    after a step; the run continues from its state, stale velocity, time and iteration
+    exactly as the run that wrote it would have.  Returns the stats dict written to
     ``stats.npz``."""
     g = dict(g)
     for k, v in _DEFAULTS.items():
@@ -56,16 +62,26 @@ def run_case(g: dict, outdir: str = "3DSolution", verbose: bool = True, max_step
     np.savez(os.path.join(outdir, "runinfo"), turb_model=g["turb_model"], dt=g["dt"], save_freq=g["save_freq"],
              nu=g["nu"])
     t0 = time.time()
-    main.U2Q()
     t = float(g["t"])
+    iteration = 0
+    if resume is None:
+        main.U2Q()
+    else:
+        with np.load(resume) as f:
+            if "Qhat" not in f.files:
+                raise ValueError(f"{resume} holds no Qhat: it was not written for a restart")
+            main.Qd.copy_(torch.from_numpy(np.ascontiguousarray(f["Qhat"])))
+            main.Ud.copy_(torch.from_numpy(np.ascontiguousarray(np.stack([f["uhat"], f["vhat"], f["what"]]))))
+            t = float(f["t_Qhat"])
+            iteration = int(f["iteration"]) + 1  # the file was written after that step
     et = float(g["et"])
     save_freq = int(g["save_freq"])
     savehook = g.get("savehook")
     s = utilities._stats(main)
     Energy, Enstrophy, Energy_resolved, t_hist = [s[0]], [s[1]], [s[2]], [t]
-    iteration = 0
+    first = iteration
     while t <= et:
-        if max_steps is not None and iteration >= max_steps:
+        if max_steps is not None and iteration - first >= max_steps:
             break
         main.t = t
         main.iteration = iteration

@@ -3,9 +3,6 @@
 
 namespace psx {
 
-#define PSX_LENGTHS(X) \
-  X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768)
-
 bool length_supported(int L) {
   switch (L) {
 #define PSX_CASE(LL) case LL:

@@ -34,7 +34,16 @@ template <int MODE> constexpr int z_NT(int L) {
   return nt < 64 ? 64 : (nt > 256 ? 256 : nt);
 }
 
+// Every transform length the kernels are instantiated for.
+#define PSX_LENGTHS(X) \
+  X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768)
+
 bool length_supported(int L);
+
+// r2c along z (the contiguous axis) of nfields real fields phys[f*in_sf + pos*L + z]
+// into out[f*out_sf + pos*(L/2+1) + k], k = 0..L/2, unnormalised (k_spectral.hip)
+hipError_t launch_zreal_forward(int L, const double* phys, long long in_sf, double2* out, long long out_sf, int npos,
+                                int nfields, const double2* tw, hipStream_t s);
 
 // T: tile width (0 = the length's default pencil_T); pencil_tile(L, T) gives the width
 // used, or -1 when the width is not available for that length

@@ -1355,7 +1355,7 @@ int psx_stats(psx_context* c, const double* uhat, double* out_dev) {
       hipLaunchKernelGGL(stats_partial, dim3(nblk), dim3(NT), 0, c->stream, u, c->part, c->N1, c->N2, c->N3, c->Nc,
                          2.0 * kPi / c->p.L1, 2.0 * kPi / c->p.L2, 2.0 * kPi / c->p.L3, c->p.kc);
       check(c, hipGetLastError(), "stats_partial");
-      hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, c->stream, c->part, nblk, out_dev + 3 * m, 1.0 / Nt,
+      hipLaunchKernelGGL(stats_final, dim3(1), dim3(NT), 0, c->stream, c->part, nblk, out_dev + 3 * m, 1.0 / Nt,
                          c->nf_dev);
       check(c, hipGetLastError(), "stats_final");
     }
@@ -1411,7 +1411,7 @@ extern "C" int psx_stats_noalloc(const double* uhat, int members, int n1, int n2
     hipLaunchKernelGGL(stats_partial, dim3(nblk), dim3(NT), 0, s, u, part, n1, n2, n3, nc, 2.0 * kPi / L1,
                        2.0 * kPi / L2, 2.0 * kPi / L3, kc);
     if (hipGetLastError() != hipSuccess) return PSX_E_HIP;
-    hipLaunchKernelGGL(stats_final, dim3(1), dim3(1), 0, s, (const double*)part, nblk, out_dev + 3 * m, 1.0 / Nt,
+    hipLaunchKernelGGL(stats_final, dim3(1), dim3(NT), 0, s, (const double*)part, nblk, out_dev + 3 * m, 1.0 / Nt,
                        (int*)nullptr);
     if (hipGetLastError() != hipSuccess) return PSX_E_HIP;
   }
@@ -1582,6 +1582,61 @@ int psx_physical(psx_context* c, const double* fhat, int nfields, double* out) {
       z.nphys = nj;
       TimeScope ts(c, K_OTHER);
       check(c, launch_zpass(c->N3, Z_PHYS, z, tw, c->stream, 0), "z_pass(physical)");
+    }
+  });
+}
+
+int psx_spectral(psx_context* c, const double* phys, int nfields, double* fhat) {
+  return guarded(c, [&] {
+    if (c->members > 1) fail(c, PSX_E_UNSUPPORTED, "one instance per call: use a single-member context");
+    if (!phys || !fhat || nfields < 1) fail(c, PSX_E_ARG, "null pointer or no fields");
+    const int N1 = c->N1, N2 = c->N2, N3 = c->N3, Nc = c->Nc;
+    const long long field = (long long)N1 * N2 * Nc, grid = (long long)N1 * N2 * N3;
+    const double2* tw3 = twiddles(c, N3);
+    const double2* tw2 = twiddles(c, N2);
+    const double2* tw1 = twiddles(c, N1);
+    for (int f0 = 0; f0 < nfields; f0 += 3) {
+      const int nj = std::min(3, nfields - f0);
+      {
+        TimeScope ts(c, K_OTHER);
+        check(c, launch_zreal_forward(N3, phys + f0 * grid, grid, c->bufA, field, N1 * N2, nj, tw3, c->stream),
+              "z_forward(real)");
+      }
+      // y then x, c2c forward on the N grid (every index kept)
+      PencilArgs a = pencil_base();
+      a.in = c->bufA;
+      a.out = c->bufB;
+      a.in_sf = a.out_sf = field;
+      a.in_sa = a.out_sa = (long long)N2 * Nc;
+      a.in_sl = a.out_sl = Nc;
+      a.A = N1;
+      a.C = Nc;
+      a.Nspec = N2;
+      a.keep_nyq = 1;
+      for (int j = 0; j < nj; ++j) a.job_in[j] = a.job_plain[j] = j;
+      set_members(c, a, nj);
+      {
+        TimeScope ts(c, K_OTHER);
+        check(c, launch_pencil_forward(N2, a, tw2, nj, c->stream), "y_forward(spectral)");
+      }
+      PencilArgs b = pencil_base();
+      b.in = c->bufB;
+      b.out = (double2*)fhat + f0 * field;
+      b.in_sf = b.out_sf = field;
+      b.in_sa = b.out_sa = Nc;
+      b.in_sl = b.out_sl = (long long)N2 * Nc;
+      b.A = N2;
+      b.C = Nc;
+      b.Nspec = N1;
+      b.keep_nyq = 1;
+      b.scale = 1.0 / std::sqrt((double)N1 * N2 * N3);  // uhat = rfftn(u) / sqrt(Nt)
+      for (int j = 0; j < nj; ++j) b.job_in[j] = b.job_plain[j] = j;
+      b.jpm = nj;
+      b.in_ms = b.out_ms = 0;
+      {
+        TimeScope ts(c, K_OTHER);
+        check(c, launch_pencil_forward(N1, b, tw1, nj, c->stream), "x_forward(spectral)");
+      }
     }
   });
 }

@@ -114,20 +114,34 @@ __global__ __launch_bounds__(NT) void stats_partial(const double2* __restrict__ 
 }
 
 // A NaN or infinite energy or enstrophy raises the context's non-finite flag (when
-// flag is set).
-__global__ void stats_final(const double* __restrict__ part, int nblk, double* __restrict__ out, double scale,
-                            int* flag) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double e = 0.0, o = 0.0, er = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-      e += part[3 * b];
-      o += part[3 * b + 1];
-      er += part[3 * b + 2];
+// flag is set).  One block of NT threads: thread i sums partials i, i + NT, ... in
+// order, then a fixed tree, so the result is bitwise reproducible.
+__global__ __launch_bounds__(NT) void stats_final(const double* __restrict__ part, int nblk,
+                                                  double* __restrict__ out, double scale, int* flag) {
+  __shared__ double sh[3][NT];
+  double e = 0.0, o = 0.0, er = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += NT) {
+    e += part[3 * b];
+    o += part[3 * b + 1];
+    er += part[3 * b + 2];
+  }
+  sh[0][threadIdx.x] = e;
+  sh[1][threadIdx.x] = o;
+  sh[2][threadIdx.x] = er;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+      sh[2][threadIdx.x] += sh[2][threadIdx.x + s];
     }
-    out[0] = 0.5 * e * scale;
-    out[1] = 0.5 * o * scale;
-    out[2] = 0.5 * er * scale;
-    if (flag && (!isfinite(e) || !isfinite(o))) *flag = 1;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = 0.5 * sh[0][0] * scale;
+    out[1] = 0.5 * sh[1][0] * scale;
+    out[2] = 0.5 * sh[2][0] * scale;
+    if (flag && (!isfinite(sh[0][0]) || !isfinite(sh[1][0]))) *flag = 1;
   }
 }
 

@@ -212,6 +212,19 @@ class SpectralSolver:
         _lib.check(self.ctx, self.lib.psx_physical(self.ctx, _ptr(fhat), int(nf), _ptr(out)), "psx_physical")
         return out
 
+    def spectral(self, phys: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """rfftn(phys) / sqrt(Nt) of (nf, N1, N2, N3) real fields: (nf, N1, N2, N3/2+1) complex128
+        (the inverse of ``physical``)."""
+        nf = phys.shape[0]
+        if phys.dtype != torch.float64 or tuple(phys.shape[1:]) != (self.N1, self.N2, self.N3) \
+                or not phys.is_contiguous():
+            raise ValueError(f"phys must be a contiguous float64 (nf, {self.N1}, {self.N2}, {self.N3}) tensor")
+        out = torch.empty((nf,) + self.shape[1:], dtype=torch.complex128, device=self.device) \
+            if out is None else out
+        self._sync_stream()
+        _lib.check(self.ctx, self.lib.psx_spectral(self.ctx, _ptr(phys), int(nf), _ptr(out)), "psx_spectral")
+        return out
+
     def gradients(self, uhat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """du_i/dx_j at the grid points: float32 (N1*N2*N3, 9), column 3i+j."""
         self._state_arg(uhat, "uhat")

@@ -62,5 +62,7 @@ def test_grid_from_file(tmp_path):
         assert np.array_equal(getattr(gg, name), np.round(getattr(gg, name)))
     assert np.array_equal(gg.Gf, o.Gf)
     assert gg.Delta == np.pi / kc and gg.N3 == N
-    with pytest.raises(NotImplementedError):
-        pp.FFTFromFile(None, 1)
+    # the transform objects are built without a GPU (contexts are created on first call)
+    fft = pp.FFTFromFile({"k1": np.zeros((N, N, N // 2 + 1))}, 1)
+    assert fft.ifftT_obj.dims == (3 * N // 2,) * 3 and fft.fft_obj2.dims == (2 * N,) * 3
+    assert fft.ifftT_obj3.inverse and not fft.fft_obj3.inverse

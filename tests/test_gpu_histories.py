@@ -1,0 +1,101 @@
+"""Longer decaying-turbulence LES runs against the oracle's histories.
+
+The oracle's runs of the seeded k^4 exp(-2 (k/4)^2) field (u_rms = 1, nu = 1e-3,
+CFL 0.5) are committed as small fixtures (``tests/golden/make_histories.py`` wrote
+them): per step the CFL dt, the time, the energy E, the enstrophy Omega and the
+resolved energy that the reference's driver records after every advanceQ_RK4 on the
+stale velocity (``src/PySpec_3d.py:121-148``), the dissipation -dE/dt formed from
+them, and the shell spectrum E(k) (``src/Classes.py:762-797``) at a few steps.  The
+GPU repeats each run from the same field and compares, step by step:
+
+* 64^3 Smagorinsky, 1000 steps (t = 0 to 45: the field decays to 0.2% of its energy);
+* 64^3 t-model (Ct = 0.1), 220 steps, up to t = 1.32: the oracle's run blows up at
+  t = 1.50, where the memory term Ct t PLQL has grown too strong;
+* 128^3 Smagorinsky, 100 steps, and 128^3 t-model, 16 steps (the oracle's cost bounds
+  these).
+
+Round-off differences (FFT summation order) grow through the chaotic dynamics, so the
+bar is per quantity and per run: relative differences of dt, t, E, Omega, E_res and
+the dissipation history below ``HIST_TOL``, and of the spectra below ``SPEC_TOL``
+(relative to the spectrum's peak, per shell, for shells holding more than 1e-12 of
+the energy).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import spectral_oracle as so
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+HIST_TOL = {"smag64": 1e-10, "tmodel64": 1e-9, "smag128": 1e-10, "tmodel128": 1e-10}
+SPEC_TOL = {"smag64": 1e-10, "tmodel64": 1e-9, "smag128": 1e-10, "tmodel128": 1e-10}
+
+
+def _golden(golden_dir, name):
+    path = os.path.join(golden_dir, f"history_{name}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not generated")
+    with np.load(path) as f:
+        return {k: f[k] for k in f.files}
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("name", ["smag64", "tmodel64", "smag128", "tmodel128"])
+def test_history_matches_oracle(golden_dir, name):
+    from pyspectral_amd.solver import SpectralSolver
+
+    h = _golden(golden_dir, name)
+    N, model, nu, cfl = int(h["N"]), int(h["turb_model"]), float(h["nu"]), float(h["cfl"])
+    steps = len(h["dt"])
+    spec_steps = [int(k) for k in h["spec_steps"]]
+    print(f"\nhistory {name}: {N}^3 model {model}, {steps} steps", flush=True)
+    u0 = np.stack(so.random_isotropic(N, seed=int(h["seed"])))
+    s = SpectralSolver(N, N, N, model, nu=nu, cfl=cfl, Ct=0.1)
+    q = torch.from_numpy(u0).to("cuda")
+    qs = q.clone()
+    t = 0.0
+    st0 = s.stats(qs).cpu().numpy()
+    E, Om, Er, T, DT = [st0[0]], [st0[1]], [st0[2]], [t], []
+    spectra = {}
+    if 0 in spec_steps:
+        spectra[0] = s.spectrum(qs).cpu().numpy()
+    for it in range(1, steps + 1):
+        dt = float(s.step(q, qs, t).item())
+        t += dt
+        st = s.stats(qs).cpu().numpy()
+        E.append(st[0])
+        Om.append(st[1])
+        Er.append(st[2])
+        T.append(t)
+        DT.append(dt)
+        if it in spec_steps:
+            spectra[it] = s.spectrum(qs).cpu().numpy()
+    assert s.finite()
+    s.close()
+    E, Om, Er, T = map(np.array, (E, Om, Er, T))
+    D = (E[1:] - E[:-1]) / (T[1:] - T[:-1])
+    errs = {"dt": _rel(DT, h["dt"]), "t": _rel(T[1:], h["t"][1:]), "E": _rel(E, h["Energy"]),
+            "Omega": _rel(Om, h["Enstrophy"]), "E_res": _rel(Er, h["Energy_resolved"]),
+            "dissipation": _rel(D, h["Dissipation"])}
+    # growth of the difference along the run
+    q4 = [int(x) for x in np.linspace(0, steps, 5)]
+    grow = [f"{_rel(E[a:b + 1], h['Energy'][a:b + 1]):.1e}" for a, b in zip(q4[:-1], q4[1:])]
+    spec_err = 0.0
+    for k, ref in zip(spec_steps, h["spectra"]):
+        got = spectra[k]
+        tot, ref_tot = got.sum(axis=1), ref.sum(axis=1)
+        keep = ref_tot > 1e-12 * ref_tot.sum()
+        spec_err = max(spec_err, float(np.max(np.abs(tot - ref_tot)[keep] / ref_tot.max())))
+    print(f"{name}: max relative differences {', '.join(f'{k} {v:.2e}' for k, v in errs.items())}; "
+          f"E by quarter of the run {grow}; spectra {spec_err:.2e}", flush=True)
+    for k, v in errs.items():
+        assert v < HIST_TOL[name], (k, v)
+    assert spec_err < SPEC_TOL[name]
