@@ -36,15 +36,18 @@ def fused_closure(N, tuning):
     return M % 64 == 0 and M // 64 in (1, 2, 3, 4, 6, 8, 12) and not tuning.get("closure_separate", 0)
 
 
-def kernel_bytes(N, closure, fused=False):
+def kernel_bytes(N, closure, fused=False, sums_in_y=False):
     """Compulsory (algorithmic) bytes per launch of each kernel class at N^3, 3/2 rule:
     every input field read once, every output field written once.  The learned closure fused
     into the z pass moves what the Smagorinsky z pass moves (9 fields in, 12 out); run
     as its own kernel it has two z passes per stage (gradients out as float32, then
-    products with the network's float32 stresses in), and the z_pass figure is their mean."""
+    products with the network's float32 stresses in), and the z_pass figure is their mean.
+    With the stress sums in the y-forward (tuning stress_sums_in_y_forward = 1) the
+    Smagorinsky-form y-forward writes 9 fields (D_mn, A, B, C) and the epilogue reads 9."""
     M, H, Nc = 3 * N // 2, N // 2, N // 2 + 1
     smag = closure in ("smagorinsky", "learned")
     nX, nY, nZ = (6, 9, 12) if smag else (3, 3, 6)
+    nW = 9 if smag and sums_in_y else nZ
     if closure == "smagorinsky":
         nY = 8  # u v w u_x v_y (u_y + v_x)/2 w_x w_y: the strain needs no separate u_y, v_x
     state = N * N * Nc * C16
@@ -56,8 +59,8 @@ def kernel_bytes(N, closure, fused=False):
         "x_inverse": 3 * N * N * H * C16 + nX * M * N * H * C16,
         "y_inverse": nX * M * N * H * C16 + nY * MMH,
         "z_pass": int(z),
-        "y_forward": nZ * MMH + nZ * M * N * H * C16,
-        "x_forward_epilogue": nZ * M * N * H * C16 + 3 * 3 * state,
+        "y_forward": nZ * MMH + nW * M * N * H * C16,
+        "x_forward_epilogue": nW * M * N * H * C16 + 3 * 3 * state,
         "cfl_x": 6 * state,
         "cfl_y": 6 * state,
         "cfl_z": 3 * state,
@@ -681,6 +684,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="timed steps of the 96-member ensemble reported beside the LES (0: skip)")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager psx_step calls instead of replays of the captured step")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel shapes")
     ap.add_argument("--tuning", default="",
                     help="kernel shapes as key=value,... (SpectralSolver.tuning() keys); set instead of autotuning, "
@@ -751,13 +756,27 @@ def main():
         tuning = solver.tuning()
     else:
         tuning = solver.tuning() if args.no_autotune else solver.autotune(q, reps=5)
-    for _ in range(args.warmup):
-        solver.step(q, qs)
-    torch.cuda.synchronize()
+    # the timed steps: one replay of the captured step (psx_step_graph, CFL dt and
+    # step-start time on the device) per step, or eager psx_step calls with --eager
+    side = torch.cuda.Stream()
+    dt_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
+    t_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
 
+    def one_step():
+        if args.eager:
+            solver.step(q, qs)
+        else:
+            solver.step_graph(q, qs, dt_dev, t_dev)
+
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        for _ in range(args.warmup):
+            one_step()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        solver.step(q, qs)
+    with torch.cuda.stream(side):
+        for _ in range(args.steps):
+            one_step()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms_per_step = el / args.steps * 1e3
@@ -765,7 +784,7 @@ def main():
 
     # per-kernel durations from HIP events on the library's stream
     fused = args.closure == "learned" and fused_closure(N, tuning)
-    kb = kernel_bytes(N, args.closure, fused)
+    kb = kernel_bytes(N, args.closure, fused, bool(tuning.get("stress_sums_in_y_forward", 0)))
     # the chunked y/z pipeline splits each of those passes into yz_chunks launches
     nch = max(1, int(tuning.get("yz_chunks", 0)))
     for name in ("y_inverse", "z_pass", "y_forward"):
@@ -818,6 +837,7 @@ def main():
                           "frac_of_copy": round(sb / (ms_per_step * 1e-3) / 1e9 / copy_gbs, 4)},
         "kernels": kernels,
         "tuning": tuning,
+        "launch": "eager psx_step" if args.eager else "psx_step_graph replays (one hipGraph per step)",
         "stats_after": {"E": E[0], "Omega": E[1], "finite": finite},
     }
     if closure_info is not None:

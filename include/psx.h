@@ -215,13 +215,16 @@ int psx_tau(psx_context* ctx, double* out_host8);
  * inside the z pass where the padded z length is 64 * {1,2,3,4,6,8,12}, 1 always as its
  * own kernel between a gradient and a products z pass; tune[7..9] = block order of the
  * x-inverse, y-inverse and y-forward passes (0: each XCD runs a contiguous range of
- * blocks, 1: dispatch order).  psx_autotune times every candidate of each
+ * blocks, 1: dispatch order); tune[10] = where the Smagorinsky-form stresses are combined
+ * into the three k1-polynomial coefficients the projection needs (0: the epilogue reads
+ * all 12 fields and sums them, 1: the y-forward sums them and hands the epilogue 9).
+ * psx_autotune times every candidate of each
  * knob on reps RHS evaluations of uhat with HIP events, keeps the fastest (knob by
  * knob), and returns the choice in tune_out (may be NULL); it allocates one temporary
  * state, and the chunked pipeline's buffers when it tries chunks, as psx_set_tuning
  * does when it sets them.  DNS, Smagorinsky, DSMAG and learned-closure contexts (the
  * learned stage does not chunk). */
-#define PSX_TUNE_N 10
+#define PSX_TUNE_N 11
 int psx_set_tuning(psx_context* ctx, const int* tune);
 int psx_get_tuning(const psx_context* ctx, int* tune);
 int psx_autotune(psx_context* ctx, const double* uhat, int reps, int* tune_out);

@@ -71,6 +71,19 @@ hipError_t launch_epilogue(int L, int ns1, int epi, bool w0, const EpiArgs& a, c
         return hipErrorInvalidValue;
     }
   }
+  if (epi == E_SMAG9) {
+    if (w0) return hipErrorInvalidValue;  // the folded fields hold no separate stress
+    switch (key) {
+#define PSX_CASE(LL, NN)                                                                                     \
+  case LL * 4096 + NN:                                                                                       \
+    hipLaunchKernelGGL((xforward_epilogue<LL, NN, epi_T(LL), E_SMAG9, false>), grid, dim3(NT), 0, s, a, tw); \
+    return hipGetLastError();
+      PSX_PAIRS32(PSX_CASE)
+#undef PSX_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
   if (epi == E_TAU2) {
     switch (key) {
 #define PSX_CASE(LL, NN)                                                                                    \

@@ -83,4 +83,33 @@ hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, 
   }
 }
 
+template <int L, int T>
+static hipError_t launch_sum_tile(const SumArgs& a, const double2* tw, int njobs, hipStream_t s) {
+  if constexpr (tile_ok<L, T>()) {
+    const dim3 grid((unsigned)(a.p.A * ((a.p.C + T - 1) / T)), (unsigned)njobs);
+    hipLaunchKernelGGL((pencil_forward_sum<L, T>), grid, dim3(NT), 0, s, a, tw);
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_pencil_forward_sum(int L, const SumArgs& a, const double2* tw, int njobs, hipStream_t s, int T) {
+  const int t = pencil_tile(L, T);
+  switch (L) {
+#define PSX_CASE(LL)                                                     \
+  case LL:                                                               \
+    switch (t) {                                                         \
+      case 4: return launch_sum_tile<LL, 4>(a, tw, njobs, s);            \
+      case 8: return launch_sum_tile<LL, 8>(a, tw, njobs, s);            \
+      case 16: return launch_sum_tile<LL, 16>(a, tw, njobs, s);          \
+      default: return hipErrorInvalidValue;                              \
+    }
+    PSX_LENGTHS(PSX_CASE)
+#undef PSX_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace psx

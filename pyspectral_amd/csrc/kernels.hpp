@@ -254,6 +254,82 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   }
 }
 
+// Forward along a strided axis where an output field may be a weighted sum of up to
+// three transformed inputs, the weight of term t at output index i (signed frequency
+// s2 = signed_freq(i, Nspec)) and column c (k3 = ks3 (c0 + c)) being
+// coef_t (ks2 s2)^p2_t k3^p3_t.  The Smagorinsky-form y-forward uses it to hand the
+// epilogue A = tau_11, B = 2 k2 tau_12 + 2 k3 tau_13 and C = k2^2 tau_22 +
+// 2 k2 k3 tau_23 + k3^2 tau_33 (3 fields) instead of the 6 stresses.  The transform is
+// linear, so each term is transformed in turn and summed in registers at the output.
+constexpr int MAXTERMS = 3;
+struct SumArgs {
+  PencilArgs p;                      // strides, extents, job_plain (output field), scale
+  int nterm[MAXJOBS];
+  int tf[MAXJOBS][MAXTERMS];         // input field of each term
+  double tc[MAXJOBS][MAXTERMS];      // coefficient
+  int tp2[MAXJOBS][MAXTERMS];        // power of k2 (0..2)
+  int tp3[MAXJOBS][MAXTERMS];        // power of k3 (0..2)
+  double ks2, ks3;                   // 2 pi / L2, 2 pi / L3
+};
+
+__device__ __forceinline__ double ipow2(double x, int p) { return p == 0 ? 1.0 : (p == 1 ? x : x * x); }
+
+template <int L, int T>
+__global__ __launch_bounds__(NT) void pencil_forward_sum(SumArgs sa, const double2* __restrict__ tw) {
+  const PencilArgs& a = sa.p;
+  constexpr int LS = Row<L>::LS;
+  __shared__ double2 buf[T * LS];
+  const int tid = threadIdx.x;
+  PSX_STAGE_TW(NT)
+  const int ct = (a.C + T - 1) / T;
+  const int bid = a.order ? (int)blockIdx.x : xcd_block<PSX_XCD != 0>();
+  const int aidx = bid / ct;
+  const int c0 = (bid - aidx * ct) * T;
+  const int mem = pencil_member(a);
+  const int job = blockIdx.y - mem * a.jpm;
+  const double2* base = a.in + mem * a.in_ms + (long long)aidx * a.in_sa + c0;
+  constexpr int IO = (L * T + NT - 1) / NT;  // >= the outputs a thread stores
+  double2 acc[IO];
+#pragma unroll
+  for (int it = 0; it < IO; ++it) acc[it] = make_double2(0.0, 0.0);
+  const int nt = sa.nterm[job];
+#pragma unroll 1
+  for (int t = 0; t < nt; ++t) {
+    const double2* src = base + (long long)sa.tf[job][t] * a.in_sf;
+    if (t > 0) __syncthreads();  // every thread has read the previous term's results
+    for (int e = tid; e < L * T; e += NT) {
+      const int l = e / T, c = e - (e / T) * T;
+      buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * a.in_sl + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
+    const double coef = sa.tc[job][t];
+    const int p2 = sa.tp2[job][t], p3 = sa.tp3[job][t];
+#pragma unroll
+    for (int it = 0; it < IO; ++it) {
+      const int e = tid + it * NT;
+      if (e < a.Nspec * T) {
+        const int i = e / T, c = e - (e / T) * T;
+        const int p = pad_slot(i, a.Nspec, L, a.keep_nyq);
+        if (p >= 0) {
+          const double w = coef * ipow2(sa.ks2 * signed_freq(i, a.Nspec), p2) * ipow2(sa.ks3 * (c0 + c), p3);
+          acc[it] = cadd(acc[it], cscale(buf[c * LS + RW(p)], w));
+        }
+      }
+    }
+  }
+  const int fo = a.job_plain[job];
+  double2* dst = a.out + mem * a.out_ms + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
+#pragma unroll
+  for (int it = 0; it < IO; ++it) {
+    const int e = tid + it * NT;
+    if (e < a.Nspec * T) {
+      const int i = e / T, c = e - (e / T) * T;
+      if (c0 + c < a.C) dst[(long long)i * a.out_sl + c] = cscale(acc[it], a.scale);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ fused z pass
 enum ZMode { Z_DNS = 0, Z_SMAG = 1, Z_TM2 = 2, Z_DT = 3, Z_SGS = 4, Z_PHYS = 5, Z_GRAD = 6, Z_LRN = 7, Z_LRNF = 8 };
 
@@ -660,7 +736,7 @@ __device__ __forceinline__ double2 block_sum2(double2 v, double2* scratch, int t
 }
 
 // ------------------------------------------------------------------ x-forward + epilogue
-enum EpiMode { E_DNS = 0, E_SMAG = 1, E_TAU2 = 2, E_SMAGF = 3 };
+enum EpiMode { E_DNS = 0, E_SMAG = 1, E_TAU2 = 2, E_SMAGF = 3, E_SMAG9 = 4 };
 template <int EPI> struct EpiShape;
 template <> struct EpiShape<E_DNS> { static constexpr int nf = 6; };
 template <> struct EpiShape<E_SMAG> { static constexpr int nf = 12; };
@@ -671,6 +747,9 @@ template <> struct EpiShape<E_SMAG> { static constexpr int nf = 12; };
 // formed before its x transform (k2 and k3 are constants of a block column), so that
 // sum_mn s_mn k_m k_n tau_mn = k1^2 A + k1 B + C (s = 1 on the diagonal, 2 off it).
 template <> struct EpiShape<E_SMAGF> { static constexpr int nf = 9; };
+// E_SMAG9: the same 9 fields with A, B and C already formed by the y-forward
+// (pencil_forward_sum), so the epilogue reads 9 fields instead of 12.
+template <> struct EpiShape<E_SMAG9> { static constexpr int nf = 9; };
 // The epilogue reads its input one field per step.  E_SMAGF has 12 steps for its 9
 // transformed fields: steps 0..6 are D_mn and A; steps 7, 8 (tau_12, tau_13) build B
 // and steps 9, 10, 11 (tau_22, tau_23, tau_33) build C in LDS, and only the last step
@@ -721,7 +800,7 @@ struct EpiArgs {
 template <int EPI>
 __device__ __forceinline__ void epi_accumulate(int f, const double* k, double kinv, double2 v, double2* acc,
                                                double2* acc_sgs) {
-  if constexpr (EPI == E_SMAGF) {
+  if constexpr (EPI == E_SMAGF || EPI == E_SMAG9) {
     if (f >= 6) {
       // the folded stress fields A, B, C (xforward_epilogue): with the products'
       // projection applied to D = N + tau, the stress's own term as written becomes
@@ -733,7 +812,7 @@ __device__ __forceinline__ void epi_accumulate(int f, const double* k, double ki
       return;
     }
   }
-  if constexpr (EPI == E_DNS || EPI == E_SMAG || EPI == E_SMAGF) {
+  if constexpr (EPI == E_DNS || EPI == E_SMAG || EPI == E_SMAGF || EPI == E_SMAG9) {
     // fields 0..5: N_mn = u_m u_n (E_SMAGF: D_mn = N_mn + tau_mn), 6..11: tau_mn;
     // (mn) in 11,22,33,12,13,23
     const int g = f % 6;

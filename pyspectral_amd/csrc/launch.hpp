@@ -50,6 +50,7 @@ hipError_t launch_zreal_forward(int L, const double* phys, long long in_sf, doub
 int pencil_tile(int L, int T);
 hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T = 0);
 hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T = 0);
+hipError_t launch_pencil_forward_sum(int L, const SumArgs& a, const double2* tw, int njobs, hipStream_t s, int T = 0);
 // mode: ZMode; returns hipErrorInvalidValue for an unsupported (L, mode)
 // variant: 0 = the one-wavefront kernel where the length allows it, 1 = the block kernel
 hipError_t launch_zpass(int L, int mode, const ZArgs& a, const double2* tw, hipStream_t s, int variant = 0);

@@ -96,7 +96,7 @@ struct psx_context {
   double* nu_m = nullptr;
   // kernel shapes (psx_autotune / psx_set_tuning): tile widths of the x-inverse,
   // y-inverse and y-forward passes of the 3/2 pipeline (0 = default), z-pass variant
-  int tune[PSX_TUNE_N] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int tune[PSX_TUNE_N] = {};
   // x-chunk pipelining of the y-inverse / z / y-forward passes over two streams
   // (tune[4] chunks): separate X and W buffers so that chunks never alias, the
   // second stream and its fork/stagger/join events
@@ -385,6 +385,65 @@ void y_forward(psx_context* c, const double2* Z, double2* W, int L1, int L2, int
   check(c, launch_pencil_forward(L2, a, tw, nf * c->members, c->stream, c->tune[2]), "y_forward");
 }
 
+// y-forward of the folded Smagorinsky-form fields (D_mn at 0..5, tau_mn at 6..11, mn =
+// 11 22 33 12 13 23) into the 9 fields the epilogue transforms: D_mn, A = tau_11,
+// B = 2 k2 tau_12 + 2 k3 tau_13, C = k2^2 tau_22 + 2 k2 k3 tau_23 + k3^2 tau_33.  The
+// three-term job runs first, so the longest blocks start earliest.
+void y_forward_sum(psx_context* c, const double2* Z, double2* W, int L1, int L2, int C, int x0 = 0, int x1 = -1) {
+  if (x1 < 0) x1 = L1;
+  SumArgs sa;
+  std::memset(&sa, 0, sizeof(sa));
+  sa.p = pencil_base();
+  PencilArgs& a = sa.p;
+  a.in = Z;
+  a.out = W;
+  a.in_sf = (long long)L1 * L2 * C;
+  a.in_sa = (long long)L2 * C;
+  a.in_sl = C;
+  a.out_sf = (long long)L1 * c->N2 * C;
+  a.out_sa = (long long)c->N2 * C;
+  a.out_sl = C;
+  a.A = x1 - x0;
+  a.C = C;
+  a.Nspec = c->N2;
+  a.keep_nyq = 0;
+  sa.ks2 = 2.0 * kPi / c->p.L2;
+  sa.ks3 = 2.0 * kPi / c->p.L3;
+  struct Term { int f; double coef; int p2, p3; };
+  const std::vector<std::pair<int, std::vector<Term>>> jobs = {
+      {8, {{7, 1.0, 2, 0}, {11, 2.0, 1, 1}, {8, 1.0, 0, 2}}},  // C
+      {7, {{9, 2.0, 1, 0}, {10, 2.0, 0, 1}}},                  // B
+      {0, {{0, 1.0, 0, 0}}}, {1, {{1, 1.0, 0, 0}}}, {2, {{2, 1.0, 0, 0}}},
+      {3, {{3, 1.0, 0, 0}}}, {4, {{4, 1.0, 0, 0}}}, {5, {{5, 1.0, 0, 0}}},
+      {6, {{6, 1.0, 0, 0}}}};                                  // D_mn, A
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    a.job_in[j] = jobs[j].second[0].f;
+    a.job_plain[j] = jobs[j].first;
+    sa.nterm[j] = (int)jobs[j].second.size();
+    for (size_t t = 0; t < jobs[j].second.size(); ++t) {
+      sa.tf[j][t] = jobs[j].second[t].f;
+      sa.tc[j][t] = jobs[j].second[t].coef;
+      sa.tp2[j][t] = jobs[j].second[t].p2;
+      sa.tp3[j][t] = jobs[j].second[t].p3;
+    }
+  }
+  const int nj = (int)jobs.size();
+  set_members(c, a, nj);
+  a.in += x0 * a.in_sa;
+  a.out += x0 * a.out_sa;
+  const double2* tw = twiddles(c, L2);
+  a.order = c->tune[9];
+  TimeScope ts(c, K_YFWD);
+  check(c, launch_pencil_forward_sum(L2, sa, tw, nj * c->members, c->stream, c->tune[2]), "y_forward(stress sums)");
+}
+
+// Whether the folded Smagorinsky-form fields leave the y-forward as all 12, combined by
+// the epilogue (tune[10] = 0), or as the 9 fields of y_forward_sum (tune[10] = 1).  At
+// 128^3 the 12-field form is faster: the summing y-forward holds its sums in registers
+// (148 VGPRs, 3 waves per SIMD against 5) and costs more (205 against 161 us) than the
+// epilogue saves (91 against 114 us).
+bool sums_in_y(const psx_context* c) { return c->tune[10] == 1; }
+
 // K1-K4 of the 3/2-rule pipeline; returns the buffer holding W (the y-forward of the
 // z-pass products of uin).  smag adds the strain-rate pencils and the stress
 // -2 c2 |S| S_ij (c2 = (Cs Delta)^2 for the static closure); fold (with smag) has the
@@ -480,7 +539,11 @@ double2* front_32(psx_context* c, const double2* uin, bool smag, double c2, bool
       check(c, launch_zpass(M3, mode, zc, tw3, c->stream, c->tune[3]), "z_pass");
     }
     // K4: y-forward + truncation
-    y_forward(c, Z, W, M1, M2, H, 0, nF, 0, x0, x1);
+    if (smag && fold && sums_in_y(c)) {
+      y_forward_sum(c, Z, W, M1, M2, H, x0, x1);
+    } else {
+      y_forward(c, Z, W, M1, M2, H, 0, nF, 0, x0, x1);
+    }
   }
   c->stream = main;
   if (two) {
@@ -535,7 +598,8 @@ void epilogue_32(psx_context* c, const double2* B, const double2* uin, const dou
   e.nu_m = c->nu_m;
   const double2* tw = twiddles(c, M1);
   TimeScope ts(c, K_XEPI);
-  check(c, launch_epilogue(M1, c->N1, folded ? E_SMAGF : (smag ? E_SMAG : E_DNS), !folded && smag && w0 != nullptr,
+  const int epi = folded ? (sums_in_y(c) ? E_SMAG9 : E_SMAGF) : (smag ? E_SMAG : E_DNS);
+  check(c, launch_epilogue(M1, c->N1, epi, !folded && smag && w0 != nullptr,
                            e, tw, c->stream),
         "x_forward_epilogue");
 }
@@ -610,7 +674,11 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
       TimeScope ts(c, K_ZPASS);
       check(c, launch_zpass(M3, Z_LRNF, q, tw3, c->stream, 0), "z_pass(fused closure)");
     }
-    y_forward(c, Z, W, M1, M2, H, 0, 12);
+    if (w0 == nullptr && sums_in_y(c)) {
+      y_forward_sum(c, Z, W, M1, M2, H);
+    } else {
+      y_forward(c, Z, W, M1, M2, H, 0, 12);
+    }
     epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true, w0 == nullptr);
     return;
   }
@@ -648,7 +716,11 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
     TimeScope ts(c, K_ZPASS);
     check(c, launch_zpass(M3, Z_LRN, q, tw3, c->stream, c->tune[3]), "z_pass(learned)");
   }
-  y_forward(c, Z, W, M1, M2, H, 0, 12);
+  if (w0 == nullptr && sums_in_y(c)) {
+    y_forward_sum(c, Z, W, M1, M2, H);
+  } else {
+    y_forward(c, Z, W, M1, M2, H, 0, 12);
+  }
   epilogue_32(c, W, uin, q0, out, dt, coef, w0, zero_nyq_out, true, w0 == nullptr);
 }
 
@@ -1777,6 +1849,7 @@ int psx_set_tuning(psx_context* c, const int* tune) {
     if (tune[6] != 0 && tune[6] != 1) fail(c, PSX_E_ARG, "closure placement must be 0 or 1");
     for (int k = 7; k < 10; ++k)
       if (tune[k] != 0 && tune[k] != 1) fail(c, PSX_E_ARG, "block order must be 0 or 1");
+    if (tune[10] != 0 && tune[10] != 1) fail(c, PSX_E_ARG, "stress sums must be 0 (epilogue) or 1 (y-forward)");
     if (tune[4] > 1) {
       if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
         fail(c, PSX_E_UNSUPPORTED, "chunked pipeline: DNS, Smagorinsky and DSMAG");
@@ -1804,7 +1877,8 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
     double2* tmp = nullptr;
     check(c, hipMalloc(&tmp, state * sizeof(double2)), "hipMalloc(autotune)");
     const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1, -1, -1,  // -1: the whole stage
-                                   K_XINV, K_YINV, K_YFWD};
+                                   K_XINV, K_YINV, K_YFWD, -1};
+    const bool smag_form = c->model != PSX_MODEL_DNS;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     check(c, hipEventCreate(&t0), "hipEventCreate");
     check(c, hipEventCreate(&t1), "hipEventCreate");
@@ -1814,6 +1888,7 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
       for (int knob = 0; knob < PSX_TUNE_N; ++knob) {
         if (learned && (knob == 4 || knob == 5)) continue;  // the learned stage is not chunked
         if (!learned && knob == 6) continue;                 // closure placement: learned only
+        if (!smag_form && knob == 10) continue;              // stress sums: Smagorinsky-form stages
         // knob 4: chunk count and streams together (v = chunks + 16 * streams)
         if (knob == 5) continue;
         std::vector<int> cand;
@@ -1822,6 +1897,8 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
             if (pencil_tile(L[knob], T) == T) cand.push_back(T);
         } else if (knob == 3 || knob >= 6) {
           cand = {0, 1};
+          // the stress sums change the folded stage only (the autotuner's RHS calls pass
+          // no w0, so they time exactly that stage)
         } else {
           cand = {1 + 16 * 2, 2 + 16 * 2, 4 + 16 * 2, 2 + 16 * 1, 4 + 16 * 1, 8 + 16 * 1};
           ensure_pipeline(c);

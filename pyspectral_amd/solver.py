@@ -155,7 +155,8 @@ class SpectralSolver:
                    "psx_step_graph")
 
     TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant", "yz_chunks", "yz_streams",
-                   "closure_separate", "x_inverse_order", "y_inverse_order", "y_forward_order")
+                   "closure_separate", "x_inverse_order", "y_inverse_order", "y_forward_order",
+                   "stress_sums_in_y_forward")
 
     def tuning(self) -> dict:
         t = (C.c_int * len(self.TUNING_KEYS))()
