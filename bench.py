@@ -622,6 +622,10 @@ def bench_tmodel(args, uh, copy_gbs):
     N, nu, cfl = args.N, 1e-3, 0.5
     s = SpectralSolver(N, N, N, 2, nu=nu, cfl=cfl, Ct=0.1)
     q = torch.from_numpy(uh).to("cuda")
+    if args.closure == "tmodel" and args.tuning:
+        s.set_tuning({k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
+    elif not args.no_autotune:
+        s.autotune(q, reps=3)
     qs = q.clone()
     t = 0.0
     for _ in range(args.tmodel_warmup):
@@ -647,6 +651,7 @@ def bench_tmodel(args, uh, copy_gbs):
     ach = kernels[dom]["GBs"]
     sb = TMODEL_BYTES_PER_POINT_STEP * N ** 3
     E = s.stats(qs).cpu().tolist()
+    tun = s.tuning()
     s.close()
     out = {"value": round(N ** 3 * args.tmodel_steps / el / 1e6, 2), "unit": "Mpts*RK4-steps/s",
            "steps": args.tmodel_steps, "warmup": args.tmodel_warmup, "ms_per_step": round(ms_step, 4),
@@ -659,7 +664,7 @@ def bench_tmodel(args, uh, copy_gbs):
            "step_roofline": {"model_bytes_per_step": int(sb), "bytes_per_point_step": TMODEL_BYTES_PER_POINT_STEP,
                              "achieved": round(sb / (ms_step * 1e-3) / 1e9, 1), "unit": "GB/s",
                              "frac": round(sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-           "kernels": kernels, "stats_after": {"E": E[0], "Omega": E[1]},
+           "kernels": kernels, "tuning": tun, "stats_after": {"E": E[0], "Omega": E[1]},
            "stages_timed_per_kernel": stages}
     return out
 

@@ -222,8 +222,9 @@ int psx_tau(psx_context* ctx, double* out_host8);
  * knob on reps RHS evaluations of uhat with HIP events, keeps the fastest (knob by
  * knob), and returns the choice in tune_out (may be NULL); it allocates one temporary
  * state, and the chunked pipeline's buffers when it tries chunks, as psx_set_tuning
- * does when it sets them.  DNS, Smagorinsky, DSMAG and learned-closure contexts (the
- * learned stage does not chunk). */
+ * does when it sets them.  Every model: the t-model family tunes the tiles and block
+ * orders of its 2x-grid pencil passes (tile widths are checked against those lengths), the
+ * learned stage does not chunk, chunking is for the DNS, Smagorinsky and DSMAG stages. */
 #define PSX_TUNE_N 11
 int psx_set_tuning(psx_context* ctx, const int* tune);
 int psx_get_tuning(const psx_context* ctx, int* tune);

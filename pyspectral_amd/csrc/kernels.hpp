@@ -1266,8 +1266,11 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int ky = blockIdx.x / ct;
-  const int c0 = (blockIdx.x - ky * ct) * T;
+  // contiguous block ranges per XCD: the column tiles of one k2 row (T = 4 at L = 256:
+  // 64-byte row segments, two tiles per 128-byte line) share one L2
+  const int bid = xcd_block<true>();
+  const int ky = bid / ct;
+  const int c0 = (bid - ky * ct) * T;
   const long long in_sl = (long long)a.P2 * a.C;
   const double2* src0 = a.in + (long long)ky * a.C + c0;
   const int s2 = signed_freq(ky, a.P2);
@@ -1283,13 +1286,28 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   for (int it = 0; it < (DYN ? IT : 1); ++it)
 #pragma unroll
     for (int i = 0; i < 3; ++i) Pj[it][i] = make_double2(0.0, 0.0);
+  // field f + 1 is loaded into registers while field f is transformed
+  constexpr int LT = (L * T + NT - 1) / NT;
+  double2 pre[LT];
+  auto load_field = [&](int f) {
+    const double2* src = src0 + (long long)f * a.in_sf;
+#pragma unroll
+    for (int i = 0; i < LT; ++i) {
+      const int e = tid + i * NT;
+      const int l = e / T, c = e - (e / T) * T;
+      pre[i] = ((L * T) % NT == 0 || e < L * T) && (c0 + c < a.C) ? src[(long long)l * in_sl + c]
+                                                                    : make_double2(0.0, 0.0);
+    }
+  };
+  load_field(0);
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
-    const double2* src = src0 + (long long)f * a.in_sf;
-    for (int e = tid; e < L * T; e += NT) {
-      const int l = e / T, c = e - (e / T) * T;
-      buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < LT; ++i) {
+      const int e = tid + i * NT;
+      if ((L * T) % NT == 0 || e < L * T) buf[(e - (e / T) * T) * LS + RW(e / T)] = pre[i];
     }
+    if (f + 1 < 6) load_field(f + 1);
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
     const int m = sym_m(f), n = sym_n(f);
@@ -1371,8 +1389,8 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
     if (a.part) {
       const double2 tot = block_sum2(red, buf, tid);
       if (tid == 0) {
-        a.part[2 * blockIdx.x] = tot.x;
-        a.part[2 * blockIdx.x + 1] = tot.y;
+        a.part[2 * bid] = tot.x;
+        a.part[2 * bid + 1] = tot.y;
       }
     }
   }

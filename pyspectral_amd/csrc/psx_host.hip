@@ -1837,10 +1837,22 @@ void ensure_pipeline(psx_context* c) {
 // ---- kernel-shape autotuning -----------------------------------------------------
 extern "C" {
 
+namespace {
+bool tau_family(int model) { return model >= PSX_MODEL_TMODEL && model <= PSX_MODEL_DYNAMIC_TAU; }
+// Transform lengths of the tuned x-inverse, y-inverse and y-forward passes: the 3/2-rule
+// lengths, or the 2x lengths of the t-model family.
+void tuned_lengths(const psx_context* c, int* L) {
+  const bool tau = tau_family(c->model);
+  L[0] = tau ? 2 * c->N1 : c->M1;
+  L[1] = L[2] = tau ? 2 * c->N2 : c->M2;
+}
+}  // namespace
+
 int psx_set_tuning(psx_context* c, const int* tune) {
   return guarded(c, [&] {
     if (!tune) fail(c, PSX_E_ARG, "null pointer");
-    const int L[3] = {c->M1, c->M2, c->M2};
+    int L[3];
+    tuned_lengths(c, L);
     for (int k = 0; k < 3; ++k)
       if (tune[k] != 0 && pencil_tile(L[k], tune[k]) < 0) fail(c, PSX_E_ARG, "tile width not available");
     if (tune[3] != 0 && tune[3] != 1) fail(c, PSX_E_ARG, "z variant must be 0 or 1");
@@ -1870,25 +1882,28 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
   return guarded(c, [&] {
     if (!uhat) fail(c, PSX_E_ARG, "null pointer");
     const bool learned = c->model == PSX_MODEL_LEARNED;
-    if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG && !learned)
-      fail(c, PSX_E_UNSUPPORTED, "autotuning covers the 3/2-rule pipeline (DNS, Smagorinsky, DSMAG, learned)");
+    const bool tau = tau_family(c->model);
     if (reps < 1) reps = 1;
     const size_t state = (size_t)c->members * 3 * c->N1 * c->N2 * c->Nc;
     double2* tmp = nullptr;
     check(c, hipMalloc(&tmp, state * sizeof(double2)), "hipMalloc(autotune)");
     const int kinds[PSX_TUNE_N] = {K_XINV, K_YINV, K_YFWD, K_ZPASS, -1, -1, -1,  // -1: the whole stage
                                    K_XINV, K_YINV, K_YFWD, -1};
-    const bool smag_form = c->model != PSX_MODEL_DNS;
+    const bool smag_form = c->model != PSX_MODEL_DNS && !tau;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     check(c, hipEventCreate(&t0), "hipEventCreate");
     check(c, hipEventCreate(&t1), "hipEventCreate");
-    const int L[3] = {c->M1, c->M2, c->M2};
+    int L[3];
+    tuned_lengths(c, L);
     const bool was_timing = c->timing;
     try {
       for (int knob = 0; knob < PSX_TUNE_N; ++knob) {
         if (learned && (knob == 4 || knob == 5)) continue;  // the learned stage is not chunked
         if (!learned && knob == 6) continue;                 // closure placement: learned only
         if (!smag_form && knob == 10) continue;              // stress sums: Smagorinsky-form stages
+        // the t-model family tunes its pencil passes' tiles and block orders (its z passes
+        // have one variant, and it is not chunked)
+        if (tau && (knob >= 3 && knob <= 6)) continue;
         // knob 4: chunk count and streams together (v = chunks + 16 * streams)
         if (knob == 5) continue;
         std::vector<int> cand;
