@@ -321,6 +321,14 @@ int psx_restrict(psx_context* ctx, const double* src, int s1, int s2, int s3, in
 int psx_timing_begin(psx_context* ctx);
 int psx_timing_end(psx_context* ctx, double* ms, int* count);
 
+/* Streaming bandwidth probe (instrumentation, no counterpart in the reference): mode 0
+ * copies n complex128 values src -> dst with the workgroups striding through the array,
+ * mode 2 with each workgroup copying contiguous 16 KiB chunks, mode 3 as mode 2 with
+ * non-temporal stores; mode 1 reads them and writes one partial sum per workgroup into
+ * dst[0 .. blocks).  `blocks` workgroups of 256 lanes, four 16-byte loads in flight per
+ * lane.  Asynchronous on `stream`. */
+int psx_stream_probe(const double* src, double* dst, long long n, int mode, int blocks, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
