@@ -2008,14 +2008,48 @@ struct psx_ensemble {
   long long total = 0;
   int count = 0;
   hipStream_t stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
   const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<unsigned long long> gens;  // each class context's settings generation at capture
+  // size classes run concurrently: class k on cls_stream[k], forked from and joined to the
+  // ensemble stream (each class's launches fill only part of the GPU)
+  std::vector<hipStream_t> cls_stream;
+  std::vector<hipEvent_t> cls_join;
+  hipEvent_t fork = nullptr;
+  // psx_ensemble_step_graph: one graph per class, captured on the class's stream (a
+  // single-stream capture, as psx_step_graph's), replayed concurrently
+  std::vector<hipGraph_t> cls_graph;
+  std::vector<hipGraphExec_t> cls_exec;
   std::string err;
 };
 
 namespace {
+// One ensemble step of every class, each class on its own stream forked from the
+// ensemble stream and joined back (t: step-start time for the eager step).
+int ensemble_step_classes(psx_ensemble* e, double* q, double* qs, double* dt_dev, double t) {
+  if (hipEventRecord(e->fork, e->stream) != hipSuccess) {
+    e->err = "hipEventRecord failed";
+    return PSX_E_HIP;
+  }
+  int rc = PSX_OK;
+  for (size_t k = 0; k < e->cls.size() && rc == PSX_OK; ++k) {
+    psx_context* c = e->cls[k];
+    if (hipStreamWaitEvent(e->cls_stream[k], e->fork, 0) != hipSuccess) {
+      e->err = "hipStreamWaitEvent failed";
+      return PSX_E_HIP;
+    }
+    c->stream = e->cls_stream[k];
+    rc = guarded(c, [&] {
+      step_body(c, (double2*)q + e->cls_offset[k], (double2*)qs + e->cls_offset[k], dt_dev + e->cls_first_slot[k], t,
+                nullptr);
+      check(c, hipEventRecord(e->cls_join[k], c->stream), "hipEventRecord");
+      check(c, hipStreamWaitEvent(e->stream, e->cls_join[k], 0), "hipStreamWaitEvent");
+    });
+    c->stream = e->stream;
+    if (rc != PSX_OK) e->err = c->err;
+  }
+  return rc;
+}
+
 // PSX_E_NONFINITE when any class context's non-finite flag is set
 int ensemble_status(const psx_ensemble* e, int rc) {
   for (const psx_context* c : e->cls) rc = with_status(c, rc);
@@ -2076,15 +2110,33 @@ int psx_ensemble_create(psx_ensemble** out, int count, const int* dims, const do
     next_slot += (int)members[k].size();
     e->total += per * (long long)members[k].size();
   }
+  bool ok = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+  for (size_t k = 0; k < e->cls.size() && ok; ++k) {
+    hipStream_t s = nullptr;
+    hipEvent_t j = nullptr;
+    ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+    if (ok) e->cls_stream.push_back(s);
+    ok = ok && hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
+    if (ok) e->cls_join.push_back(j);
+  }
+  if (!ok) {
+    psx_ensemble_destroy(e);
+    return PSX_E_HIP;
+  }
   *out = e;
   return PSX_OK;
 }
 
 int psx_ensemble_destroy(psx_ensemble* e) {
   if (!e) return PSX_E_ARG;
-  if (e->exec) hipGraphExecDestroy(e->exec);
-  if (e->graph) hipGraphDestroy(e->graph);
+  for (hipGraphExec_t x : e->cls_exec)
+    if (x) hipGraphExecDestroy(x);
+  for (hipGraph_t g : e->cls_graph)
+    if (g) hipGraphDestroy(g);
   for (psx_context* c : e->cls) psx_destroy(c);
+  for (hipStream_t s : e->cls_stream) hipStreamDestroy(s);
+  for (hipEvent_t j : e->cls_join) hipEventDestroy(j);
+  if (e->fork) hipEventDestroy(e->fork);
   delete e;
   return PSX_OK;
 }
@@ -2121,18 +2173,8 @@ int psx_ensemble_set_stream(psx_ensemble* e, void* stream) {
 int psx_ensemble_step(psx_ensemble* e, double* q, double* qs, double* dt_dev, double t) {
   if (!e) return PSX_E_ARG;
   if (!q || !qs || !dt_dev || q == qs) return PSX_E_ARG;
-  for (size_t k = 0; k < e->cls.size(); ++k) {
-    psx_context* c = e->cls[k];
-    const int rc = guarded(c, [&] {
-      step_body(c, (double2*)q + e->cls_offset[k], (double2*)qs + e->cls_offset[k], dt_dev + e->cls_first_slot[k], t,
-                nullptr);
-    });
-    if (rc != PSX_OK) {
-      e->err = c->err;
-      return rc;
-    }
-  }
-  return ensemble_status(e, PSX_OK);
+  const int rc = ensemble_step_classes(e, q, qs, dt_dev, t);
+  return rc != PSX_OK ? rc : ensemble_status(e, PSX_OK);
 }
 
 int psx_ensemble_status(psx_ensemble* e) {
@@ -2152,49 +2194,66 @@ int psx_ensemble_step_graph(psx_ensemble* e, double* q, double* qs, double* dt_d
     return PSX_E_ARG;
   }
   const void* key[4] = {q, qs, dt_dev, e->stream};
-  bool same = e->exec != nullptr && e->gens.size() == e->cls.size();
+  bool same = e->cls_exec.size() == e->cls.size() && e->gens.size() == e->cls.size();
   for (int i = 0; i < 4 && same; ++i) same = e->key[i] == key[i];
   for (size_t k = 0; k < e->cls.size() && same; ++k) same = e->gens[k] == e->cls[k]->gen;
   if (!same) {
-    if (e->exec) hipGraphExecDestroy(e->exec);
-    if (e->graph) hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
-    if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      e->err = "hipStreamBeginCapture failed";
-      return PSX_E_HIP;
-    }
-    int rc = PSX_OK;
-    for (size_t k = 0; k < e->cls.size() && rc == PSX_OK; ++k) {
+    for (hipGraphExec_t x : e->cls_exec)
+      if (x) hipGraphExecDestroy(x);
+    for (hipGraph_t g : e->cls_graph)
+      if (g) hipGraphDestroy(g);
+    e->cls_exec.assign(e->cls.size(), nullptr);
+    e->cls_graph.assign(e->cls.size(), nullptr);
+    e->gens.clear();
+    // each class's step captured on its own stream (the ensemble stream is not in the capture)
+    for (size_t k = 0; k < e->cls.size(); ++k) {
       psx_context* c = e->cls[k];
+      const hipStream_t cs = e->cls_stream[k];
       const bool timing = c->timing;
       c->timing = false;
-      rc = guarded(c, [&] {
-        step_body(c, (double2*)q + e->cls_offset[k], (double2*)qs + e->cls_offset[k], dt_dev + e->cls_first_slot[k],
-                  0.0, nullptr);
-      });
+      c->stream = cs;
+      int rc = PSX_OK;
+      if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        e->err = "hipStreamBeginCapture failed";
+        rc = PSX_E_HIP;
+      } else {
+        rc = guarded(c, [&] {
+          step_body(c, (double2*)q + e->cls_offset[k], (double2*)qs + e->cls_offset[k], dt_dev + e->cls_first_slot[k],
+                    0.0, nullptr);
+        });
+        if (rc != PSX_OK) e->err = c->err;
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(cs, &g);
+        if (rc == PSX_OK && ec != hipSuccess) {
+          e->err = "hipStreamEndCapture failed";
+          rc = PSX_E_HIP;
+        }
+        e->cls_graph[k] = g;
+        if (rc == PSX_OK && hipGraphInstantiate(&e->cls_exec[k], g, nullptr, nullptr, 0) != hipSuccess) {
+          e->err = "hipGraphInstantiate failed";
+          rc = PSX_E_HIP;
+        }
+      }
       c->timing = timing;
-      if (rc != PSX_OK) e->err = c->err;
-    }
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(e->stream, &g);
-    if (rc != PSX_OK || ec != hipSuccess) {
-      if (g) hipGraphDestroy(g);
-      if (rc == PSX_OK) e->err = "hipStreamEndCapture failed";
-      return rc != PSX_OK ? rc : PSX_E_HIP;
-    }
-    e->graph = g;
-    if (hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0) != hipSuccess) {
-      e->err = "hipGraphInstantiate failed";
-      return PSX_E_HIP;
+      c->stream = e->stream;
+      if (rc != PSX_OK) return rc;  // gens stays short: the next call captures again
     }
     for (int i = 0; i < 4; ++i) e->key[i] = key[i];
-    e->gens.clear();
     for (const psx_context* c : e->cls) e->gens.push_back(c->gen);
   }
-  if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) {
-    e->err = "hipGraphLaunch failed";
+  // the class graphs run concurrently, forked from and joined to the ensemble stream
+  if (hipEventRecord(e->fork, e->stream) != hipSuccess) {
+    e->err = "hipEventRecord failed";
     return PSX_E_HIP;
+  }
+  for (size_t k = 0; k < e->cls.size(); ++k) {
+    if (hipStreamWaitEvent(e->cls_stream[k], e->fork, 0) != hipSuccess ||
+        hipGraphLaunch(e->cls_exec[k], e->cls_stream[k]) != hipSuccess ||
+        hipEventRecord(e->cls_join[k], e->cls_stream[k]) != hipSuccess ||
+        hipStreamWaitEvent(e->stream, e->cls_join[k], 0) != hipSuccess) {
+      e->err = "hipGraphLaunch failed";
+      return PSX_E_HIP;
+    }
   }
   return ensemble_status(e, PSX_OK);
 }

@@ -9,7 +9,7 @@ them, and the shell spectrum E(k) (``src/Classes.py:762-797``) at a few steps.  
 GPU repeats each run from the same field and compares, step by step:
 
 * 64^3 Smagorinsky, 1000 steps (t = 0 to 45: the field decays to 0.2% of its energy);
-* 64^3 t-model (Ct = 0.1), 220 steps, up to t = 1.32: the oracle's run blows up at
+* 64^3 t-model (Ct = 0.1), 220 steps, up to t = 1.47: the oracle's run blows up at
   t = 1.50, where the memory term Ct t PLQL has grown too strong;
 * 128^3 Smagorinsky, 100 steps, and 128^3 t-model, 16 steps (the oracle's cost bounds
   these).

@@ -74,3 +74,33 @@ def test_training_reduces_loss():
     print(f"\nloss {first:.4f} -> {last:.4f}")
     assert last < 0.2 * first
     net.close()
+
+
+def test_adam_steps_match_torch_adam():
+    """The Adam update kernel (beta 0.9/0.999, eps 1e-8, FP32 moments) against
+    torch.optim.Adam fed the same gradients: five steps at two learning rates, each
+    step's parameter update compared (float32 rounding of the two formulas only)."""
+    net, p, norm, rng = _net(5)
+    n = 2048
+    x = torch.from_numpy(rng.normal(0, 1.0, (n, 9)).astype(np.float32)).cuda()
+    t = torch.from_numpy(rng.normal(0, 1.0, (n, 6)).astype(np.float32)).cuda()
+    w = torch.nn.Parameter(torch.from_numpy(net.params().copy()))
+    opt = torch.optim.Adam([w], lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    for step, lr in enumerate((1e-3, 1e-3, 1e-3, 3e-4, 3e-4)):
+        before = net.params().copy()
+        _, g = net.grad(x, t)  # the gradient train_step computes (deterministic reductions)
+        for grp in opt.param_groups:
+            grp["lr"] = lr
+        w.grad = g.cpu().clone()
+        opt.step()
+        net.train_step(x, t, lr)
+        after = net.params()
+        d_ours = after.astype(np.float64) - before
+        d_ref = w.detach().numpy().astype(np.float64) - before
+        rel = np.linalg.norm(d_ours - d_ref) / np.linalg.norm(d_ref)
+        print(f"\nAdam step {step + 1} (lr {lr}): update relative L2 {rel:.2e}, |update| {np.linalg.norm(d_ref):.3e}")
+        assert rel < 1e-4
+        # both continue from the kernel's weights, so float32 rounding does not accumulate
+        with torch.no_grad():
+            w.copy_(torch.from_numpy(after.copy()))
+    net.close()
