@@ -36,13 +36,13 @@ def fused_closure(N, tuning):
     return M % 64 == 0 and M // 64 in (1, 2, 3, 4, 6, 8, 12) and not tuning.get("closure_separate", 0)
 
 
-def kernel_bytes(N, closure, fused=False, sums_in_y=False):
+def kernel_bytes(N, closure, fused=False, sums_in_y=True):
     """Compulsory (algorithmic) bytes per launch of each kernel class at N^3, 3/2 rule:
     every input field read once, every output field written once.  The learned closure fused
     into the z pass moves what the Smagorinsky z pass moves (9 fields in, 12 out); run
     as its own kernel it has two z passes per stage (gradients out as float32, then
     products with the network's float32 stresses in), and the z_pass figure is their mean.
-    With the stress sums in the y-forward (tuning stress_sums_in_y_forward = 1) the
+    With the stress sums in the y-forward (tuning stress_sums_in_epilogue = 0) the
     Smagorinsky-form y-forward writes 9 fields (D_mn, A, B, C) and the epilogue reads 9."""
     M, H, Nc = 3 * N // 2, N // 2, N // 2 + 1
     smag = closure in ("smagorinsky", "learned")
@@ -816,7 +816,7 @@ def main():
 
     # per-kernel durations from HIP events on the library's stream
     fused = args.closure == "learned" and fused_closure(N, tuning)
-    kb = kernel_bytes(N, args.closure, fused, bool(tuning.get("stress_sums_in_y_forward", 0)))
+    kb = kernel_bytes(N, args.closure, fused, not tuning.get("stress_sums_in_epilogue", 0))
     # the chunked y/z pipeline splits each of those passes into yz_chunks launches
     nch = max(1, int(tuning.get("yz_chunks", 0)))
     for name in ("y_inverse", "z_pass", "y_forward"):

@@ -216,8 +216,8 @@ int psx_tau(psx_context* ctx, double* out_host8);
  * own kernel between a gradient and a products z pass; tune[7..9] = block order of the
  * x-inverse, y-inverse and y-forward passes (0: each XCD runs a contiguous range of
  * blocks, 1: dispatch order); tune[10] = where the Smagorinsky-form stresses are combined
- * into the three k1-polynomial coefficients the projection needs (0: the epilogue reads
- * all 12 fields and sums them, 1: the y-forward sums them and hands the epilogue 9).
+ * into the three k1-polynomial coefficients the projection needs (0: the y-forward sums
+ * them and hands the epilogue 9 fields, 1: the epilogue reads all 12 and sums them).
  * psx_autotune times every candidate of each
  * knob on reps RHS evaluations of uhat with HIP events, keeps the fastest (knob by
  * knob), and returns the choice in tune_out (may be NULL); it allocates one temporary

@@ -95,6 +95,7 @@ static hipError_t launch_sum_tile(const SumArgs& a, const double2* tw, int njobs
 }
 
 hipError_t launch_pencil_forward_sum(int L, const SumArgs& a, const double2* tw, int njobs, hipStream_t s, int T) {
+  if (3 * a.p.Nspec != 2 * L) return hipErrorInvalidValue;  // the kernel's compile-time truncation
   const int t = pencil_tile(L, T);
   switch (L) {
 #define PSX_CASE(LL)                                                     \

@@ -277,6 +277,8 @@ __device__ __forceinline__ double ipow2(double x, int p) { return p == 0 ? 1.0 :
 template <int L, int T>
 __global__ __launch_bounds__(NT) void pencil_forward_sum(SumArgs sa, const double2* __restrict__ tw) {
   const PencilArgs& a = sa.p;
+  // the 3/2 rule's truncation: Nspec = 2L/3 spectral outputs (the launcher checks)
+  constexpr int NS = 2 * L / 3;
   constexpr int LS = Row<L>::LS;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(NT) void pencil_forward_sum(SumArgs sa, const doubl
   const int mem = pencil_member(a);
   const int job = blockIdx.y - mem * a.jpm;
   const double2* base = a.in + mem * a.in_ms + (long long)aidx * a.in_sa + c0;
-  constexpr int IO = (L * T + NT - 1) / NT;  // >= the outputs a thread stores
+  constexpr int IO = (NS * T + NT - 1) / NT;  // the outputs a thread stores
   double2 acc[IO];
 #pragma unroll
   for (int it = 0; it < IO; ++it) acc[it] = make_double2(0.0, 0.0);
@@ -308,11 +310,15 @@ __global__ __launch_bounds__(NT) void pencil_forward_sum(SumArgs sa, const doubl
 #pragma unroll
     for (int it = 0; it < IO; ++it) {
       const int e = tid + it * NT;
-      if (e < a.Nspec * T) {
+      if (e < NS * T) {
         const int i = e / T, c = e - (e / T) * T;
-        const int p = pad_slot(i, a.Nspec, L, a.keep_nyq);
+        const int p = pad_slot(i, NS, L, a.keep_nyq);
         if (p >= 0) {
-          const double w = coef * ipow2(sa.ks2 * signed_freq(i, a.Nspec), p2) * ipow2(sa.ks3 * (c0 + c), p3);
+          // recomputed per term (an empty asm keeps the factors from being hoisted and
+          // held across the transform)
+          double k2 = sa.ks2 * signed_freq(i, NS), k3 = sa.ks3 * (c0 + c);
+          asm volatile("" : "+v"(k2), "+v"(k3));
+          const double w = coef * ipow2(k2, p2) * ipow2(k3, p3);
           acc[it] = cadd(acc[it], cscale(buf[c * LS + RW(p)], w));
         }
       }
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(NT) void pencil_forward_sum(SumArgs sa, const doubl
 #pragma unroll
   for (int it = 0; it < IO; ++it) {
     const int e = tid + it * NT;
-    if (e < a.Nspec * T) {
+    if (e < NS * T) {
       const int i = e / T, c = e - (e / T) * T;
       if (c0 + c < a.C) dst[(long long)i * a.out_sl + c] = cscale(acc[it], a.scale);
     }

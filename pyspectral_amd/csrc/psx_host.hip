@@ -437,12 +437,11 @@ void y_forward_sum(psx_context* c, const double2* Z, double2* W, int L1, int L2,
   check(c, launch_pencil_forward_sum(L2, sa, tw, nj * c->members, c->stream, c->tune[2]), "y_forward(stress sums)");
 }
 
-// Whether the folded Smagorinsky-form fields leave the y-forward as all 12, combined by
-// the epilogue (tune[10] = 0), or as the 9 fields of y_forward_sum (tune[10] = 1).  At
-// 128^3 the 12-field form is faster: the summing y-forward holds its sums in registers
-// (148 VGPRs, 3 waves per SIMD against 5) and costs more (205 against 161 us) than the
-// epilogue saves (91 against 114 us).
-bool sums_in_y(const psx_context* c) { return c->tune[10] == 1; }
+// Whether the folded Smagorinsky-form fields leave the y-forward as the 9 fields of
+// y_forward_sum (tune[10] = 0) or as all 12, combined by the epilogue (tune[10] = 1).  At
+// 128^3 the 9-field form saves 150 MB per stage: y-forward 161 -> 166 us, epilogue
+// 114 -> 92 us.
+bool sums_in_y(const psx_context* c) { return c->tune[10] == 0; }
 
 // K1-K4 of the 3/2-rule pipeline; returns the buffer holding W (the y-forward of the
 // z-pass products of uin).  smag adds the strain-rate pencils and the stress
@@ -1861,7 +1860,7 @@ int psx_set_tuning(psx_context* c, const int* tune) {
     if (tune[6] != 0 && tune[6] != 1) fail(c, PSX_E_ARG, "closure placement must be 0 or 1");
     for (int k = 7; k < 10; ++k)
       if (tune[k] != 0 && tune[k] != 1) fail(c, PSX_E_ARG, "block order must be 0 or 1");
-    if (tune[10] != 0 && tune[10] != 1) fail(c, PSX_E_ARG, "stress sums must be 0 (epilogue) or 1 (y-forward)");
+    if (tune[10] != 0 && tune[10] != 1) fail(c, PSX_E_ARG, "stress sums must be 0 (y-forward) or 1 (epilogue)");
     if (tune[4] > 1) {
       if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
         fail(c, PSX_E_UNSUPPORTED, "chunked pipeline: DNS, Smagorinsky and DSMAG");

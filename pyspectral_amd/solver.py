@@ -156,7 +156,7 @@ class SpectralSolver:
 
     TUNING_KEYS = ("x_inverse_tile", "y_inverse_tile", "y_forward_tile", "z_variant", "yz_chunks", "yz_streams",
                    "closure_separate", "x_inverse_order", "y_inverse_order", "y_forward_order",
-                   "stress_sums_in_y_forward")
+                   "stress_sums_in_epilogue")
 
     def tuning(self) -> dict:
         t = (C.c_int * len(self.TUNING_KEYS))()

@@ -38,7 +38,7 @@ def test_every_kernel_shape_matches_oracle(model):
                 print(f"\nmodel={model} tile={T} z={zv} order={order}", flush=True)
                 s.set_tuning({"x_inverse_tile": T, "y_inverse_tile": T, "y_forward_tile": T, "z_variant": zv,
                               "x_inverse_order": order, "y_inverse_order": order, "y_forward_order": order,
-                              "stress_sums_in_y_forward": order})
+                              "stress_sums_in_epilogue": order})
                 r = s.rhs(ud, 0.0).cpu().numpy()
                 err = np.linalg.norm(r - ref) / np.linalg.norm(ref)
                 assert err < 1e-12, (T, zv, order, err)
