@@ -53,8 +53,8 @@ def _compile(src: str, hipcc: str, build_dir: str, defines) -> str:
 
 
 def build(jobs: int | None = None, verbose: bool = True, out: str = OUT, defines=()) -> str:
-    """Compile every csrc/*.hip and link ``out``; ``defines`` are extra -D macros
-    (e.g. PSX_LDS_PAD=0 for the unpadded LDS layout)."""
+    """Compile every csrc/*.hip and link ``out``; ``defines`` are extra -D macros for
+    experimental builds (compiled into a separate build folder)."""
     hipcc = _hipcc()
     build_dir = BUILD if not defines else BUILD + "_" + "_".join(d.replace("=", "") for d in defines)
     os.makedirs(build_dir, exist_ok=True)

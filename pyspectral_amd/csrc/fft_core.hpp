@@ -31,58 +31,34 @@ __host__ __device__ __forceinline__ double2 cmul_i(double2 a) {
 }
 
 // ---------------------------------------------------------------- radix plans
-// Two plan families per length: V = 0 uses radices <= 8, V = 1 prefers radix 16
-// and 12 (fewer LDS passes, more work per butterfly).  Plan<L> is the family
-// selected at build time by PSX_FFT_PLAN.
-template <int L, int V> struct PlanV;
-#define PSX_PLAN(V, LEN, ...)                                     \
-  template <> struct PlanV<LEN, V> {                              \
+// The Stockham stages of each length, radices <= 8 (a family preferring radices 16
+// and 12 -- fewer LDS passes, more registers per butterfly -- measured slower).
+template <int L> struct Plan;
+#define PSX_PLAN(LEN, ...)                                        \
+  template <> struct Plan<LEN> {                                  \
     static constexpr int r[] = {__VA_ARGS__};                     \
     static constexpr int n = sizeof(r) / sizeof(int);             \
   };
-PSX_PLAN(0, 8, 8)
-PSX_PLAN(0, 12, 4, 3)
-PSX_PLAN(0, 16, 4, 4)
-PSX_PLAN(0, 24, 8, 3)
-PSX_PLAN(0, 32, 8, 4)
-PSX_PLAN(0, 36, 4, 3, 3)
-PSX_PLAN(0, 48, 4, 4, 3)
-PSX_PLAN(0, 64, 8, 8)
-PSX_PLAN(0, 72, 8, 3, 3)
-PSX_PLAN(0, 96, 8, 4, 3)
-PSX_PLAN(0, 128, 8, 4, 4)
-PSX_PLAN(0, 144, 4, 4, 3, 3)
-PSX_PLAN(0, 192, 8, 8, 3)
-PSX_PLAN(0, 256, 8, 8, 4)
-PSX_PLAN(0, 288, 8, 4, 3, 3)
-PSX_PLAN(0, 384, 8, 4, 4, 3)
-PSX_PLAN(0, 512, 8, 8, 8)
-PSX_PLAN(0, 576, 8, 8, 3, 3)
-PSX_PLAN(0, 768, 8, 8, 4, 3)
-PSX_PLAN(1, 8, 8)
-PSX_PLAN(1, 12, 12)
-PSX_PLAN(1, 16, 16)
-PSX_PLAN(1, 24, 6, 4)
-PSX_PLAN(1, 32, 8, 4)
-PSX_PLAN(1, 36, 12, 3)
-PSX_PLAN(1, 48, 16, 3)
-PSX_PLAN(1, 64, 8, 8)
-PSX_PLAN(1, 72, 12, 6)
-PSX_PLAN(1, 96, 16, 6)
-PSX_PLAN(1, 128, 16, 8)
-PSX_PLAN(1, 144, 12, 12)
-PSX_PLAN(1, 192, 16, 12)
-PSX_PLAN(1, 256, 16, 16)
-PSX_PLAN(1, 288, 16, 6, 3)
-PSX_PLAN(1, 384, 16, 8, 3)
-PSX_PLAN(1, 512, 8, 8, 8)
-PSX_PLAN(1, 576, 16, 12, 3)
-PSX_PLAN(1, 768, 16, 16, 3)
+PSX_PLAN(8, 8)
+PSX_PLAN(12, 4, 3)
+PSX_PLAN(16, 4, 4)
+PSX_PLAN(24, 8, 3)
+PSX_PLAN(32, 8, 4)
+PSX_PLAN(36, 4, 3, 3)
+PSX_PLAN(48, 4, 4, 3)
+PSX_PLAN(64, 8, 8)
+PSX_PLAN(72, 8, 3, 3)
+PSX_PLAN(96, 8, 4, 3)
+PSX_PLAN(128, 8, 4, 4)
+PSX_PLAN(144, 4, 4, 3, 3)
+PSX_PLAN(192, 8, 8, 3)
+PSX_PLAN(256, 8, 8, 4)
+PSX_PLAN(288, 8, 4, 3, 3)
+PSX_PLAN(384, 8, 4, 4, 3)
+PSX_PLAN(512, 8, 8, 8)
+PSX_PLAN(576, 8, 8, 3, 3)
+PSX_PLAN(768, 8, 8, 4, 3)
 #undef PSX_PLAN
-#ifndef PSX_FFT_PLAN
-#define PSX_FFT_PLAN 0
-#endif
-template <int L> struct Plan : PlanV<L, PSX_FFT_PLAN> {};
 
 // ---------------------------------------------------------------- small DFTs
 template <int R, int SIGN> struct Dft;
@@ -200,54 +176,13 @@ template <int SIGN> struct Dft<12, SIGN> {
   }
 };
 
-// 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2; DFT4 over n1, twiddle W16^(n2 k1), DFT4 over n2.
-template <int SIGN> struct Dft<16, SIGN> {
-  static __host__ __device__ __forceinline__ void run(double2* v) {
-    constexpr double c1 = 0.92387953251128675612818318939679;  // cos(pi/8)
-    constexpr double s1 = 0.38268343236508977172845998403040;  // sin(pi/8)
-    constexpr double r = 0.70710678118654752440084436210485;   // 1/sqrt(2)
-    double2 b[4][4];
-#pragma unroll
-    for (int n2 = 0; n2 < 4; ++n2) {
-      double2 t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
-      Dft<4, SIGN>::run(t);
-#pragma unroll
-      for (int k1 = 0; k1 < 4; ++k1) b[n2][k1] = t[k1];
-    }
-    // W16^m, m = n2 k1: (1,2,3), (2,4,6), (3,6,9)
-    b[1][1] = cmul(b[1][1], make_double2(c1, SIGN * s1));
-    b[1][2] = cmul(b[1][2], make_double2(r, SIGN * r));
-    b[1][3] = cmul(b[1][3], make_double2(s1, SIGN * c1));
-    b[2][1] = cmul(b[2][1], make_double2(r, SIGN * r));
-    b[2][2] = cmul_i<SIGN>(b[2][2]);
-    b[2][3] = cmul(b[2][3], make_double2(-r, SIGN * r));
-    b[3][1] = cmul(b[3][1], make_double2(s1, SIGN * c1));
-    b[3][2] = cmul(b[3][2], make_double2(-r, SIGN * r));
-    b[3][3] = cmul(b[3][3], make_double2(-c1, -SIGN * s1));
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-      double2 t[4] = {b[0][k1], b[1][k1], b[2][k1], b[3][k1]};
-      Dft<4, SIGN>::run(t);
-      v[k1] = t[0];
-      v[k1 + 4] = t[1];
-      v[k1 + 8] = t[2];
-      v[k1 + 12] = t[3];
-    }
-  }
-};
 
 // ---------------------------------------------------------------- LDS row layout
 // Rows of L complex values are stored with one pad slot after every 8 values (so
 // the stride-R writes of a Stockham stage with Ns = 1 fall in distinct banks) and
-// one extra slot per row (so column-wise accesses across rows do too).
-#ifndef PSX_LDS_PAD
-#define PSX_LDS_PAD 1
-#endif
-// Twiddles as powers of one table entry per butterfly (1) or one load each (0).
-#ifndef PSX_TW_POW
-#define PSX_TW_POW 1
-#endif
-template <int L, bool PAD = (PSX_LDS_PAD != 0)> struct Row {
+// one extra slot per row (so column-wise accesses across rows do too); PAD = false
+// (L + 1 entries per row) for kernels whose access pattern measured faster unpadded.
+template <int L, bool PAD = true> struct Row {
   static constexpr int LS = PAD ? L + L / 8 + 1 : L + 1;
   static __device__ __forceinline__ int at(int i) { return PAD ? i + (i >> 3) : i; }
 };
@@ -274,7 +209,6 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
         for (int r = 0; r < R; ++r) v[it][r] = src[Row<L, PAD>::at(jj + r * NBF)];
         if constexpr (NS > 1) {
           const int k = jj % NS;
-#if PSX_TW_POW
           // one table load per butterfly; the other twiddles are its running powers
           // (at most R - 2 products, a few ulp; two live values instead of R - 1)
           double2 w1 = tw[k * TWS];
@@ -285,14 +219,6 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
             v[it][r] = cmul(v[it][r], wc);
             if (r + 1 < R) wc = cmul(wc, w1);
           }
-#else
-#pragma unroll
-          for (int r = 1; r < R; ++r) {
-            double2 w = tw[k * r * TWS];
-            if constexpr (SIGN > 0) w.y = -w.y;
-            v[it][r] = cmul(v[it][r], w);
-          }
-#endif
         }
         Dft<R, SIGN>::run(v[it]);
       }
@@ -318,7 +244,7 @@ __device__ __forceinline__ void fft_stages(double2* buf, const double2* __restri
 
 // Run the batch.  Caller must __syncthreads() before (data staged) — the routine
 // ends with a barrier, so results are visible on return.
-template <int L, int B, int NT, int LS, int SIGN, bool PAD = (PSX_LDS_PAD != 0)>
+template <int L, int B, int NT, int LS, int SIGN, bool PAD = true>
 __device__ __forceinline__ void fft_batch(double2* buf, const double2* __restrict__ tw, int tid) {
   fft_stages<L, B, NT, LS, SIGN, 0, 1, PAD>(buf, tw, tid);
 }

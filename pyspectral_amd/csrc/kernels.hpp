@@ -22,31 +22,15 @@ namespace psx {
 
 constexpr int NT = 256;       // threads per block
 
-// Twiddle tables read from global memory (0) or staged in LDS per block (1).  A table
-// read from global memory inside a transform waits on the vector memory counter,
-// which retires loads and stores in order, so it also waits for loads the block has
-// in flight for its next field (the epilogue's prefetch, the second input of a
-// pencil job) and for the stores of its previous transform; reads from LDS wait
-// only on the LDS counter.
-#ifndef PSX_TW_LDS
-#define PSX_TW_LDS 1
-#endif
-#if PSX_TW_LDS
+// Twiddle tables are staged in LDS per block.  A table read from global memory inside
+// a transform waits on the vector memory counter, which retires loads and stores in
+// order, so it would also wait for loads the block has in flight for its next field
+// (the epilogue's prefetch, the second input of a pencil job) and for the stores of its
+// previous transform; reads from LDS wait only on the LDS counter.
 #define PSX_STAGE_TW(NTH)                                              \
   __shared__ double2 tw_lds[L];                                        \
   for (int i_ = threadIdx.x; i_ < L; i_ += (NTH)) tw_lds[i_] = tw[i_]; \
   const double2* twp = tw_lds;
-#else
-#define PSX_STAGE_TW(NTH) const double2* twp = tw;
-#endif
-#ifndef PSX_ZPAD
-#define PSX_ZPAD 0
-#endif
-// z pass: stage the block's input spectra in LDS with one coalesced load pass (1)
-// or load them inside the recipe loop (0).
-#ifndef PSX_ZSTAGE
-#define PSX_ZSTAGE 1
-#endif
 constexpr int MAXJOBS = 16;   // input fields per pencil pass
 constexpr int MAXREC = 12;    // real pencils built by the z pass
 
@@ -83,17 +67,10 @@ __device__ __forceinline__ int pad_slot(int i, int n, int L, int keep_nyq) {
 // own L2.  xcd_block gives every XCD a contiguous range of logical block ids, so
 // neighbouring column tiles -- which share the cache lines of rows that are not
 // 128-byte aligned, like the state's (N3/2+1)-entry rows -- run on one L2.
-// Measured: it helps the pencil passes that read the state (the CFL passes most)
-// and slows the x-forward epilogue, whose W-layout rows are aligned.
-#ifndef PSX_XCD
-#define PSX_XCD 1
-#endif
-// The x-forward epilogue: 0 dispatch order, 1 contiguous ranges (fixes its traffic but
-// measured 255 vs 140 us), 2 one k2 row's column tiles per XCD with rows round-robin
-// (traffic of the shared lines stays in one L2; measured 126 vs 131 us).
-#ifndef PSX_XCD_EPI
-#define PSX_XCD_EPI 2
-#endif
+// Measured: it helps the pencil passes that read the state (the CFL passes most).
+// The x-forward epilogue uses its own order: one k2 row's column tiles per XCD with
+// rows round-robin (traffic of the shared lines stays in one L2; measured 126 against
+// 131 us in dispatch order and 255 us with contiguous ranges).
 template <bool ON>
 __device__ __forceinline__ int xcd_block() {
   if constexpr (ON) {
@@ -128,7 +105,7 @@ struct PencilArgs {
   // in + m*in_ms and writes at out + m*out_ms (jpm = 0: one member)
   int jpm;
   long long in_ms, out_ms;
-  // block order: 0 gives each XCD a contiguous range of blocks (PSX_XCD), 1 keeps the
+  // block order: 0 gives each XCD a contiguous range of blocks (xcd_block), 1 keeps the
   // dispatch order (which of the two is faster depends on the pass and the job count)
   int order;
 };
@@ -139,27 +116,14 @@ __device__ __forceinline__ int pencil_member(const PencilArgs& a) { return a.jpm
 // Inverse along a strided axis: Nspec spectral entries -> L physical points, with
 // the zero-pad fused in; optionally a second output multiplied by i*k_axis.
 // grid = (A * ceil(C/T), njobs)
-#ifndef PSX_PINV_MINW
-#define PSX_PINV_MINW 1
-#endif
-#ifndef PSX_EPI_PREFETCH
-#define PSX_EPI_PREFETCH 1
-#endif
-#ifndef PSX_EPI_MINW
-#define PSX_EPI_MINW 1
-#endif
-// x-forward epilogue: load the state entries of the final update in one batch (1)
-#ifndef PSX_EPI_STATE_PRELOAD
-#define PSX_EPI_STATE_PRELOAD 1
-#endif
 template <int L, int T>
-__global__ __launch_bounds__(NT, PSX_PINV_MINW) void pencil_inverse(PencilArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2* __restrict__ tw) {
   constexpr int LS = Row<L>::LS;
   __shared__ double2 buf[T * LS];
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int bid = a.order ? (int)blockIdx.x : xcd_block<PSX_XCD != 0>();
+  const int bid = a.order ? (int)blockIdx.x : xcd_block<true>();
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
   const int mem = pencil_member(a);
@@ -231,7 +195,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int bid = a.order ? (int)blockIdx.x : xcd_block<PSX_XCD != 0>();
+  const int bid = a.order ? (int)blockIdx.x : xcd_block<true>();
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
   const int mem = pencil_member(a);
@@ -284,7 +248,7 @@ __global__ __launch_bounds__(NT) void pencil_forward_sum(SumArgs sa, const doubl
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int bid = a.order ? (int)blockIdx.x : xcd_block<PSX_XCD != 0>();
+  const int bid = a.order ? (int)blockIdx.x : xcd_block<true>();
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
   const int mem = pencil_member(a);
@@ -530,7 +494,7 @@ __device__ __forceinline__ void zpoint_fold(const ZArgs& a, double* out) {
 // unpadded (PAD = false measured faster for this kernel).
 template <int L, int P, int MODE, int NTZ>
 __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict__ tw) {
-  constexpr bool PADZ = PSX_ZPAD != 0;
+  constexpr bool PADZ = false;  // unpadded rows measured faster in this kernel
   constexpr int NIN = ZShape<MODE>::nin;
   constexpr int NOUT = ZShape<MODE>::nout;
   constexpr int NRI = NIN * P, NRO = NOUT * P;  // real pencils in / out
@@ -549,7 +513,8 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
   // --- build the inverse spectra
   constexpr int HS = L2 + 1;  // staged k3 modes per real input field
   constexpr int NRS = ZFields<MODE>::n * P;  // staged rows: input fields x positions
-  constexpr bool STAGE = (PSX_ZSTAGE != 0) && (NRS * HS <= NCMAX * LS);
+  // stage the block's input spectra in LDS with one coalesced load pass when they fit
+  constexpr bool STAGE = NRS * HS <= NCMAX * LS;
   if constexpr (STAGE) {
     // one coalesced pass of independent global loads into LDS, then the recipe
     // combinations from LDS into registers, then the Hermitian pencils
@@ -863,7 +828,7 @@ __device__ __forceinline__ void epi_accumulate(int f, const double* k, double ki
 // accumulating the spectral RHS in registers; blocks past N2*ceil(H/T) handle the
 // k3 >= H planes (no transformed data there) elementwise.
 template <int L, int NS1, int T, int EPI, bool W0>
-__global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2* __restrict__ tw) {
   constexpr int LS = Row<L>::LS;
   constexpr int NF = EpiShape<EPI>::nf;
   constexpr int IT = (NS1 * T + NT - 1) / NT;
@@ -872,13 +837,13 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   PSX_STAGE_TW(NT)
   const int ct = (a.H + T - 1) / T;
   const int nmain = a.N2 * ct;
-  // PSX_XCD_EPI = 2: the ct column tiles of one k2 row run on one XCD (they share the
-  // cache lines of the state's unaligned rows) and consecutive rows go round-robin
-  // over the 8 XCDs.  (Dispatching the blocks of the k3 >= H planes first measured
-  // slower: 119 against 115 us at 128^3.)
-  const int bid = (PSX_XCD_EPI == 2 && (int)blockIdx.x < nmain && (a.N2 & 7) == 0)
+  // the ct column tiles of one k2 row run on one XCD (they share the cache lines of
+  // the state's unaligned rows) and consecutive rows go round-robin over the 8 XCDs.
+  // (Dispatching the blocks of the k3 >= H planes first measured slower: 119 against
+  // 115 us at 128^3.)
+  const int bid = ((int)blockIdx.x < nmain && (a.N2 & 7) == 0)
                       ? (8 * (((int)blockIdx.x >> 3) / ct) + ((int)blockIdx.x & 7)) * ct + ((int)blockIdx.x >> 3) % ct
-                      : xcd_block<PSX_XCD_EPI == 1>();
+                      : (int)blockIdx.x;
   const int mem = blockIdx.y;
   const double2* const in_ = a.in + mem * a.ms_in;
   const double2* const uprev_ = a.uprev + mem * a.ms_state;
@@ -989,29 +954,18 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
     const bool add = fold && (s == 8 || s == 10 || s == 11);
     const bool transform = !fold || s == 8 || s == 11;
     const int f = fold ? (s <= 8 ? 7 : 8) : s;  // the transformed field this step feeds
-    if constexpr (PSX_EPI_PREFETCH != 0) {
 #pragma unroll
-      for (int i = 0; i < LT; ++i) {
-        const int e = tid + i * NT;
-        const int l = e / T, c = e - (e / T) * T;
-        if ((L * T) % NT == 0 || e < L * T) {
-          double2 v = pre[i];
-          if (fold) v = cscale(v, weight(s, c));
-          if (add) v = cadd(v, buf[c * LS + RW(l)]);
-          buf[c * LS + RW(l)] = v;
-        }
-      }
-      if (snext < NSTEP) load_field(snext, pre);
-    } else {
-      const double2* src = src0 + (long long)epi_step_field(EPI, s) * a.in_sf;
-      for (int e = tid; e < L * T; e += NT) {
-        const int l = e / T, c = e - (e / T) * T;
-        double2 v = (c0 + c < a.H) ? src[(long long)l * in_sl + c] : make_double2(0.0, 0.0);
+    for (int i = 0; i < LT; ++i) {
+      const int e = tid + i * NT;
+      const int l = e / T, c = e - (e / T) * T;
+      if ((L * T) % NT == 0 || e < L * T) {
+        double2 v = pre[i];
         if (fold) v = cscale(v, weight(s, c));
         if (add) v = cadd(v, buf[c * LS + RW(l)]);
         buf[c * LS + RW(l)] = v;
       }
     }
+    if (snext < NSTEP) load_field(snext, pre);
     if (!transform) return;
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
@@ -1030,10 +984,10 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
     }
     __syncthreads();
   };
-  // with PSX_EPI_PREFETCH, step s+1 loads while step s is transformed (a second
-  // buffer, two steps ahead, needs 256 VGPRs: it spilled and ran 12% slower)
+  // step s+1 loads while step s is transformed (a second buffer, two steps ahead,
+  // needs 256 VGPRs: it spilled and ran 12% slower)
   double2 pa[LT];
-  if (PSX_EPI_PREFETCH != 0) load_field(0, pa);
+  load_field(0, pa);
 #pragma unroll 1
   for (int s = 0; s < NSTEP; ++s) run_step(s, pa, s + 1);
 
@@ -1043,7 +997,7 @@ __global__ __launch_bounds__(NT, PSX_EPI_MINW) void xforward_epilogue(EpiArgs a,
   // the state entries this thread updates (uprev, and q0 when set), all loaded before
   // any is used: one memory round trip instead of one per component (not with the
   // SGS output, whose accumulators leave no room for them)
-  constexpr bool PRE = PSX_EPI_STATE_PRELOAD != 0 && EPI != E_TAU2 && !SGS;
+  constexpr bool PRE = EPI != E_TAU2 && !SGS;
   double2 su[PRE ? IT : 1][3], sq[PRE ? IT : 1][3];
   if constexpr (PRE) {
 #pragma unroll

@@ -10,14 +10,8 @@ namespace psx {
 
 // Tile widths (contiguous k3 columns per block) and z-pass position counts,
 // sized so a block's LDS buffer stays near 48 KiB.
-#ifndef PSX_PT_SMALL
-#define PSX_PT_SMALL 8
-#endif
-constexpr int pencil_T(int L) { return L <= 96 ? 16 : (L <= 192 ? PSX_PT_SMALL : (L <= 384 ? 8 : 4)); }
-#ifndef PSX_ET_SMALL
-#define PSX_ET_SMALL 8
-#endif
-constexpr int epi_T(int L) { return L <= 64 ? 8 : (L <= 192 ? PSX_ET_SMALL : (L <= 384 ? 4 : 2)); }
+constexpr int pencil_T(int L) { return L <= 96 ? 16 : (L <= 384 ? 8 : 4); }
+constexpr int epi_T(int L) { return L <= 192 ? 8 : (L <= 384 ? 4 : 2); }
 // z pass: positions per block (pairing two positions, or two fields when P = 1)
 // and threads per block, sized to the first (radix-8 or -4) stage's butterflies.
 template <int MODE> constexpr int z_P(int L) {

@@ -5,21 +5,13 @@
 #include "launch.hpp"
 #include "zwave.hpp"
 
-#ifndef PSX_ZWAVE
-#define PSX_ZWAVE 1
-#endif
-// the compile-time half-spectrum instantiations below (0: runtime lengths only)
-#ifndef PSX_ZW_HK
-#define PSX_ZW_HK 1
-#endif
-
 namespace psx {
 
 // The 3/2-rule modes with L/3 half-spectrum entries in and out (the padded length a
 // multiple of 192) run an instantiation with that length fixed at compile time (not
 // the fused closure at 768, which spills more that way).
 template <int L, int MODE> constexpr bool zw_third_mode() {
-  return PSX_ZW_HK != 0 && ZWave<L>::ok && L % 192 == 0 &&
+  return ZWave<L>::ok && L % 192 == 0 &&
          (MODE == Z_DNS || MODE == Z_SMAG || (MODE == Z_LRNF && L < 768));
 }
 template <int L, int MODE> bool zw_third(const ZArgs& a) {
@@ -45,7 +37,7 @@ hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int va
       return hipErrorInvalidValue;
     }
   } else {
-    if constexpr (PSX_ZWAVE != 0 && ZWave<L>::ok) {
+    if constexpr (ZWave<L>::ok) {
       if (variant == 0) {
         if constexpr (zw_third_mode<L, MODE>()) {
           if (zw_third<L, MODE>(a)) {

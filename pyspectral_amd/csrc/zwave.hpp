@@ -16,32 +16,12 @@
 
 namespace psx {
 
-#ifndef PSX_ZW_PREFETCH
-#define PSX_ZW_PREFETCH 1
-#endif
-// minimum waves per SIMD the compiler must allow (register cap)
-#ifndef PSX_ZW_MINW
-#define PSX_ZW_MINW 1
-#endif
-// skip the forward transform's rows that the r2c split never reads (compile-time
-// half spectra only)
-#ifndef PSX_ZW_PRUNE
-#define PSX_ZW_PRUNE 1
-#endif
-// the unpadded natural-order row layout at L = 192 (ZWave::FLAT)
-#ifndef PSX_ZW_NAT192
-#define PSX_ZW_NAT192 1
-#endif
-// 16-point network tiles evaluated together in the fused closure pass
-#ifndef PSX_ZW_FUSED_TILES
-#define PSX_ZW_FUSED_TILES 1
-#endif
 
 // Waves per SIMD the register allocation must allow: the Smagorinsky pass at L = 192
 // with 64-entry half spectra fits 4 (128 VGPRs, no scratch); every other pass keeps
-// PSX_ZW_MINW.
+// 1.
 template <int L, int MODE, int HK> constexpr int zw_min_waves() {
-  return (MODE == Z_SMAG && L == 192 && HK == 64 && PSX_ZW_MINW < 4) ? 4 : PSX_ZW_MINW;
+  return (MODE == Z_SMAG && L == 192 && HK == 64) ? 4 : 1;
 }
 
 template <int L> struct ZWave {
@@ -52,11 +32,11 @@ template <int L> struct ZWave {
   static constexpr int G = R0 >= 8 ? 1 : 8 / R0;
   // LDS slot of one 64-point sub-transform: 64 entries, pass-1 output at b1*9 + j
   static constexpr int SUB = 72;
-  // natural-order rows: one pad entry per 16, except at L = 192 (with PSX_ZW_NAT192):
+  // natural-order rows: one pad entry per 16, except at L = 192:
   // no pad inside the row and a row stride of 202 entries, which a bank model of the
   // second pass's stride-3 writes and the lanes' contiguous reads finds conflict-free
   // where the padded layout puts n and n + 16 (or 17) in the same banks
-  static constexpr bool FLAT = L == 192 && PSX_ZW_NAT192 != 0;
+  static constexpr bool FLAT = L == 192;
   static constexpr int NATS = FLAT ? 202 : L + L / 16;
   static __device__ __forceinline__ int nat(int n) { return FLAT ? n : n + (n >> 4); }
 };
@@ -253,7 +233,7 @@ __device__ __forceinline__ void zw_sub64(double2* buf, double2 w8, int l, int nr
     const int u = l + 64 * it, f = u >> 3, b1 = u & 7;
     if ((NB % 64 == 0 || u < NB) && f < nrows * R0) {
       const int g = f / R0, av = f - g * R0;
-      if constexpr (SIGN < 0 && HK > 0 && PSX_ZW_PRUNE != 0) {
+      if constexpr (SIGN < 0 && HK > 0) {  // rows the r2c split never reads are skipped
 #pragma unroll
         for (int b2 = 0; b2 < 8; ++b2)
           if (zw_row_read<L, HK>(b2)) buf[g * W::NATS + W::nat(av + R0 * (b1 + 8 * b2))] = y[it][b2];
@@ -292,7 +272,7 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
   constexpr int NF = ZFields<MODE>::n;
   constexpr ZGroupUse<MODE, P, NRI, G> use{};
   // every raw entry a pencil group reads, loaded before any is used; with
-  // PSX_ZW_PREFETCH the next group's loads are issued before this group's LDS passes
+  // the next group's loads are issued before this group's LDS passes
   auto load_group = [&](int g0, double2 (&raw)[R0][P][NF]) {
 #pragma unroll
     for (int m = 0; m < R0; ++m) {
@@ -313,7 +293,6 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
 #pragma unroll
   for (int g0 = 0; g0 < NCI; g0 += G) {
     const int cur = (g0 / G) & 1;
-    if (!PSX_ZW_PREFETCH && g0 > 0) load_group(g0, raw[cur]);
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
       const int q = g0 + gi;
@@ -331,7 +310,7 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
         for (int av = 1; av < R0; ++av) buf[(gi * R0 + av) * SUB + l] = cmul(v[av], cconj(t1[av]));
       }
     }
-    if (PSX_ZW_PREFETCH && g0 + G < NCI) load_group(g0 + G, raw[cur ^ 1]);
+    if (g0 + G < NCI) load_group(g0 + G, raw[cur ^ 1]);
     wave_sync();
     zw_sub64<L, +1>(buf, w8, l, NCI - g0 < G ? NCI - g0 : G);
 #pragma unroll
@@ -390,7 +369,7 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
     // with the points along N; the stresses go back into the same records.
     [[maybe_unused]] float* rec = nullptr;
     if constexpr (MODE == Z_LRNF) {
-      static_assert(P == 1 && (L / 16) % PSX_ZW_FUSED_TILES == 0, "fused closure: one position, whole tiles");
+      static_assert(P == 1, "fused closure: one position");
       __shared__ float recs[L * 16];
       rec = recs;
 #pragma unroll
@@ -406,7 +385,7 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
       const int g = l >> 4, rr = l & 15;
       psx_mlp_impl::Net R;
       psx_mlp_impl::net_load(R, a.npk, a.nw, a.nnorm, l);
-      constexpr int TP = PSX_ZW_FUSED_TILES;
+      constexpr int TP = 1;  // one 16-point tile at a time (two interleaved needed 256 VGPRs)
 #pragma unroll 1
       for (int t0 = 0; t0 < L / 16; t0 += TP) {
         psx_mlp_impl::f4 xn[TP], out[TP];
