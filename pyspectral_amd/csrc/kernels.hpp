@@ -186,6 +186,33 @@ __global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2
   }
 }
 
+// A block's L x T tile of a strided pencil pass, loaded into registers as one batch
+// of IT loads per thread (element e = l T + c: row l, column c; columns at or beyond
+// ncol read as zero), then staged into the LDS rows (column c at buf[c LS ..]).
+// Issuing every load before the first use keeps IT requests in flight per thread,
+// where a load-then-store loop would wait for each one in turn.
+template <int L, int T> struct PencilLoad {
+  static constexpr int IT = (L * T + NT - 1) / NT;
+  static __device__ __forceinline__ void load(double2* r, const double2* src, long long sl, int ncol, int tid) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      const int l = e / T, c = e - (e / T) * T;
+      r[it] = make_double2(0.0, 0.0);
+      if (((L * T) % NT == 0 || e < L * T) && c < ncol) r[it] = src[(long long)l * sl + c];
+    }
+  }
+  static __device__ __forceinline__ void stage(const double2* r, double2* buf, int tid) {
+    constexpr int LS = Row<L>::LS;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      const int l = e / T, c = e - (e / T) * T;
+      if ((L * T) % NT == 0 || e < L * T) buf[c * LS + RW(l)] = r[it];
+    }
+  }
+};
+
 // Forward along a strided axis: L physical points -> Nspec spectral entries, with
 // the truncation fused in.  grid = (A * ceil(C/T), njobs)
 template <int L, int T>
@@ -201,10 +228,9 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   const int mem = pencil_member(a);
   const int job = blockIdx.y - mem * a.jpm;
   const double2* src = a.in + mem * a.in_ms + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
-  for (int e = tid; e < L * T; e += NT) {
-    const int l = e / T, c = e - (e / T) * T;
-    buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * a.in_sl + c] : make_double2(0.0, 0.0);
-  }
+  double2 r[PencilLoad<L, T>::IT];
+  PencilLoad<L, T>::load(r, src, a.in_sl, a.C - c0, tid);
+  PencilLoad<L, T>::stage(r, buf, tid);
   __syncthreads();
   fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
   const int fo = a.job_plain[job];
@@ -259,15 +285,15 @@ __global__ __launch_bounds__(NT) void pencil_forward_sum(SumArgs sa, const doubl
 #pragma unroll
   for (int it = 0; it < IO; ++it) acc[it] = make_double2(0.0, 0.0);
   const int nt = sa.nterm[job];
+  // term t+1's pencils are loaded into registers while term t is transformed
+  double2 r[PencilLoad<L, T>::IT];
+  PencilLoad<L, T>::load(r, base + (long long)sa.tf[job][0] * a.in_sf, a.in_sl, a.C - c0, tid);
 #pragma unroll 1
   for (int t = 0; t < nt; ++t) {
-    const double2* src = base + (long long)sa.tf[job][t] * a.in_sf;
     if (t > 0) __syncthreads();  // every thread has read the previous term's results
-    for (int e = tid; e < L * T; e += NT) {
-      const int l = e / T, c = e - (e / T) * T;
-      buf[c * LS + RW(l)] = (c0 + c < a.C) ? src[(long long)l * a.in_sl + c] : make_double2(0.0, 0.0);
-    }
+    PencilLoad<L, T>::stage(r, buf, tid);
     __syncthreads();
+    if (t + 1 < nt) PencilLoad<L, T>::load(r, base + (long long)sa.tf[job][t + 1] * a.in_sf, a.in_sl, a.C - c0, tid);
     fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
     const double coef = sa.tc[job][t];
     const int p2 = sa.tp2[job][t], p3 = sa.tp3[job][t];
@@ -994,11 +1020,29 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
   const double dtc = q0_ ? (*dt_) * a.c : 0.0;
   const double coef = (EPI == E_TAU2 && a.store_only) ? 1.0 : (a.coef_dev ? *a.coef_dev : a.coef);
   double2 red = make_double2(0.0, 0.0);
-  // the state entries this thread updates (uprev, and q0 when set), all loaded before
-  // any is used: one memory round trip instead of one per component (not with the
-  // SGS output, whose accumulators leave no room for them)
-  constexpr bool PRE = EPI != E_TAU2 && !SGS;
+  // the state entries this thread updates (uprev, and q0 when set; for the t-model
+  // also PL and the reduction's velocity), all loaded before any is used: one memory
+  // round trip instead of one per component (not with the SGS output, whose
+  // accumulators leave no room for them)
+  constexpr bool PRE = !SGS;
+  constexpr bool PRET = EPI == E_TAU2;
   double2 su[PRE ? IT : 1][3], sq[PRE ? IT : 1][3];
+  double2 sp[PRET ? IT : 1][3], sr[PRET ? IT : 1][3];
+  if constexpr (PRET) {
+    const bool want_pl = pl_ && !a.store_only;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      const int kx = e / T, c = e - (e / T) * T;
+      const bool ok = e < NS1 * T && c0 + c < a.H;
+      const long long idx = (long long)kx * plane + (long long)ky * a.Nc + c0 + c;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        sp[it][cc] = ok && want_pl ? pl_[cc * fstride + idx] : make_double2(0.0, 0.0);
+        sr[it][cc] = ok && a.red_u ? a.red_u[cc * fstride + idx] : make_double2(0.0, 0.0);
+      }
+    }
+  }
   if constexpr (PRE) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -1042,12 +1086,12 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
             if (w0_) w0_[cc * fstride + idx] = w;
             if (a.red_u) {
               const double wz = kz == 0 ? 1.0 : (kz < a.N3 / 2 ? 2.0 : 0.0);
-              const double2 r = a.red_u[cc * fstride + idx];
+              const double2 r = sr[PRET ? it : 0][cc];
               red.x += wz * (w.x * r.x + w.y * r.y);
               red.y += wz * (w.y * r.x - w.x * r.y);
             }
             rhs[cc] = a.store_only ? make_double2(0.0, 0.0)
-                                   : cadd(cadd(pl_[cc * fstride + idx], w), cscale(u[cc], a.force));
+                                   : cadd(cadd(sp[PRET ? it : 0][cc], w), cscale(u[cc], a.force));
           } else {
             rhs[cc] = cadd(acc[it][cc], cscale(u[cc], a.force - nu_ * (ksq + 1e-50)));
           }
