@@ -1,9 +1,11 @@
 // Launch one z-pass instantiation: the one-wavefront kernel (zwave.hpp) for
-// lengths 64 * {1,2,3,4,6,8,12}, the block-wide Stockham kernel otherwise (the fused
+// lengths 64 * {1,2,3,4,6,8,12}, the lane-group kernel (zsub.hpp) for the 3/2-rule
+// products at 48, 72 and 96, the block-wide Stockham kernel otherwise (the fused
 // learned closure has only the one-wavefront form).
 #pragma once
 #include "launch.hpp"
 #include "zwave.hpp"
+#include "zsub.hpp"
 
 namespace psx {
 
@@ -37,6 +39,14 @@ hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int va
       return hipErrorInvalidValue;
     }
   } else {
+    if constexpr (ZSub<L>::ok && (MODE == Z_DNS || MODE == Z_SMAG)) {
+      // the short lengths of small grids: a lane group per position
+      constexpr int S = ZSubShape<L>::S;
+      if (variant == 0 && zw_third<L, MODE>(a) && a.npos % S == 0) {
+        hipLaunchKernelGGL((zsub<L, MODE, L / 3>), dim3((unsigned)(a.npos / S)), dim3(64), 0, s, a, tw);
+        return hipGetLastError();
+      }
+    }
     if constexpr (ZWave<L>::ok) {
       if (variant == 0) {
         if constexpr (zw_third_mode<L, MODE>()) {

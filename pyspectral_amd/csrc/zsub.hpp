@@ -1,0 +1,204 @@
+// Fused z pass for the short z lengths of small grids: L = 3 W with W = 16, 24 or 32
+// lanes per position and S = 64 / W positions per wavefront.
+//
+// Same contract as zwave (zwave.hpp) with one position per lane group, and the same
+// data path scaled down.  Lane l < W of group s owns the entries k = l + W m (m < 3)
+// of every complex pencil of position blockIdx.x * S + s, and a length-L transform
+// x_n = sum_k Z_k w_L^{e n k} (e = +1 inverse, -1 forward) runs as a radix-3 register
+// step and a W-point transform over the group's lanes in two LDS passes, W = A B with
+// l = j + A r and b = b1 + B b2:
+//   Y_a(l)                = w_L^{e a l} sum_m Z_{l + W m} w_3^{e a m}      (registers)
+//   U_a(j, b1)            = w_W^{e j b1} sum_r Y_a(j + A r) w_B^{e b1 r}  (LDS pass 1)
+//   x_{a + 3 (b1 + B b2)} = sum_j U_a(j, b1) w_A^{e b2 j}                 (LDS pass 2)
+// The block-wide Stockham kernel (zpass) spends most of its time on barriers and idle
+// threads at these lengths; one wavefront per S positions needs no workgroup barrier.
+// At W = 24 the lanes 48..63 form a third group that computes on the first group's
+// position in LDS of its own and stores nothing.
+#pragma once
+#include "zwave.hpp"
+
+namespace psx {
+
+template <int L> struct ZSub { static constexpr bool ok = false; };
+template <> struct ZSub<48> { static constexpr bool ok = true; static constexpr int W = 16, A = 4, B = 4; };
+template <> struct ZSub<72> { static constexpr bool ok = true; static constexpr int W = 24, A = 8, B = 3; };
+template <> struct ZSub<96> { static constexpr bool ok = true; static constexpr int W = 32, A = 8, B = 4; };
+
+template <int L> struct ZSubShape {
+  static constexpr int W = ZSub<L>::W, A = ZSub<L>::A, B = ZSub<L>::B;
+  static constexpr int R0 = L / W;                      // 3
+  static constexpr int S = 64 / W;                      // positions per wavefront
+  static constexpr int NG = (64 + W - 1) / W;           // lane groups (S, or S + 1 idle)
+  static constexpr int SUBW = B * (A + 1);              // sub-transform slot; pass-1 output at b1 (A + 1) + j
+  static constexpr int NATS = L + L / 16 + 1;           // natural-order row, one pad entry per 16
+  static constexpr int SG = R0 * SUBW > NATS ? R0 * SUBW : NATS;  // LDS entries of one group
+  static __device__ __forceinline__ int nat(int n) { return n + (n >> 4); }
+  static_assert(W <= SUBW, "register-step slots");
+};
+
+// Entries k = l + W m (every l < W) of chunk m all have min(k, L - k) >= HK: the 3/2
+// rule's zero pad, neither loaded nor transformed.
+template <int L, int W, int HK>
+__host__ __device__ constexpr bool zs_zero_chunk(int m) {
+  if (HK <= 0) return false;
+  for (int l = 0; l < W; ++l) {
+    const int k = l + W * m, kk = k > L / 2 ? L - k : k;
+    if (kk < HK) return false;
+  }
+  return true;
+}
+
+// The W-point transforms of the R0 sub-transforms held at sb[a SUBW + l]; the result
+// x_{a + R0 b} lands at sb[nat(a + R0 b)].  twl: the length-L table w_L^k in LDS.
+template <int L, int SIGN>
+__device__ __forceinline__ void zs_subW(double2* sb, const double2* twl, int l) {
+  using Z = ZSubShape<L>;
+  constexpr int A = Z::A, B = Z::B, R0 = Z::R0, SUBW = Z::SUBW;
+  // pass 1: lane l < R0 A is (a, j); a B-point transform over r, twiddle w_W^{e j b1}
+  const bool p1 = l < R0 * A;
+  const int a1 = l / A, j = l - (l / A) * A;
+  double2 y[B];
+  if (p1) {
+#pragma unroll
+    for (int r = 0; r < B; ++r) y[r] = sb[a1 * SUBW + j + A * r];
+    Dft<B, SIGN>::run(y);
+#pragma unroll
+    for (int b1 = 1; b1 < B; ++b1) {
+      double2 w = twl[R0 * j * b1];  // w_L^{R0 j b1} = w_W^{j b1}
+      if constexpr (SIGN > 0) w.y = -w.y;
+      y[b1] = cmul(y[b1], w);
+    }
+  }
+  wave_sync();
+  if (p1) {
+#pragma unroll
+    for (int b1 = 0; b1 < B; ++b1) sb[a1 * SUBW + b1 * (A + 1) + j] = y[b1];
+  }
+  wave_sync();
+  // pass 2: lane l < R0 B is (a, b1); an A-point transform over j
+  const bool p2 = l < R0 * B;
+  const int a2 = l / B, b1 = l - (l / B) * B;
+  double2 z[A];
+  if (p2) {
+#pragma unroll
+    for (int jj = 0; jj < A; ++jj) z[jj] = sb[a2 * SUBW + b1 * (A + 1) + jj];
+    Dft<A, SIGN>::run(z);
+  }
+  wave_sync();
+  if (p2) {
+#pragma unroll
+    for (int b2 = 0; b2 < A; ++b2) sb[Z::nat(a2 + R0 * (b1 + B * b2))] = z[b2];
+  }
+  wave_sync();
+}
+
+// HK: the input and output half spectra hold HK = L / 3 entries (the launcher checks).
+template <int L, int MODE, int HK>
+__global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ tw) {
+  using Z = ZSubShape<L>;
+  static_assert(ZSub<L>::ok, "zsub: L must be 48, 72 or 96");
+  static_assert(MODE == Z_DNS || MODE == Z_SMAG, "zsub: product modes only");
+  constexpr int W = Z::W, R0 = Z::R0, S = Z::S, SUBW = Z::SUBW, SG = Z::SG;
+  constexpr int NIN = ZShape<MODE>::nin, NOUT = ZShape<MODE>::nout;
+  constexpr int NRI = NIN, NRO = NOUT;
+  constexpr int NCI = (NRI + 1) / 2, NCO = (NRO + 1) / 2;
+  constexpr int NF = ZFields<MODE>::n;
+  __shared__ double2 buf[Z::NG * SG];
+  __shared__ double2 twl[L];
+  const int lane = threadIdx.x;
+  const int s = lane / W, l = lane - (lane / W) * W;
+  const bool act = s < S;
+  double2* const sb = buf + s * SG;
+  const ZMember zm = zmember(a, (long long)blockIdx.x * S + (act ? s : 0));
+  const long long pos0 = zm.pos0;
+
+  // register-step twiddles w_L^{-a l} (the inverse uses the conjugates)
+  double2 t1[R0];
+#pragma unroll
+  for (int av = 0; av < R0; ++av) t1[av] = tw[av * l];
+  for (int i = lane; i < L; i += 64) twl[i] = tw[i];
+
+  // ---- inverse transforms; lane l keeps x_n, n = l + W m, of every pencil
+  constexpr ZGroupUse<MODE, 1, NRI, 1> use{};
+  auto load_pencil = [&](int q, double2 (&raw)[R0][1][NF]) {
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      if (zs_zero_chunk<L, W, HK>(m)) continue;
+      const int k = l + W * m, kk = k > L / 2 ? L - k : k;
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi)
+        if ((use.bits[q][0] >> fi) & 1u) raw[m][0][fi] = zw_in<HK>(a, zm, fi, pos0, kk);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  double2 x[NCI][R0];
+  double2 raw[2][R0][1][NF];
+  load_pencil(0, raw[0]);
+  wave_sync();  // the LDS twiddle table
+#pragma unroll
+  for (int q = 0; q < NCI; ++q) {
+    const int cur = q & 1;
+    double2 v[R0];
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      const int k = l + W * m, kk = k > L / 2 ? L - k : k;
+      v[m] = zs_zero_chunk<L, W, HK>(m) ? make_double2(0.0, 0.0)
+                                        : zw_hermitian<L, MODE, 1, NRI, NF>(q, k, a.kzscale * kk, raw[cur][m]);
+    }
+    Dft<R0, +1>::run(v);
+    sb[l] = v[0];
+#pragma unroll
+    for (int av = 1; av < R0; ++av) sb[av * SUBW + l] = cmul(v[av], cconj(t1[av]));
+    if (q + 1 < NCI) load_pencil(q + 1, raw[cur ^ 1]);
+    wave_sync();
+    zs_subW<L, +1>(sb, twl, l);
+#pragma unroll
+    for (int m = 0; m < R0; ++m) x[q][m] = sb[Z::nat(l + W * m)];
+    wave_sync();
+  }
+
+  // ---- pointwise products / closure on the lane's R0 z points
+  double2 xo[NCO][R0];
+#pragma unroll
+  for (int m = 0; m < R0; ++m) {
+    double in[NIN], o[NOUT];
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) in[f] = (f & 1) ? x[f >> 1][m].y : x[f >> 1][m].x;
+    zpoint<MODE>(a, in, o);
+    zpoint_fold<MODE>(a, o);
+#pragma unroll
+    for (int q = 0; q < NCO; ++q) {
+      xo[q][m].x = o[2 * q];
+      xo[q][m].y = 2 * q + 1 < NRO ? o[2 * q + 1] : 0.0;
+    }
+  }
+
+  // ---- forward transforms, truncation to HK modes, split of the pencil pairs
+#pragma unroll
+  for (int q = 0; q < NCO; ++q) {
+    double2 v[R0];
+#pragma unroll
+    for (int m = 0; m < R0; ++m) v[m] = xo[q][m];
+    Dft<R0, -1>::run(v);
+    sb[l] = v[0];
+#pragma unroll
+    for (int av = 1; av < R0; ++av) sb[av * SUBW + l] = cmul(v[av], t1[av]);
+    wave_sync();
+    zs_subW<L, -1>(sb, twl, l);
+    if (act) {
+      const int r0 = 2 * q, r1 = 2 * q + 1;
+#pragma unroll
+      for (int k = l; k < HK; k += W) {
+        const double2 z1 = sb[Z::nat(k)];
+        const double2 z2 = sb[Z::nat(k == 0 ? 0 : L - k)];
+        zm.out[(long long)r0 * a.out_sf + pos0 * a.Hout + k] = make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
+        if (r1 < NRO)
+          zm.out[(long long)r1 * a.out_sf + pos0 * a.Hout + k] =
+              make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
+      }
+    }
+    wave_sync();
+  }
+}
+
+}  // namespace psx
