@@ -208,6 +208,9 @@ def bench_ensemble(args):
     q = ens.new_state()
     for i, n in enumerate(sizes):
         ens.member(q, i).copy_(torch.from_numpy(cases.decaying_isotropic(n, seed=args.seed + i, k0=k0s[i])))
+    tuning = None
+    if not args.no_autotune:
+        tuning = ens.autotune(q, reps=3)   # per size class, on the members' own states
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         qs, dt = q.clone(), ens.new_dt()
@@ -260,6 +263,7 @@ def bench_ensemble(args):
                           "achieved": round(bytes_step / (el / args.steps) / 1e9, 1), "unit": "GB/s",
                           "frac": round(bytes_step / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)},
         "unbatched": unbatched,
+        "tuning": {"per_size_class": tuning, "classes": sorted(set(sizes), key=sizes.index)} if tuning else None,
         "stats_after": {"E_mean": float(E[:, 0].mean()), "finite": bool(finite and np.isfinite(E).all())},
     }
 
@@ -742,7 +746,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build advances one grid on one GPU; runs on more than one GPU are not part "
-              "of it (see DESIGN.md for how the grid would shard).", file=sys.stderr)
+              "of it (the design notes describe how the grid would shard).", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -878,10 +882,11 @@ def main():
     if args.closure == "smagorinsky" and args.tmodel_steps > 0:
         out["tmodel"] = bench_tmodel(args, uh, copy_gbs)
     if args.closure == "smagorinsky" and args.ensemble_steps > 0:
-        ea = argparse.Namespace(seed=args.seed, steps=args.ensemble_steps, warmup=5, unbatched_steps=0)
+        ea = argparse.Namespace(seed=args.seed, steps=args.ensemble_steps, warmup=5, unbatched_steps=0,
+                                no_autotune=args.no_autotune)
         ens = bench_ensemble(ea)
         out["ensemble"] = {k: ens[k] for k in ("value", "unit", "steps", "warmup", "ms_per_step", "config",
-                                              "step_roofline", "stats_after")}
+                                              "step_roofline", "tuning", "stats_after")}
     if args.cpu_steps > 0 and args.closure != "learned":
         out["cpu_baseline"] = cpu_baseline(N, args.closure, nu, cfl, args.cpu_steps, args.seed)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

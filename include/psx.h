@@ -101,6 +101,17 @@ int psx_ensemble_set_stream(psx_ensemble* ens, void* stream);
 int psx_ensemble_step(psx_ensemble* ens, double* q, double* qs, double* dt_dev, double t);
 int psx_ensemble_step_graph(psx_ensemble* ens, double* q, double* qs, double* dt_dev);
 int psx_ensemble_stats(psx_ensemble* ens, const double* q, double* out_dev);
+/* Number of size classes (distinct member grids, in order of first appearance). */
+int psx_ensemble_classes(const psx_ensemble* ens);
+/* Kernel shapes for every size class (no counterpart in the reference): coordinate
+ * descent on the time of the whole ensemble step, the classes running concurrently as
+ * psx_ensemble_step runs them -- for each class, the tile widths, z-pass variant, block
+ * orders and stress-sum placement of psx_set_tuning (tune[0..3], [7..10]) are tried in
+ * turn, each candidate timed on reps steps from a copy of q (the ensemble state; q is
+ * not changed), and the fastest kept (a change must gain 1%).  Allocates two ensemble
+ * states for the trials.  tune_out (may be NULL) receives PSX_TUNE_N values per class,
+ * class-major.  The captured ensemble step is re-captured by its next call. */
+int psx_ensemble_autotune(psx_ensemble* ens, const double* q, int reps, int* tune_out);
 /* psx_status over every member: waits for the ensemble's stream. */
 int psx_ensemble_status(psx_ensemble* ens);
 /* psx_set_closure for every member of a PSX_MODEL_LEARNED ensemble (one network). */

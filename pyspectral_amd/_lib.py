@@ -36,7 +36,7 @@ EXPORTS = (
     "psx_mlp_set_params", "psx_mlp_set_norm", "psx_mlp_forward", "psx_mlp_grad", "psx_mlp_gradient",
     "psx_mlp_train_step", "psx_set_closure", "psx_physical", "psx_gradients", "psx_restrict",
     "psx_ensemble_set_closure", "psx_mlp_closure", "psx_status", "psx_clear_status", "psx_ensemble_status",
-    "psx_stats_noalloc", "psx_spectral", "psx_stream_probe",
+    "psx_stats_noalloc", "psx_spectral", "psx_stream_probe", "psx_ensemble_classes", "psx_ensemble_autotune",
 )
 
 MLP_NPARAM = 9350
@@ -139,6 +139,8 @@ def load() -> C.CDLL:
         "psx_stats_noalloc": (i, [dp, i, i, i, i, d, d, d, i, dp, vp]),
         "psx_spectral": (i, [vp, dp, i, dp]),
         "psx_stream_probe": (i, [dp, dp, C.c_longlong, i, i, vp]),
+        "psx_ensemble_classes": (i, [vp]),
+        "psx_ensemble_autotune": (i, [vp, dp, i, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

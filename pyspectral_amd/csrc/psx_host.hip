@@ -2154,6 +2154,94 @@ int psx_ensemble_set_closure(psx_ensemble* e, psx_mlp* net, double bound, int fl
   // the captured ensemble step is rebuilt: psx_set_closure advanced every class's generation
   return PSX_OK;
 }
+int psx_ensemble_classes(const psx_ensemble* e) { return e ? (int)e->cls.size() : PSX_E_ARG; }
+
+int psx_ensemble_autotune(psx_ensemble* e, const double* q, int reps, int* tune_out) {
+  if (!e || !q) return PSX_E_ARG;
+  if (reps < 1) reps = 1;
+  // Coordinate descent on the time of the whole ensemble step, the classes running
+  // concurrently as psx_ensemble_step runs them (a class timed alone favours shapes
+  // that fill the GPU by themselves): for each class and knob, every candidate is timed
+  // on reps steps from q (copied, so q is not changed) and the fastest kept.
+  const size_t bytes = (size_t)e->total * sizeof(double2);
+  double2* tq = nullptr;
+  double2* tqs = nullptr;
+  double* tdt = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  int rc = PSX_OK;
+  if (hipMalloc(&tq, bytes) != hipSuccess || hipMalloc(&tqs, bytes) != hipSuccess ||
+      hipMalloc(&tdt, sizeof(double) * e->count) != hipSuccess || hipEventCreate(&t0) != hipSuccess ||
+      hipEventCreate(&t1) != hipSuccess) {
+    e->err = "psx_ensemble_autotune: allocation failed";
+    rc = PSX_E_HIP;
+  }
+  auto trial = [&](double& ms) -> int {
+    if (hipMemcpyAsync(tq, q, bytes, hipMemcpyDeviceToDevice, e->stream) != hipSuccess ||
+        hipMemcpyAsync(tqs, q, bytes, hipMemcpyDeviceToDevice, e->stream) != hipSuccess) {
+      e->err = "psx_ensemble_autotune: hipMemcpyAsync failed";
+      return PSX_E_HIP;
+    }
+    int r = ensemble_step_classes(e, (double*)tq, (double*)tqs, tdt, 0.0);  // warm-up
+    if (r != PSX_OK) return r;
+    if (hipEventRecord(t0, e->stream) != hipSuccess) return PSX_E_HIP;
+    for (int i = 0; i < reps && r == PSX_OK; ++i) r = ensemble_step_classes(e, (double*)tq, (double*)tqs, tdt, 0.0);
+    if (r != PSX_OK) return r;
+    float f = 0.f;
+    if (hipEventRecord(t1, e->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+        hipEventElapsedTime(&f, t0, t1) != hipSuccess) {
+      e->err = "psx_ensemble_autotune: event timing failed";
+      return PSX_E_HIP;
+    }
+    ms = f;
+    return PSX_OK;
+  };
+  // tiles, z-pass variant, block orders, stress sums (chunking and the closure's
+  // placement stay as they are)
+  const int knobs[] = {0, 1, 2, 3, 7, 8, 9, 10};
+  for (size_t k = 0; k < e->cls.size() && rc == PSX_OK; ++k) {
+    psx_context* c = e->cls[k];
+    int L[3];
+    tuned_lengths(c, L);
+    const bool smag_form = c->model != PSX_MODEL_DNS && !tau_family(c->model);
+    for (int knob : knobs) {
+      if (rc != PSX_OK) break;
+      if (knob == 10 && !smag_form) continue;
+      std::vector<int> cand{c->tune[knob]};  // the current choice first: others must beat it
+      if (knob < 3) {
+        for (int T : {4, 8, 16})
+          if (pencil_tile(L[knob], T) == T && T != c->tune[knob]) cand.push_back(T);
+      } else {
+        cand.push_back(c->tune[knob] ? 0 : 1);
+      }
+      double best = 1e300;
+      int pick = c->tune[knob];
+      for (int v : cand) {
+        c->tune[knob] = v;
+        double ms = 0.0;
+        rc = trial(ms);
+        if (rc != PSX_OK) break;
+        if (ms < (best < 1e300 ? 0.99 * best : best)) {
+          best = ms;
+          pick = v;
+        }
+      }
+      c->tune[knob] = pick;
+    }
+  }
+  for (size_t k = 0; k < e->cls.size(); ++k) {
+    psx_context* c = e->cls[k];
+    drop_graph(c);  // the captured ensemble step is rebuilt: every class's generation advanced
+    if (tune_out)
+      for (int i = 0; i < PSX_TUNE_N; ++i) tune_out[k * PSX_TUNE_N + i] = c->tune[i];
+  }
+  if (hipStreamSynchronize(e->stream) != hipSuccess && rc == PSX_OK) rc = PSX_E_HIP;
+  hipFree(tq);
+  hipFree(tqs);
+  hipFree(tdt);
+  if (t0) hipEventDestroy(t0);
+  if (t1) hipEventDestroy(t1);
+  return rc;
+}
 long long psx_ensemble_size(const psx_ensemble* e) { return e ? e->total : PSX_E_ARG; }
 long long psx_ensemble_offset(const psx_ensemble* e, int member) {
   return (e && member >= 0 && member < e->count) ? e->offset[member] : PSX_E_ARG;

@@ -110,6 +110,23 @@ class Ensemble:
         a = self._args(q, qs, dt)
         self._check(self.lib.psx_ensemble_step_graph(self.ens, *a), "psx_ensemble_step_graph")
 
+    def autotune(self, q, reps: int = 3) -> list:
+        """Choose each size class's kernel shapes by the time of the whole ensemble step
+        (classes running concurrently), one knob at a time, on steps from a copy of
+        ``q``; returns one tuning dict per class, classes in order of their first
+        members."""
+        from .solver import SpectralSolver
+
+        n = self.lib.psx_ensemble_classes(self.ens)
+        if n < 0:
+            raise _lib.PsxError(f"psx_ensemble_classes failed ({n})")
+        keys = SpectralSolver.TUNING_KEYS
+        out = (C.c_int * (n * len(keys)))()
+        a = self._args(q, q, self.new_dt())
+        self._check(self.lib.psx_ensemble_autotune(self.ens, a[0], int(reps), out), "psx_ensemble_autotune")
+        torch.cuda.current_stream(self.device).synchronize()
+        return [dict(zip(keys, list(out)[k * len(keys):(k + 1) * len(keys)])) for k in range(n)]
+
     def set_closure(self, net, bound: float = float("inf"), clip_backscatter: bool = False) -> None:
         """Attach a learned-closure network to every member of a ``"learned"`` ensemble."""
         self._closure = net

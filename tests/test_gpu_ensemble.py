@@ -83,3 +83,42 @@ def test_ensemble_graph_replays_match_eager():
         assert torch.equal(d1, d2), step
     e1.close()
     e2.close()
+
+
+def test_ensemble_autotune_keeps_the_results():
+    """psx_ensemble_autotune times each size class's kernel shapes on ensemble steps
+    from a copy of the state (the state itself is unchanged); the replay captured after
+    it matches the step of an untuned ensemble to round-off, and a graph captured before
+    the tuning is re-captured rather than replayed with the old shapes."""
+    from pyspectral_amd.ensemble import Ensemble
+    from pyspectral_amd.solver import SpectralSolver
+
+    sizes = [32, 16, 48, 16, 64]
+    nus = [1e-3, 4e-3, 2e-3, 6e-4, 1e-3]
+    e1 = Ensemble([(n, n, n) for n in sizes], nus, model=1, cfl=0.5)
+    e2 = Ensemble([(n, n, n) for n in sizes], nus, model=1, cfl=0.5)
+    q1 = e1.new_state()
+    for i, n in enumerate(sizes):
+        e1.member(q1, i).copy_(torch.from_numpy(np.stack(so.random_isotropic(n, seed=i, kp=3.0))))
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        q2, s2, d2 = q1.clone(), q1.clone(), e2.new_dt()
+        e2.step_graph(q2, s2, d2)          # captured with the built-in shapes
+        q2.copy_(q1)
+        s2.copy_(q1)
+        q_before = q1.clone()
+        tuned = e2.autotune(q1, reps=2)
+        assert torch.equal(q1, q_before)
+        e2.step_graph(q2, s2, d2)          # re-captured with the tuned shapes
+    stream.synchronize()
+    print("\n", tuned, flush=True)
+    assert len(tuned) == 4 and all(set(t) == set(SpectralSolver.TUNING_KEYS) for t in tuned)
+    s1, d1 = q1.clone(), e1.new_dt()
+    e1.step(q1, s1, d1)
+    torch.cuda.synchronize()
+    # the CFL z pass of a retuned class may use the other kernel variant: dt to round-off
+    assert (d1 - d2).abs().max().item() <= 1e-14 * d1.abs().max().item()
+    err = (q1 - q2).abs().max().item() / q1.abs().max().item()
+    assert err < 1e-13, err
+    e1.close()
+    e2.close()
