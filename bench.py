@@ -324,66 +324,86 @@ def _spectral_errors(E, ref, NL):
     return out
 
 
+def _closure_a_priori(net, x_te, t_te, c2_les, bound, rel, corr):
+    """A-priori stress errors of one fitted network on held-out pairs."""
+    import numpy as np
+    import torch
+
+    from pyspectral_amd.learned import LAYOUT
+
+    y = net.forward(x_te).double()
+    yc = net.closure(x_te, c2_les, bound, True).double()
+    p = torch.from_numpy(net.params().astype(np.float64)).cuda()
+    P = {k: p[o:o + int(np.prod(sh))].reshape(sh) for k, o, sh in LAYOUT}
+    mx, sx, my, sy = (torch.from_numpy(np.asarray(a, np.float64)).cuda() for a in net.norm)
+    h = (x_te.double() - mx) / sx
+    for i in (1, 2, 3):
+        h = torch.tanh(h @ P[f"W{i}"].T + P[f"b{i}"])
+    y64 = (h @ P["W4"].T + P["b4"]) * sy + my
+    g = x_te.double()
+    S = torch.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
+                     0.5 * (g[:, 5] + g[:, 7])], 1)
+    Smag = torch.sqrt(2 * (S[:, :3] ** 2).sum(1) + 4 * (S[:, 3:] ** 2).sum(1))
+    tau_sm = -2 * c2_les * Smag[:, None] * S
+    T = t_te.double()
+    w6 = torch.tensor([1, 1, 1, 2, 2, 2.], device=T.device, dtype=T.dtype)
+    diss = lambda tau: float(-(tau * S * w6).sum(1).mean())  # noqa: E731  mean SGS dissipation
+    return {
+        "held_out_points": int(x_te.shape[0]),
+        "bf16_kernel_vs_fp64_network_rel_L2": round(rel(y, y64), 5),
+        "network_vs_filtered_dns_rel_L2": round(rel(y, T), 4),
+        "network_corr_per_component": corr(y, T),
+        "clipped_closure_vs_filtered_dns_rel_L2": round(rel(yc, T), 4),
+        "smagorinsky_corr_per_component": corr(tau_sm, T),
+        "mean_dissipation": {"filtered_dns": round(diss(T), 5), "network": round(diss(y), 5),
+                             "clipped_closure": round(diss(yc), 5), "smagorinsky": round(diss(tau_sm), 5)},
+        "backscatter_fraction_network": round(float(((y * S * w6).sum(1) > 0).double().mean()), 4),
+    }
+
+
 def bench_closure(args):
-    """The learned closure against filtered DNS.  Training data: forced isotropic DNS
-    (linear forcing A = 0.1, nu = 5e-4, CFL 0.5) at --train-N^3 for --train-steps steps
-    from the seeded field, filtered and restricted to the LES grid every 20 steps with
-    two filters: the box of width 2 dx_LES and the sharp cutoff at the LES grid's
-    Nyquist (kc = N_LES/2).  One network is fitted per filter with Adam (lr 1e-3 decaying
-    to 1e-4, batch 65536) on all but the last tenth of the snapshots, which are held out
-    for the a-priori stress errors.  Judge: forced DNS at --judge-N^3 from the same
-    seeded field to the times --times.  LES at --les-N^3 from the filtered initial field:
-    each network with and without the backscatter clip, Smagorinsky, dynamic
-    Smagorinsky and the t-model, each timed on the GPU; E(k) errors against the
-    box-filtered and the sharp-cut DNS spectra."""
+    """The learned closure against filtered DNS, on each LES grid of --les-N (one training
+    DNS and one judge DNS serve them all).  Training data: forced isotropic DNS (linear
+    forcing A = 0.1, nu = 5e-4, CFL 0.5) at --train-N^3 for --train-steps steps from the
+    seeded field, filtered and restricted to each LES grid every 20 steps with two
+    filters: the box of width 2 dx_LES and the sharp cutoff at the LES grid's Nyquist
+    (kc = N_LES/2).  One network is fitted per filter (and per target form: full or
+    deviatoric stress) with Adam (lr 1e-3 decaying to 1e-4, batch 65536) on all but the
+    last tenth of the snapshots, which are held out for the a-priori stress errors.
+    A-posteriori fit: the output scale of each filter's clipped network is chosen with
+    the LES in the loop (golden-section search on log scale over [0.02, 2]) to minimise
+    the rms shell-by-shell relative E(k) error up to the cutoff at t = --aposteriori-time
+    against the training DNS's filtered spectrum at that time -- the judge DNS takes no
+    part in it.  Judge: forced DNS at --judge-N^3 from the
+    same seeded field to the times --times.  LES from the filtered initial field: each
+    network with and without the backscatter clip, Smagorinsky, dynamic Smagorinsky and
+    the t-model, each timed on the GPU; E(k) errors against the box-filtered and the
+    sharp-cut DNS spectra."""
+    import math
+
     import numpy as np
     import torch
 
     from pyspectral_amd import cases
-    from pyspectral_amd.learned import MLP, LAYOUT, calibrate_dissipation, fit, input_range, training_pairs
+    from pyspectral_amd.learned import MLP, calibrate_dissipation, fit, input_range, training_pairs
     from pyspectral_amd.solver import SpectralSolver
 
     nu, A, cfl = 5e-4, 0.1, 0.5
-    NL, NT, NJ = args.les_N, args.train_N, args.judge_N
-    delta = 2.0 * (2.0 * np.pi / NL)
-    c2_les = (0.16 * 2.0 * np.pi / NL) ** 2
+    NLs = [int(x) for x in str(args.les_N).split(",")]
+    NT, NJ = args.train_N, args.judge_N
     times = [float(x) for x in args.times.split(",")]
+    t_ap = args.aposteriori_time
     filters = ("box", "sharp")
-    u0 = cases.decaying_isotropic(NL, seed=args.seed)  # the seeded field, fully resolved at NL
+    delta = {NL: 2.0 * (2.0 * np.pi / NL) for NL in NLs}
+    c2_les = {NL: (0.16 * 2.0 * np.pi / NL) ** 2 for NL in NLs}
+    u0 = cases.decaying_isotropic(max(NLs), seed=args.seed)  # the seeded field, fully resolved
     out = {"metric": "learned-LES speedup over DNS at <=5% E(k) error", "unit": "x (DNS GPU time / LES GPU time)",
            "n_gpus": 1, "higher_is_better": True, "dtype": "f64 (network bf16/fp32)", "data": "synthetic",
            "config": {"workload": f"forced isotropic turbulence, A={A}, nu={nu}, CFL={cfl}; training DNS {NT}^3 "
                                   f"x {args.train_steps} steps, filters: box width 2 dx_LES and sharp cutoff at "
-                                  f"k = {NL // 2}; judge DNS {NJ}^3; LES {NL}^3 to t = {times}",
+                                  f"k = N_LES/2; judge DNS {NJ}^3; LES {NLs} to t = {times}",
                       "parallelism": "single GPU"}}
-
-    # ---- training data from the forced DNS (both filters from the same snapshots)
-    t0 = time.perf_counter()
-    les = SpectralSolver(NL, NL, NL, "learned", nu=nu, cfl=cfl, force=A)
-    dns = SpectralSolver(NT, NT, NT, 0, nu=nu, cfl=cfl, force=A, kc=NL // 2)
-    q = torch.from_numpy(cases.prolong(u0, NT)).to("cuda")
-    qs = q.clone()
-    pairs = {f: ([], []) for f in filters}
-    t = 0.0
-    for step in range(args.train_steps + 1):
-        if step % 20 == 0:
-            for f in filters:
-                x, tt = training_pairs(dns, q, les, delta, f)
-                pairs[f][0].append(x)
-                pairs[f][1].append(tt)
-        if step == args.train_steps:
-            break
-        t += float(dns.step(q, qs, t).item())
-        if step % 200 == 0:
-            _log(f"training DNS step {step}, t = {t:.3f}")
-    torch.cuda.synchronize()
-    dns.close()
-    del q, qs
-    t_data = time.perf_counter() - t0
-    nsnap = len(pairs["box"][0])
-    out["training_dns"] = {"grid": NT, "steps": args.train_steps, "t_end": round(t, 4), "snapshots": nsnap,
-                           "seconds": round(t_data, 2)}
-    n_hold = max(1, nsnap // 10)
+    les = {NL: SpectralSolver(NL, NL, NL, "learned", nu=nu, cfl=cfl, force=A) for NL in NLs}
 
     def rel(a, b):
         return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
@@ -397,69 +417,92 @@ def bench_closure(args):
         d[:, :3] -= d[:, :3].mean(dim=1, keepdim=True)
         return d
 
-    nets, bounds, out["a_priori"] = {}, {}, {}
-    for f0, dev in ((f, d) for f in filters for d in (False, True)):
-        f = f0 + ("_dev" if dev else "")
-        xs, ts = pairs[f0]
-        x_tr, t_tr = torch.cat(xs[args.skip_snapshots:-n_hold]), torch.cat(ts[args.skip_snapshots:-n_hold])
-        x_te, t_te = torch.cat(xs[-n_hold:]), torch.cat(ts[-n_hold:])
-        if dev:
-            t_tr, t_te = deviatoric(t_tr), deviatoric(t_te)
-        s0 = time.perf_counter()
-        net = MLP(seed=args.seed)
-        losses = fit(net, x_tr, t_tr, steps=args.fit_steps, batch=65536, lr=1e-3, lr_final=1e-4, seed=args.seed)
-        torch.cuda.synchronize()
-        t_fit = time.perf_counter() - s0
-        bound = input_range(net, x_tr)
-        nets[f], bounds[f] = net, bound
-        # a copy whose outputs are scaled to the filtered-DNS mean subgrid dissipation
-        cal = MLP(seed=args.seed)
-        cal.set_params(net.params())
-        cal.set_norm(*net.norm)
-        alpha = calibrate_dissipation(cal, x_tr, t_tr, c2_les, bound, True)
-        nets[f + "_cal"], bounds[f + "_cal"] = cal, bound
-        _log(f"fit ({f}): loss {np.mean(losses[:20]):.4f} -> {np.mean(losses[-20:]):.4f} in {t_fit:.1f} s")
-        # a-priori stress errors on the held-out snapshots
-        y = net.forward(x_te).double()
-        yc = net.closure(x_te, c2_les, bound, True).double()
-        p = torch.from_numpy(net.params().astype(np.float64)).cuda()
-        P = {k: p[o:o + int(np.prod(sh))].reshape(sh) for k, o, sh in LAYOUT}
-        mx, sx, my, sy = (torch.from_numpy(np.asarray(a, np.float64)).cuda() for a in net.norm)
-        h = (x_te.double() - mx) / sx
-        for i in (1, 2, 3):
-            h = torch.tanh(h @ P[f"W{i}"].T + P[f"b{i}"])
-        y64 = (h @ P["W4"].T + P["b4"]) * sy + my
-        g = x_te.double()
-        S = torch.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
-                         0.5 * (g[:, 5] + g[:, 7])], 1)
-        Smag = torch.sqrt(2 * (S[:, :3] ** 2).sum(1) + 4 * (S[:, 3:] ** 2).sum(1))
-        tau_sm = -2 * c2_les * Smag[:, None] * S
-        T = t_te.double()
-        w6 = torch.tensor([1, 1, 1, 2, 2, 2.], device=T.device, dtype=T.dtype)
-        diss = lambda tau: float(-(tau * S * w6).sum(1).mean())  # noqa: E731  mean SGS dissipation
-        out["a_priori"][f] = {
-            "held_out_points": int(x_te.shape[0]),
-            "bf16_kernel_vs_fp64_network_rel_L2": round(rel(y, y64), 5),
-            "network_vs_filtered_dns_rel_L2": round(rel(y, T), 4),
-            "network_corr_per_component": corr(y, T),
-            "clipped_closure_vs_filtered_dns_rel_L2": round(rel(yc, T), 4),
-            "smagorinsky_corr_per_component": corr(tau_sm, T),
-            "mean_dissipation": {"filtered_dns": round(diss(T), 5), "network": round(diss(y), 5),
-                                 "clipped_closure": round(diss(yc), 5), "smagorinsky": round(diss(tau_sm), 5)},
-            "backscatter_fraction_network": round(float(((y * S * w6).sum(1) > 0).double().mean()), 4),
-            "dissipation_calibration_factor": round(alpha, 4),
-            "fit": {"points": int(x_tr.shape[0]), "steps": args.fit_steps,
-                    "loss_first": round(float(np.mean(losses[:20])), 4),
-                    "loss_last": round(float(np.mean(losses[-20:])), 4), "seconds": round(t_fit, 2),
-                    "bound": round(bound, 3)},
-        }
-        del x_tr, t_tr, x_te, t_te, y, yc, y64, g, S, T
+    def spectra(dns, q, n_src):
+        """The box-filtered and sharp-cut shell spectra of a DNS state on every LES grid."""
+        r = {}
+        for NL in NLs:
+            fb = dns.filter(q, "box", delta[NL])
+            r[NL] = {"box": _shell_energy(les[NL], les[NL].restrict(fb, (n_src,) * 3)),
+                     "sharp": _shell_energy(les[NL], les[NL].restrict(q, (n_src,) * 3))}
+            del fb
+        return r
+
+    # ---- training data from the forced DNS (every grid and filter from the same snapshots)
+    t0 = time.perf_counter()
+    dns = SpectralSolver(NT, NT, NT, 0, nu=nu, cfl=cfl, force=A, kc=min(NLs) // 2)
+    q = torch.from_numpy(cases.prolong(u0, NT)).to("cuda")
+    qs = q.clone()
+    pairs = {(NL, f): ([], []) for NL in NLs for f in filters}
+    ap_ref = None
+    t = 0.0
+    for step in range(args.train_steps + 1):
+        if step % 20 == 0:
+            for NL in NLs:
+                dns.set_params(kc=NL // 2)  # the sharp filter's cutoff: this LES grid's Nyquist
+                for f in filters:
+                    x, tt = training_pairs(dns, q, les[NL], delta[NL], f)
+                    pairs[(NL, f)][0].append(x)
+                    pairs[(NL, f)][1].append(tt)
+        if step == args.train_steps:
+            break
+        dt = float(dns.compute_dt(qs).item())
+        if ap_ref is None and t + dt >= t_ap - 1e-12:
+            # land on the a-posteriori fitting time and record the target spectra there
+            dns.set_params(cfl=-(t_ap - t))
+            dns.step(q, qs, t)
+            dns.set_params(cfl=cfl)
+            t = t_ap
+            ap_ref = spectra(dns, q, NT)
+        else:
+            t += float(dns.step(q, qs, t).item())
+        if step % 200 == 0:
+            _log(f"training DNS step {step}, t = {t:.3f}")
+    torch.cuda.synchronize()
+    dns.close()
+    del q, qs
+    t_data = time.perf_counter() - t0
+    nsnap = len(pairs[(NLs[0], "box")][0])
+    out["training_dns"] = {"grid": NT, "steps": args.train_steps, "t_end": round(t, 4), "snapshots": nsnap,
+                           "seconds": round(t_data, 2), "aposteriori_target_time": t_ap}
+    n_hold = max(1, nsnap // 10)
+
+    nets, bounds, a_priori = {}, {}, {NL: {} for NL in NLs}
+    for NL in NLs:
+        for f0, dev in ((f, d) for f in filters for d in (False, True)):
+            f = f0 + ("_dev" if dev else "")
+            xs, ts = pairs[(NL, f0)]
+            x_tr, t_tr = torch.cat(xs[args.skip_snapshots:-n_hold]), torch.cat(ts[args.skip_snapshots:-n_hold])
+            x_te, t_te = torch.cat(xs[-n_hold:]), torch.cat(ts[-n_hold:])
+            if dev:
+                t_tr, t_te = deviatoric(t_tr), deviatoric(t_te)
+            s0 = time.perf_counter()
+            net = MLP(seed=args.seed)
+            losses = fit(net, x_tr, t_tr, steps=args.fit_steps, batch=65536, lr=1e-3, lr_final=1e-4, seed=args.seed)
+            torch.cuda.synchronize()
+            t_fit = time.perf_counter() - s0
+            bound = input_range(net, x_tr)
+            nets[(NL, f)], bounds[(NL, f)] = net, bound
+            # a copy whose outputs are scaled to the filtered-DNS mean subgrid dissipation
+            cal = MLP(seed=args.seed)
+            cal.set_params(net.params())
+            cal.set_norm(*net.norm)
+            alpha = calibrate_dissipation(cal, x_tr, t_tr, c2_les[NL], bound, True)
+            nets[(NL, f + "_cal")], bounds[(NL, f + "_cal")] = cal, bound
+            _log(f"fit ({NL}^3, {f}): loss {np.mean(losses[:20]):.4f} -> {np.mean(losses[-20:]):.4f} "
+                 f"in {t_fit:.1f} s")
+            a_priori[NL][f] = dict(_closure_a_priori(net, x_te, t_te, c2_les[NL], bound, rel, corr),
+                                   dissipation_calibration_factor=round(alpha, 4),
+                                   fit={"points": int(x_tr.shape[0]), "steps": args.fit_steps,
+                                        "loss_first": round(float(np.mean(losses[:20])), 4),
+                                        "loss_last": round(float(np.mean(losses[-20:])), 4),
+                                        "seconds": round(t_fit, 2), "bound": round(bound, 3)})
+            del x_tr, t_tr, x_te, t_te
     del pairs
     torch.cuda.empty_cache()
 
     # ---- judge DNS, or its spectra from an earlier run of the same configuration
-    refs = {"box": {}, "sharp": {}}
-    key = {"NJ": NJ, "NL": NL, "times": times, "seed": args.seed, "nu": nu, "A": A, "cfl": cfl}
+    refs = {NL: {"box": {}, "sharp": {}} for NL in NLs}
+    key = {"NJ": NJ, "NL": NLs, "times": times, "seed": args.seed, "nu": nu, "A": A, "cfl": cfl}
     cache = None
     if args.judge_cache and os.path.exists(args.judge_cache):
         with open(args.judge_cache) as fh:
@@ -467,8 +510,9 @@ def bench_closure(args):
         if cache.get("key") != key:
             cache = None
     if cache is not None:
-        for kind in refs:
-            refs[kind] = {float(t_): np.asarray(v) for t_, v in cache[kind].items()}
+        for NL in NLs:
+            for kind in ("box", "sharp"):
+                refs[NL][kind] = {float(t_): np.asarray(v) for t_, v in cache[str(NL)][kind].items()}
         out["judge_dns"] = dict(cache["judge_dns"], from_cache=True)
         _log("judge DNS spectra from the cache")
     else:
@@ -482,10 +526,10 @@ def bench_closure(args):
                                   on_step=lambda n, tt: _log(f"judge DNS t = {tt:.3f}") if n % 200 == 0 else None)
             wall += time.perf_counter() - s0
             nsteps += n
-            fb = judge.filter(qj, "box", delta)
-            refs["box"][T_] = _shell_energy(les, les.restrict(fb, (NJ, NJ, NJ)))
-            del fb
-            refs["sharp"][T_] = _shell_energy(les, les.restrict(qj, (NJ, NJ, NJ)))
+            sp = spectra(judge, qj, NJ)
+            for NL in NLs:
+                for kind in ("box", "sharp"):
+                    refs[NL][kind][T_] = sp[NL][kind]
             _log(f"judge DNS reached t = {T_} ({nsteps} steps, {wall:.1f} s)")
         judge.close()
         del qj, qjs
@@ -494,59 +538,119 @@ def bench_closure(args):
         if args.judge_cache:
             with open(args.judge_cache, "w") as fh:
                 json.dump({"key": key, "judge_dns": out["judge_dns"],
-                           **{kind: {str(t_): v.tolist() for t_, v in refs[kind].items()} for kind in refs}}, fh)
+                           **{str(NL): {kind: {str(t_): v.tolist() for t_, v in refs[NL][kind].items()}
+                                        for kind in ("box", "sharp")} for NL in NLs}}, fh)
 
-    # ---- LES runs
-    ud = torch.from_numpy(u0).to("cuda")
-    init = {"box": les.filter(ud, "box", delta), "sharp": ud}
-    runs = {}
-    variants = [("learned_box_plain", "learned", "box", False)]
-    variants += [(f"learned_{f}{d}{c}_clip", "learned", f + d + c, True)
-                 for f in filters for d in ("", "_dev") for c in ("", "_cal")]
-    variants += [("smagorinsky", 1, None, False), ("dsmag", 7, None, False), ("tmodel", 2, None, False)]
-    for name, model, f, clip in variants:
+    def run_les(NL, model, net, bound, clip, ref_kind, until):
+        """One LES from the filtered initial field to the times in `until`: (E(k) at each
+        time or None after a blow-up, steps, GPU seconds)."""
         s = SpectralSolver(NL, NL, NL, model, nu=nu, cfl=cfl, force=A, Ct=0.1)
         if model == "learned":
-            s.set_closure(nets[f], bounds[f], clip_backscatter=clip)
-        ref_kind = (f or "box").split("_")[0]
-        q = init[ref_kind].clone()
+            s.set_closure(net, bound, clip_backscatter=clip)
+        ud = torch.from_numpy(u0).to("cuda")
+        if NL < u0.shape[1]:  # the seeded field (made on the largest LES grid) restricted
+            ud = les[NL].restrict(ud, u0.shape[1:3] + (u0.shape[1],))
+        q = les[NL].filter(ud, "box", delta[NL]) if ref_kind == "box" else ud
         qs = q.clone()
         s.step(q.clone(), qs.clone(), 0.0)  # warm-up (first-call setup)
         torch.cuda.synchronize()
-        t, wall, nsteps, errs, ok = 0.0, 0.0, 0, {}, True
-        for T_ in times:
+        t, wall, nsteps, Es = 0.0, 0.0, 0, []
+        for T_ in until:
             s0 = time.perf_counter()
             t, n, ok = _advance_to(s, q, qs, t, T_)
             wall += time.perf_counter() - s0
             nsteps += n
+            Es.append(_shell_energy(les[NL], q) if ok else None)
             if not ok:
-                errs[str(T_)] = "blew up"
                 break
-            E = _shell_energy(les, q)
-            errs[str(T_)] = {"vs_box": _spectral_errors(E, refs["box"][T_], NL),
-                             "vs_sharp": _spectral_errors(E, refs["sharp"][T_], NL),
-                             "E_les": round(float(E[1:NL // 2].sum()), 5),
-                             "E_dns_box": round(float(refs["box"][T_][1:NL // 2].sum()), 5),
-                             "E_dns_sharp": round(float(refs["sharp"][T_][1:NL // 2].sum()), 5)}
-        runs[name] = {"initial": f"{ref_kind}-filtered", "steps": nsteps, "gpu_seconds": round(wall, 3),
-                      "stable": ok, "errors": errs}
-        _log(f"LES {name}: {nsteps} steps in {wall:.2f} s; last {errs.get(str(times[-1]), errs)}")
         s.close()
-    out["les"] = runs
-    best = None
-    for name, r in runs.items():
-        if not name.startswith("learned") or not r["stable"]:
-            continue
-        kind = "vs_box" if "_box_" in name else "vs_sharp"
-        worst = max(e[kind][f"max_k<={NL // 2 - 1}"] or 1e9 for e in r["errors"].values())
-        if best is None or worst < best[1]:
-            best = (name, worst)
-    out["best_learned"] = {"run": best[0], "worst_max_rel_err_to_cutoff": best[1]} if best else None
-    if best:
-        out["value"] = round(out["judge_dns"]["gpu_seconds"] / runs[best[0]]["gpu_seconds"], 1)
-        out["within_5pct_to_cutoff"] = bool(best[1] <= 0.05)
-    else:
-        out["value"] = None
+        return Es, nsteps, wall
+
+    def scaled(NL, f, a):
+        net = MLP(seed=args.seed)
+        net.set_params(nets[(NL, f)].params())
+        mx, sx, my, sy = nets[(NL, f)].norm
+        net.set_norm(mx, sx, [v * a for v in my], [v * a for v in sy])
+        return net
+
+    out["by_les_grid"] = {}
+    for NL in NLs:
+        res = {"a_priori": a_priori[NL]}
+        # ---- a-posteriori fit of each filter's clipped network: the output scale a
+        # minimising the LES's shell-by-shell E(k) error up to the cutoff (the rms of
+        # E_les/E_ref - 1 over 1 <= k < N/2) at t_ap against the training DNS
+        ap = {}
+        for f in filters:
+            def err_of(la):
+                Es, _, _ = run_les(NL, "learned", scaled(NL, f, math.exp(la)), bounds[(NL, f)], True, f, [t_ap])
+                if Es[-1] is None:
+                    return 1e9
+                kk = np.arange(1, NL // 2)
+                e = float(np.sqrt(np.mean((Es[-1][kk] / ap_ref[NL][f][kk] - 1.0) ** 2)))
+                return e if math.isfinite(e) else 1e9
+            lo, hi = math.log(0.02), math.log(2.0)
+            g = (math.sqrt(5) - 1) / 2
+            c, d = hi - g * (hi - lo), lo + g * (hi - lo)
+            fc, fd = err_of(c), err_of(d)
+            evals = 2
+            for _ in range(args.aposteriori_evals - 2):
+                if fc < fd:
+                    hi, d, fd = d, c, fc
+                    c = hi - g * (hi - lo)
+                    fc = err_of(c)
+                else:
+                    lo, c, fc = c, d, fd
+                    d = lo + g * (hi - lo)
+                    fd = err_of(d)
+                evals += 1
+            la, fbest = (c, fc) if fc < fd else (d, fd)
+            ap[f] = {"output_scale": round(math.exp(la), 4), "search": [0.02, 2.0],
+                     "rms_shell_rel_err_at_fit_time_vs_training_dns": round(fbest, 4), "les_runs": evals}
+            nets[(NL, f + "_ap")], bounds[(NL, f + "_ap")] = scaled(NL, f, math.exp(la)), bounds[(NL, f)]
+            _log(f"a-posteriori fit ({NL}^3, {f}): scale {math.exp(la):.3f}, error {fbest:.4f} at t = {t_ap}")
+        res["aposteriori_fit"] = ap
+
+        # ---- LES runs, judged against the judge DNS
+        runs = {}
+        variants = [("learned_box_plain", "learned", "box", False)]
+        variants += [(f"learned_{f}{d}{c}_clip", "learned", f + d + c, True)
+                     for f in filters for d in ("", "_dev") for c in ("", "_cal")]
+        variants += [(f"learned_{f}_clip_aposteriori", "learned", f + "_ap", True) for f in filters]
+        variants += [("smagorinsky", 1, None, False), ("dsmag", 7, None, False), ("tmodel", 2, None, False)]
+        for name, model, f, clip in variants:
+            ref_kind = (f or "box").split("_")[0]
+            Es, nsteps, wall = run_les(NL, model, nets.get((NL, f)), bounds.get((NL, f)), clip, ref_kind, times)
+            errs, ok = {}, True
+            for T_, E in zip(times, Es + [None] * (len(times) - len(Es))):
+                if E is None:
+                    errs[str(T_)] = "blew up"
+                    ok = False
+                    break
+                errs[str(T_)] = {"vs_box": _spectral_errors(E, refs[NL]["box"][T_], NL),
+                                 "vs_sharp": _spectral_errors(E, refs[NL]["sharp"][T_], NL),
+                                 "E_les": round(float(E[1:NL // 2].sum()), 5),
+                                 "E_dns_box": round(float(refs[NL]["box"][T_][1:NL // 2].sum()), 5),
+                                 "E_dns_sharp": round(float(refs[NL]["sharp"][T_][1:NL // 2].sum()), 5)}
+            runs[name] = {"initial": f"{ref_kind}-filtered", "steps": nsteps, "gpu_seconds": round(wall, 3),
+                          "stable": ok, "errors": errs}
+            _log(f"LES {NL}^3 {name}: {nsteps} steps in {wall:.2f} s; last {errs.get(str(times[-1]), errs)}")
+        res["les"] = runs
+        best = None
+        for name, r in runs.items():
+            if not name.startswith("learned") or not r["stable"]:
+                continue
+            kind = "vs_box" if "_box_" in name else "vs_sharp"
+            worst = max(e[kind][f"max_k<={NL // 2 - 1}"] or 1e9 for e in r["errors"].values())
+            if best is None or worst < best[1]:
+                best = (name, worst)
+        res["best_learned"] = {"run": best[0], "worst_max_rel_err_to_cutoff": best[1]} if best else None
+        res["within_5pct_to_cutoff"] = bool(best and best[1] <= 0.05)
+        res["value"] = round(out["judge_dns"]["gpu_seconds"] / runs[best[0]]["gpu_seconds"], 1) if best else None
+        out["by_les_grid"][str(NL)] = res
+    for s_ in les.values():
+        s_.close()
+    out["within_5pct_to_cutoff"] = {str(NL): out["by_les_grid"][str(NL)]["within_5pct_to_cutoff"] for NL in NLs}
+    out["value"] = out["by_les_grid"][str(NLs[0])]["value"]
     return out
 
 
@@ -729,7 +833,10 @@ def main():
     ap.add_argument("--workload", default="les128", choices=["les128", "ensemble", "closure"],
                     help="les128: the 128^3 LES metric (default); ensemble: 96 members of 32^3/48^3/64^3; "
                          "closure: the learned closure trained on and judged against forced DNS")
-    ap.add_argument("--les-N", type=int, default=64)
+    ap.add_argument("--les-N", default="64,128", help="closure: LES grids, comma-separated")
+    ap.add_argument("--aposteriori-time", type=float, default=0.5,
+                    help="closure: the time at which the a-posteriori fit matches the training DNS spectrum")
+    ap.add_argument("--aposteriori-evals", type=int, default=10, help="closure: LES runs of the a-posteriori fit")
     ap.add_argument("--train-N", type=int, default=256)
     ap.add_argument("--judge-N", type=int, default=512)
     ap.add_argument("--train-steps", type=int, default=2000)

@@ -114,9 +114,13 @@ hipError_t launch_zpass_lrn(int L, const ZArgs& a, const double2* tw, hipStream_
   }
 }
 
-// the fused closure (network inside the pass): one-wavefront lengths only
+// the fused closure (network inside the pass): the one-wavefront lengths, and the
+// lane-group lengths 48, 72, 96 (positions per wavefront)
 int zpass_positions_lrnf(int L) {
   switch (L) {
+    case 48: return ZSubShape<48>::S;
+    case 72: return ZSubShape<72>::S;
+    case 96: return ZSubShape<96>::S;
     case 64: return z_P<Z_LRNF>(64);
     case 128: return z_P<Z_LRNF>(128);
     case 192: return z_P<Z_LRNF>(192);
@@ -130,6 +134,9 @@ int zpass_positions_lrnf(int L) {
 
 hipError_t launch_zpass_lrnf(int L, const ZArgs& a, const double2* tw, hipStream_t s, int variant) {
   switch (L) {
+    case 48: return zpass_launch<48, Z_LRNF>(a, tw, s, variant);
+    case 72: return zpass_launch<72, Z_LRNF>(a, tw, s, variant);
+    case 96: return zpass_launch<96, Z_LRNF>(a, tw, s, variant);
     case 64: return zpass_launch<64, Z_LRNF>(a, tw, s, variant);
     case 128: return zpass_launch<128, Z_LRNF>(a, tw, s, variant);
     case 192: return zpass_launch<192, Z_LRNF>(a, tw, s, variant);

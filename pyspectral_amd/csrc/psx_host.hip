@@ -651,7 +651,8 @@ void stage_learned(psx_context* c, const double2* uin, const double2* q0, double
   const double cd = c->p.Cs * delta_grid(c);
   const int clip = (c->net_flags & PSX_CLOSURE_CLIP_BACKSCATTER) ? 1 : 0;
   const int src[9] = {3, 6, 0, 4, 7, 1, 5, 8, 2};
-  if (c->tune[6] == 0 && zpass_positions(M3, Z_LRNF) > 0) {
+  const int PF = zpass_positions(M3, Z_LRNF);
+  if (c->tune[6] == 0 && PF > 0 && M2 % PF == 0) {
     // the network inside the z pass: velocity and gradient pencils in, products and
     // stresses out, no gradient or stress arrays in HBM
     ZArgs q = z;

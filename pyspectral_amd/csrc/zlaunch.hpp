@@ -25,8 +25,15 @@ hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int va
   constexpr int P = z_P<MODE>(L);
   const dim3 grid((unsigned)(a.npos / P));
   if constexpr (MODE == Z_LRNF) {
-    // the fused closure exists as the one-wavefront kernel only
-    if constexpr (ZWave<L>::ok) {
+    // the fused closure exists as the one-wavefront and the lane-group kernels only
+    if constexpr (ZSub<L>::ok) {
+      constexpr int S = ZSubShape<L>::S;
+      if (zw_third<L, MODE>(a) && a.npos % S == 0) {
+        hipLaunchKernelGGL((zsub<L, MODE, L / 3>), dim3((unsigned)(a.npos / S)), dim3(64), 0, s, a, tw);
+        return hipGetLastError();
+      }
+      return hipErrorInvalidValue;
+    } else if constexpr (ZWave<L>::ok) {
       if constexpr (zw_third_mode<L, MODE>()) {
         if (zw_third<L, MODE>(a)) {
           hipLaunchKernelGGL((zwave<L, P, MODE, L / 3>), grid, dim3(64), 0, s, a, tw);

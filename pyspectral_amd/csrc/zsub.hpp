@@ -97,7 +97,7 @@ template <int L, int MODE, int HK>
 __global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ tw) {
   using Z = ZSubShape<L>;
   static_assert(ZSub<L>::ok, "zsub: L must be 48, 72 or 96");
-  static_assert(MODE == Z_DNS || MODE == Z_SMAG, "zsub: product modes only");
+  static_assert(MODE == Z_DNS || MODE == Z_SMAG || MODE == Z_LRNF, "zsub: product modes only");
   constexpr int W = Z::W, R0 = Z::R0, S = Z::S, SUBW = Z::SUBW, SG = Z::SG;
   constexpr int NIN = ZShape<MODE>::nin, NOUT = ZShape<MODE>::nout;
   constexpr int NRI = NIN, NRO = NOUT;
@@ -157,6 +157,42 @@ __global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ 
     wave_sync();
   }
 
+  // ---- Z_LRNF: the closure network on the positions' own gradients, as zwave runs
+  // it: float point records of stride 16 in LDS (the nine gradients at 0..8, the six
+  // stresses written back at 9..14), point p = s L + z of the wavefront's S positions,
+  // 16-point MFMA tiles over all 64 lanes
+  [[maybe_unused]] float* rec = nullptr;
+  if constexpr (MODE == Z_LRNF) {
+    __shared__ float recs[Z::NG * L * 16];
+    rec = recs + s * L * 16;
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      const int zz = l + W * m;
+#pragma unroll
+      for (int f = 0; f < 9; ++f) {
+        const int r = 3 + f;
+        rec[zz * 16 + f] = (float)(((r & 1) ? x[r >> 1][m].y : x[r >> 1][m].x) * a.gscale);
+      }
+    }
+    wave_sync();
+    const int g = lane >> 4, rr = lane & 15;
+    psx_mlp_impl::Net R;
+    psx_mlp_impl::net_load(R, a.npk, a.nw, a.nnorm, lane);
+    static_assert((S * L) % 16 == 0, "whole 16-point tiles");
+#pragma unroll 1
+    for (int t0 = 0; t0 < S * L / 16; ++t0) {
+      float* gp = &recs[(t0 * 16 + rr) * 16];
+      psx_mlp_impl::f4 xn[1], out[1];
+      xn[0] = psx_mlp_impl::net_input(R, gp, true, g);
+      psx_mlp_impl::net_forward<1>(R, xn, out);
+      const psx_mlp_impl::f4 res = psx_mlp_impl::net_closure(R, out[0], xn[0], gp, true, g, a.nc2, a.nbound, a.nclip);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (4 * g + j < 6) gp[9 + 4 * g + j] = res[j];
+    }
+    wave_sync();
+  }
+
   // ---- pointwise products / closure on the lane's R0 z points
   double2 xo[NCO][R0];
 #pragma unroll
@@ -164,7 +200,14 @@ __global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ 
     double in[NIN], o[NOUT];
 #pragma unroll
     for (int f = 0; f < NIN; ++f) in[f] = (f & 1) ? x[f >> 1][m].y : x[f >> 1][m].x;
-    zpoint<MODE>(a, in, o);
+    if constexpr (MODE == Z_LRNF) {
+      // the products, then the stresses joined as zwave's fused pass does
+      zpoint<Z_DNS>(a, in, o);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) o[6 + c] = a.c2 * (double)rec[(l + W * m) * 16 + 9 + c];
+    } else {
+      zpoint<MODE>(a, in, o);
+    }
     zpoint_fold<MODE>(a, o);
 #pragma unroll
     for (int q = 0; q < NCO; ++q) {

@@ -219,10 +219,11 @@ def test_learned_rhs_with_clip_matches_oracle():
     net.close()
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 128), (16, 32, 256)])
+@pytest.mark.parametrize("shape", [(16, 16, 128), (16, 32, 256), (16, 16, 32), (16, 32, 48), (32, 16, 64)])
 @pytest.mark.parametrize("bound,clip", [(float("inf"), False), (1.5, True)])
 def test_fused_closure_pass_matches_separate_network_kernel(shape, bound, clip):
-    """Padded z lengths 192 and 384 run the network inside the z pass; the separate
+    """Padded z lengths 192 and 384 (one wavefront per position) and 48, 72 and 96 (4,
+    2 and 2 positions per wavefront) run the network inside the z pass; the separate
     network kernel between a gradient and a products pass gives the same RHS (the two
     pair their pencils differently, so float casts of the gradients may differ in the
     last bit at a few points)."""
@@ -249,7 +250,8 @@ def test_fused_closure_pass_matches_separate_network_kernel(shape, bound, clip):
     net.close()
 
 
-@pytest.mark.parametrize("shape,separate", [((16, 16, 128), 0), ((16, 16, 128), 1), ((32, 16, 48), 1)])
+@pytest.mark.parametrize("shape,separate", [((16, 16, 128), 0), ((16, 16, 128), 1), ((32, 16, 48), 1),
+                                            ((32, 16, 48), 0)])
 def test_learned_rhs_folded_matches_twelve_fields(shape, separate):
     """Without a w0 output the learned closure's y-forward folds its 12 fields into 9;
     the RHS equals the 12-field one (which the oracle pins) to rounding."""
