@@ -7,10 +7,10 @@
 * Forced isotropic DNS at 256^3 (384^3 with the 3/2 rule; A = 0.1, nu = 5e-4): RHS and
   one CFL step against the oracle, and the filtered-DNS training pairs 256^3 -> 64^3
   against the oracle's box filter / stress / restriction chain.
-* Learned-closure LES RHS at 64^3 (padded z length 96: gradient z pass, network
-  kernel, products z pass) and 128^3 (192: the network inside the z pass) against the
-  oracle's learned RHS fed by the same network kernel on the oracle's gradients
-  (stresses pass through float32: 1e-6 of the subgrid term).
+* Learned-closure LES RHS at 64^3 (padded z length 96: the network inside the
+  lane-group z pass) and 128^3 (192: the network inside the one-wavefront z pass)
+  against the oracle's learned RHS fed by the same network kernel on the oracle's
+  gradients (stresses pass through float32: 1e-6 of the subgrid term).
 * The 512^3 judge DNS (768^3 padded): too large for the oracle, so two
   size-independent properties of the plain RHS are checked on the GPU result: k . RHS
   = 0 and, at nu = 0, zero energy transfer over the half spectrum.
@@ -197,9 +197,10 @@ def _net_for(G, seed=2):
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("N", [64, 128])
 def test_learned_les_rhs_matches_oracle(N):
-    """64^3 pads z to 96 (separate network kernel); 128^3 pads z to 192 (network fused
-    into the z pass).  Both against the oracle's learned RHS with the network kernel's
-    stresses on the oracle's gradients, with the range guard active."""
+    """64^3 pads z to 96 (network fused into the lane-group z pass); 128^3 pads z to 192
+    (network fused into the one-wavefront z pass).  Both against the oracle's learned
+    RHS with the network kernel's stresses on the oracle's gradients, with the range
+    guard active."""
     from pyspectral_amd import cases
     from pyspectral_amd.solver import SpectralSolver
 
