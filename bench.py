@@ -687,8 +687,8 @@ def kernel_bytes_tmodel(N):
     class, summed over that class's launches in the stage, on the 2x grid P = 2N with
     H = N/2 carried k3 modes (16 B per complex value, every input read once and every
     output written once): x-inverse 3 state fields -> X (3 P N H); y-inverse X -> Y1
-    (3 P P H) and, in round 2, PL_q (3 P P N -> 3 P P N); z pass 3 P P H -> 6 P P N
-    products (round 1) and Y1 + PL_q -> 9 P P H (round 2); y-forward 6 P P N -> 6 P P N
+    (3 P P H) and, in the second 2x-grid sweep, PL_q (3 P P N -> 3 P P N); z pass 3 P P H -> 6 P P N
+    products (first sweep) and Y1 + PL_q -> 9 P P H (second sweep); y-forward 6 P P N -> 6 P P N
     and 9 P P H -> 9 P N H; the PL pass reads the 6 products and u, writes PL (base grid)
     and the x-inverse of PL_q (3 P P N); the epilogue reads 9 P N H and PL, u, q0 and
     writes the state.  Returns {class: (launches, bytes)}."""
@@ -709,7 +709,7 @@ def kernel_bytes_tmodel(N):
     }
 
 
-# the step model of the t-model (two 2x-grid rounds per stage): 3698 N^3 bytes per
+# the step model of the t-model (two 2x-grid sweeps per stage): 3698 N^3 bytes per
 # stage in the 3-pass pencil convention, 14 792 bytes per point and RK4 step
 TMODEL_BYTES_PER_POINT_STEP = 14792
 

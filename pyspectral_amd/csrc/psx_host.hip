@@ -806,8 +806,8 @@ void stage_dsmag(psx_context* c, const double2* uin, const double2* q0, double2*
 }
 
 // One pass of the 2x-grid t-model pipeline on velocity u (already filtered for the
-// test-filtered pass of the dynamic model): round 1 (products on the 2x grid, PL,
-// PL_q), round 2 (u_j (PL_q)_i on the 2x grid) and the final x-forward epilogue.
+// test-filtered pass of the dynamic model): a first 2x-grid sweep (products, PL,
+// PL_q), a second (u_j (PL_q)_i on the 2x grid) and the final x-forward epilogue.
 // src/RHSfunctions.py:240-363 (t-model), 366-497 (static tau), 592-878 (dynamic).
 void tau_pass(psx_context* c, const double2* u, bool filtered, const double2* uin, const double2* q0, double2* out,
               const double* dt, double stage_c, double2* w0, int zero_nyq_out, double tcoef,
@@ -912,7 +912,7 @@ void tau_pass(psx_context* c, const double2* u, bool filtered, const double2* ui
     if (P == 0 || (z.npos % P) != 0) fail(c, PSX_E_UNSUPPORTED, "z pass: unsupported 2x length");
     const double2* tw = twiddles(c, P3);
     TimeScope ts(c, K_ZPASS);
-    check(c, launch_zpass(P3, Z_TM2, z, tw, c->stream), "z_pass(2x, round 2)");
+    check(c, launch_zpass(P3, Z_TM2, z, tw, c->stream), "z_pass(2x, second sweep)");
   }
   y_forward(c, A, B, P1, P2, H, 1, 9, N2);
   {
