@@ -226,7 +226,8 @@ int psx_tau(psx_context* ctx, double* out_host8);
  * inside the z pass where the padded z length is 64 * {1,2,3,4,6,8,12}, 1 always as its
  * own kernel between a gradient and a products z pass; tune[7..9] = block order of the
  * x-inverse, y-inverse and y-forward passes (0: each XCD runs a contiguous range of
- * blocks, 1: dispatch order); tune[10] = where the Smagorinsky-form stresses are combined
+ * blocks, 1: dispatch order, 2: each outer row's column tiles on one XCD, consecutive
+ * rows round-robin over the XCDs); tune[10] = where the Smagorinsky-form stresses are combined
  * into the three k1-polynomial coefficients the projection needs (0: the y-forward sums
  * them and hands the epilogue 9 fields, 1: the epilogue reads all 12 and sums them).
  * psx_autotune times every candidate of each

@@ -93,6 +93,20 @@ __device__ __forceinline__ int xcd_block() {
   }
 }
 
+// Logical block of a pencil pass of A outer rows with ct column tiles each, by block
+// order: 0 = contiguous per-XCD ranges (xcd_block), 1 = dispatch order, 2 = the ct
+// tiles of one outer row on one XCD, consecutive rows round-robin over the 8 XCDs (the
+// epilogue's order; it needs A % 8 == 0, else order 0 is used).  With gridDim.x = A ct
+// a multiple of 8, block b runs on XCD b & 7, which takes rows x, x + 8, ... in turn.
+__device__ __forceinline__ int pencil_block(int order, int ct, int A) {
+  if (order == 1) return blockIdx.x;
+  if (order == 2 && (A & 7) == 0) {
+    const int b = blockIdx.x, s = b >> 3;
+    return (8 * (s / ct) + (b & 7)) * ct + s % ct;
+  }
+  return xcd_block<true>();
+}
+
 // ------------------------------------------------------------------ pencil passes
 struct PencilArgs {
   const double2* in;
@@ -116,8 +130,9 @@ struct PencilArgs {
   // in + m*in_ms and writes at out + m*out_ms (jpm = 0: one member)
   int jpm;
   long long in_ms, out_ms;
-  // block order: 0 gives each XCD a contiguous range of blocks (xcd_block), 1 keeps the
-  // dispatch order (which of the two is faster depends on the pass and the job count)
+  // block order (pencil_block): 0 gives each XCD a contiguous range of blocks, 1 keeps
+  // the dispatch order, 2 keeps each outer row's column tiles on one XCD (which is
+  // fastest depends on the pass and the job count)
   int order;
 };
 
@@ -134,7 +149,7 @@ __global__ __launch_bounds__(NT) void pencil_inverse(PencilArgs a, const double2
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int bid = a.order ? (int)blockIdx.x : xcd_block<true>();
+  const int bid = pencil_block(a.order, ct, a.A);
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
   const int mem = pencil_member(a);
@@ -233,7 +248,7 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int bid = a.order ? (int)blockIdx.x : xcd_block<true>();
+  const int bid = pencil_block(a.order, ct, a.A);
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
   const int mem = pencil_member(a);
@@ -285,7 +300,7 @@ __global__ __launch_bounds__(NT) void pencil_forward_sum(SumArgs sa, const doubl
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
-  const int bid = a.order ? (int)blockIdx.x : xcd_block<true>();
+  const int bid = pencil_block(a.order, ct, a.A);
   const int aidx = bid / ct;
   const int c0 = (bid - aidx * ct) * T;
   const int mem = pencil_member(a);
@@ -803,61 +818,82 @@ struct EpiArgs {
   double force;  // linear forcing coefficient: RHS += force * u (u truncated where trunc_u)
 };
 
-// Accumulate the contribution of transformed field f (value v) to the three RHS
-// components at wavenumber (k1, k2, k3), kinv = 1/(|k|^2 + 1e-50).
+// The epilogue's RHS is linear in its transformed fields, and every projection in it
+// has the form -i V_i + i k_i (k.V)/k^2 of a vector V_i = sum_j X_ij k_j of a symmetric
+// tensor X.  So each field only adds its k-weighted value to the sums of its tensor,
+// and epi_project applies the projections once per output.  Sums per output:
+//   V   -- X = the products N_mn (E_DNS, E_SMAG), D_mn = N_mn + tau_mn (E_SMAGF,
+//          E_SMAG9), or A_ij + A_ji with A_ij = u_j (PL_q)_i (E_TAU2);
+//   Vt  -- X = tau_mn (E_SMAG: the stresses keep their own projection);
+//   St  -- k.tau.k = k1^2 A + k1 B + C (E_SMAGF, E_SMAG9: fields 6, 7, 8).
+// Fields 0..5 (and E_SMAG's 6..11) hold the symmetric components mn = 11 22 33 12 13
+// 23; E_TAU2's field 3 i + j holds A_ij.
 template <int EPI>
-__device__ __forceinline__ void epi_accumulate(int f, const double* k, double kinv, double2 v, double2* acc,
-                                               double2* acc_sgs) {
+__device__ __forceinline__ void epi_accumulate(int f, const double* k, double2 v, double2* V, double2* Vt,
+                                               double2& St) {
   if constexpr (EPI == E_SMAGF || EPI == E_SMAG9) {
     if (f >= 6) {
-      // the folded stress fields A, B, C (xforward_epilogue): with the products'
-      // projection applied to D = N + tau, the stress's own term as written becomes
-      // -(1 + i) k_i (k1^2 A + k1 B + C) / k^2
-      const double w = (f == 6 ? k[0] * k[0] : (f == 7 ? k[0] : 1.0)) * kinv;
-      const double2 t = make_double2(v.x - v.y, v.x + v.y);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) acc[i] = csub(acc[i], cscale(t, k[i] * w));
+      const double w = f == 6 ? k[0] * k[0] : (f == 7 ? k[0] : 1.0);
+      St = make_double2(St.x + w * v.x, St.y + w * v.y);
       return;
     }
   }
-  if constexpr (EPI == E_DNS || EPI == E_SMAG || EPI == E_SMAGF || EPI == E_SMAG9) {
-    // fields 0..5: N_mn = u_m u_n (E_SMAGF: D_mn = N_mn + tau_mn), 6..11: tau_mn;
-    // (mn) in 11,22,33,12,13,23
+  if constexpr (EPI == E_TAU2) {
+    const int i = f / 3, j = f - 3 * (f / 3);
+    const double ki = sel3(i, k[0], k[1], k[2]), kj = sel3(j, k[0], k[1], k[2]);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const double w = (m == i ? kj : 0.0) + (m == j ? ki : 0.0);
+      V[m] = make_double2(V[m].x + w * v.x, V[m].y + w * v.y);
+    }
+  } else {
     const int g = f % 6;
     const int m = sym_m(g), n = sym_n(g);
     const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
-    const double s = (m == n) ? 1.0 : 2.0;
-    const double kk = s * km * kn * kinv;
-    const double2 mi = cmul_i<-1>(v);  // -i v
+    double2* X = (EPI == E_SMAG && f >= 6) ? Vt : V;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      double2 d = make_double2(0.0, 0.0);
-      if (i == m) d = cadd(d, cscale(mi, kn));
-      if (i == n && m != n) d = cadd(d, cscale(mi, km));
-      if (f < 6) {
-        // + i k_i (k_m k_n N_mn)/k^2: the Leray projection (src/RHSfunctions.py:29-45)
-        d = cadd(d, cscale(cmul_i<1>(v), k[i] * kk));
-      } else {
-        // + 1j*k_i*(1j*ksqr_i*k_m k_n tau_mn) as written (src/RHSfunctions.py:215-222)
-        d = csub(d, cscale(v, k[i] * kk));
-      }
-      acc[i] = cadd(acc[i], d);
-      if (acc_sgs != nullptr && f >= 6) acc_sgs[i] = cadd(acc_sgs[i], d);
+      // X_mn adds k_n to V_m and, off the diagonal, X_nm = X_mn adds k_m to V_n
+      const double w = (i == m ? kn : 0.0) + (i == n && m != n ? km : 0.0);
+      X[i] = make_double2(X[i].x + w * v.x, X[i].y + w * v.y);
     }
-  } else if constexpr (EPI == E_TAU2) {
-    // field 3 i + j holds u_j (PL_q)_i; PLQL_m = sum_j -i k_j (A_ji + A_ij) + i k_m pterm,
-    // pterm = 2/k^2 sum_mn k_m k_n A_mn (src/RHSfunctions.py:338-352)
-    const int i = f / 3, j = f - 3 * (f / 3);
-    const double ki = sel3(i, k[0], k[1], k[2]), kj = sel3(j, k[0], k[1], k[2]);
-    const double2 mi = cmul_i<-1>(v);
-    const double pk = 2.0 * ki * kj * kinv;
+  }
+}
+
+// The RHS (or E_TAU2's PLQL) at wavenumber k from an output's sums, kinv = 1/(|k|^2 +
+// 1e-50):
+//   out_i = -i V_i + i k_i (k.V) kinv     the Leray projection of the products'
+//                                         divergence (src/RHSfunctions.py:29-45), or
+//                                         PLQL (src/RHSfunctions.py:338-352)
+//   sgs_i = -i Vt_i - k_i (k.Vt) kinv     E_SMAG: the stress term with its 1j*1j as
+//                                         written (src/RHSfunctions.py:215-222)
+//   out_i -= (1 + i) k_i St kinv          E_SMAGF, E_SMAG9: with the products'
+//                                         projection applied to D = N + tau, the
+//                                         stress term as written leaves this
+// E_SMAG's out includes sgs.
+template <int EPI>
+__device__ __forceinline__ void epi_project(const double* k, double kinv, const double2* V, const double2* Vt,
+                                            double2 St, double2* out, double2* sgs) {
+  const double2 kV = make_double2(k[0] * V[0].x + k[1] * V[1].x + k[2] * V[2].x,
+                                  k[0] * V[0].y + k[1] * V[1].y + k[2] * V[2].y);
+  const double2 w = cscale(kV, kinv);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      double2 d = cscale(cmul_i<1>(v), k[m] * pk);
-      if (m == i) d = cadd(d, cscale(mi, kj));
-      if (m == j) d = cadd(d, cscale(mi, ki));
-      acc[m] = cadd(acc[m], d);
+  for (int i = 0; i < 3; ++i) out[i] = cmul_i<1>(csub(cscale(w, k[i]), V[i]));
+  if constexpr (EPI == E_SMAG) {
+    const double2 kT = make_double2(k[0] * Vt[0].x + k[1] * Vt[1].x + k[2] * Vt[2].x,
+                                    k[0] * Vt[0].y + k[1] * Vt[1].y + k[2] * Vt[2].y);
+    const double2 wt = cscale(kT, kinv);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      sgs[i] = csub(cmul_i<-1>(Vt[i]), cscale(wt, k[i]));
+      out[i] = cadd(out[i], sgs[i]);
     }
+  }
+  if constexpr (EPI == E_SMAGF || EPI == E_SMAG9) {
+    const double2 ws = cscale(St, kinv);
+    const double2 t = make_double2(ws.x - ws.y, ws.x + ws.y);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] = csub(out[i], cscale(t, k[i]));
   }
 }
 
@@ -952,13 +988,19 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
   const long long in_sl = (long long)a.N2 * a.H;  // x stride in W layout
   const double2* src0 = in_ + (long long)ky * a.H + c0;
   constexpr bool SGS = W0 && (EPI == E_SMAG);
+  constexpr bool VT = EPI == E_SMAG;                      // the stresses' own sums
+  constexpr bool ST = EPI == E_SMAGF || EPI == E_SMAG9;   // k.tau.k from A, B, C
   static_assert(EPI != E_TAU2 || T * LS >= NT, "block_sum2 needs NT LDS entries");
+  // per output: the k-weighted sums of epi_accumulate
   double2 acc[IT][3];
-  double2 accs[SGS ? IT : 1][3];
+  double2 accs[VT ? IT : 1][3];
+  double2 acct[ST ? IT : 1];
 #pragma unroll
   for (int it = 0; it < IT; ++it) acc[it][0] = acc[it][1] = acc[it][2] = make_double2(0.0, 0.0);
 #pragma unroll
-  for (int it = 0; it < (SGS ? IT : 1); ++it) accs[it][0] = accs[it][1] = accs[it][2] = make_double2(0.0, 0.0);
+  for (int it = 0; it < (VT ? IT : 1); ++it) accs[it][0] = accs[it][1] = accs[it][2] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int it = 0; it < (ST ? IT : 1); ++it) acct[it] = make_double2(0.0, 0.0);
 
   const double k2 = a.ks[1] * signed_freq(ky, a.N2);
   // One step: stage input field epi_step_field(s) in LDS (from the registers `pre`,
@@ -1014,8 +1056,7 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
         const int p = pad_slot(kx, NS1, L, a.keep_nyq);
         if (p >= 0 && c0 + c < a.H) {
           const double k[3] = {a.ks[0] * signed_freq(kx, NS1), k2, a.ks[2] * (c0 + c)};
-          const double kinv = 1.0 / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + 1e-50);
-          epi_accumulate<EPI>(f, k, kinv, buf[c * LS + RW(p)], acc[it], SGS ? accs[SGS ? it : 0] : nullptr);
+          epi_accumulate<EPI>(f, k, buf[c * LS + RW(p)], acc[it], accs[VT ? it : 0], acct[ST ? it : 0]);
         }
       }
     }
@@ -1086,14 +1127,16 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
         const long long idx = (long long)kx * plane + (long long)ky * a.Nc + kz;
         const double k[3] = {a.ks[0] * signed_freq(kx, NS1), k2, a.ks[2] * kz};
         const double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const double kinv = 1.0 / (ksq + 1e-50);
         const bool nyq = (kx == a.N1 / 2) || (ky == a.N2 / 2) || (kz == a.N3 / 2);
-        double2 u[3], rhs[3];
+        double2 u[3], rhs[3], nl[3], sgs[3];
+        epi_project<EPI>(k, kinv, acc[it], accs[VT ? it : 0], acct[ST ? it : 0], nl, sgs);
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
           u[cc] = (a.trunc_u && nyq) ? make_double2(0.0, 0.0)
                                      : (PRE ? su[PRE ? it : 0][cc] : uprev_[cc * fstride + idx]);
           if constexpr (EPI == E_TAU2) {
-            const double2 w = cscale(acc[it][cc], coef);
+            const double2 w = cscale(nl[cc], coef);
             if (w0_) w0_[cc * fstride + idx] = w;
             if (a.red_u) {
               const double wz = kz == 0 ? 1.0 : (kz < a.N3 / 2 ? 2.0 : 0.0);
@@ -1104,15 +1147,14 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
             rhs[cc] = a.store_only ? make_double2(0.0, 0.0)
                                    : cadd(cadd(sp[PRET ? it : 0][cc], w), cscale(u[cc], a.force));
           } else {
-            rhs[cc] = cadd(acc[it][cc], cscale(u[cc], a.force - nu_ * (ksq + 1e-50)));
+            rhs[cc] = cadd(nl[cc], cscale(u[cc], a.force - nu_ * (ksq + 1e-50)));
           }
         }
         if constexpr (SGS) {
 #pragma unroll
-          for (int cc = 0; cc < 3; ++cc) w0_[cc * fstride + idx] = accs[it][cc];
+          for (int cc = 0; cc < 3; ++cc) w0_[cc * fstride + idx] = sgs[cc];
         }
         if (a.rotate) {
-          const double kinv = 1.0 / (ksq + 1e-50);
           double2 cr[3];
           cr[0] = csub(cscale(u[1], a.Om[2]), cscale(u[2], a.Om[1]));
           cr[1] = csub(cscale(u[2], a.Om[0]), cscale(u[0], a.Om[2]));
@@ -1290,17 +1332,13 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   const double2* src0 = a.in + (long long)ky * a.C + c0;
   const int s2 = signed_freq(ky, a.P2);
   const double k2 = a.ks[1] * s2;
-  // acc: divergence part D (+ projection part when !DYN); Pj: projection part (DYN)
+  // acc: V_i = sum_j N_ij k_j of the products; the divergence part of PL is -i V_i and
+  // its projection part i k_i (k.V)/k^2 (see epi_accumulate)
   double2 acc[IT][3];
-  double2 Pj[DYN ? IT : 1][3];
 #pragma unroll
   for (int it = 0; it < IT; ++it)
 #pragma unroll
     for (int i = 0; i < 3; ++i) acc[it][i] = make_double2(0.0, 0.0);
-#pragma unroll
-  for (int it = 0; it < (DYN ? IT : 1); ++it)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) Pj[it][i] = make_double2(0.0, 0.0);
   // field f + 1 is loaded into registers while field f is transformed
   constexpr int LT = (L * T + NT - 1) / NT;
   double2 pre[LT];
@@ -1314,16 +1352,6 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
                                                                     : make_double2(0.0, 0.0);
     }
   };
-  // 1/|k|^2 of the thread's outputs, held across the fields (the rest of the index
-  // arithmetic is recomputed per field)
-  double kinvs[IT];
-#pragma unroll
-  for (int it = 0; it < IT; ++it) {
-    const int e = tid + it * NT;
-    const int kx = e / T, c = e - (e / T) * T;
-    const double k1 = a.ks[0] * signed_freq(kx, L), k3 = a.ks[2] * (c0 + c);
-    kinvs[it] = 1.0 / (k1 * k1 + k2 * k2 + k3 * k3 + 1e-50);
-  }
   load_field(0, tid);
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
@@ -1337,28 +1365,18 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, twp, tl);
     const int m = sym_m(f), n = sym_n(f);
-    const double s = (m == n) ? 1.0 : 2.0;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = tl + it * NT;
       const int kx = e / T, c = e - (e / T) * T;
       if (e < L * T && c0 + c < a.C) {
         const double k[3] = {a.ks[0] * signed_freq(kx, L), k2, a.ks[2] * (c0 + c)};
-        const double kinv = kinvs[it];
         const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
         const double2 v = buf[c * LS + RW(kx)];
-        const double2 mi = cmul_i<-1>(v);
-        const double kk = s * km * kn * kinv;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          if (i == m) acc[it][i] = cadd(acc[it][i], cscale(mi, kn));
-          if (i == n && m != n) acc[it][i] = cadd(acc[it][i], cscale(mi, km));
-          const double2 pv = cscale(cmul_i<1>(v), k[i] * kk);
-          if constexpr (DYN) {
-            Pj[it][i] = cadd(Pj[it][i], pv);
-          } else {
-            acc[it][i] = cadd(acc[it][i], pv);
-          }
+          const double w = (i == m ? kn : 0.0) + (i == n && m != n ? km : 0.0);
+          acc[it][i] = make_double2(acc[it][i].x + w * v.x, acc[it][i].y + w * v.y);
         }
       }
     }
@@ -1383,6 +1401,16 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
                                         (long long)((s2 + a.N2) % a.N2) * a.Nc + kz)
                                      : 0;
       const bool has_u = in_base && s1 > -a.N1 / 2 && s2 > -a.N2 / 2;
+      // divergence part D_i = -i V_i and projection part P_i = i k_i (k.V)/k^2
+      const double kinv = 1.0 / (ksq + 1e-50);
+      const double2 kV = make_double2(k[0] * acc[it][0].x + k[1] * acc[it][1].x + k[2] * acc[it][2].x,
+                                      k[0] * acc[it][0].y + k[1] * acc[it][1].y + k[2] * acc[it][2].y);
+      double2 Dv[3], Pv[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        Dv[i] = cmul_i<-1>(acc[it][i]);
+        Pv[i] = cmul_i<1>(cscale(kV, k[i] * kinv));
+      }
       if constexpr (DYN) {
         const bool in_f = in_base && fabs(k[0]) < a.kf && fabs(k[1]) < a.kf && fabs(k[2]) < a.kf;
         if (in_f) {
@@ -1390,7 +1418,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
 #pragma unroll
           for (int i = 0; i < 3; ++i) {
             // G = -i k_j N_ij + i k_i p = D - P
-            const double2 G = csub(acc[it][i], Pj[it][i]);
+            const double2 G = csub(Dv[i], Pv[i]);
             if (a.g_out) a.g_out[i * fs + bidx] = G;
             if (a.g_ref) {
               const double2 d = csub(G, a.g_ref[i * fs + bidx]);
@@ -1403,8 +1431,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       }
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        double2 pl = acc[it][i];
-        if constexpr (DYN) pl = cadd(pl, Pj[it][i]);
+        double2 pl = cadd(Dv[i], Pv[i]);
         if (has_u) pl = csub(pl, cscale(a.u[i * fs + bidx], a.nu * (ksq + 1e-50)));
         if (in_base && a.pln) a.pln[i * fs + bidx] = pl;
         acc[it][i] = pl;

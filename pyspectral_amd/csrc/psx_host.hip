@@ -1860,7 +1860,7 @@ int psx_set_tuning(psx_context* c, const int* tune) {
     if (tune[5] < 0 || tune[5] > 2) fail(c, PSX_E_ARG, "chunk streams must be 0, 1 or 2");
     if (tune[6] != 0 && tune[6] != 1) fail(c, PSX_E_ARG, "closure placement must be 0 or 1");
     for (int k = 7; k < 10; ++k)
-      if (tune[k] != 0 && tune[k] != 1) fail(c, PSX_E_ARG, "block order must be 0 or 1");
+      if (tune[k] < 0 || tune[k] > 2) fail(c, PSX_E_ARG, "block order must be 0, 1 or 2");
     if (tune[10] != 0 && tune[10] != 1) fail(c, PSX_E_ARG, "stress sums must be 0 (y-forward) or 1 (epilogue)");
     if (tune[4] > 1) {
       if (c->model != PSX_MODEL_DNS && c->model != PSX_MODEL_SMAGORINSKY && c->model != PSX_MODEL_DSMAG)
@@ -1910,6 +1910,8 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
         if (knob < 3) {
           for (int T : {4, 8, 16})
             if (pencil_tile(L[knob], T) == T) cand.push_back(T);
+        } else if (knob >= 7 && knob <= 9) {
+          cand = {0, 1, 2};  // block orders
         } else if (knob == 3 || knob >= 6) {
           cand = {0, 1};
           // the stress sums change the folded stage only (the autotuner's RHS calls pass
@@ -2211,6 +2213,9 @@ int psx_ensemble_autotune(psx_ensemble* e, const double* q, int reps, int* tune_
       if (knob < 3) {
         for (int T : {4, 8, 16})
           if (pencil_tile(L[knob], T) == T && T != c->tune[knob]) cand.push_back(T);
+      } else if (knob >= 7 && knob <= 9) {
+        for (int v : {0, 1, 2})
+          if (v != c->tune[knob]) cand.push_back(v);
       } else {
         cand.push_back(c->tune[knob] ? 0 : 1);
       }
