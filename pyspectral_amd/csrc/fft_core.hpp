@@ -24,6 +24,13 @@ __host__ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __host__ __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __host__ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// Store with the nontemporal hint (streaming: no allocation in the last-level cache).
+// The z passes' outputs are read by the next kernel only after the whole pass, and
+// measured faster downstream so stored (y-forward 131.5 -> 129.3 us at 128^3).
+__device__ __forceinline__ void store_nt(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
 // multiply by +i (S = +1) or -i (S = -1)
 template <int S>
 __host__ __device__ __forceinline__ double2 cmul_i(double2 a) {

@@ -1028,7 +1028,37 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
                  : (s == 7 ? 2.0 * k2
                            : (s == 8 ? 2.0 * k3 : (s == 9 ? k2 * k2 : (s == 10 ? 2.0 * k2 * k3 : k3 * k3))));
   };
-  auto run_step = [&](int s, double2 (&pre)[LT], int snext) {
+  // The state entries this thread updates, uprev and q0 (when set), all loaded before
+  // any is used: the last step issues the uprev loads before its transform and the q0
+  // loads before its accumulation, so that the round trips overlap that work (not with
+  // the SGS output, whose accumulators leave no room for them)
+  constexpr bool PRE = !SGS;
+  double2 su[PRE ? IT : 1][3], sq[PRE ? IT : 1][3];
+  auto load_state = [&](int which) {
+    if constexpr (PRE) {
+      const double2* src = which == 0 ? uprev_ : q0_;
+      if (src) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+          const int e = tid + it * NT;
+          const int kx = e / T, c = e - (e / T) * T;
+          const bool ok = e < NS1 * T && c0 + c < a.H;
+          const long long idx = (long long)kx * plane + (long long)ky * a.Nc + c0 + c;
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) {
+            const double2 v = ok ? src[cc * fstride + idx] : make_double2(0.0, 0.0);
+            if (which == 0)
+              su[it][cc] = v;
+            else
+              sq[it][cc] = v;
+          }
+        }
+      }
+    }
+  };
+  // LAST: the final step, which issues the state loads in place of a next field's
+  // (a compile-time flag, so that those registers are live in that step only)
+  auto run_step = [&](auto last, int s, double2 (&pre)[LT], int snext) {
     const bool fold = EPI == E_SMAGF && s >= 7;
     const bool add = fold && (s == 8 || s == 10 || s == 11);
     const bool transform = !fold || s == 8 || s == 11;
@@ -1044,10 +1074,14 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
         buf[c * LS + RW(l)] = v;
       }
     }
-    if (snext < NSTEP) load_field(snext, pre);
+    if constexpr (decltype(last)::value)
+      load_state(0);
+    else if (snext < NSTEP)
+      load_field(snext, pre);
     if (!transform) return;
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
+    if constexpr (decltype(last)::value) load_state(1);
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = tid + it * NT;
@@ -1063,22 +1097,18 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
     __syncthreads();
   };
   // step s+1 loads while step s is transformed (a second buffer, two steps ahead,
-  // needs 256 VGPRs: it spilled and ran 12% slower)
+  // measured no faster)
   double2 pa[LT];
   load_field(0, pa);
 #pragma unroll 1
-  for (int s = 0; s < NSTEP; ++s) run_step(s, pa, s + 1);
+  for (int s = 0; s < NSTEP - 1; ++s) run_step(std::false_type{}, s, pa, s + 1);
+  run_step(std::true_type{}, NSTEP - 1, pa, NSTEP);
 
   const double dtc = q0_ ? (*dt_) * a.c : 0.0;
   const double coef = (EPI == E_TAU2 && a.store_only) ? 1.0 : (a.coef_dev ? *a.coef_dev : a.coef);
   double2 red = make_double2(0.0, 0.0);
-  // the state entries this thread updates (uprev, and q0 when set; for the t-model
-  // also PL and the reduction's velocity), all loaded before any is used: one memory
-  // round trip instead of one per component (not with the SGS output, whose
-  // accumulators leave no room for them)
-  constexpr bool PRE = !SGS;
+  // the t-model's PL and reduction velocity, loaded before any is used
   constexpr bool PRET = EPI == E_TAU2;
-  double2 su[PRE ? IT : 1][3], sq[PRE ? IT : 1][3];
   double2 sp[PRET ? IT : 1][3], sr[PRET ? IT : 1][3];
   if constexpr (PRET) {
     const bool want_pl = pl_ && !a.store_only;
@@ -1092,28 +1122,6 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
       for (int cc = 0; cc < 3; ++cc) {
         sp[it][cc] = ok && want_pl ? pl_[cc * fstride + idx] : make_double2(0.0, 0.0);
         sr[it][cc] = ok && a.red_u ? a.red_u[cc * fstride + idx] : make_double2(0.0, 0.0);
-      }
-    }
-  }
-  if constexpr (PRE) {
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * NT;
-      const int kx = e / T, c = e - (e / T) * T;
-      const bool ok = e < NS1 * T && c0 + c < a.H;
-      const long long idx = (long long)kx * plane + (long long)ky * a.Nc + c0 + c;
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) su[it][cc] = ok ? uprev_[cc * fstride + idx] : make_double2(0.0, 0.0);
-    }
-    if (q0_) {
-#pragma unroll
-      for (int it = 0; it < IT; ++it) {
-        const int e = tid + it * NT;
-        const int kx = e / T, c = e - (e / T) * T;
-        const bool ok = e < NS1 * T && c0 + c < a.H;
-        const long long idx = (long long)kx * plane + (long long)ky * a.Nc + c0 + c;
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) sq[it][cc] = ok ? q0_[cc * fstride + idx] : make_double2(0.0, 0.0);
       }
     }
   }

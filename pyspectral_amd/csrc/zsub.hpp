@@ -234,10 +234,11 @@ __global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ 
       for (int k = l; k < HK; k += W) {
         const double2 z1 = sb[Z::nat(k)];
         const double2 z2 = sb[Z::nat(k == 0 ? 0 : L - k)];
-        zm.out[(long long)r0 * a.out_sf + pos0 * a.Hout + k] = make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
+        store_nt(&zm.out[(long long)r0 * a.out_sf + pos0 * a.Hout + k],
+                 make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y)));
         if (r1 < NRO)
-          zm.out[(long long)r1 * a.out_sf + pos0 * a.Hout + k] =
-              make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
+          store_nt(&zm.out[(long long)r1 * a.out_sf + pos0 * a.Hout + k],
+                   make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x)));
       }
     }
     wave_sync();

@@ -462,11 +462,11 @@ __global__ __launch_bounds__(64, (zw_min_waves<L, MODE, HK>())) void zwave(ZArgs
           for (int k = l; k < Ho; k += 64) {
             const double2 z1 = buf[gi * W::NATS + W::nat(k)];
             const double2 z2 = buf[gi * W::NATS + W::nat(k == 0 ? 0 : L - k)];
-            zm.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k] =
-                make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y));
+            store_nt(&zm.out[(long long)(r0 / P) * a.out_sf + (pos0 + r0 % P) * a.Hout + k],
+                     make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y)));
             if (r1 < NRO)
-              zm.out[(long long)(r1 / P) * a.out_sf + (pos0 + r1 % P) * a.Hout + k] =
-                  make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x));
+              store_nt(&zm.out[(long long)(r1 / P) * a.out_sf + (pos0 + r1 % P) * a.Hout + k],
+                       make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x)));
           }
         }
       }
