@@ -29,9 +29,11 @@ template <int L> struct ZSubShape {
   static constexpr int R0 = L / W;                      // 3
   static constexpr int S = 64 / W;                      // positions per wavefront
   static constexpr int NG = (64 + W - 1) / W;           // lane groups (S, or S + 1 idle)
+  static constexpr int G = 2;                           // complex pencils per round of LDS passes
   static constexpr int SUBW = B * (A + 1);              // sub-transform slot; pass-1 output at b1 (A + 1) + j
   static constexpr int NATS = L + L / 16 + 1;           // natural-order row, one pad entry per 16
-  static constexpr int SG = R0 * SUBW > NATS ? R0 * SUBW : NATS;  // LDS entries of one group
+  static constexpr int SP = R0 * SUBW > NATS ? R0 * SUBW : NATS;  // LDS entries of one pencil
+  static constexpr int SG = G * SP;                     // LDS entries of one group
   static __device__ __forceinline__ int nat(int n) { return n + (n >> 4); }
   static_assert(W <= SUBW, "register-step slots");
 };
@@ -48,46 +50,68 @@ __host__ __device__ constexpr bool zs_zero_chunk(int m) {
   return true;
 }
 
-// The W-point transforms of the R0 sub-transforms held at sb[a SUBW + l]; the result
-// x_{a + R0 b} lands at sb[nat(a + R0 b)].  twl: the length-L table w_L^k in LDS.
+// The W-point transforms of the R0 sub-transforms of np <= G pencils, pencil g's held
+// at sb[g SP + a SUBW + l]; the result x_{a + R0 b} of pencil g lands at
+// sb[g SP + nat(a + R0 b)].  twl: the length-L table w_L^k in LDS.  The group's W
+// lanes share the butterflies of all np pencils in each pass, so a round of passes
+// (and its wave syncs) serves two pencils.
 template <int L, int SIGN>
-__device__ __forceinline__ void zs_subW(double2* sb, const double2* twl, int l) {
+__device__ __forceinline__ void zs_subW(double2* sb, const double2* twl, int l, int np) {
   using Z = ZSubShape<L>;
-  constexpr int A = Z::A, B = Z::B, R0 = Z::R0, SUBW = Z::SUBW;
-  // pass 1: lane l < R0 A is (a, j); a B-point transform over r, twiddle w_W^{e j b1}
-  const bool p1 = l < R0 * A;
-  const int a1 = l / A, j = l - (l / A) * A;
-  double2 y[B];
-  if (p1) {
+  constexpr int A = Z::A, B = Z::B, R0 = Z::R0, SUBW = Z::SUBW, W = Z::W, SP = Z::SP, G = Z::G;
+  // pass 1: butterfly u = (g, a, j), u = (g R0 + a) A + j; a B-point transform over r,
+  // twiddle w_W^{e j b1}
+  constexpr int N1 = G * R0 * A, IT1 = (N1 + W - 1) / W;
+  double2 y[IT1][B];
 #pragma unroll
-    for (int r = 0; r < B; ++r) y[r] = sb[a1 * SUBW + j + A * r];
-    Dft<B, SIGN>::run(y);
+  for (int it = 0; it < IT1; ++it) {
+    const int u = l + W * it, ga = u / A, j = u - ga * A;
+    if ((N1 % W == 0 || u < N1) && ga < np * R0) {
+      const int g = ga / R0, a1 = ga - g * R0;
 #pragma unroll
-    for (int b1 = 1; b1 < B; ++b1) {
-      double2 w = twl[R0 * j * b1];  // w_L^{R0 j b1} = w_W^{j b1}
-      if constexpr (SIGN > 0) w.y = -w.y;
-      y[b1] = cmul(y[b1], w);
+      for (int r = 0; r < B; ++r) y[it][r] = sb[g * SP + a1 * SUBW + j + A * r];
+      Dft<B, SIGN>::run(y[it]);
+#pragma unroll
+      for (int b1 = 1; b1 < B; ++b1) {
+        double2 w = twl[R0 * j * b1];  // w_L^{R0 j b1} = w_W^{j b1}
+        if constexpr (SIGN > 0) w.y = -w.y;
+        y[it][b1] = cmul(y[it][b1], w);
+      }
     }
   }
   wave_sync();
-  if (p1) {
 #pragma unroll
-    for (int b1 = 0; b1 < B; ++b1) sb[a1 * SUBW + b1 * (A + 1) + j] = y[b1];
+  for (int it = 0; it < IT1; ++it) {
+    const int u = l + W * it, ga = u / A, j = u - ga * A;
+    if ((N1 % W == 0 || u < N1) && ga < np * R0) {
+      const int g = ga / R0, a1 = ga - g * R0;
+#pragma unroll
+      for (int b1 = 0; b1 < B; ++b1) sb[g * SP + a1 * SUBW + b1 * (A + 1) + j] = y[it][b1];
+    }
   }
   wave_sync();
-  // pass 2: lane l < R0 B is (a, b1); an A-point transform over j
-  const bool p2 = l < R0 * B;
-  const int a2 = l / B, b1 = l - (l / B) * B;
-  double2 z[A];
-  if (p2) {
+  // pass 2: butterfly u = (g, a, b1), u = (g R0 + a) B + b1; an A-point transform over j
+  constexpr int N2 = G * R0 * B, IT2 = (N2 + W - 1) / W;
+  double2 z[IT2][A];
 #pragma unroll
-    for (int jj = 0; jj < A; ++jj) z[jj] = sb[a2 * SUBW + b1 * (A + 1) + jj];
-    Dft<A, SIGN>::run(z);
+  for (int it = 0; it < IT2; ++it) {
+    const int u = l + W * it, ga = u / B, b1 = u - ga * B;
+    if ((N2 % W == 0 || u < N2) && ga < np * R0) {
+      const int g = ga / R0, a2 = ga - g * R0;
+#pragma unroll
+      for (int jj = 0; jj < A; ++jj) z[it][jj] = sb[g * SP + a2 * SUBW + b1 * (A + 1) + jj];
+      Dft<A, SIGN>::run(z[it]);
+    }
   }
   wave_sync();
-  if (p2) {
 #pragma unroll
-    for (int b2 = 0; b2 < A; ++b2) sb[Z::nat(a2 + R0 * (b1 + B * b2))] = z[b2];
+  for (int it = 0; it < IT2; ++it) {
+    const int u = l + W * it, ga = u / B, b1 = u - ga * B;
+    if ((N2 % W == 0 || u < N2) && ga < np * R0) {
+      const int g = ga / R0, a2 = ga - g * R0;
+#pragma unroll
+      for (int b2 = 0; b2 < A; ++b2) sb[g * SP + Z::nat(a2 + R0 * (b1 + B * b2))] = z[it][b2];
+    }
   }
   wave_sync();
 }
@@ -118,42 +142,59 @@ __global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ 
   for (int av = 0; av < R0; ++av) t1[av] = tw[av * l];
   for (int i = lane; i < L; i += 64) twl[i] = tw[i];
 
-  // ---- inverse transforms; lane l keeps x_n, n = l + W m, of every pencil
-  constexpr ZGroupUse<MODE, 1, NRI, 1> use{};
-  auto load_pencil = [&](int q, double2 (&raw)[R0][1][NF]) {
+  // ---- inverse transforms, G pencils per round; lane l keeps x_n, n = l + W m, of
+  // every pencil
+  constexpr int G = Z::G, SP = Z::SP;
+  constexpr ZGroupUse<MODE, 1, NRI, G> use{};
+  // every raw entry of a round's pencils, loaded before any is used; the next round's
+  // loads are issued before this round's LDS passes
+  auto load_round = [&](int q0, double2 (&raw)[R0][1][NF]) {
 #pragma unroll
     for (int m = 0; m < R0; ++m) {
       if (zs_zero_chunk<L, W, HK>(m)) continue;
       const int k = l + W * m, kk = k > L / 2 ? L - k : k;
 #pragma unroll
       for (int fi = 0; fi < NF; ++fi)
-        if ((use.bits[q][0] >> fi) & 1u) raw[m][0][fi] = zw_in<HK>(a, zm, fi, pos0, kk);
+        if ((use.bits[q0 / G][0] >> fi) & 1u) raw[m][0][fi] = zw_in<HK>(a, zm, fi, pos0, kk);
     }
     __builtin_amdgcn_sched_barrier(0);
   };
   double2 x[NCI][R0];
   double2 raw[2][R0][1][NF];
-  load_pencil(0, raw[0]);
+  load_round(0, raw[0]);
   wave_sync();  // the LDS twiddle table
 #pragma unroll
-  for (int q = 0; q < NCI; ++q) {
-    const int cur = q & 1;
-    double2 v[R0];
+  for (int q0 = 0; q0 < NCI; q0 += G) {
+    const int cur = (q0 / G) & 1;
+    const int np = NCI - q0 < G ? NCI - q0 : G;
 #pragma unroll
-    for (int m = 0; m < R0; ++m) {
-      const int k = l + W * m, kk = k > L / 2 ? L - k : k;
-      v[m] = zs_zero_chunk<L, W, HK>(m) ? make_double2(0.0, 0.0)
-                                        : zw_hermitian<L, MODE, 1, NRI, NF>(q, k, a.kzscale * kk, raw[cur][m]);
+    for (int g = 0; g < G; ++g) {
+      const int q = q0 + g;
+      if (q < NCI) {
+        double2 v[R0];
+#pragma unroll
+        for (int m = 0; m < R0; ++m) {
+          const int k = l + W * m, kk = k > L / 2 ? L - k : k;
+          v[m] = zs_zero_chunk<L, W, HK>(m) ? make_double2(0.0, 0.0)
+                                            : zw_hermitian<L, MODE, 1, NRI, NF>(q, k, a.kzscale * kk, raw[cur][m]);
+        }
+        Dft<R0, +1>::run(v);
+        sb[g * SP + l] = v[0];
+#pragma unroll
+        for (int av = 1; av < R0; ++av) sb[g * SP + av * SUBW + l] = cmul(v[av], cconj(t1[av]));
+      }
     }
-    Dft<R0, +1>::run(v);
-    sb[l] = v[0];
-#pragma unroll
-    for (int av = 1; av < R0; ++av) sb[av * SUBW + l] = cmul(v[av], cconj(t1[av]));
-    if (q + 1 < NCI) load_pencil(q + 1, raw[cur ^ 1]);
+    if (q0 + G < NCI) load_round(q0 + G, raw[cur ^ 1]);
     wave_sync();
-    zs_subW<L, +1>(sb, twl, l);
+    zs_subW<L, +1>(sb, twl, l, np);
 #pragma unroll
-    for (int m = 0; m < R0; ++m) x[q][m] = sb[Z::nat(l + W * m)];
+    for (int g = 0; g < G; ++g) {
+      const int q = q0 + g;
+      if (q < NCI) {
+#pragma unroll
+        for (int m = 0; m < R0; ++m) x[q][m] = sb[g * SP + Z::nat(l + W * m)];
+      }
+    }
     wave_sync();
   }
 
@@ -216,29 +257,43 @@ __global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ 
     }
   }
 
-  // ---- forward transforms, truncation to HK modes, split of the pencil pairs
+  // ---- forward transforms, G pencils per round, truncation to HK modes, split of the
+  // pencil pairs
 #pragma unroll
-  for (int q = 0; q < NCO; ++q) {
-    double2 v[R0];
+  for (int q0 = 0; q0 < NCO; q0 += G) {
+    const int np = NCO - q0 < G ? NCO - q0 : G;
 #pragma unroll
-    for (int m = 0; m < R0; ++m) v[m] = xo[q][m];
-    Dft<R0, -1>::run(v);
-    sb[l] = v[0];
+    for (int g = 0; g < G; ++g) {
+      const int q = q0 + g;
+      if (q < NCO) {
+        double2 v[R0];
 #pragma unroll
-    for (int av = 1; av < R0; ++av) sb[av * SUBW + l] = cmul(v[av], t1[av]);
+        for (int m = 0; m < R0; ++m) v[m] = xo[q][m];
+        Dft<R0, -1>::run(v);
+        sb[g * SP + l] = v[0];
+#pragma unroll
+        for (int av = 1; av < R0; ++av) sb[g * SP + av * SUBW + l] = cmul(v[av], t1[av]);
+      }
+    }
     wave_sync();
-    zs_subW<L, -1>(sb, twl, l);
+    zs_subW<L, -1>(sb, twl, l, np);
     if (act) {
-      const int r0 = 2 * q, r1 = 2 * q + 1;
 #pragma unroll
-      for (int k = l; k < HK; k += W) {
-        const double2 z1 = sb[Z::nat(k)];
-        const double2 z2 = sb[Z::nat(k == 0 ? 0 : L - k)];
-        store_nt(&zm.out[(long long)r0 * a.out_sf + pos0 * a.Hout + k],
-                 make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y)));
-        if (r1 < NRO)
-          store_nt(&zm.out[(long long)r1 * a.out_sf + pos0 * a.Hout + k],
-                   make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x)));
+      for (int g = 0; g < G; ++g) {
+        const int q = q0 + g;
+        if (q < NCO) {
+          const int r0 = 2 * q, r1 = 2 * q + 1;
+#pragma unroll
+          for (int k = l; k < HK; k += W) {
+            const double2 z1 = sb[g * SP + Z::nat(k)];
+            const double2 z2 = sb[g * SP + Z::nat(k == 0 ? 0 : L - k)];
+            store_nt(&zm.out[(long long)r0 * a.out_sf + pos0 * a.Hout + k],
+                     make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y - z2.y)));
+            if (r1 < NRO)
+              store_nt(&zm.out[(long long)r1 * a.out_sf + pos0 * a.Hout + k],
+                       make_double2(0.5 * (z1.y + z2.y), 0.5 * (z2.x - z1.x)));
+          }
+        }
       }
     }
     wave_sync();
