@@ -32,6 +32,15 @@ int pencil_tile(int L, int T) {
   }
 }
 
+// The width a pass launches with: T when given, else the length's default, narrowed
+// to 8 where 16 does not divide the column count C but 8 does (C = 24: two 16-wide
+// tiles would leave a quarter of the block's columns empty).
+static int launch_width(int L, int T, int C) {
+  if (T > 0) return pencil_tile(L, T);
+  const int t = pencil_T(L);
+  return (t == 16 && C % 16 != 0 && C % 8 == 0) ? 8 : t;
+}
+
 // kind: 0 forward, 1 inverse
 template <int L, int T, int KIND>
 static hipError_t launch_tile(const PencilArgs& a, const double2* tw, int njobs, hipStream_t s) {
@@ -58,7 +67,7 @@ static hipError_t launch_len(int T, const PencilArgs& a, const double2* tw, int 
 }
 
 hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T) {
-  const int t = pencil_tile(L, T);
+  const int t = launch_width(L, T, a.C);
   switch (L) {
 #define PSX_CASE(LL) \
   case LL:           \
@@ -71,7 +80,7 @@ hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, 
 }
 
 hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T) {
-  const int t = pencil_tile(L, T);
+  const int t = launch_width(L, T, a.C);
   switch (L) {
 #define PSX_CASE(LL) \
   case LL:           \
@@ -96,7 +105,7 @@ static hipError_t launch_sum_tile(const SumArgs& a, const double2* tw, int njobs
 
 hipError_t launch_pencil_forward_sum(int L, const SumArgs& a, const double2* tw, int njobs, hipStream_t s, int T) {
   if (3 * a.p.Nspec != 2 * L) return hipErrorInvalidValue;  // the kernel's compile-time truncation
-  const int t = pencil_tile(L, T);
+  const int t = launch_width(L, T, a.p.C);
   switch (L) {
 #define PSX_CASE(LL)                                                     \
   case LL:                                                               \
