@@ -118,9 +118,9 @@ hipError_t launch_zpass_lrn(int L, const ZArgs& a, const double2* tw, hipStream_
 // lane-group lengths 48, 72, 96 (positions per wavefront)
 int zpass_positions_lrnf(int L) {
   switch (L) {
-    case 48: return ZSubShape<48>::S;
-    case 72: return ZSubShape<72>::S;
-    case 96: return ZSubShape<96>::S;
+    case 48: return ZSubGroups<48, zs_waves<48, Z_LRNF>()>::S;
+    case 72: return ZSubGroups<72, zs_waves<72, Z_LRNF>()>::S;
+    case 96: return ZSubGroups<96, zs_waves<96, Z_LRNF>()>::S;
     case 64: return z_P<Z_LRNF>(64);
     case 128: return z_P<Z_LRNF>(128);
     case 192: return z_P<Z_LRNF>(192);

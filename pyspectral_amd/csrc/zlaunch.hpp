@@ -27,9 +27,9 @@ hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int va
   if constexpr (MODE == Z_LRNF) {
     // the fused closure exists as the one-wavefront and the lane-group kernels only
     if constexpr (ZSub<L>::ok) {
-      constexpr int S = ZSubShape<L>::S;
+      constexpr int NW = zs_waves<L, MODE>(), S = ZSubGroups<L, NW>::S;
       if (zw_third<L, MODE>(a) && a.npos % S == 0) {
-        hipLaunchKernelGGL((zsub<L, MODE, L / 3>), dim3((unsigned)(a.npos / S)), dim3(64), 0, s, a, tw);
+        hipLaunchKernelGGL((zsub<L, MODE, L / 3>), dim3((unsigned)(a.npos / S)), dim3(64 * NW), 0, s, a, tw);
         return hipGetLastError();
       }
       return hipErrorInvalidValue;
@@ -48,9 +48,9 @@ hipError_t zpass_launch(const ZArgs& a, const double2* tw, hipStream_t s, int va
   } else {
     if constexpr (ZSub<L>::ok && (MODE == Z_DNS || MODE == Z_SMAG)) {
       // the short lengths of small grids: a lane group per position
-      constexpr int S = ZSubShape<L>::S;
+      constexpr int NW = zs_waves<L, MODE>(), S = ZSubGroups<L, NW>::S;
       if (variant == 0 && zw_third<L, MODE>(a) && a.npos % S == 0) {
-        hipLaunchKernelGGL((zsub<L, MODE, L / 3>), dim3((unsigned)(a.npos / S)), dim3(64), 0, s, a, tw);
+        hipLaunchKernelGGL((zsub<L, MODE, L / 3>), dim3((unsigned)(a.npos / S)), dim3(64 * NW), 0, s, a, tw);
         return hipGetLastError();
       }
     }

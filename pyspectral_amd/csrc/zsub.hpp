@@ -1,5 +1,5 @@
 // Fused z pass for the short z lengths of small grids: L = 3 W with W = 16, 24 or 32
-// lanes per position and S = 64 / W positions per wavefront.
+// lanes per position and S = 64 NW / W positions per workgroup of NW wavefronts.
 //
 // Same contract as zwave (zwave.hpp) with one position per lane group, and the same
 // data path scaled down.  Lane l < W of group s owns the entries k = l + W m (m < 3)
@@ -12,23 +12,27 @@
 //   x_{a + 3 (b1 + B b2)} = sum_j U_a(j, b1) w_A^{e b2 j}                 (LDS pass 2)
 // The block-wide Stockham kernel (zpass) spends most of its time on barriers and idle
 // threads at these lengths; one wavefront per S positions needs no workgroup barrier.
-// At W = 24 the lanes 48..63 form a third group that computes on the first group's
-// position in LDS of its own and stores nothing.
+// At W = 24 the products modes run three wavefronts per workgroup (eight lane groups,
+// no idle lanes) and pay a barrier per LDS pass; the fused closure runs one wavefront,
+// whose lanes 48..63 form a third group that computes on the first group's position
+// in LDS of its own and stores nothing.
 #pragma once
 #include "zwave.hpp"
 
 namespace psx {
 
+// NW: wavefronts per workgroup of the products modes
 template <int L> struct ZSub { static constexpr bool ok = false; };
-template <> struct ZSub<48> { static constexpr bool ok = true; static constexpr int W = 16, A = 4, B = 4; };
-template <> struct ZSub<72> { static constexpr bool ok = true; static constexpr int W = 24, A = 8, B = 3; };
-template <> struct ZSub<96> { static constexpr bool ok = true; static constexpr int W = 32, A = 8, B = 4; };
+template <> struct ZSub<48> { static constexpr bool ok = true; static constexpr int W = 16, A = 4, B = 4, NW = 1; };
+template <> struct ZSub<72> { static constexpr bool ok = true; static constexpr int W = 24, A = 8, B = 3, NW = 3; };
+template <> struct ZSub<96> { static constexpr bool ok = true; static constexpr int W = 32, A = 8, B = 4, NW = 1; };
+// the fused closure keeps one wavefront (its register use leaves room for one
+// three-wavefront workgroup per CU)
+template <int L, int MODE> constexpr int zs_waves() { return MODE == Z_LRNF ? 1 : ZSub<L>::NW; }
 
 template <int L> struct ZSubShape {
   static constexpr int W = ZSub<L>::W, A = ZSub<L>::A, B = ZSub<L>::B;
   static constexpr int R0 = L / W;                      // 3
-  static constexpr int S = 64 / W;                      // positions per wavefront
-  static constexpr int NG = (64 + W - 1) / W;           // lane groups (S, or S + 1 idle)
   static constexpr int G = 2;                           // complex pencils per round of LDS passes
   static constexpr int SUBW = B * (A + 1);              // sub-transform slot; pass-1 output at b1 (A + 1) + j
   static constexpr int NATS = L + L / 16 + 1;           // natural-order row, one pad entry per 16
@@ -36,6 +40,12 @@ template <int L> struct ZSubShape {
   static constexpr int SG = G * SP;                     // LDS entries of one group
   static __device__ __forceinline__ int nat(int n) { return n + (n >> 4); }
   static_assert(W <= SUBW, "register-step slots");
+};
+// the lane groups of a workgroup of NW wavefronts
+template <int L, int NW> struct ZSubGroups {
+  static constexpr int W = ZSub<L>::W;
+  static constexpr int S = 64 * NW / W;                 // positions per workgroup
+  static constexpr int NG = (64 * NW + W - 1) / W;      // lane groups (S, or S + 1 idle)
 };
 
 // Entries k = l + W m (every l < W) of chunk m all have min(k, L - k) >= HK: the 3/2
@@ -118,16 +128,18 @@ __device__ __forceinline__ void zs_subW(double2* sb, const double2* twl, int l, 
 
 // HK: the input and output half spectra hold HK = L / 3 entries (the launcher checks).
 template <int L, int MODE, int HK>
-__global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__((64 * zs_waves<L, MODE>())) void zsub(ZArgs a, const double2* __restrict__ tw) {
   using Z = ZSubShape<L>;
   static_assert(ZSub<L>::ok, "zsub: L must be 48, 72 or 96");
   static_assert(MODE == Z_DNS || MODE == Z_SMAG || MODE == Z_LRNF, "zsub: product modes only");
-  constexpr int W = Z::W, R0 = Z::R0, S = Z::S, SUBW = Z::SUBW, SG = Z::SG;
+  constexpr int NW = zs_waves<L, MODE>(), NTB = 64 * NW;
+  using ZG = ZSubGroups<L, NW>;
+  constexpr int W = Z::W, R0 = Z::R0, S = ZG::S, SUBW = Z::SUBW, SG = Z::SG;
   constexpr int NIN = ZShape<MODE>::nin, NOUT = ZShape<MODE>::nout;
   constexpr int NRI = NIN, NRO = NOUT;
   constexpr int NCI = (NRI + 1) / 2, NCO = (NRO + 1) / 2;
   constexpr int NF = ZFields<MODE>::n;
-  __shared__ double2 buf[Z::NG * SG];
+  __shared__ double2 buf[ZG::NG * SG];
   __shared__ double2 twl[L];
   const int lane = threadIdx.x;
   const int s = lane / W, l = lane - (lane / W) * W;
@@ -140,7 +152,7 @@ __global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ 
   double2 t1[R0];
 #pragma unroll
   for (int av = 0; av < R0; ++av) t1[av] = tw[av * l];
-  for (int i = lane; i < L; i += 64) twl[i] = tw[i];
+  for (int i = lane; i < L; i += NTB) twl[i] = tw[i];
 
   // ---- inverse transforms, G pencils per round; lane l keeps x_n, n = l + W m, of
   // every pencil
@@ -204,7 +216,8 @@ __global__ __launch_bounds__(64) void zsub(ZArgs a, const double2* __restrict__ 
   // 16-point MFMA tiles over all 64 lanes
   [[maybe_unused]] float* rec = nullptr;
   if constexpr (MODE == Z_LRNF) {
-    __shared__ float recs[Z::NG * L * 16];
+    static_assert(NW == 1, "the fused closure runs one wavefront per workgroup");
+    __shared__ float recs[ZG::NG * L * 16];
     rec = recs + s * L * 16;
 #pragma unroll
     for (int m = 0; m < R0; ++m) {
