@@ -215,9 +215,9 @@ int psx_sgs_dns(psx_context* ctx, const double* uhat, double* w0_out, double* ta
 int psx_tau(psx_context* ctx, double* out_host8);
 
 /* Kernel shapes (no counterpart in the reference): tune[0..2] = tile widths (k3 columns
- * per block: 4, 8 or 16; 0 = the built-in default: 16 up to length 96, or 8 where 16
- * does not divide the column count but 8 does; 8 up to 384; 4 beyond) of the x-inverse,
- * y-inverse and y-forward passes, tune[3] = z-pass variant (0: one wavefront per position where the
+ * per block: 4, 8 or 16, and 24 or 32 up to length 96; 0 = the built-in default: 16 up
+ * to length 96, or 24, else 8, where 16 does not divide the column count but that width
+ * does; 8 up to 384; 4 beyond) of the x-inverse, y-inverse and y-forward passes, tune[3] = z-pass variant (0: one wavefront per position where the
  * z length is a multiple of 64, 1: the block-wide Stockham kernel), tune[4] = number of
  * x-row chunks (0 or 1: none; at most 8) in which the y-inverse, z and y-forward passes
  * run alternately on the context stream and a second library stream, so that their

@@ -39,9 +39,9 @@ bool length_supported(int L);
 hipError_t launch_zreal_forward(int L, const double* phys, long long in_sf, double2* out, long long out_sf, int npos,
                                 int nfields, const double2* tw, hipStream_t s);
 
-// T: tile width (0 = the length's default pencil_T, or 8 where 16 does not divide the
-// column count but 8 does); pencil_tile(L, T) gives the width used for T > 0, or -1
-// when the width is not available for that length
+// T: tile width (0 = the length's default pencil_T, or 24 or 8 where 16 does not
+// divide the column count but 24 or 8 does); pencil_tile(L, T) gives the width used
+// for T > 0, or -1 when the width is not available for that length
 int pencil_tile(int L, int T);
 hipError_t launch_pencil_inverse(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T = 0);
 hipError_t launch_pencil_forward(int L, const PencilArgs& a, const double2* tw, int njobs, hipStream_t s, int T = 0);

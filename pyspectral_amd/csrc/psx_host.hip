@@ -1908,7 +1908,7 @@ int psx_autotune(psx_context* c, const double* uhat, int reps, int* tune_out) {
         if (knob == 5) continue;
         std::vector<int> cand;
         if (knob < 3) {
-          for (int T : {4, 8, 16})
+          for (int T : {4, 8, 16, 24, 32})
             if (pencil_tile(L[knob], T) == T) cand.push_back(T);
         } else if (knob >= 7 && knob <= 9) {
           cand = {0, 1, 2};  // block orders
@@ -2211,7 +2211,7 @@ int psx_ensemble_autotune(psx_ensemble* e, const double* q, int reps, int* tune_
       if (knob == 10 && !smag_form) continue;
       std::vector<int> cand{c->tune[knob]};  // the current choice first: others must beat it
       if (knob < 3) {
-        for (int T : {4, 8, 16})
+        for (int T : {4, 8, 16, 24, 32})
           if (pencil_tile(L[knob], T) == T && T != c->tune[knob]) cand.push_back(T);
       } else if (knob >= 7 && knob <= 9) {
         for (int v : {0, 1, 2})

@@ -32,7 +32,9 @@ def test_every_kernel_shape_matches_oracle(model):
     ref = st.compact()
     s = SpectralSolver(*shape, model, nu=0.01)
     ud = torch.from_numpy(u).cuda()
-    for T in (4, 8, 16):
+    # lengths 48 (x) and 24 (y) take every width; 24 and 32 leave a partial last tile
+    # of the 64 k3 columns
+    for T in (4, 8, 16, 24, 32):
         for zv in (0, 1):
             for order in (0, 1, 2):
                 print(f"\nmodel={model} tile={T} z={zv} order={order}", flush=True)
@@ -44,7 +46,7 @@ def test_every_kernel_shape_matches_oracle(model):
                 assert err < 1e-12, (T, zv, order, err)
     tuned = s.autotune(ud, reps=2)
     assert set(tuned) == set(s.TUNING_KEYS)
-    assert all(tuned[k] in (4, 8, 16) for k in s.TUNING_KEYS[:3]) and tuned["z_variant"] in (0, 1)
+    assert all(tuned[k] in (4, 8, 16, 24, 32) for k in s.TUNING_KEYS[:3]) and tuned["z_variant"] in (0, 1)
     r = s.rhs(ud, 0.0).cpu().numpy()
     assert np.linalg.norm(r - ref) / np.linalg.norm(ref) < 1e-12
     s.close()
@@ -54,9 +56,11 @@ def test_every_kernel_shape_matches_oracle(model):
 @pytest.mark.parametrize("model", [1, 0])
 def test_short_z_lengths_match_oracle(shape, model):
     """Padded z lengths 48, 72 and 96 (the small grids of the ensembles and the 64^3
-    LES): the lane-group z pass (z_variant 0: 16, 24 or 32 lanes per position, 4, 2 or
-    2 positions per wavefront) and the block-wide one (z_variant 1) both give the
-    oracle's RHS, and one RK4 step with either gives the same state to round-off."""
+    LES): the lane-group z pass (z_variant 0: 16, 24 or 32 lanes per position; 4
+    positions per wavefront, 8 per three-wavefront workgroup, 2 per wavefront) and the
+    block-wide one (z_variant 1) both give the oracle's RHS, and one RK4 step with
+    either gives the same state to round-off.  The 24 k3 columns of the z = 48 grids
+    take the 24-column pencil tiles by default."""
     from pyspectral_amd.solver import SpectralSolver
 
     u = _field(*shape, sum(shape) + model)
