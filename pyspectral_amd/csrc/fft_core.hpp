@@ -256,4 +256,103 @@ __device__ __forceinline__ void fft_batch(double2* buf, const double2* __restric
   fft_stages<L, B, NT, LS, SIGN, 0, 1, PAD>(buf, tw, tid);
 }
 
+// ------------------------------------------------------- register-resident batch
+// Radix sequences whose first and last stages are radix 4, for batches where each
+// thread holds exactly four entries (B L = 4 NT).
+template <int L> struct RegPlan { static constexpr bool ok = false; };
+#define PSX_REGPLAN(LEN, ...)                                     \
+  template <> struct RegPlan<LEN> {                               \
+    static constexpr bool ok = true;                              \
+    static constexpr int r[] = {__VA_ARGS__};                     \
+    static constexpr int n = sizeof(r) / sizeof(int);             \
+  };
+PSX_REGPLAN(128, 4, 8, 4)
+PSX_REGPLAN(256, 4, 4, 4, 4)
+PSX_REGPLAN(512, 4, 8, 4, 4)
+#undef PSX_REGPLAN
+
+// One middle stage S of fft_rows_reg: read `in`, twiddle, radix R, write `out`, barrier;
+// the next stage reads `out`.
+template <int L, int B, int NT, int LS, int SIGN, int S, int NS>
+__device__ __forceinline__ void reg_mid(double2* in, double2* out, const double2* __restrict__ tw, int tid) {
+  using P = RegPlan<L>;
+  if constexpr (S < P::n - 1) {
+    constexpr int R = P::r[S];
+    constexpr int NBF = L / R, TOT = B * NBF, IT = (TOT + NT - 1) / NT, TWS = L / (NS * R);
+    double2 v[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int j = tid + it * NT;
+      if ((TOT % NT) == 0 || j < TOT) {
+        const int b = j / NBF, jj = j - b * NBF;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[it][r] = in[b * LS + Row<L>::at(jj + r * NBF)];
+        double2 w1 = tw[(jj % NS) * TWS];
+        if constexpr (SIGN > 0) w1.y = -w1.y;
+        double2 wc = w1;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          v[it][r] = cmul(v[it][r], wc);
+          if (r + 1 < R) wc = cmul(wc, w1);
+        }
+        Dft<R, SIGN>::run(v[it]);
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int j = tid + it * NT;
+      if ((TOT % NT) == 0 || j < TOT) {
+        const int b = j / NBF, jj = j - b * NBF, k = jj % NS;
+        const int base = (jj / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[b * LS + Row<L>::at(base + r * NS)] = v[it][r];
+      }
+    }
+    __syncthreads();
+    reg_mid<L, B, NT, LS, SIGN, S + 1, NS * R>(out, in, tw, tid);
+  }
+}
+
+// Stockham transforms of B rows of length L (B L = 4 NT) with the data in registers on
+// entry and exit: thread t owns row b = t % B and holds its entries
+// n = t / B + (L / 4) e, e < 4 -- on entry the inputs, on return the outputs at the
+// same indices (the layout of a row-batched load of B contiguous columns, e = t + e NT
+// split as n = e / B, b = e % B).  The radix-4 first and last stages run on the
+// registers; the middle stages pass through the LDS buffers x0 and x1 (B rows of stride
+// LS each, Row<L> padding) in turn, one barrier per stage.  No barrier is needed before
+// the first stage: it writes x0, which no thread still reads, provided the previous
+// call's last stage read x1 -- so consecutive calls pass the buffers swapped when the
+// plan has an even number of stages (fft_rows_reg_swap).
+template <int L, int B, int NT, int LS, int SIGN>
+__device__ __forceinline__ void fft_rows_reg(double2 (&v)[4], double2* x0, double2* x1,
+                                             const double2* __restrict__ tw, int tid) {
+  using P = RegPlan<L>;
+  static_assert(P::ok && B * L == 4 * NT, "four entries per thread");
+  static_assert(P::r[0] == 4 && P::r[P::n - 1] == 4, "radix-4 first and last stages");
+  constexpr int Q = L / 4;
+  const int b = tid % B, jj = tid / B;
+  // first stage (NS = 1): no twiddle; outputs at 4 jj + r
+  Dft<4, SIGN>::run(v);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) x0[b * LS + Row<L>::at(4 * jj + r)] = v[r];
+  __syncthreads();
+  reg_mid<L, B, NT, LS, SIGN, 1, 4>(x0, x1, tw, tid);
+  // last stage (NS = Q): inputs jj + Q r from the buffer the last middle stage wrote,
+  // twiddle w_L^{r jj}, outputs jj + Q r
+  const double2* src = (P::n % 2 == 0) ? x0 : x1;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) v[r] = src[b * LS + Row<L>::at(jj + Q * r)];
+  double2 w1 = tw[jj];
+  if constexpr (SIGN > 0) w1.y = -w1.y;
+  double2 wc = w1;
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    v[r] = cmul(v[r], wc);
+    if (r + 1 < 4) wc = cmul(wc, w1);
+  }
+  Dft<4, SIGN>::run(v);
+}
+// whether consecutive fft_rows_reg calls swap x0 and x1 (the last stage read x0)
+template <int L> constexpr bool fft_rows_reg_swap() { return RegPlan<L>::n % 2 == 0; }
+
 }  // namespace psx

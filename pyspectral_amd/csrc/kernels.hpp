@@ -1326,8 +1326,13 @@ template <int L, int T, bool DYN>
 __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restrict__ tw) {
   constexpr int LS = Row<L>::LS;
   constexpr int IT = (L * T + NT - 1) / NT;
+  // four entries per thread (L = 128, 256, 512): the transforms keep their first and
+  // last stages in registers (fft_rows_reg) on two LDS buffers, so the loaded field
+  // entries feed the first stage and the last stage's outputs feed the accumulation,
+  // with no staging pass in between
+  constexpr bool REG = RegPlan<L>::ok && L * T == 4 * NT;
   static_assert(T * LS >= NT, "block_sum2 needs NT LDS entries");
-  __shared__ double2 buf[T * LS];
+  __shared__ double2 buf[(REG ? 2 : 1) * T * LS];
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.C + T - 1) / T;
@@ -1361,6 +1366,41 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
     }
   };
   load_field(0, tid);
+  [[maybe_unused]] double2* xa = buf;
+  [[maybe_unused]] double2* xb = buf + (REG ? T * LS : 0);
+  if constexpr (REG) {
+    static_assert(LT == 4 && IT == 4, "four entries per thread");
+#pragma unroll 1
+    for (int f = 0; f < 6; ++f) {
+      const int tl = opaque(tid);  // addresses are recomputed for every field
+      // thread tl holds entries kx = tl / T + (L / 4) it of column tl % T, as loaded
+      double2 v[4];
+#pragma unroll
+      for (int it = 0; it < 4; ++it) v[it] = pre[it];
+      if (f + 1 < 6) load_field(f + 1, tl);
+      fft_rows_reg<L, T, NT, LS, -1>(v, xa, xb, twp, tl);
+      if constexpr (fft_rows_reg_swap<L>()) {
+        double2* t = xa;
+        xa = xb;
+        xb = t;
+      }
+      const int m = sym_m(f), n = sym_n(f);
+      const int c = tl % T;
+      if (c0 + c < a.C) {
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+          const int kx = tl / T + (L / 4) * it;
+          const double k[3] = {a.ks[0] * signed_freq(kx, L), k2, a.ks[2] * (c0 + c)};
+          const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const double w = (i == m ? kn : 0.0) + (i == n && m != n ? km : 0.0);
+            acc[it][i] = make_double2(acc[it][i].x + w * v[it].x, acc[it][i].y + w * v[it].y);
+          }
+        }
+      }
+    }
+  } else {
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
     const int tl = opaque(tid);  // addresses are recomputed for every field
@@ -1389,6 +1429,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       }
     }
     __syncthreads();
+  }
   }
 
   // PL = D + P - nu k^2 u (left in acc); base-grid copy; Leonard reduction
@@ -1457,6 +1498,33 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   }
   // x-inverse of PL_q (PL outside the resolved cube), one component at a time
   // (unrolled: acc is indexed by the component)
+  if constexpr (REG) {
+    __syncthreads();  // block_sum2's last reads of buf
+    const int b = tid % T;
+    const bool valid = c0 + b < a.C;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 v[4];
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int s1 = signed_freq(tid / T + (L / 4) * it, L);
+        const bool inside = (s1 > -a.kpx && s1 < a.kpx && s2 > -a.kpy && s2 < a.kpy && c0 + b < a.kpz);
+        v[it] = (valid && !inside) ? acc[it][i] : make_double2(0.0, 0.0);
+      }
+      fft_rows_reg<L, T, NT, LS, +1>(v, xa, xb, twp, tid);
+      if constexpr (fft_rows_reg_swap<L>()) {
+        double2* t = xa;
+        xa = xb;
+        xb = t;
+      }
+      double2* dst = a.xq + (long long)i * a.P1 * a.P2 * a.C + (long long)ky * a.C + c0;
+      if (valid) {
+#pragma unroll
+        for (int it = 0; it < 4; ++it) dst[(long long)(tid / T + (L / 4) * it) * in_sl + b] = v[it];
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     __syncthreads();

@@ -101,6 +101,18 @@ def _field_shape(N1, N2, N3, seed):
 @pytest.mark.parametrize("shape,models", [((16, 16, 128), MODELS), ((16, 32, 64), MODELS),
                                           ((16, 16, 256), [0, 1, 2, "Static tau-model"]), ((16, 16, 512), [0])])
 def test_rhs_long_z_matches_oracle(shape, models):
+    _check_rhs_and_dt(shape, models)
+
+
+# x lengths 64 and 128: the t-model family's PL pass runs the 2x-grid x lengths 128 and
+# 256 with each thread's four entries in registers through the first and last stages
+@pytest.mark.parametrize("shape,models", [((64, 16, 16), [2, "Static tau-model", "Dynamic tau-model"]),
+                                          ((128, 16, 32), [2, "Dynamic tau-model"])])
+def test_rhs_long_x_tau_family_matches_oracle(shape, models):
+    _check_rhs_and_dt(shape, models)
+
+
+def _check_rhs_and_dt(shape, models):
     N1, N2, N3 = shape
     nu, t = 0.011, 0.29
     u = _field_shape(N1, N2, N3, 41 + N3)
