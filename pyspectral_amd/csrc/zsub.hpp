@@ -62,11 +62,15 @@ __host__ __device__ constexpr bool zs_zero_chunk(int m) {
 
 // The W-point transforms of the R0 sub-transforms of np <= G pencils, pencil g's held
 // at sb[g SP + a SUBW + l]; the result x_{a + R0 b} of pencil g lands at
-// sb[g SP + nat(a + R0 b)].  twl: the length-L table w_L^k in LDS.  The group's W
+// sb[g SP + nat(a + R0 b)].  Pass-1 twiddles w_W^{j b1} (forward sign) of the lane's
+// column j = l mod A, which is the same for every butterfly the lane runs (W is a
+// multiple of A): from the registers wj[b1] (WJ), or from the length-L table tw in LDS
+// (the fused closure, which has no registers to spare).  The group's W
 // lanes share the butterflies of all np pencils in each pass, so a round of passes
 // (and its wave syncs) serves two pencils.
-template <int L, int SIGN>
-__device__ __forceinline__ void zs_subW(double2* sb, const double2* twl, int l, int np) {
+template <int L, int SIGN, bool WJ>
+__device__ __forceinline__ void zs_subW(double2* sb, const double2 (&wj)[ZSub<L>::B], const double2* tw, int l,
+                                        int np) {
   using Z = ZSubShape<L>;
   constexpr int A = Z::A, B = Z::B, R0 = Z::R0, SUBW = Z::SUBW, W = Z::W, SP = Z::SP, G = Z::G;
   // pass 1: butterfly u = (g, a, j), u = (g R0 + a) A + j; a B-point transform over r,
@@ -83,9 +87,8 @@ __device__ __forceinline__ void zs_subW(double2* sb, const double2* twl, int l, 
       Dft<B, SIGN>::run(y[it]);
 #pragma unroll
       for (int b1 = 1; b1 < B; ++b1) {
-        double2 w = twl[R0 * j * b1];  // w_L^{R0 j b1} = w_W^{j b1}
-        if constexpr (SIGN > 0) w.y = -w.y;
-        y[it][b1] = cmul(y[it][b1], w);
+        const double2 w = WJ ? wj[b1] : tw[R0 * j * b1];
+        y[it][b1] = cmul(y[it][b1], SIGN > 0 ? cconj(w) : w);
       }
     }
   }
@@ -140,7 +143,8 @@ __global__ __launch_bounds__((64 * zs_waves<L, MODE>())) void zsub(ZArgs a, cons
   constexpr int NCI = (NRI + 1) / 2, NCO = (NRO + 1) / 2;
   constexpr int NF = ZFields<MODE>::n;
   __shared__ double2 buf[ZG::NG * SG];
-  __shared__ double2 twl[L];
+  constexpr bool WJ = MODE != Z_LRNF;
+  __shared__ double2 twl[WJ ? 1 : L];
   const int lane = threadIdx.x;
   const int s = lane / W, l = lane - (lane / W) * W;
   const bool act = s < S;
@@ -148,11 +152,16 @@ __global__ __launch_bounds__((64 * zs_waves<L, MODE>())) void zsub(ZArgs a, cons
   const ZMember zm = zmember(a, (long long)blockIdx.x * S + (act ? s : 0));
   const long long pos0 = zm.pos0;
 
-  // register-step twiddles w_L^{-a l} (the inverse uses the conjugates)
-  double2 t1[R0];
+  // register-step twiddles w_L^{-a l} and the W-point transforms' pass-1 twiddles
+  // w_L^{-R0 j b1} = w_W^{-j b1}, j = l mod A (the inverse uses the conjugates)
+  double2 t1[R0], wj[Z::B];
 #pragma unroll
   for (int av = 0; av < R0; ++av) t1[av] = tw[av * l];
-  for (int i = lane; i < L; i += NTB) twl[i] = tw[i];
+#pragma unroll
+  for (int b1 = 0; b1 < Z::B; ++b1) wj[b1] = WJ ? tw[R0 * (l % Z::A) * b1] : make_double2(1.0, 0.0);
+  static_assert(W % Z::A == 0 && R0 * (Z::A - 1) * (Z::B - 1) < L, "pass-1 twiddles");
+  if constexpr (!WJ)
+    for (int i = lane; i < L; i += NTB) twl[i] = tw[i];
 
   // ---- inverse transforms, G pencils per round; lane l keeps x_n, n = l + W m, of
   // every pencil
@@ -174,7 +183,7 @@ __global__ __launch_bounds__((64 * zs_waves<L, MODE>())) void zsub(ZArgs a, cons
   double2 x[NCI][R0];
   double2 raw[2][R0][1][NF];
   load_round(0, raw[0]);
-  wave_sync();  // the LDS twiddle table
+  if constexpr (!WJ) wave_sync();  // the LDS twiddle table
 #pragma unroll
   for (int q0 = 0; q0 < NCI; q0 += G) {
     const int cur = (q0 / G) & 1;
@@ -198,7 +207,7 @@ __global__ __launch_bounds__((64 * zs_waves<L, MODE>())) void zsub(ZArgs a, cons
     }
     if (q0 + G < NCI) load_round(q0 + G, raw[cur ^ 1]);
     wave_sync();
-    zs_subW<L, +1>(sb, twl, l, np);
+    zs_subW<L, +1, WJ>(sb, wj, twl, l, np);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int q = q0 + g;
@@ -289,7 +298,7 @@ __global__ __launch_bounds__((64 * zs_waves<L, MODE>())) void zsub(ZArgs a, cons
       }
     }
     wave_sync();
-    zs_subW<L, -1>(sb, twl, l, np);
+    zs_subW<L, -1, WJ>(sb, wj, twl, l, np);
     if (act) {
 #pragma unroll
       for (int g = 0; g < G; ++g) {
