@@ -60,6 +60,16 @@ __host__ __device__ constexpr bool zs_zero_chunk(int m) {
   return true;
 }
 
+// Forward transforms store only the outputs n < L/3 and n > 2L/3 (the 3/2 rule's kept
+// modes k and L - k, k < L/3); pass 2 writes x_n, n = a + R0 (b1 + B b2), for every a,
+// b1 of a given b2 -- that write is skipped when all of those n fall in between.
+template <int L, int SIGN>
+__host__ __device__ constexpr bool zs_unread(int b2) {
+  using Z = ZSubShape<L>;
+  const int lo = Z::R0 * Z::B * b2, hi = lo + Z::R0 * Z::B - 1;
+  return SIGN < 0 && lo >= L / 3 && hi <= L - L / 3;
+}
+
 // The W-point transforms of the R0 sub-transforms of np <= G pencils, pencil g's held
 // at sb[g SP + a SUBW + l]; the result x_{a + R0 b} of pencil g lands at
 // sb[g SP + nat(a + R0 b)].  Pass-1 twiddles w_W^{j b1} (forward sign) of the lane's
@@ -123,7 +133,8 @@ __device__ __forceinline__ void zs_subW(double2* sb, const double2 (&wj)[ZSub<L>
     if ((N2 % W == 0 || u < N2) && ga < np * R0) {
       const int g = ga / R0, a2 = ga - g * R0;
 #pragma unroll
-      for (int b2 = 0; b2 < A; ++b2) sb[g * SP + Z::nat(a2 + R0 * (b1 + B * b2))] = z[it][b2];
+      for (int b2 = 0; b2 < A; ++b2)
+        if (!zs_unread<L, SIGN>(b2)) sb[g * SP + Z::nat(a2 + R0 * (b1 + B * b2))] = z[it][b2];
     }
   }
   wave_sync();
