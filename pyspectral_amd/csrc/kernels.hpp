@@ -35,12 +35,25 @@ constexpr int MAXJOBS = 16;   // input fields per pencil pass
 constexpr int MAXREC = 12;    // real pencils built by the z pass
 
 // ------------------------------------------------------------------ small selects
-// Component index pairs (11, 22, 33, 12, 13, 23) of the symmetric tensors, and a
-// select of one of three values, written as selects so no per-thread array is
+// A select of one of three values, written as selects so no per-thread array is
 // indexed at run time (which would place it in scratch memory).
-__device__ __forceinline__ int sym_m(int g) { return g == 1 ? 1 : (g == 2 ? 2 : (g == 5 ? 1 : 0)); }
-__device__ __forceinline__ int sym_n(int g) { return g == 0 ? 0 : (g == 1 ? 1 : (g == 3 ? 1 : 2)); }
 __device__ __forceinline__ double sel3(int i, double a, double b, double c) { return i == 0 ? a : (i == 1 ? b : c); }
+
+__device__ __forceinline__ void cfma(double2& x, double w, double2 v) { x = make_double2(x.x + w * v.x, x.y + w * v.y); }
+// The k-weighted sums V_i = sum_j X_ij k_j of a symmetric tensor, one component at a
+// time: X_mn (g = 0..5: mn = 11 22 33 12 13 23) adds k_n v to V_m and, off the diagonal,
+// X_nm = X_mn adds k_m v to V_n.  g is the same for the whole block, so the branch is
+// taken by the scalar unit and only those one or two complex multiply-adds run.
+__device__ __forceinline__ void sym_accumulate(int g, const double* k, double2 v, double2* V) {
+  switch (g) {
+    case 0: cfma(V[0], k[0], v); break;
+    case 1: cfma(V[1], k[1], v); break;
+    case 2: cfma(V[2], k[2], v); break;
+    case 3: cfma(V[0], k[1], v); cfma(V[1], k[0], v); break;
+    case 4: cfma(V[0], k[2], v); cfma(V[2], k[0], v); break;
+    default: cfma(V[1], k[2], v); cfma(V[2], k[1], v); break;
+  }
+}
 
 // Maximum that propagates NaN, as numpy's amax does (fmax drops a NaN operand, so a
 // NaN state would give a finite CFL limit where src/Classes.py:693 gives NaN).
@@ -839,24 +852,21 @@ __device__ __forceinline__ void epi_accumulate(int f, const double* k, double2 v
     }
   }
   if constexpr (EPI == E_TAU2) {
-    const int i = f / 3, j = f - 3 * (f / 3);
-    const double ki = sel3(i, k[0], k[1], k[2]), kj = sel3(j, k[0], k[1], k[2]);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const double w = (m == i ? kj : 0.0) + (m == j ? ki : 0.0);
-      V[m] = make_double2(V[m].x + w * v.x, V[m].y + w * v.y);
+    // A_ij adds k_j v to V_i and k_i v to V_j (2 k_i v on the diagonal); f is the same
+    // for the whole block (a scalar branch)
+    switch (f) {
+      case 0: cfma(V[0], 2.0 * k[0], v); break;
+      case 1: cfma(V[0], k[1], v); cfma(V[1], k[0], v); break;
+      case 2: cfma(V[0], k[2], v); cfma(V[2], k[0], v); break;
+      case 3: cfma(V[1], k[0], v); cfma(V[0], k[1], v); break;
+      case 4: cfma(V[1], 2.0 * k[1], v); break;
+      case 5: cfma(V[1], k[2], v); cfma(V[2], k[1], v); break;
+      case 6: cfma(V[2], k[0], v); cfma(V[0], k[2], v); break;
+      case 7: cfma(V[2], k[1], v); cfma(V[1], k[2], v); break;
+      default: cfma(V[2], 2.0 * k[2], v); break;
     }
   } else {
-    const int g = f % 6;
-    const int m = sym_m(g), n = sym_n(g);
-    const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
-    double2* X = (EPI == E_SMAG && f >= 6) ? Vt : V;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      // X_mn adds k_n to V_m and, off the diagonal, X_nm = X_mn adds k_m to V_n
-      const double w = (i == m ? kn : 0.0) + (i == n && m != n ? km : 0.0);
-      X[i] = make_double2(X[i].x + w * v.x, X[i].y + w * v.y);
-    }
+    sym_accumulate(f % 6, k, v, (EPI == E_SMAG && f >= 6) ? Vt : V);
   }
 }
 
@@ -1384,19 +1394,13 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
         xa = xb;
         xb = t;
       }
-      const int m = sym_m(f), n = sym_n(f);
       const int c = tl % T;
       if (c0 + c < a.C) {
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
           const int kx = tl / T + (L / 4) * it;
           const double k[3] = {a.ks[0] * signed_freq(kx, L), k2, a.ks[2] * (c0 + c)};
-          const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            const double w = (i == m ? kn : 0.0) + (i == n && m != n ? km : 0.0);
-            acc[it][i] = make_double2(acc[it][i].x + w * v[it].x, acc[it][i].y + w * v[it].y);
-          }
+          sym_accumulate(f, k, v[it], acc[it]);
         }
       }
     }
@@ -1412,20 +1416,13 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
     if (f + 1 < 6) load_field(f + 1, tl);
     __syncthreads();
     fft_batch<L, T, NT, LS, -1>(buf, twp, tl);
-    const int m = sym_m(f), n = sym_n(f);
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = tl + it * NT;
       const int kx = e / T, c = e - (e / T) * T;
       if (e < L * T && c0 + c < a.C) {
         const double k[3] = {a.ks[0] * signed_freq(kx, L), k2, a.ks[2] * (c0 + c)};
-        const double km = sel3(m, k[0], k[1], k[2]), kn = sel3(n, k[0], k[1], k[2]);
-        const double2 v = buf[c * LS + RW(kx)];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const double w = (i == m ? kn : 0.0) + (i == n && m != n ? km : 0.0);
-          acc[it][i] = make_double2(acc[it][i].x + w * v.x, acc[it][i].y + w * v.y);
-        }
+        sym_accumulate(f, k, buf[c * LS + RW(kx)], acc[it]);
       }
     }
     __syncthreads();
