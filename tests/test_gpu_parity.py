@@ -104,10 +104,12 @@ def test_rhs_long_z_matches_oracle(shape, models):
     _check_rhs_and_dt(shape, models)
 
 
-# x lengths 64 and 128: the t-model family's PL pass runs the 2x-grid x lengths 128 and
-# 256 with each thread's four entries in registers through the first and last stages
+# x lengths 64, 128 and 256: the t-model family's PL pass runs the 2x-grid x lengths 128,
+# 256 and 512 with each thread's four entries in registers through the first and last
+# stages (radix plans 4 8 4, 4 4 4 4 and 4 8 4 4)
 @pytest.mark.parametrize("shape,models", [((64, 16, 16), [2, "Static tau-model", "Dynamic tau-model"]),
-                                          ((128, 16, 32), [2, "Dynamic tau-model"])])
+                                          ((128, 16, 32), [2, "Dynamic tau-model"]),
+                                          ((256, 16, 16), [2, "Dynamic tau-model"])])
 def test_rhs_long_x_tau_family_matches_oracle(shape, models):
     _check_rhs_and_dt(shape, models)
 
