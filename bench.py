@@ -850,6 +850,9 @@ def main():
                     help="first run one device-to-device copy of this many MiB (a known-size dispatch "
                          "that calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.event_steps < 1 or args.event_steps_tmodel < 1:
+        ap.error("--event-steps and --event-steps-tmodel must be at least 1: the roofline comes from the "
+                 "per-kernel HIP-event timings of those steps")
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build advances one grid on one GPU; runs on more than one GPU are not part "
