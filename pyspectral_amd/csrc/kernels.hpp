@@ -915,7 +915,12 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
   constexpr int LS = Row<L>::LS;
   constexpr int NF = EpiShape<EPI>::nf;
   constexpr int IT = (NS1 * T + NT - 1) / NT;
-  __shared__ double2 buf[T * LS];
+  // four entries per thread and NS1 = L / 2 (the t-model family's 2x lengths 128, 256,
+  // 512): the transforms keep their first and last stages in registers (fft_rows_reg),
+  // fed by the loaded field entries; the kept outputs kx < NS1 / 2 and kx >= NS1 / 2
+  // are the last stage's outputs 0 and 3 of the thread
+  constexpr bool REG = RegPlan<L>::ok && L * T == 4 * NT && 2 * NS1 == L && EPI != E_SMAGF;
+  __shared__ double2 buf[(REG ? 2 : 1) * T * LS];
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
   const int ct = (a.H + T - 1) / T;
@@ -1068,7 +1073,38 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
   };
   // LAST: the final step, which issues the state loads in place of a next field's
   // (a compile-time flag, so that those registers are live in that step only)
+  [[maybe_unused]] double2* xa = buf;
+  [[maybe_unused]] double2* xb = buf + (REG ? T * LS : 0);
   auto run_step = [&](auto last, int s, double2 (&pre)[LT], int snext) {
+    if constexpr (REG) {
+      static_assert(LT == 4 && IT == 2, "four entries per thread, two kept");
+      double2 v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = pre[i];
+      if constexpr (decltype(last)::value)
+        load_state(0);
+      else if (snext < NSTEP)
+        load_field(snext, pre);
+      fft_rows_reg<L, T, NT, LS, -1>(v, xa, xb, twp, tid);
+      if constexpr (fft_rows_reg_swap<L>()) {
+        double2* t = xa;
+        xa = xb;
+        xb = t;
+      }
+      if constexpr (decltype(last)::value) load_state(1);
+      const int c = tid % T;
+      if (c0 + c < a.H) {
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+          const int kx = tid / T + (L / 4) * it;  // slot kx (it = 0) or L / 2 + kx (it = 1)
+          if (pad_slot(kx, NS1, L, a.keep_nyq) >= 0) {
+            const double k[3] = {a.ks[0] * signed_freq(kx, NS1), k2, a.ks[2] * (c0 + c)};
+            epi_accumulate<EPI>(s, k, v[it == 0 ? 0 : 3], acc[it], accs[VT ? it : 0], acct[ST ? it : 0]);
+          }
+        }
+      }
+      return;
+    }
     const bool fold = EPI == E_SMAGF && s >= 7;
     const bool add = fold && (s == 8 || s == 10 || s == 11);
     const bool transform = !fold || s == 8 || s == 11;
