@@ -257,25 +257,28 @@ __device__ __forceinline__ void fft_batch(double2* buf, const double2* __restric
 }
 
 // ------------------------------------------------------- register-resident batch
-// Radix sequences whose first and last stages are radix 4, for batches where each
-// thread holds exactly four entries (B L = 4 NT).
-template <int L> struct RegPlan { static constexpr bool ok = false; };
-#define PSX_REGPLAN(LEN, ...)                                     \
-  template <> struct RegPlan<LEN> {                               \
+// Radix sequences whose first and last stages are radix E, for batches where each
+// thread holds exactly E entries (B L = E NT).
+template <int L, int E> struct RegPlan { static constexpr bool ok = false; };
+#define PSX_REGPLAN(LEN, E, ...)                                  \
+  template <> struct RegPlan<LEN, E> {                            \
     static constexpr bool ok = true;                              \
     static constexpr int r[] = {__VA_ARGS__};                     \
     static constexpr int n = sizeof(r) / sizeof(int);             \
   };
-PSX_REGPLAN(128, 4, 8, 4)
-PSX_REGPLAN(256, 4, 4, 4, 4)
-PSX_REGPLAN(512, 4, 8, 4, 4)
+PSX_REGPLAN(128, 4, 4, 8, 4)
+PSX_REGPLAN(256, 4, 4, 4, 4, 4)
+PSX_REGPLAN(512, 4, 4, 8, 4, 4)
+PSX_REGPLAN(256, 8, 8, 4, 8)
+PSX_REGPLAN(512, 8, 8, 8, 8)
 #undef PSX_REGPLAN
 
 // One middle stage S of fft_rows_reg: read `in`, twiddle, radix R, write `out`, barrier;
-// the next stage reads `out`.
-template <int L, int B, int NT, int LS, int SIGN, int S, int NS>
+// the next stage reads `out`.  In place (in == out), a barrier separates the reads from
+// the writes.
+template <int L, int E, int B, int NT, int LS, int SIGN, bool INPLACE, int S, int NS>
 __device__ __forceinline__ void reg_mid(double2* in, double2* out, const double2* __restrict__ tw, int tid) {
-  using P = RegPlan<L>;
+  using P = RegPlan<L, E>;
   if constexpr (S < P::n - 1) {
     constexpr int R = P::r[S];
     constexpr int NBF = L / R, TOT = B * NBF, IT = (TOT + NT - 1) / NT, TWS = L / (NS * R);
@@ -298,6 +301,7 @@ __device__ __forceinline__ void reg_mid(double2* in, double2* out, const double2
         Dft<R, SIGN>::run(v[it]);
       }
     }
+    if constexpr (INPLACE) __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int j = tid + it * NT;
@@ -309,50 +313,55 @@ __device__ __forceinline__ void reg_mid(double2* in, double2* out, const double2
       }
     }
     __syncthreads();
-    reg_mid<L, B, NT, LS, SIGN, S + 1, NS * R>(out, in, tw, tid);
+    reg_mid<L, E, B, NT, LS, SIGN, INPLACE, S + 1, NS * R>(out, in, tw, tid);
   }
 }
 
-// Stockham transforms of B rows of length L (B L = 4 NT) with the data in registers on
+// Stockham transforms of B rows of length L (B L = E NT) with the data in registers on
 // entry and exit: thread t owns row b = t % B and holds its entries
-// n = t / B + (L / 4) e, e < 4 -- on entry the inputs, on return the outputs at the
-// same indices (the layout of a row-batched load of B contiguous columns, e = t + e NT
-// split as n = e / B, b = e % B).  The radix-4 first and last stages run on the
-// registers; the middle stages pass through the LDS buffers x0 and x1 (B rows of stride
-// LS each, Row<L> padding) in turn, one barrier per stage.  No barrier is needed before
-// the first stage: it writes x0, which no thread still reads, provided the previous
-// call's last stage read x1 -- so consecutive calls pass the buffers swapped when the
-// plan has an even number of stages (fft_rows_reg_swap).
-template <int L, int B, int NT, int LS, int SIGN>
-__device__ __forceinline__ void fft_rows_reg(double2 (&v)[4], double2* x0, double2* x1,
+// n = t / B + (L / E) e, e < E -- on entry the inputs, on return the outputs at the
+// same indices (the layout of a row-batched load of B contiguous columns, element
+// t + e NT split as n = e / B, b = e % B).  The radix-E first and last stages run on
+// the registers; the middle stages pass through LDS (B rows of stride LS each, Row<L>
+// padding), one barrier per stage.  Two buffers x0, x1 are used in turn; no barrier is
+// needed before the first stage, which writes x0, provided the previous call's last
+// stage read x1 -- so consecutive calls pass the buffers swapped when the plan has an
+// even number of stages (fft_rows_reg_swap).  INPLACE: one buffer (x1 = x0), a barrier
+// between each middle stage's reads and writes, and one on entry.
+template <int L, int E, int B, int NT, int LS, int SIGN, bool INPLACE = false>
+__device__ __forceinline__ void fft_rows_reg(double2 (&v)[E], double2* x0, double2* x1,
                                              const double2* __restrict__ tw, int tid) {
-  using P = RegPlan<L>;
-  static_assert(P::ok && B * L == 4 * NT, "four entries per thread");
-  static_assert(P::r[0] == 4 && P::r[P::n - 1] == 4, "radix-4 first and last stages");
-  constexpr int Q = L / 4;
+  using P = RegPlan<L, E>;
+  static_assert(P::ok && B * L == E * NT, "E entries per thread");
+  static_assert(P::r[0] == E && P::r[P::n - 1] == E, "radix-E first and last stages");
+  constexpr int Q = L / E;
   const int b = tid % B, jj = tid / B;
-  // first stage (NS = 1): no twiddle; outputs at 4 jj + r
-  Dft<4, SIGN>::run(v);
+  if constexpr (INPLACE) {
+    x1 = x0;
+    __syncthreads();  // the previous call's last-stage reads
+  }
+  // first stage (NS = 1): no twiddle; outputs at E jj + r
+  Dft<E, SIGN>::run(v);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) x0[b * LS + Row<L>::at(4 * jj + r)] = v[r];
+  for (int r = 0; r < E; ++r) x0[b * LS + Row<L>::at(E * jj + r)] = v[r];
   __syncthreads();
-  reg_mid<L, B, NT, LS, SIGN, 1, 4>(x0, x1, tw, tid);
+  reg_mid<L, E, B, NT, LS, SIGN, INPLACE, 1, E>(x0, x1, tw, tid);
   // last stage (NS = Q): inputs jj + Q r from the buffer the last middle stage wrote,
   // twiddle w_L^{r jj}, outputs jj + Q r
   const double2* src = (P::n % 2 == 0) ? x0 : x1;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) v[r] = src[b * LS + Row<L>::at(jj + Q * r)];
+  for (int r = 0; r < E; ++r) v[r] = src[b * LS + Row<L>::at(jj + Q * r)];
   double2 w1 = tw[jj];
   if constexpr (SIGN > 0) w1.y = -w1.y;
   double2 wc = w1;
 #pragma unroll
-  for (int r = 1; r < 4; ++r) {
+  for (int r = 1; r < E; ++r) {
     v[r] = cmul(v[r], wc);
-    if (r + 1 < 4) wc = cmul(wc, w1);
+    if (r + 1 < E) wc = cmul(wc, w1);
   }
-  Dft<4, SIGN>::run(v);
+  Dft<E, SIGN>::run(v);
 }
-// whether consecutive fft_rows_reg calls swap x0 and x1 (the last stage read x0)
-template <int L> constexpr bool fft_rows_reg_swap() { return RegPlan<L>::n % 2 == 0; }
+// whether consecutive two-buffer fft_rows_reg calls swap x0 and x1 (the last stage read x0)
+template <int L, int E> constexpr bool fft_rows_reg_swap() { return RegPlan<L, E>::n % 2 == 0; }
 
 }  // namespace psx

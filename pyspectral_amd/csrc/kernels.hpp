@@ -269,11 +269,32 @@ __global__ __launch_bounds__(NT) void pencil_forward(PencilArgs a, const double2
   const double2* src = a.in + mem * a.in_ms + (long long)a.job_in[job] * a.in_sf + (long long)aidx * a.in_sa + c0;
   double2 r[PencilLoad<L, T>::IT];
   PencilLoad<L, T>::load(r, src, a.in_sl, a.C - c0, tid);
+  const int fo = a.job_plain[job];
+  double2* dst = a.out + mem * a.out_ms + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
+  // eight entries per thread (2x lengths 256, 512 at 8 and 4 columns): the loaded
+  // entries n = t / T + (L / 8) e feed a radix-8 first stage and the radix-8 last
+  // stage's outputs p = t / T + (L / 8) e go straight to their output rows (in-place
+  // middle stages: the one LDS buffer keeps the block's occupancy)
+  if constexpr (RegPlan<L, 8>::ok && L * T == 8 * NT) {
+    fft_rows_reg<L, 8, T, NT, LS, -1, true>(r, buf, buf, twp, tid);
+    const int b = tid % T, jj = tid / T;
+    if (c0 + b < a.C) {
+      const int half = a.Nspec / 2, gap = L - a.Nspec;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        // the output row whose slot is p (pad_slot's inverse), or -1 for a dropped slot
+        const int p = jj + (L / 8) * e;
+        const int i = p < half ? p : (p >= L - half ? p - gap : -1);
+        if (i >= 0)
+          dst[(long long)i * a.out_sl + b] =
+              pad_slot(i, a.Nspec, L, a.keep_nyq) == p ? cscale(r[e], a.scale) : make_double2(0.0, 0.0);
+      }
+    }
+    return;
+  }
   PencilLoad<L, T>::stage(r, buf, tid);
   __syncthreads();
   fft_batch<L, T, NT, LS, -1>(buf, twp, tid);
-  const int fo = a.job_plain[job];
-  double2* dst = a.out + mem * a.out_ms + (long long)fo * a.out_sf + (long long)aidx * a.out_sa + c0;
   for (int e = tid; e < a.Nspec * T; e += NT) {
     const int i = e / T, c = e - (e / T) * T;
     if (c0 + c < a.C) {
@@ -919,7 +940,7 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
   // 512): the transforms keep their first and last stages in registers (fft_rows_reg),
   // fed by the loaded field entries; the kept outputs kx < NS1 / 2 and kx >= NS1 / 2
   // are the last stage's outputs 0 and 3 of the thread
-  constexpr bool REG = RegPlan<L>::ok && L * T == 4 * NT && 2 * NS1 == L && EPI != E_SMAGF;
+  constexpr bool REG = RegPlan<L, 4>::ok && L * T == 4 * NT && 2 * NS1 == L && EPI != E_SMAGF;
   __shared__ double2 buf[(REG ? 2 : 1) * T * LS];
   const int tid = threadIdx.x;
   PSX_STAGE_TW(NT)
@@ -1085,8 +1106,8 @@ __global__ __launch_bounds__(NT) void xforward_epilogue(EpiArgs a, const double2
         load_state(0);
       else if (snext < NSTEP)
         load_field(snext, pre);
-      fft_rows_reg<L, T, NT, LS, -1>(v, xa, xb, twp, tid);
-      if constexpr (fft_rows_reg_swap<L>()) {
+      fft_rows_reg<L, 4, T, NT, LS, -1>(v, xa, xb, twp, tid);
+      if constexpr (fft_rows_reg_swap<L, 4>()) {
         double2* t = xa;
         xa = xb;
         xb = t;
@@ -1376,7 +1397,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   // last stages in registers (fft_rows_reg) on two LDS buffers, so the loaded field
   // entries feed the first stage and the last stage's outputs feed the accumulation,
   // with no staging pass in between
-  constexpr bool REG = RegPlan<L>::ok && L * T == 4 * NT;
+  constexpr bool REG = RegPlan<L, 4>::ok && L * T == 4 * NT;
   static_assert(T * LS >= NT, "block_sum2 needs NT LDS entries");
   __shared__ double2 buf[(REG ? 2 : 1) * T * LS];
   const int tid = threadIdx.x;
@@ -1424,8 +1445,8 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
 #pragma unroll
       for (int it = 0; it < 4; ++it) v[it] = pre[it];
       if (f + 1 < 6) load_field(f + 1, tl);
-      fft_rows_reg<L, T, NT, LS, -1>(v, xa, xb, twp, tl);
-      if constexpr (fft_rows_reg_swap<L>()) {
+      fft_rows_reg<L, 4, T, NT, LS, -1>(v, xa, xb, twp, tl);
+      if constexpr (fft_rows_reg_swap<L, 4>()) {
         double2* t = xa;
         xa = xb;
         xb = t;
@@ -1544,8 +1565,8 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
         const bool inside = (s1 > -a.kpx && s1 < a.kpx && s2 > -a.kpy && s2 < a.kpy && c0 + b < a.kpz);
         v[it] = (valid && !inside) ? acc[it][i] : make_double2(0.0, 0.0);
       }
-      fft_rows_reg<L, T, NT, LS, +1>(v, xa, xb, twp, tid);
-      if constexpr (fft_rows_reg_swap<L>()) {
+      fft_rows_reg<L, 4, T, NT, LS, +1>(v, xa, xb, twp, tid);
+      if constexpr (fft_rows_reg_swap<L, 4>()) {
         double2* t = xa;
         xa = xb;
         xb = t;
