@@ -6,7 +6,7 @@ namespace psx {
 #define PSX_LENGTHS2X(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
 unsigned pl_pass_blocks(int L, const PLArgs& a) {
-  const int T = epi_T(L);
+  const int T = pl_T(L);
   return (unsigned)(a.P2 * ((a.C + T - 1) / T));
 }
 
@@ -16,7 +16,7 @@ hipError_t launch_pl_pass(int L, bool dyn, const PLArgs& a, const double2* tw, h
     switch (L) {
 #define PSX_CASE(LL)                                                                           \
   case LL:                                                                                     \
-    hipLaunchKernelGGL((pl_pass<LL, epi_T(LL), true>), grid, dim3(NT), 0, s, a, tw);           \
+    hipLaunchKernelGGL((pl_pass<LL, pl_T(LL), true, pl_NT(LL)>), grid, dim3(pl_NT(LL)), 0, s, a, tw);           \
     return hipGetLastError();
       PSX_LENGTHS2X(PSX_CASE)
 #undef PSX_CASE
@@ -27,7 +27,7 @@ hipError_t launch_pl_pass(int L, bool dyn, const PLArgs& a, const double2* tw, h
   switch (L) {
 #define PSX_CASE(LL)                                                                           \
   case LL:                                                                                     \
-    hipLaunchKernelGGL((pl_pass<LL, epi_T(LL), false>), grid, dim3(NT), 0, s, a, tw);          \
+    hipLaunchKernelGGL((pl_pass<LL, pl_T(LL), false, pl_NT(LL)>), grid, dim3(pl_NT(LL)), 0, s, a, tw);          \
     return hipGetLastError();
     PSX_LENGTHS2X(PSX_CASE)
 #undef PSX_CASE

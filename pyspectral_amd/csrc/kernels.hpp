@@ -781,11 +781,12 @@ __global__ __launch_bounds__(NTZ) void zpass(ZArgs a, const double2* __restrict_
 
 // Sum a complex value over the block (deterministic tree order); uses >= NT
 // entries of the scratch LDS buffer, which must be free.  Result valid in tid 0.
+template <int NTH = NT>
 __device__ __forceinline__ double2 block_sum2(double2 v, double2* scratch, int tid) {
   __syncthreads();
   scratch[tid] = v;
   __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
+  for (int s = NTH / 2; s > 0; s >>= 1) {
     if (tid < s) scratch[tid] = cadd(scratch[tid], scratch[tid + s]);
     __syncthreads();
   }
@@ -1389,19 +1390,21 @@ struct PLArgs {
   double* part;           // per-block (re, im) partial of sum_w (G - g_ref) . conj(uf)
 };
 
-template <int L, int T, bool DYN>
-__global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restrict__ tw) {
+template <int L, int T, bool DYN, int NTH>
+__global__ __launch_bounds__(NTH) void pl_pass(PLArgs a, const double2* __restrict__ tw) {
   constexpr int LS = Row<L>::LS;
-  constexpr int IT = (L * T + NT - 1) / NT;
+  constexpr int IT = (L * T + NTH - 1) / NTH;
   // four entries per thread (L = 128, 256, 512): the transforms keep their first and
   // last stages in registers (fft_rows_reg) on two LDS buffers, so the loaded field
   // entries feed the first stage and the last stage's outputs feed the accumulation,
   // with no staging pass in between
-  constexpr bool REG = RegPlan<L, 4>::ok && L * T == 4 * NT;
-  static_assert(T * LS >= NT, "block_sum2 needs NT LDS entries");
-  __shared__ double2 buf[(REG ? 2 : 1) * T * LS];
+  constexpr bool REG = RegPlan<L, 4>::ok && L * T == 4 * NTH;
+  // one buffer used in place where two would exceed 64 KiB (8 columns at length 256)
+  constexpr bool INPL = REG && 2 * T * LS * 16 > 65536;
+  static_assert(T * LS >= NTH, "block_sum2 needs NTH LDS entries");
+  __shared__ double2 buf[(REG && !INPL ? 2 : 1) * T * LS];
   const int tid = threadIdx.x;
-  PSX_STAGE_TW(NT)
+  PSX_STAGE_TW(NTH)
   const int ct = (a.C + T - 1) / T;
   // contiguous block ranges per XCD: the column tiles of one k2 row (T = 4 at L = 256:
   // 64-byte row segments, two tiles per 128-byte line) share one L2
@@ -1420,21 +1423,21 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
 #pragma unroll
     for (int i = 0; i < 3; ++i) acc[it][i] = make_double2(0.0, 0.0);
   // field f + 1 is loaded into registers while field f is transformed
-  constexpr int LT = (L * T + NT - 1) / NT;
+  constexpr int LT = (L * T + NTH - 1) / NTH;
   double2 pre[LT];
   auto load_field = [&](int f, int tl) {
     const double2* src = src0 + (long long)f * a.in_sf;
 #pragma unroll
     for (int i = 0; i < LT; ++i) {
-      const int e = tl + i * NT;
+      const int e = tl + i * NTH;
       const int l = e / T, c = e - (e / T) * T;
-      pre[i] = ((L * T) % NT == 0 || e < L * T) && (c0 + c < a.C) ? src[(long long)l * in_sl + c]
+      pre[i] = ((L * T) % NTH == 0 || e < L * T) && (c0 + c < a.C) ? src[(long long)l * in_sl + c]
                                                                     : make_double2(0.0, 0.0);
     }
   };
   load_field(0, tid);
   [[maybe_unused]] double2* xa = buf;
-  [[maybe_unused]] double2* xb = buf + (REG ? T * LS : 0);
+  [[maybe_unused]] double2* xb = buf + (REG && !INPL ? T * LS : 0);
   if constexpr (REG) {
     static_assert(LT == 4 && IT == 4, "four entries per thread");
 #pragma unroll 1
@@ -1445,8 +1448,8 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
 #pragma unroll
       for (int it = 0; it < 4; ++it) v[it] = pre[it];
       if (f + 1 < 6) load_field(f + 1, tl);
-      fft_rows_reg<L, 4, T, NT, LS, -1>(v, xa, xb, twp, tl);
-      if constexpr (fft_rows_reg_swap<L, 4>()) {
+      fft_rows_reg<L, 4, T, NTH, LS, -1, INPL>(v, xa, xb, twp, tl);
+      if constexpr (!INPL && fft_rows_reg_swap<L, 4>()) {
         double2* t = xa;
         xa = xb;
         xb = t;
@@ -1467,15 +1470,15 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
     const int tl = opaque(tid);  // addresses are recomputed for every field
 #pragma unroll
     for (int i = 0; i < LT; ++i) {
-      const int e = tl + i * NT;
-      if ((L * T) % NT == 0 || e < L * T) buf[(e - (e / T) * T) * LS + RW(e / T)] = pre[i];
+      const int e = tl + i * NTH;
+      if ((L * T) % NTH == 0 || e < L * T) buf[(e - (e / T) * T) * LS + RW(e / T)] = pre[i];
     }
     if (f + 1 < 6) load_field(f + 1, tl);
     __syncthreads();
-    fft_batch<L, T, NT, LS, -1>(buf, twp, tl);
+    fft_batch<L, T, NTH, LS, -1>(buf, twp, tl);
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int e = tl + it * NT;
+      const int e = tl + it * NTH;
       const int kx = e / T, c = e - (e / T) * T;
       if (e < L * T && c0 + c < a.C) {
         const double k[3] = {a.ks[0] * signed_freq(kx, L), k2, a.ks[2] * (c0 + c)};
@@ -1492,7 +1495,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   double2 red = make_double2(0.0, 0.0);
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    const int e = tid + it * NT;
+    const int e = tid + it * NTH;
     const int kx = e / T, c = e - (e / T) * T;
     const int kz = c0 + c;
     if (e < L * T && kz < a.C) {
@@ -1543,7 +1546,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
   }
   if constexpr (DYN) {
     if (a.part) {
-      const double2 tot = block_sum2(red, buf, tid);
+      const double2 tot = block_sum2<NTH>(red, buf, tid);
       if (tid == 0) {
         a.part[2 * bid] = tot.x;
         a.part[2 * bid + 1] = tot.y;
@@ -1565,8 +1568,8 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
         const bool inside = (s1 > -a.kpx && s1 < a.kpx && s2 > -a.kpy && s2 < a.kpy && c0 + b < a.kpz);
         v[it] = (valid && !inside) ? acc[it][i] : make_double2(0.0, 0.0);
       }
-      fft_rows_reg<L, 4, T, NT, LS, +1>(v, xa, xb, twp, tid);
-      if constexpr (fft_rows_reg_swap<L, 4>()) {
+      fft_rows_reg<L, 4, T, NTH, LS, +1, INPL>(v, xa, xb, twp, tid);
+      if constexpr (!INPL && fft_rows_reg_swap<L, 4>()) {
         double2* t = xa;
         xa = xb;
         xb = t;
@@ -1584,7 +1587,7 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * NT;
+      const int e = tid + it * NTH;
       if (e < L * T) {
         const int kx = e / T, c = e - (e / T) * T;
         const int s1 = signed_freq(kx, L);
@@ -1594,9 +1597,9 @@ __global__ __launch_bounds__(NT) void pl_pass(PLArgs a, const double2* __restric
       }
     }
     __syncthreads();
-    fft_batch<L, T, NT, LS, +1>(buf, twp, tid);
+    fft_batch<L, T, NTH, LS, +1>(buf, twp, tid);
     double2* dst = a.xq + (long long)i * a.P1 * a.P2 * a.C + (long long)ky * a.C + c0;
-    for (int e = tid; e < L * T; e += NT) {
+    for (int e = tid; e < L * T; e += NTH) {
       const int x = e / T, c = e - (e / T) * T;
       if (c0 + c < a.C) dst[(long long)x * in_sl + c] = buf[c * LS + RW(x)];
     }

@@ -12,6 +12,10 @@ namespace psx {
 // sized so a block's LDS buffer stays near 48 KiB.
 constexpr int pencil_T(int L) { return L <= 96 ? 16 : (L <= 384 ? 8 : 4); }
 constexpr int epi_T(int L) { return L <= 192 ? 8 : (L <= 384 ? 4 : 2); }
+// PL pass (t-model family): columns and threads per block -- at length 256, 8 columns
+// and 512 threads (four entries per thread, 128-byte row segments)
+constexpr int pl_T(int L) { return L == 256 ? 8 : epi_T(L); }
+constexpr int pl_NT(int L) { return L == 256 ? 512 : NT; }
 // z pass: positions per block (pairing two positions, or two fields when P = 1)
 // and threads per block, sized to the first (radix-8 or -4) stage's butterflies.
 template <int MODE> constexpr int z_P(int L) {
