@@ -998,7 +998,7 @@ void stage_tau(psx_context* c, const double2* uin, const double2* q0, double2* o
   ea.H = c->H;
   const int nLE = (int)pl_pass_blocks(P1, pa);
   const int nE = (int)epilogue_blocks(P1, ea);
-  hipLaunchKernelGGL(tau_finalize, dim3(1), dim3(1), 0, c->stream, c->partLE, nLE, c->partA, nE, c->partB, nE,
+  hipLaunchKernelGGL(tau_finalize, dim3(1), dim3(NT), 0, c->stream, c->partLE, nLE, c->partA, nE, c->partB, nE,
                      1.0 / ((double)N1 * N2 * N3), c->scal + 1);
   check(c, hipGetLastError(), "tau_finalize");
   hipLaunchKernelGGL(tau_update, dim3(eb), dim3(256), 0, c->stream, c->pln, c->plql, c->scal + 2, uin, q0, out, w0, dt,

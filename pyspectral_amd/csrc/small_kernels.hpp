@@ -196,13 +196,30 @@ __global__ void filter_copy(const double2* __restrict__ in, double2* __restrict_
 // Dynamic tau from the three reductions (src/RHSfunctions.py:822-861):
 // tau = |LE| / (|PLQLE_filt| - |filt_PLQLE| + 1e-5), used clamped at 0.
 // scal[0] raw tau, scal[1] clamped tau, scal[2..7] LE, PLQLE_filt, filt_PLQLE (re, im).
-__global__ void tau_finalize(const double* __restrict__ pLE, int nLE, const double* __restrict__ pA, int nA,
-                             const double* __restrict__ pB, int nB, double inv_nt, double* __restrict__ scal) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double le[2] = {0.0, 0.0}, a[2] = {0.0, 0.0}, b[2] = {0.0, 0.0};
-    for (int i = 0; i < nLE; ++i) { le[0] += pLE[2 * i]; le[1] += pLE[2 * i + 1]; }
-    for (int i = 0; i < nA; ++i) { a[0] += pA[2 * i]; a[1] += pA[2 * i + 1]; }
-    for (int i = 0; i < nB; ++i) { b[0] += pB[2 * i]; b[1] += pB[2 * i + 1]; }
+// One block of NT threads: each thread sums a strided share of each partial array,
+// then a fixed-order tree over the block (the same result for a given partial count).
+__global__ __launch_bounds__(NT) void tau_finalize(const double* __restrict__ pLE, int nLE,
+                                                   const double* __restrict__ pA, int nA,
+                                                   const double* __restrict__ pB, int nB, double inv_nt,
+                                                   double* __restrict__ scal) {
+  __shared__ double sh[6][NT];
+  const int tid = threadIdx.x;
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = tid; i < nLE; i += NT) { v[0] += pLE[2 * i]; v[1] += pLE[2 * i + 1]; }
+  for (int i = tid; i < nA; i += NT) { v[2] += pA[2 * i]; v[3] += pA[2 * i + 1]; }
+  for (int i = tid; i < nB; i += NT) { v[4] += pB[2 * i]; v[5] += pB[2 * i + 1]; }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sh[k][tid] = v[k];
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sh[k][tid] += sh[k][tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double le[2] = {sh[0][0], sh[1][0]}, a[2] = {sh[2][0], sh[3][0]}, b[2] = {sh[4][0], sh[5][0]};
     for (int j = 0; j < 2; ++j) { le[j] *= inv_nt; a[j] *= inv_nt; b[j] *= inv_nt; }
     const double tau = hypot(le[0], le[1]) / (hypot(a[0], a[1]) - hypot(b[0], b[1]) + 1.0e-5);
     scal[0] = tau;
