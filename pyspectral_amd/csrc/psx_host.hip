@@ -771,7 +771,7 @@ void stage_dsmag(psx_context* c, const double2* uin, const double2* q0, double2*
     hipLaunchKernelGGL(dsmag_partials, dim3(c->ndsPart), dim3(NT), 0, c->stream, (const double2*)c->dsD, c->dsPart,
                        c->N1, c->N2, c->H, kL);
     check(c, hipGetLastError(), "dsmag_partials");
-    hipLaunchKernelGGL(dsmag_finalize, dim3(1), dim3(1), 0, c->stream, (const double*)c->dsPart, c->ndsPart,
+    hipLaunchKernelGGL(dsmag_finalize, dim3(1), dim3(NT), 0, c->stream, (const double*)c->dsPart, c->ndsPart,
                        c->scal + 1);
     check(c, hipGetLastError(), "dsmag_finalize");
   }

@@ -399,16 +399,30 @@ __global__ __launch_bounds__(NT) void dsmag_partials(const double2* __restrict__
   }
 }
 
-// cs2[0] = cs2[1] = sum num / sum den (partials summed in a fixed order)
-__global__ void dsmag_finalize(const double* __restrict__ part, int nparts, double* __restrict__ cs2) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double n = 0.0, d = 0.0;
-    for (int i = 0; i < nparts; ++i) {
-      n += part[2 * i];
-      d += part[2 * i + 1];
+// cs2[0] = cs2[1] = sum num / sum den; one block of NT threads sums strided shares of the
+// partials, then combines them in a fixed-order tree.
+__global__ __launch_bounds__(NT) void dsmag_finalize(const double* __restrict__ part, int nparts,
+                                                     double* __restrict__ cs2) {
+  __shared__ double sh[2][NT];
+  const int tid = threadIdx.x;
+  double n = 0.0, d = 0.0;
+  for (int i = tid; i < nparts; i += NT) {
+    n += part[2 * i];
+    d += part[2 * i + 1];
+  }
+  sh[0][tid] = n;
+  sh[1][tid] = d;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      sh[0][tid] += sh[0][tid + st];
+      sh[1][tid] += sh[1][tid + st];
     }
-    cs2[0] = n / d;
-    cs2[1] = n / d;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    cs2[0] = sh[0][0] / sh[1][0];
+    cs2[1] = sh[0][0] / sh[1][0];
   }
 }
 
