@@ -68,7 +68,7 @@ def kernel_bytes(N, closure, fused=False, sums_in_y=True):
     }
 
 
-def survey_bytes_per_step(N, closure):
+def pencil_model_bytes_per_step(N, closure):
     """The 3-pass pencil model's algorithmic bytes per RK4 step (4 stages):
     DNS 16[h(3N^2+18MN+18M^2)+9N^2Nc], Smagorinsky 16[h(3N^2+24MN+30M^2)+9N^2Nc];
     the learned closure adds a second read of 3 velocity pencils, the float32
@@ -249,7 +249,7 @@ def bench_ensemble(args):
             s.close()
         unbatched = {"value": round(pts * ns / el1 / 1e6, 2), "ms_per_step": round(el1 / ns * 1e3, 3),
                      "how": "the same 96 members, one SpectralSolver.step each, eager launches"}
-    bytes_step = sum(survey_bytes_per_step(n, "smagorinsky") for n in sizes)
+    bytes_step = sum(pencil_model_bytes_per_step(n, "smagorinsky") for n in sizes)
     return {
         "metric": "Mpts*RK4-steps/s, ensemble of 96 LES members (32x 32^3, 48^3, 64^3)",
         "value": round(value, 2), "unit": "Mpts*RK4-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -961,7 +961,7 @@ def main():
     solver.close()
     del q, qs
 
-    sb = survey_bytes_per_step(N, args.closure)
+    sb = pencil_model_bytes_per_step(N, args.closure)
     out = {
         "metric": "Mpts*RK4-steps/s at 128^3 FP64 LES",
         "value": round(value, 2),
